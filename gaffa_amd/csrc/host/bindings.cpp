@@ -1,0 +1,693 @@
+// pybind11 module gaffa_amd._core: the Python surface of gaffa's _core for
+// the search hot path (src/gaffa/python/dedispersion_bindings.cpp:594-672,
+// ffa_bindings.cpp:169-217), running on the GPU through the C++ host layer.
+// Numpy inputs are zero-copy views that must be C-contiguous; outputs own
+// their memory through a capsule; the GIL is released around device work.
+#include "gaffa/dedispersion.h"
+#include "gaffa/ffa_gpu.h"
+#include "gaffa/pipeline.h"
+#include "gaffa/preprocessing.h"
+#include "gaffa/synth.h"
+
+#include <pybind11/numpy.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <optional>
+#include <string>
+#include <vector>
+
+namespace py = pybind11;
+using namespace gaffa;
+
+namespace {
+
+// "gpu" is the backend; "cuda" is accepted as an alias so gaffa scripts run
+// unchanged.  gaffa_amd has no CPU backend: its CPU path is the test oracle.
+std::string parse_backend(const std::string& value, const char* what) {
+  if (value == "gpu" || value == "cuda") return "gpu";
+  if (value == "cpu") {
+    throw py::value_error(std::string(what) +
+                          " backend 'cpu' is not provided by gaffa_amd; use backend='gpu'");
+  }
+  throw py::value_error(std::string("unknown ") + what + " backend: " + value);
+}
+
+template <typename T>
+py::array_t<T> owning_array(std::vector<T>&& v, std::vector<py::ssize_t> shape) {
+  auto* owner = new std::vector<T>(std::move(v));
+  py::capsule cap(owner, [](void* p) { delete static_cast<std::vector<T>*>(p); });
+  std::vector<py::ssize_t> strides(shape.size());
+  py::ssize_t s = sizeof(T);
+  for (std::size_t i = shape.size(); i-- > 0;) {
+    strides[i] = s;
+    s *= shape[i];
+  }
+  return py::array_t<T>(shape, strides, owner->data(), cap);
+}
+
+void require_c_contiguous(const py::buffer_info& info, const char* name) {
+  py::ssize_t expect = info.itemsize;
+  for (py::ssize_t i = info.ndim; i-- > 0;) {
+    if (info.shape[i] > 1 && info.strides[i] != expect) {
+      throw py::value_error(std::string(name) +
+                            " must be C-contiguous; use numpy.ascontiguousarray explicitly if a "
+                            "copy is intended");
+    }
+    expect *= info.shape[i];
+  }
+}
+
+// ------------------------------------------------------------ dedispersion --
+struct PyDedispersedResult {
+  py::array data;
+  std::string backend;
+  double dm_low = 0.0;
+  double dm_step = 0.0;
+  std::size_t ndm = 0;
+  std::size_t nsamples = 0;
+  double tsamp = 0.0;
+};
+
+struct FilterbankInput {
+  HostSampleView<std::uint8_t> samples;
+  std::vector<double> freq;
+  double tsamp = 0.0;
+  py::array keep;
+};
+
+// Accepts gaffa_amd.io.Filterbank (or any object with .header and .data):
+// data is (nsamples, nifs, nchans) C-contiguous, header carries tsamp, nbits
+// and the ascending frequency_table (dedispersion_bindings.cpp:224-259).
+FilterbankInput filterbank_input(const py::object& fb) {
+  if (!py::hasattr(fb, "header") || !py::hasattr(fb, "data")) {
+    throw py::type_error("filterbank must provide .header and .data");
+  }
+  const py::object header = fb.attr("header");
+  const int nbits = header.attr("nbits").cast<int>();
+  if (nbits != 8) {
+    throw py::type_error("GPU dedispersion supports 8-bit filterbanks (uint8 samples)");
+  }
+  FilterbankInput in;
+  in.tsamp = header.attr("tsamp").cast<double>();
+  if (!(in.tsamp > 0.0) || !std::isfinite(in.tsamp)) {
+    throw std::invalid_argument("filterbank tsamp must be positive");
+  }
+  in.freq = header.attr("frequency_table").cast<std::vector<double>>();
+  py::object data = fb.attr("data");
+  if (!py::isinstance<py::array>(data)) throw py::type_error("filterbank data must be a numpy.ndarray");
+  in.keep = py::reinterpret_borrow<py::array>(data);
+  const py::buffer_info info = in.keep.request();
+  if (info.ndim != 3) throw std::invalid_argument("filterbank data must be a 3D numpy array");
+  if (info.itemsize != 1 || info.format != py::format_descriptor<std::uint8_t>::format()) {
+    throw std::invalid_argument("filterbank data dtype does not match header");
+  }
+  if (info.strides[2] != 1 || info.strides[1] != info.shape[2] ||
+      info.strides[0] != info.shape[1] * info.shape[2]) {
+    throw std::invalid_argument("filterbank data must be C-contiguous in (time, if, channel) order");
+  }
+  const SampleShape shape{static_cast<std::size_t>(info.shape[0]),
+                          static_cast<std::size_t>(info.shape[1]),
+                          static_cast<std::size_t>(info.shape[2])};
+  if (in.freq.size() != shape.nchans) {
+    throw std::invalid_argument("filterbank frequency table length must match nchans");
+  }
+  in.samples = make_host_sample_view<std::uint8_t>(
+      std::span<const std::uint8_t>(static_cast<const std::uint8_t*>(info.ptr),
+                                    sample_element_count(shape)),
+      shape);
+  return in;
+}
+
+double reference_frequency(const std::vector<double>& f) {
+  if (f.empty()) throw std::invalid_argument("dedispersion requires at least one channel");
+  return *std::max_element(f.begin(), f.end());
+}
+
+PyDedispersedResult make_result(DedispersedResult<std::uint32_t>&& r, double dm_low,
+                                double dm_step, double tsamp) {
+  PyDedispersedResult p;
+  p.ndm = r.shape.ndm;
+  p.nsamples = r.shape.nsamples;
+  p.data = owning_array(std::move(r.data), {static_cast<py::ssize_t>(p.ndm),
+                                            static_cast<py::ssize_t>(p.nsamples)});
+  p.backend = "gpu";
+  p.dm_low = dm_low;
+  p.dm_step = dm_step;
+  p.tsamp = tsamp;
+  return p;
+}
+
+PyDedispersedResult py_single(const py::object& fb, double dm, const std::string& backend,
+                              int device_id, std::size_t tpb, std::size_t tile) {
+  parse_backend(backend, "dedispersion");
+  FilterbankInput in = filterbank_input(fb);
+  const SingleDmDedispersionPlan plan{dm, reference_frequency(in.freq), in.tsamp, 0,
+                                      in.samples.shape.nchans};
+  DedispersedResult<std::uint32_t> r;
+  {
+    py::gil_scoped_release release;
+    r = dedisperse_single_dm_gpu(in.samples, in.freq, plan,
+                                 GpuDedispersionOptions{device_id, tpb, tile, nullptr});
+  }
+  return make_result(std::move(r), dm, 0.0, in.tsamp);
+}
+
+PyDedispersedResult py_multi(const py::object& fb, double dm_low, double dm_step, std::size_t ndm,
+                             const std::string& backend, int device_id, std::size_t tpb,
+                             std::size_t tile) {
+  parse_backend(backend, "dedispersion");
+  FilterbankInput in = filterbank_input(fb);
+  const MultiDmDedispersionPlan plan{dm_low, dm_step, ndm, reference_frequency(in.freq),
+                                     in.tsamp, 0, in.samples.shape.nchans};
+  DedispersedResult<std::uint32_t> r;
+  {
+    py::gil_scoped_release release;
+    r = dedisperse_multi_dm_gpu(in.samples, in.freq, plan,
+                                GpuDedispersionOptions{device_id, tpb, tile, nullptr});
+  }
+  return make_result(std::move(r), dm_low, dm_step, in.tsamp);
+}
+
+PyDedispersedResult py_subband(const py::object& fb, double dm_low, double dm_step,
+                               std::size_t ndm, const std::string& backend,
+                               std::size_t subband_channels, std::size_t ndm_per_nominal,
+                               int device_id, std::size_t tpb, std::size_t tile) {
+  parse_backend(backend, "dedispersion");
+  FilterbankInput in = filterbank_input(fb);
+  const MultiDmDedispersionPlan plan{dm_low, dm_step, ndm, reference_frequency(in.freq),
+                                     in.tsamp, 0, in.samples.shape.nchans};
+  DedispersedResult<std::uint32_t> r;
+  {
+    py::gil_scoped_release release;
+    r = dedisperse_subband_gpu(in.samples, in.freq, plan,
+                               SubbandDedispersionOptions{subband_channels, ndm_per_nominal},
+                               GpuDedispersionOptions{device_id, tpb, tile, nullptr});
+  }
+  return make_result(std::move(r), dm_low, dm_step, in.tsamp);
+}
+
+// ----------------------------------------------------------------------- FFA --
+const py::array& require_series(const py::object& obj, py::array& holder, int ndim) {
+  if (!py::isinstance<py::array>(obj)) throw py::type_error("FFA time_series must be a numpy.ndarray");
+  holder = py::reinterpret_borrow<py::array>(obj);
+  const py::buffer_info info = holder.request();
+  if (info.ndim != ndim) {
+    throw py::value_error(ndim == 1 ? "FFA time_series must be a 1D array"
+                                    : "FFA batch must be a 2D array");
+  }
+  if (info.size <= 0) throw py::value_error("FFA time_series must not be empty");
+  if (info.itemsize != 4 || info.format != py::format_descriptor<float>::format()) {
+    throw py::type_error("FFA time_series must have dtype float32");
+  }
+  require_c_contiguous(info, "FFA time_series");
+  return holder;
+}
+
+std::size_t peak_limit(const py::object& max_peaks) {
+  if (max_peaks.is_none()) return 0;
+  const auto v = max_peaks.cast<long long>();
+  if (v <= 0) throw py::value_error("max_peaks must be positive or None");
+  return static_cast<std::size_t>(v);
+}
+
+std::vector<FfaPeak> py_ffa_search(const py::object& ts, const FfaSearchPlan& plan, int device_id,
+                                   float snr_threshold, const py::object& max_peaks) {
+  py::array holder;
+  require_series(ts, holder, 1);
+  const py::buffer_info info = holder.request();
+  const std::size_t limit = peak_limit(max_peaks);
+  const std::span<const float> s(static_cast<const float*>(info.ptr),
+                                 static_cast<std::size_t>(info.shape[0]));
+  py::gil_scoped_release release;
+  return search_ffa_gpu_host(s, plan, FfaSearchOptions{snr_threshold, limit}, device_id).peaks;
+}
+
+// Batch of equal-length series (rows of a 2D float32 array) -> list of
+// (series_index, FfaPeak), sorted by (series, peak order).
+std::vector<std::pair<std::size_t, FfaPeak>> py_ffa_search_batch(
+    const py::object& batch, const FfaSearchPlan& plan, int device_id, float snr_threshold,
+    const py::object& max_peaks, std::size_t series_tile) {
+  py::array holder;
+  require_series(batch, holder, 2);
+  const py::buffer_info info = holder.request();
+  const std::size_t ns = static_cast<std::size_t>(info.shape[0]);
+  const std::size_t n = static_cast<std::size_t>(info.shape[1]);
+  const std::size_t limit = peak_limit(max_peaks);
+  std::vector<std::pair<std::size_t, FfaPeak>> out;
+  {
+    py::gil_scoped_release release;
+    GpuDeviceScope scope(device_id);
+    GpuDeviceBuffer<float> d = upload(static_cast<const float*>(info.ptr), ns * n);
+    GpuFfaExecutionOptions eo;
+    eo.series_tile_size = series_tile;
+    GpuFfaProgram program(plan, GpuFfaProgramOptions{device_id}, eo);
+    const FfaBatchSearchResult r = run_ffa_batch_gpu(
+        program, GpuTimeSeriesBatchView{d.data(), ns, n, n, device_id},
+        FfaSearchOptions{snr_threshold, limit});
+    out.reserve(r.peaks.size());
+    for (const auto& p : r.peaks) out.emplace_back(p.series_index, p.peak);
+  }
+  return out;
+}
+
+// rows_eval x bins final transform rows of task `task` for each series.
+py::array_t<float> py_ffa_transform_rows(const py::object& batch, const FfaSearchPlan& plan,
+                                         std::size_t task, int device_id) {
+  py::array holder;
+  require_series(batch, holder, 2);
+  const py::buffer_info info = holder.request();
+  const std::size_t ns = static_cast<std::size_t>(info.shape[0]);
+  const std::size_t n = static_cast<std::size_t>(info.shape[1]);
+  if (task >= plan.tasks.size()) throw py::index_error("FFA task index out of range");
+  const FfaSearchTask& t = plan.tasks[task];
+  std::vector<float> host(ns * t.rows_eval * t.bins);
+  {
+    py::gil_scoped_release release;
+    GpuDeviceScope scope(device_id);
+    GpuDeviceBuffer<float> d = upload(static_cast<const float*>(info.ptr), ns * n);
+    GpuDeviceBuffer<float> out(host.size());
+    GpuFfaExecutionOptions eo;
+    eo.series_tile_size = ns;
+    GpuFfaProgram program(plan, GpuFfaProgramOptions{device_id}, eo);
+    if (n != program.execution_plan().input_nsamples()) {
+      throw std::invalid_argument("FFA executor task input_nsamples must match time series size");
+    }
+    program.transform_rows(d.data(), ns, n, task, out.data());
+    copy_to_host(host.data(), out.data(), host.size() * 4, nullptr);
+  }
+  return owning_array(std::move(host), {static_cast<py::ssize_t>(ns),
+                                        static_cast<py::ssize_t>(t.rows_eval),
+                                        static_cast<py::ssize_t>(t.bins)});
+}
+
+// ---------------------------------------------------------------- preprocess --
+PreprocessPlan preprocess_plan_from(std::optional<std::size_t> window_samples,
+                                    std::size_t min_points, bool normalise, bool reject_constant) {
+  PreprocessPlan plan;
+  if (window_samples.has_value()) {
+    PreprocessStep s;
+    s.kind = PreprocessStepKind::DetrendRunningMedian;
+    s.detrend_running_median.window_samples = *window_samples;
+    s.detrend_running_median.min_points = min_points;
+    plan.steps.push_back(s);
+  }
+  if (normalise) {
+    PreprocessStep s;
+    s.kind = PreprocessStepKind::Normalise;
+    s.normalise.reject_constant = reject_constant;
+    plan.steps.push_back(s);
+  }
+  return plan;
+}
+
+// [nseries][nsamples] uint32 or float32 -> preprocessed float32 copy.
+py::array_t<float> py_preprocess_batch(const py::array& batch, std::optional<std::size_t> window,
+                                       std::size_t min_points, bool normalise,
+                                       bool reject_constant, int device_id) {
+  const py::buffer_info info = batch.request();
+  if (info.ndim != 2) throw py::value_error("preprocess batch must be a 2D array");
+  require_c_contiguous(info, "preprocess batch");
+  const bool is_u32 = info.format == py::format_descriptor<std::uint32_t>::format() ||
+                      (info.itemsize == 4 && (info.format == "I" || info.format == "L"));
+  const bool is_f32 = info.format == py::format_descriptor<float>::format();
+  if (info.itemsize != 4 || !(is_u32 || is_f32)) {
+    throw py::type_error("preprocess batch must have dtype uint32 or float32");
+  }
+  const std::size_t ns = static_cast<std::size_t>(info.shape[0]);
+  const std::size_t n = static_cast<std::size_t>(info.shape[1]);
+  std::vector<float> host(ns * n);
+  const PreprocessPlan plan = preprocess_plan_from(window, min_points, normalise, reject_constant);
+  {
+    py::gil_scoped_release release;
+    GpuDeviceScope scope(device_id);
+    GpuDeviceBuffer<std::uint32_t> d =
+        upload(static_cast<const std::uint32_t*>(info.ptr), ns * n);
+    if (plan.steps.empty()) {
+      if (!is_u32) throw std::invalid_argument("preprocess plan must contain at least one step");
+      gaffa_hip_preprocess_args a{};
+      a.data = d.data();
+      a.nseries = ns;
+      a.nsamples = n;
+      a.stride = n;
+      a.input_is_u32 = 1;
+      a.scrunch_factor = 1;
+      a.median_window = 1;
+      GpuDeviceBuffer<std::uint8_t> ws(gaffa_hip_preprocess_workspace_bytes(ns, n, 1));
+      GpuDeviceBuffer<std::int32_t> st(ns);
+      hip_ok(gaffa_hip_memset(st.data(), 0, st.bytes(), nullptr), "hipMemset");
+      a.workspace = ws.data();
+      a.workspace_bytes = ws.bytes();
+      a.status = st.data();
+      hip_ok(gaffa_hip_detrend_normalise_batch(&a, nullptr), "convert_uint32_to_float");
+      hip_ok(gaffa_hip_stream_synchronize(nullptr), "hipStreamSynchronize");
+    } else {
+      GpuPreprocessExecutionOptions pe;
+      pe.series_tile_size = std::min<std::size_t>(ns, 1024);
+      GpuPreprocessProgram prog(plan, GpuPreprocessProgramOptions{device_id}, pe);
+      prog.run(d.data(), ns, n, n, is_u32);
+      prog.synchronize();
+    }
+    copy_to_host(host.data(), d.data(), host.size() * 4, nullptr);
+  }
+  return owning_array(std::move(host), {static_cast<py::ssize_t>(ns), static_cast<py::ssize_t>(n)});
+}
+
+// ------------------------------------------------------------------ plans --
+FfaSearchPlan plan_from_arrays(const py::array& tasks, const std::vector<std::size_t>& widths) {
+  // structured array with the fields of FfaSearchTask, in order
+  const py::buffer_info info = tasks.request();
+  if (info.ndim != 1 || info.itemsize != static_cast<py::ssize_t>(sizeof(FfaSearchTask))) {
+    throw py::value_error("tasks must be a 1D structured array of FfaSearchTask records");
+  }
+  require_c_contiguous(info, "tasks");
+  FfaSearchPlan plan;
+  plan.width_trials = widths;
+  plan.tasks.resize(static_cast<std::size_t>(info.shape[0]));
+  std::memcpy(plan.tasks.data(), info.ptr, plan.tasks.size() * sizeof(FfaSearchTask));
+  return plan;
+}
+
+py::array tasks_to_array(const FfaSearchPlan& plan) {
+  std::vector<FfaSearchTask> v = plan.tasks;
+  py::dtype dt = py::dtype::of<FfaSearchTask>();
+  auto* owner = new std::vector<FfaSearchTask>(std::move(v));
+  py::capsule cap(owner, [](void* p) { delete static_cast<std::vector<FfaSearchTask>*>(p); });
+  return py::array(dt, {static_cast<py::ssize_t>(owner->size())}, {}, owner->data(), cap);
+}
+
+py::dict stage_dict(const StageTimes& t) {
+  py::dict d;
+  d["dedisperse_ms"] = t.dedisperse_ms;
+  d["preprocess_ms"] = t.preprocess_ms;
+  d["ffa_ms"] = t.ffa_ms;
+  d["total_ms"] = t.total_ms;
+  return d;
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_core, m) {
+  m.doc() = "gaffa search hot path on AMD Instinct MI355X (HIP, gfx950)";
+  PYBIND11_NUMPY_DTYPE(FfaSearchTask, downsample_factor, effective_tsamp, input_nsamples,
+                       prepared_nsamples, bins, rows, rows_eval, period_begin, period_end);
+
+  m.def("gpu_device_count", &gpu_device_count);
+  m.def("gpu_runtime_version", &gpu_runtime_version);
+  m.def("cuda_device_count", &gpu_device_count, "Alias of gpu_device_count for gaffa scripts.");
+
+  py::register_exception_translator([](std::exception_ptr p) {
+    try {
+      if (p) std::rethrow_exception(p);
+    } catch (const std::out_of_range& e) {
+      PyErr_SetString(PyExc_IndexError, e.what());
+    } catch (const std::invalid_argument& e) {
+      PyErr_SetString(PyExc_ValueError, e.what());
+    } catch (const std::overflow_error& e) {
+      PyErr_SetString(PyExc_OverflowError, e.what());
+    }
+  });
+
+  // dedispersion
+  py::class_<PyDedispersedResult>(m, "DedispersedResult")
+      .def_readonly("data", &PyDedispersedResult::data)
+      .def_readonly("backend", &PyDedispersedResult::backend)
+      .def_readonly("dm_low", &PyDedispersedResult::dm_low)
+      .def_readonly("dm_step", &PyDedispersedResult::dm_step)
+      .def_readonly("ndm", &PyDedispersedResult::ndm)
+      .def_readonly("nsamples", &PyDedispersedResult::nsamples)
+      .def_readonly("tsamp", &PyDedispersedResult::tsamp)
+      .def_property_readonly("shape", [](const PyDedispersedResult& r) {
+        return py::make_tuple(r.ndm, r.nsamples);
+      })
+      .def_property_readonly("dtype", [](const PyDedispersedResult& r) { return r.data.attr("dtype"); })
+      .def_property_readonly("nbytes", [](const PyDedispersedResult& r) { return r.data.attr("nbytes"); })
+      .def("__repr__", [](const PyDedispersedResult& r) {
+        return "<DedispersedResult shape=(" + std::to_string(r.ndm) + ", " +
+               std::to_string(r.nsamples) + ") backend=" + r.backend + ">";
+      });
+  m.def("dedisperse_single_dm", &py_single, py::arg("filterbank"), py::kw_only(), py::arg("dm"),
+        py::arg("backend") = "gpu", py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
+        py::arg("time_tile_samples") = 81920);
+  m.def("dedisperse_multi_dm", &py_multi, py::arg("filterbank"), py::kw_only(), py::arg("dm_low"),
+        py::arg("dm_step"), py::arg("ndm"), py::arg("backend") = "gpu", py::arg("device_id") = 0,
+        py::arg("threads_per_block") = 256, py::arg("time_tile_samples") = 81920);
+  m.def("dedisperse_subband", &py_subband, py::arg("filterbank"), py::kw_only(), py::arg("dm_low"),
+        py::arg("dm_step"), py::arg("ndm"), py::arg("backend") = "gpu",
+        py::arg("subband_channels") = 32, py::arg("ndm_per_nominal") = 32,
+        py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
+        py::arg("time_tile_samples") = 81920);
+  m.def("dedispersion_delay_bins", &dedispersion_delay_bins, py::arg("dm"),
+        py::arg("frequency_mhz"), py::arg("ref_frequency_mhz"), py::arg("tsamp"));
+
+  // FFA
+  py::class_<FfaSearchPlan>(m, "FfaPlan")
+      .def(py::init(&plan_from_arrays), py::arg("tasks"), py::arg("width_trials"))
+      .def_property_readonly("task_count", [](const FfaSearchPlan& p) { return p.tasks.size(); })
+      .def_property_readonly("width_trials",
+                             [](const FfaSearchPlan& p) { return py::tuple(py::cast(p.width_trials)); })
+      .def_property_readonly("tasks", &tasks_to_array)
+      .def("__repr__", [](const FfaSearchPlan& p) {
+        return "<FfaPlan task_count=" + std::to_string(p.tasks.size()) +
+               " width_trials=" + std::to_string(p.width_trials.size()) + ">";
+      });
+  py::class_<FfaPeak>(m, "FfaPeak")
+      .def_readonly("period", &FfaPeak::period)
+      .def_readonly("frequency", &FfaPeak::frequency)
+      .def_readonly("snr", &FfaPeak::snr)
+      .def_readonly("width", &FfaPeak::width)
+      .def_readonly("duty_cycle", &FfaPeak::duty_cycle)
+      .def_readonly("phase", &FfaPeak::phase)
+      .def_readonly("shift", &FfaPeak::shift)
+      .def_readonly("bins", &FfaPeak::bins)
+      .def_readonly("width_index", &FfaPeak::width_index)
+      .def_readonly("period_index", &FfaPeak::period_index)
+      .def("__repr__", [](const FfaPeak& p) {
+        return "<FfaPeak period=" + std::to_string(p.period) + " snr=" + std::to_string(p.snr) + ">";
+      });
+  m.def(
+      "_make_riptide_ffa_plan",
+      [](std::size_t nsamples, double tsamp, double period_min, double period_max,
+         std::size_t bins_min, std::size_t bins_max, std::size_t min_periods, double duty,
+         double spacing, std::size_t max_tasks) {
+        return make_riptide_ffa_plan(nsamples, tsamp,
+                                     RiptideFfaPlanOptions{period_min, period_max, bins_min,
+                                                           bins_max, min_periods, duty, spacing,
+                                                           max_tasks});
+      },
+      py::kw_only(), py::arg("nsamples"), py::arg("tsamp"), py::arg("period_min"),
+      py::arg("period_max"), py::arg("bins_min") = 180, py::arg("bins_max") = 256,
+      py::arg("min_periods") = 1, py::arg("duty_cycle_max") = 0.20,
+      py::arg("width_trial_spacing") = 1.5, py::arg("max_tasks") = 1'000'000);
+  m.def("_ffa_search_gpu_host", &py_ffa_search, py::arg("time_series"), py::arg("plan"),
+        py::kw_only(), py::arg("device_id") = 0, py::arg("snr_threshold") = 6.0F,
+        py::arg("max_peaks") = py::none());
+  m.def("_ffa_search_batch_gpu", &py_ffa_search_batch, py::arg("batch"), py::arg("plan"),
+        py::kw_only(), py::arg("device_id") = 0, py::arg("snr_threshold") = 6.0F,
+        py::arg("max_peaks") = py::none(), py::arg("series_tile") = 0);
+  m.def("_ffa_transform_rows_gpu", &py_ffa_transform_rows, py::arg("batch"), py::arg("plan"),
+        py::arg("task"), py::kw_only(), py::arg("device_id") = 0);
+  m.def("ffa_task_stdnoise", &ffa_task_stdnoise);
+
+  // preprocessing
+  m.def("running_median_window_samples", [](double seconds, double tsamp) {
+    return running_median_window_seconds(seconds, tsamp).window_samples;
+  });
+  m.def("_preprocess_batch_gpu", &py_preprocess_batch, py::arg("batch"), py::kw_only(),
+        py::arg("window_samples") = py::none(), py::arg("min_points") = 101,
+        py::arg("normalise") = true, py::arg("reject_constant") = true, py::arg("device_id") = 0);
+
+  // DM search pipeline
+  py::class_<DmPeak>(m, "DmPeak")
+      .def_readonly("dm", &DmPeak::dm)
+      .def_readonly("dm_index", &DmPeak::dm_index)
+      .def_readonly("peak", &DmPeak::peak)
+      .def("__repr__", [](const DmPeak& p) {
+        return "<DmPeak dm=" + std::to_string(p.dm) + " period=" + std::to_string(p.peak.period) +
+               " snr=" + std::to_string(p.peak.snr) + ">";
+      });
+  m.def(
+      "_find_dm_peaks_gpu",
+      [](const py::array& plane, const std::vector<double>& dms, double tsamp, double period_min,
+         double period_max, std::size_t bins_min, std::size_t bins_max, std::size_t min_periods,
+         double duty, double spacing, std::optional<std::size_t> window, std::size_t min_points,
+         bool normalise, float snr_threshold, const py::object& max_peaks, int device_id) {
+        const py::buffer_info info = plane.request();
+        if (info.ndim != 2 || info.itemsize != 4) throw py::value_error("plane must be 2D uint32");
+        require_c_contiguous(info, "plane");
+        const std::size_t ndm = static_cast<std::size_t>(info.shape[0]);
+        const std::size_t n = static_cast<std::size_t>(info.shape[1]);
+        DmSearchOptions o;
+        o.plan = RiptideFfaPlanOptions{period_min, period_max, bins_min, bins_max,
+                                       min_periods, duty, spacing, 1'000'000};
+        o.preprocess = preprocess_plan_from(window, min_points, normalise, true);
+        o.snr_threshold = snr_threshold;
+        o.max_peaks = peak_limit(max_peaks);
+        DmSearchResult r;
+        {
+          py::gil_scoped_release release;
+          GpuDeviceScope scope(device_id);
+          GpuDeviceBuffer<std::uint32_t> d =
+              upload(static_cast<const std::uint32_t*>(info.ptr), ndm * n);
+          r = find_dm_peaks_gpu(GpuDedispersedView<std::uint32_t>{d.data(), {ndm, n}, n, device_id},
+                                dms, tsamp, o, device_id);
+        }
+        return r.peaks;
+      },
+      py::arg("plane"), py::arg("dms"), py::arg("tsamp"), py::kw_only(), py::arg("period_min"),
+      py::arg("period_max"), py::arg("bins_min") = 180, py::arg("bins_max") = 256,
+      py::arg("min_periods") = 1, py::arg("duty_cycle_max") = 0.20,
+      py::arg("width_trial_spacing") = 1.5, py::arg("window_samples") = py::none(),
+      py::arg("min_points") = 101, py::arg("normalise") = true, py::arg("snr_threshold") = 6.0F,
+      py::arg("max_peaks") = py::none(), py::arg("device_id") = 0);
+
+  py::class_<SynthPulsar>(m, "SynthPulsar")
+      .def(py::init([](double period_s, double dm, double duty, double amp_sigma, double phase0) {
+             return SynthPulsar{period_s, dm, duty, amp_sigma, phase0};
+           }),
+           py::arg("period_s"), py::arg("dm"), py::arg("duty"), py::arg("amp_sigma"),
+           py::arg("phase0") = 0.0)
+      .def_readonly("period_s", &SynthPulsar::period_s)
+      .def_readonly("dm", &SynthPulsar::dm)
+      .def_readonly("duty", &SynthPulsar::duty)
+      .def_readonly("amp_sigma", &SynthPulsar::amp_sigma)
+      .def_readonly("phase0", &SynthPulsar::phase0);
+  py::class_<SynthFilterbankSpec>(m, "SynthFilterbankSpec")
+      .def(py::init([](std::size_t nsamples, std::size_t nchans, double fch1, double foff,
+                       double tsamp, std::uint64_t seed, std::vector<SynthPulsar> pulsars,
+                       std::vector<std::size_t> rfi, std::int32_t rfi_offset) {
+             return SynthFilterbankSpec{nsamples, nchans, fch1, foff, tsamp, seed,
+                                        std::move(pulsars), std::move(rfi), rfi_offset};
+           }),
+           py::kw_only(), py::arg("nsamples"), py::arg("nchans"), py::arg("fch1"),
+           py::arg("foff"), py::arg("tsamp"), py::arg("seed"),
+           py::arg("pulsars") = std::vector<SynthPulsar>{},
+           py::arg("rfi_channels") = std::vector<std::size_t>{}, py::arg("rfi_offset") = 0)
+      .def_readonly("nsamples", &SynthFilterbankSpec::nsamples)
+      .def_readonly("nchans", &SynthFilterbankSpec::nchans)
+      .def_readonly("tsamp", &SynthFilterbankSpec::tsamp)
+      .def_property_readonly("frequencies", &synth_frequencies);
+  m.def(
+      "_synth_filterbank_gpu",
+      [](const SynthFilterbankSpec& spec, std::size_t t_begin, std::optional<std::size_t> t_count,
+         int device_id) {
+        const std::size_t cnt = t_count.value_or(spec.nsamples - t_begin);
+        std::vector<std::uint8_t> host(cnt * spec.nchans);
+        {
+          py::gil_scoped_release release;
+          GpuDeviceScope scope(device_id);
+          GpuDeviceBuffer<std::uint8_t> d(host.size());
+          synth_filterbank_gpu(spec, d.data(), t_begin, cnt, nullptr);
+          copy_to_host(host.data(), d.data(), host.size(), nullptr);
+        }
+        return owning_array(std::move(host), {static_cast<py::ssize_t>(cnt),
+                                              static_cast<py::ssize_t>(spec.nchans)});
+      },
+      py::arg("spec"), py::kw_only(), py::arg("t_begin") = 0, py::arg("t_count") = py::none(),
+      py::arg("device_id") = 0);
+
+  py::class_<DmSegment>(m, "DmSegment")
+      .def(py::init([](double lo, double step, std::size_t n) { return DmSegment{lo, step, n}; }),
+           py::arg("dm_low"), py::arg("dm_step"), py::arg("ndm"))
+      .def_readonly("dm_low", &DmSegment::dm_low)
+      .def_readonly("dm_step", &DmSegment::dm_step)
+      .def_readonly("ndm", &DmSegment::ndm);
+  py::class_<GpuSearchPipeline>(m, "GpuSearchPipeline")
+      .def(py::init([](std::size_t nsamples, std::size_t nchans, double tsamp,
+                       std::vector<double> freq, std::vector<DmSegment> segments, bool subband,
+                       std::size_t subband_channels, std::size_t ndm_per_nominal,
+                       std::optional<std::size_t> window, std::size_t min_points, bool normalise,
+                       double period_min, double period_max, std::size_t bins_min,
+                       std::size_t bins_max, double duty, double spacing, float snr_threshold,
+                       std::size_t ffa_workspace_bytes, std::size_t ffa_series_tile,
+                       int device_id) {
+             SearchPipelineConfig c;
+             c.nsamples = nsamples;
+             c.nchans = nchans;
+             c.tsamp = tsamp;
+             c.frequency_mhz = std::move(freq);
+             c.dm_segments = std::move(segments);
+             c.subband = subband;
+             c.subband_options = SubbandDedispersionOptions{subband_channels, ndm_per_nominal};
+             c.preprocess = preprocess_plan_from(window, min_points, normalise, true);
+             c.ffa = RiptideFfaPlanOptions{period_min, period_max, bins_min, bins_max, 1, duty,
+                                           spacing, 1'000'000};
+             c.snr_threshold = snr_threshold;
+             c.ffa_workspace_bytes_limit = ffa_workspace_bytes;
+             c.ffa_series_tile = ffa_series_tile;
+             c.device_id = device_id;
+             py::gil_scoped_release release;
+             return std::make_unique<GpuSearchPipeline>(std::move(c));
+           }),
+           py::kw_only(), py::arg("nsamples"), py::arg("nchans"), py::arg("tsamp"),
+           py::arg("frequency_mhz"), py::arg("dm_segments"), py::arg("subband") = true,
+           py::arg("subband_channels") = 32, py::arg("ndm_per_nominal") = 32,
+           py::arg("window_samples") = py::none(), py::arg("min_points") = 101,
+           py::arg("normalise") = true, py::arg("period_min"), py::arg("period_max"),
+           py::arg("bins_min") = 180, py::arg("bins_max") = 256, py::arg("duty_cycle_max") = 0.20,
+           py::arg("width_trial_spacing") = 1.5, py::arg("snr_threshold") = 6.0F,
+           py::arg("ffa_workspace_bytes") = 32ULL << 30, py::arg("ffa_series_tile") = 0,
+           py::arg("device_id") = 0)
+      .def("generate", [](GpuSearchPipeline& p, const SynthFilterbankSpec& s) {
+        py::gil_scoped_release release;
+        p.generate(s);
+      })
+      .def("upload", [](GpuSearchPipeline& p, const py::array_t<std::uint8_t>& a) {
+        const py::buffer_info info = a.request();
+        require_c_contiguous(info, "filterbank data");
+        py::gil_scoped_release release;
+        p.upload(static_cast<const std::uint8_t*>(info.ptr), static_cast<std::size_t>(info.size));
+      })
+      .def("dedisperse", [](GpuSearchPipeline& p) {
+        py::gil_scoped_release release;
+        p.dedisperse();
+      })
+      .def("preprocess", [](GpuSearchPipeline& p) {
+        py::gil_scoped_release release;
+        p.preprocess();
+      })
+      .def("search", [](GpuSearchPipeline& p) {
+        py::gil_scoped_release release;
+        p.search();
+      })
+      .def("run", [](GpuSearchPipeline& p) {
+        StageTimes t;
+        {
+          py::gil_scoped_release release;
+          t = p.run();
+        }
+        return stage_dict(t);
+      })
+      .def("download_plane_u32", [](const GpuSearchPipeline& p) {
+        std::vector<std::uint32_t> host(p.ndm() * p.out_nsamples());
+        p.download_plane(host.data());
+        return owning_array(std::move(host), {static_cast<py::ssize_t>(p.ndm()),
+                                              static_cast<py::ssize_t>(p.out_nsamples())});
+      })
+      .def("download_plane_f32", [](const GpuSearchPipeline& p) {
+        std::vector<std::uint32_t> bits(p.ndm() * p.out_nsamples());
+        p.download_plane(bits.data());
+        std::vector<float> host(bits.size());
+        std::memcpy(host.data(), bits.data(), bits.size() * 4);
+        return owning_array(std::move(host), {static_cast<py::ssize_t>(p.ndm()),
+                                              static_cast<py::ssize_t>(p.out_nsamples())});
+      })
+      .def("download_filterbank", [](const GpuSearchPipeline& p, std::size_t nsamples,
+                                     std::size_t nchans) {
+        std::vector<std::uint8_t> host(nsamples * nchans);
+        p.download_filterbank(host.data());
+        return owning_array(std::move(host), {static_cast<py::ssize_t>(nsamples),
+                                              static_cast<py::ssize_t>(nchans)});
+      })
+      .def_property_readonly("ndm", &GpuSearchPipeline::ndm)
+      .def_property_readonly("out_nsamples", &GpuSearchPipeline::out_nsamples)
+      .def_property_readonly("dms", &GpuSearchPipeline::dms)
+      .def_property_readonly("ffa_plan", &GpuSearchPipeline::ffa_plan)
+      .def_property_readonly("peaks", &GpuSearchPipeline::peaks)
+      .def_property_readonly("ffa_tile", [](const GpuSearchPipeline& p) {
+        return p.ffa_program().tile_capacity();
+      });
+}

@@ -1,0 +1,357 @@
+// Host side of GPU dedispersion: validation (same checks and messages as
+// dedispersion.cpp:53-139 in gaffa), host-built delay tables, and the calls
+// into the C ABI.
+#include "gaffa/dedispersion.h"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <stdexcept>
+
+namespace gaffa {
+namespace {
+
+constexpr std::size_t kRowAlign = 64;  // device rows padded to 256 bytes
+
+std::size_t padded_stride(std::size_t n) { return (n + kRowAlign - 1) / kRowAlign * kRowAlign; }
+
+void check_arg(bool ok, const char* message) {
+  if (!ok) throw std::invalid_argument(message);
+}
+
+void validate_samples(const SampleShape& shape, bool has_data,
+                      std::span<const double> frequency_mhz) {
+  check_arg(has_data, "dedispersion samples data must not be null");
+  check_arg(shape.nifs == 1, "dedispersion requires nifs == 1");
+  check_arg(shape.nsamples != 0, "dedispersion requires at least one sample");
+  check_arg(shape.nchans != 0, "dedispersion requires at least one channel");
+  check_arg(frequency_mhz.size() == shape.nchans,
+            "frequency table length must match sample channel count");
+  for (double f : frequency_mhz) {
+    check_arg(std::isfinite(f) && f > 0.0, "frequency values must be finite and positive");
+  }
+}
+
+void validate_plan_values(double ref, double tsamp) {
+  check_arg(std::isfinite(ref) && ref > 0.0, "reference frequency must be positive");
+  check_arg(std::isfinite(tsamp) && tsamp > 0.0, "tsamp must be positive");
+}
+
+void validate_channels(std::span<const double> f, double ref, std::size_t cb, std::size_t ce,
+                       std::size_t nchans) {
+  check_arg(cb < ce, "channel range must be non-empty");
+  check_arg(ce <= nchans, "channel range exceeds sample channel count");
+  for (std::size_t c = cb; c < ce; ++c) {
+    check_arg(!(f[c] > ref),
+              "dedispersion requires selected frequencies <= reference frequency");
+  }
+}
+
+void validate_multi(const SampleShape& shape, bool has_data, std::span<const double> f,
+                    const MultiDmDedispersionPlan& p) {
+  validate_samples(shape, has_data, f);
+  validate_plan_values(p.ref_frequency_mhz, p.tsamp);
+  check_arg(std::isfinite(p.dm_low) && p.dm_low >= 0.0,
+            "dm_low must be finite and non-negative");
+  check_arg(p.ndm != 0, "dedispersion requires at least one DM");
+  check_arg(std::isfinite(p.dm_step) && p.dm_step > 0.0, "dm_step must be positive");
+  validate_channels(f, p.ref_frequency_mhz, p.chan_begin, p.chan_end, shape.nchans);
+}
+
+double min_selected(std::span<const double> f, std::size_t cb, std::size_t ce) {
+  return *std::min_element(f.begin() + static_cast<std::ptrdiff_t>(cb),
+                           f.begin() + static_cast<std::ptrdiff_t>(ce));
+}
+
+std::size_t valid_output(std::size_t n, std::int32_t max_delay) {
+  check_arg(max_delay >= 0, "dedispersion requires non-negative delays");
+  check_arg(static_cast<std::size_t>(max_delay) < n,
+            "dedispersion valid output range is empty");
+  return n - static_cast<std::size_t>(max_delay);
+}
+
+// max - min over every (16 DM x 32 channel) block of a [ndm][nc] table: the
+// LDS slab height the brute-force kernel needs.
+std::int32_t block_spread(const std::vector<std::int32_t>& d, std::size_t ndm, std::size_t nc) {
+  std::int32_t spread = 0;
+  for (std::size_t i0 = 0; i0 < ndm; i0 += 16) {
+    for (std::size_t c0 = 0; c0 < nc; c0 += 32) {
+      std::int32_t lo = std::numeric_limits<std::int32_t>::max(), hi = 0;
+      for (std::size_t i = i0; i < std::min(ndm, i0 + 16); ++i)
+        for (std::size_t c = c0; c < std::min(nc, c0 + 32); ++c) {
+          lo = std::min(lo, d[i * nc + c]);
+          hi = std::max(hi, d[i * nc + c]);
+        }
+      spread = std::max(spread, hi - lo);
+    }
+  }
+  return spread;
+}
+
+}  // namespace
+
+std::int32_t dedispersion_delay_bins(double dm, double frequency_mhz, double ref_frequency_mhz,
+                                     double tsamp) {
+  const double f2 = frequency_mhz * frequency_mhz;
+  const double r2 = ref_frequency_mhz * ref_frequency_mhz;
+  const double delay = 4148.808 * dm * (1.0 / f2 - 1.0 / r2);
+  const double bins = std::round(delay / tsamp);
+  if (bins < static_cast<double>(std::numeric_limits<std::int32_t>::min()) ||
+      bins > static_cast<double>(std::numeric_limits<std::int32_t>::max())) {
+    throw std::overflow_error("dedispersion delay exceeds int32 range");
+  }
+  return static_cast<std::int32_t>(bins);
+}
+
+std::vector<std::int32_t> make_multi_dm_delay_table(std::span<const double> f,
+                                                    const MultiDmDedispersionPlan& p) {
+  const std::size_t nc = p.chan_end - p.chan_begin;
+  if (p.ndm != 0 && nc > std::numeric_limits<std::size_t>::max() / p.ndm) {
+    throw std::overflow_error("dedispersion delay table size overflow");
+  }
+  std::vector<std::int32_t> d(p.ndm * nc);
+  for (std::size_t i = 0; i < p.ndm; ++i) {
+    const double dm = p.dm_low + static_cast<double>(i) * p.dm_step;
+    for (std::size_t c = 0; c < nc; ++c)
+      d[i * nc + c] = dedispersion_delay_bins(dm, f[p.chan_begin + c], p.ref_frequency_mhz, p.tsamp);
+  }
+  return d;
+}
+
+std::size_t multi_dm_output_nsamples(std::size_t n, std::span<const double> f,
+                                     const MultiDmDedispersionPlan& p) {
+  const double dm_max = p.dm_low + static_cast<double>(p.ndm - 1) * p.dm_step;
+  return valid_output(n, dedispersion_delay_bins(dm_max, min_selected(f, p.chan_begin, p.chan_end),
+                                                 p.ref_frequency_mhz, p.tsamp));
+}
+
+SubbandTables make_subband_tables(std::span<const double> f, const MultiDmDedispersionPlan& p,
+                                  const SubbandDedispersionOptions& o) {
+  check_arg(o.subband_channels != 0, "subband_channels must be positive");
+  check_arg(o.ndm_per_nominal != 0, "ndm_per_nominal must be positive");
+  SubbandTables t;
+  const std::size_t nc = p.chan_end - p.chan_begin;
+  t.nsub = (nc + o.subband_channels - 1) / o.subband_channels;
+  t.n_nominal = (p.ndm + o.ndm_per_nominal - 1) / o.ndm_per_nominal;
+  std::vector<double> sub_freq(t.nsub);
+  for (std::size_t s = 0; s < t.nsub; ++s) {
+    const std::size_t b = p.chan_begin + s * o.subband_channels;
+    const std::size_t e = std::min(b + o.subband_channels, p.chan_end);
+    sub_freq[s] = f[(b + e - 1) / 2];  // lower-middle channel (dedispersion.cpp:376)
+  }
+  t.coarse.resize(t.n_nominal * nc);
+  for (std::size_t k = 0; k < t.n_nominal; ++k) {
+    const double nominal =
+        p.dm_low + static_cast<double>(k * o.ndm_per_nominal) * p.dm_step;
+    for (std::size_t c = 0; c < nc; ++c)
+      t.coarse[k * nc + c] =
+          dedispersion_delay_bins(nominal, f[p.chan_begin + c], p.ref_frequency_mhz, p.tsamp);
+    for (std::size_t s = 0; s < t.nsub; ++s) {
+      const std::size_t cs = s * o.subband_channels;
+      const std::size_t ce = std::min(cs + o.subband_channels, nc);
+      for (std::size_t c0 = cs; c0 < ce; c0 += 32) {
+        std::int32_t lo = std::numeric_limits<std::int32_t>::max(), hi = 0;
+        for (std::size_t c = c0; c < std::min(ce, c0 + 32); ++c) {
+          lo = std::min(lo, t.coarse[k * nc + c]);
+          hi = std::max(hi, t.coarse[k * nc + c]);
+        }
+        t.max_coarse_spread = std::max(t.max_coarse_spread, hi - lo);
+      }
+    }
+  }
+  t.residual.resize(p.ndm * t.nsub);
+  for (std::size_t i = 0; i < p.ndm; ++i) {
+    const std::size_t k = i / o.ndm_per_nominal;
+    const double nominal = p.dm_low + static_cast<double>(k * o.ndm_per_nominal) * p.dm_step;
+    const double dm = p.dm_low + static_cast<double>(i) * p.dm_step;
+    for (std::size_t s = 0; s < t.nsub; ++s) {
+      const std::int32_t r =
+          dedispersion_delay_bins(dm - nominal, sub_freq[s], p.ref_frequency_mhz, p.tsamp);
+      check_arg(r >= 0, "dedispersion requires non-negative delays");
+      t.residual[i * t.nsub + s] = r;
+      t.max_residual = std::max(t.max_residual, r);
+    }
+  }
+  return t;
+}
+
+void dedisperse_multi_dm_gpu_into(GpuSampleView samples, std::span<const double> f,
+                                  const MultiDmDedispersionPlan& p, std::uint32_t* out,
+                                  std::size_t out_stride, const GpuDedispersionOptions& o,
+    std::size_t out_nsamples) {
+  validate_multi(samples.shape, samples.data != nullptr, f, p);
+  std::size_t n_out = multi_dm_output_nsamples(samples.shape.nsamples, f, p);
+  if (out_nsamples != 0) {
+    check_arg(out_nsamples <= n_out, "requested output length exceeds the valid range");
+    n_out = out_nsamples;
+  }
+  check_arg(out_stride >= n_out, "output stride must be >= out_nsamples");
+  const std::vector<std::int32_t> delays = make_multi_dm_delay_table(f, p);
+  for (std::int32_t d : delays) check_arg(d >= 0, "dedispersion requires non-negative delays");
+  GpuDeviceScope scope(o.device_id);
+  GpuDeviceBuffer<std::int32_t> d_delays = upload(delays.data(), delays.size());
+  gaffa_hip_dedisperse_multi_args a{};
+  a.samples = samples.data;
+  a.nsamples = samples.shape.nsamples;
+  a.nchans = samples.shape.nchans;
+  a.chan_begin = p.chan_begin;
+  a.chan_end = p.chan_end;
+  a.delays = d_delays.data();
+  a.ndm = p.ndm;
+  a.out = out;
+  a.out_nsamples = n_out;
+  a.out_stride = out_stride;
+  a.max_block_spread = block_spread(delays, p.ndm, p.chan_end - p.chan_begin);
+  hip_ok(gaffa_hip_dedisperse_multi_u8(&a, o.stream), "dedisperse_multi_u8");
+  hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_multi_u8");
+}
+
+void dedisperse_subband_gpu_into(GpuSampleView samples, std::span<const double> f,
+                                 const MultiDmDedispersionPlan& p,
+                                 const SubbandDedispersionOptions& so, std::uint32_t* out,
+                                 std::size_t out_stride, const GpuDedispersionOptions& o,
+    std::size_t out_nsamples) {
+  validate_multi(samples.shape, samples.data != nullptr, f, p);
+  check_arg(so.subband_channels != 0, "subband_channels must be positive");
+  check_arg(so.ndm_per_nominal != 0, "ndm_per_nominal must be positive");
+  std::size_t n_out = multi_dm_output_nsamples(samples.shape.nsamples, f, p);
+  if (out_nsamples != 0) {
+    check_arg(out_nsamples <= n_out, "requested output length exceeds the valid range");
+    n_out = out_nsamples;
+  }
+  check_arg(out_stride >= n_out, "output stride must be >= out_nsamples");
+  const SubbandTables t = make_subband_tables(f, p, so);
+  for (std::int32_t d : t.coarse) check_arg(d >= 0, "dedispersion requires non-negative delays");
+  GpuDeviceScope scope(o.device_id);
+  GpuDeviceBuffer<std::int32_t> d_coarse = upload(t.coarse.data(), t.coarse.size());
+  GpuDeviceBuffer<std::int32_t> d_resid = upload(t.residual.data(), t.residual.size());
+  gaffa_hip_dedisperse_subband_args a{};
+  a.samples = samples.data;
+  a.nsamples = samples.shape.nsamples;
+  a.nchans = samples.shape.nchans;
+  a.chan_begin = p.chan_begin;
+  a.chan_end = p.chan_end;
+  a.subband_channels = so.subband_channels;
+  a.ndm_per_nominal = so.ndm_per_nominal;
+  a.ndm = p.ndm;
+  a.coarse_delays = d_coarse.data();
+  a.residual_delays = d_resid.data();
+  a.out = out;
+  a.out_nsamples = n_out;
+  a.out_stride = out_stride;
+  a.max_residual = t.max_residual;
+  a.max_coarse_spread = t.max_coarse_spread;
+  hip_ok(gaffa_hip_dedisperse_subband_u8(&a, o.stream), "dedisperse_subband_u8");
+  hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_subband_u8");
+}
+
+GpuDedispersedResult<std::uint32_t> dedisperse_multi_dm_gpu_device(
+    GpuSampleView samples, std::span<const double> f, const MultiDmDedispersionPlan& p,
+    const GpuDedispersionOptions& o) {
+  validate_multi(samples.shape, samples.data != nullptr, f, p);
+  const std::size_t n_out = multi_dm_output_nsamples(samples.shape.nsamples, f, p);
+  GpuDeviceScope scope(o.device_id);
+  GpuDedispersedResult<std::uint32_t> r;
+  r.shape = DedispersedShape{p.ndm, n_out};
+  r.stride = padded_stride(n_out);
+  r.device_id = o.device_id;
+  r.data = GpuDeviceBuffer<std::uint32_t>(p.ndm * r.stride);
+  dedisperse_multi_dm_gpu_into(samples, f, p, r.data.data(), r.stride, o);
+  return r;
+}
+
+GpuDedispersedResult<std::uint32_t> dedisperse_subband_gpu_device(
+    GpuSampleView samples, std::span<const double> f, const MultiDmDedispersionPlan& p,
+    const SubbandDedispersionOptions& so, const GpuDedispersionOptions& o) {
+  validate_multi(samples.shape, samples.data != nullptr, f, p);
+  const std::size_t n_out = multi_dm_output_nsamples(samples.shape.nsamples, f, p);
+  GpuDeviceScope scope(o.device_id);
+  GpuDedispersedResult<std::uint32_t> r;
+  r.shape = DedispersedShape{p.ndm, n_out};
+  r.stride = padded_stride(n_out);
+  r.device_id = o.device_id;
+  r.data = GpuDeviceBuffer<std::uint32_t>(p.ndm * r.stride);
+  dedisperse_subband_gpu_into(samples, f, p, so, r.data.data(), r.stride, o);
+  return r;
+}
+
+GpuDedispersedResult<std::uint32_t> dedisperse_single_dm_gpu_device(
+    GpuSampleView samples, std::span<const double> f, const SingleDmDedispersionPlan& sp,
+    const GpuDedispersionOptions& o) {
+  validate_samples(samples.shape, samples.data != nullptr, f);
+  validate_plan_values(sp.ref_frequency_mhz, sp.tsamp);
+  validate_channels(f, sp.ref_frequency_mhz, sp.chan_begin, sp.chan_end, samples.shape.nchans);
+  check_arg(std::isfinite(sp.dm) && sp.dm >= 0.0, "dm must be finite and non-negative");
+  // a single DM is a one-row multi-DM plan whose output length uses this DM
+  const MultiDmDedispersionPlan p{sp.dm, 1.0, 1, sp.ref_frequency_mhz, sp.tsamp, sp.chan_begin,
+                                  sp.chan_end};
+  return dedisperse_multi_dm_gpu_device(samples, f, p, o);
+}
+
+namespace {
+
+template <typename Run>
+DedispersedResult<std::uint32_t> via_device(HostSampleView<std::uint8_t> samples,
+                                            const GpuDedispersionOptions& o, Run&& run) {
+  check_arg(!samples.data.empty(), "dedispersion samples data must not be null");
+  check_arg(samples.data.size() == samples.size(),
+            "dedispersion sample data size does not match shape");
+  GpuDeviceScope scope(o.device_id);
+  GpuDeviceBuffer<std::uint8_t> d = upload(samples.data.data(), samples.data.size());
+  GpuSampleView v{d.data(), samples.shape, o.device_id};
+  return copy_to_host(run(v));
+}
+
+}  // namespace
+
+DedispersedResult<std::uint32_t> dedisperse_single_dm_gpu(HostSampleView<std::uint8_t> samples,
+                                                          std::span<const double> f,
+                                                          const SingleDmDedispersionPlan& p,
+                                                          const GpuDedispersionOptions& o) {
+  validate_samples(samples.shape, !samples.data.empty(), f);
+  return via_device(samples, o, [&](GpuSampleView v) {
+    return dedisperse_single_dm_gpu_device(v, f, p, o);
+  });
+}
+
+DedispersedResult<std::uint32_t> dedisperse_multi_dm_gpu(HostSampleView<std::uint8_t> samples,
+                                                         std::span<const double> f,
+                                                         const MultiDmDedispersionPlan& p,
+                                                         const GpuDedispersionOptions& o) {
+  validate_multi(samples.shape, !samples.data.empty(), f, p);
+  return via_device(samples, o, [&](GpuSampleView v) {
+    return dedisperse_multi_dm_gpu_device(v, f, p, o);
+  });
+}
+
+DedispersedResult<std::uint32_t> dedisperse_subband_gpu(HostSampleView<std::uint8_t> samples,
+                                                        std::span<const double> f,
+                                                        const MultiDmDedispersionPlan& p,
+                                                        const SubbandDedispersionOptions& so,
+                                                        const GpuDedispersionOptions& o) {
+  validate_multi(samples.shape, !samples.data.empty(), f, p);
+  check_arg(so.subband_channels != 0, "subband_channels must be positive");
+  check_arg(so.ndm_per_nominal != 0, "ndm_per_nominal must be positive");
+  return via_device(samples, o, [&](GpuSampleView v) {
+    return dedisperse_subband_gpu_device(v, f, p, so, o);
+  });
+}
+
+DedispersedResult<std::uint32_t> copy_to_host(const GpuDedispersedResult<std::uint32_t>& r) {
+  DedispersedResult<std::uint32_t> h;
+  h.shape = r.shape;
+  h.data.resize(dedispersed_element_count(r.shape));
+  if (r.shape.ndm == 0) return h;
+  GpuDeviceScope scope(r.device_id);
+  if (r.stride == r.shape.nsamples) {
+    copy_to_host(h.data.data(), r.data.data(), h.data.size() * 4, nullptr);
+  } else {
+    std::vector<std::uint32_t> tmp(r.shape.ndm * r.stride);
+    copy_to_host(tmp.data(), r.data.data(), tmp.size() * 4, nullptr);
+    for (std::size_t i = 0; i < r.shape.ndm; ++i)
+      std::copy_n(tmp.data() + i * r.stride, r.shape.nsamples, h.data.data() + i * r.shape.nsamples);
+  }
+  return h;
+}
+
+}  // namespace gaffa

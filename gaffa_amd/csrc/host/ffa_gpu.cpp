@@ -1,0 +1,409 @@
+// Host side of the GPU FFA search: builds the device execution layout of an
+// FfaSearchPlan, owns workspace and the compact peak buffer, runs series
+// tiles through gaffa_hip_ffa_search_batch, grows the peak buffer and repeats
+// a tile on overflow (ffa_cuda_executor.cuh:385-531 in gaffa), and expands
+// compact peaks with make_ffa_peak.
+#include "gaffa/ffa_gpu.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+namespace gaffa {
+namespace {
+
+constexpr std::size_t kAlign = 64;               // floats
+constexpr std::size_t kSubtreeLdsFloats = 40960;  // 160 KiB: two ping-pong buffers
+constexpr std::uint32_t kLevelRows = 16;
+constexpr std::uint32_t kTerminalRows = 64;
+
+std::size_t align_up(std::size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
+
+void check_arg(bool ok, const char* message) {
+  if (!ok) throw std::invalid_argument(message);
+}
+
+void validate_task(const FfaSearchTask& t, std::size_t n) {
+  check_arg(t.input_nsamples == n, "FFA executor task input_nsamples must match time series size");
+  check_arg(std::isfinite(t.downsample_factor) && t.downsample_factor >= 1.0,
+            "FFA executor task downsample_factor must be finite and >= 1");
+  check_arg(t.effective_tsamp > 0.0 && std::isfinite(t.effective_tsamp),
+            "FFA executor task effective_tsamp must be finite and > 0");
+  check_arg(t.bins > 1, "FFA executor task bins must be > 1");
+  check_arg(t.rows != 0, "FFA executor task rows must be > 0");
+  check_arg(t.rows_eval != 0 && t.rows_eval <= t.rows,
+            "FFA executor task rows_eval must satisfy 0 < rows_eval <= rows");
+  check_arg(t.prepared_nsamples != 0, "FFA executor task prepared_nsamples must be > 0");
+  if (t.rows > std::numeric_limits<std::size_t>::max() / t.bins) {
+    throw std::overflow_error("FFA executor shape size overflow");
+  }
+  check_arg(t.rows * t.bins <= t.prepared_nsamples,
+            "FFA executor task rows * bins must be <= prepared_nsamples");
+  if (t.downsample_factor == 1.0) {
+    check_arg(t.prepared_nsamples == n,
+              "FFA executor no-downsample task prepared_nsamples must match input size");
+  } else {
+    check_arg(t.prepared_nsamples == downsampled_size(n, t.downsample_factor),
+              "FFA executor task prepared_nsamples must match downsampled_size");
+  }
+  check_arg(t.bins < 65536, "GPU FFA supports bins < 65536");
+  check_arg(t.rows < (1ULL << 31), "GPU FFA supports rows < 2^31");
+}
+
+void collect_nodes(std::uint32_t start, std::uint32_t size, std::uint32_t depth,
+                   std::uint32_t target, std::uint32_t task, std::vector<gaffa_hip_ffa_item>& out) {
+  if (depth == target) {
+    out.push_back(gaffa_hip_ffa_item{task, start, size, 0});
+    return;
+  }
+  if (size < 2) throw std::logic_error("FFA subtree depth below a leaf");
+  const std::uint32_t h = size / 2;
+  collect_nodes(start, h, depth + 1, target, task, out);
+  collect_nodes(start + h, size - h, depth + 1, target, task, out);
+}
+
+}  // namespace
+
+GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan) {
+  check_arg(!plan.tasks.empty(), "FFA executor plan must contain at least one task");
+  check_arg(!plan.width_trials.empty(), "FFA search plan width_trials must not be empty");
+  check_arg(plan.width_trials.size() <= GAFFA_HIP_FFA_MAX_WIDTHS,
+            "GPU FFA supports at most 16 width trials");
+  input_nsamples_ = plan.tasks.front().input_nsamples;
+  check_arg(input_nsamples_ > 0, "FFA executor time series must not be empty");
+  std::map<double, std::uint32_t> group_of;
+  for (const FfaSearchTask& t : plan.tasks) {
+    validate_task(t, input_nsamples_);
+    if (group_of.count(t.downsample_factor) == 0) {
+      gaffa_hip_ffa_group g{};
+      g.factor = t.downsample_factor;
+      g.prepared_nsamples = t.prepared_nsamples;
+      if (t.downsample_factor != 1.0) {
+        g.prep_offset = prepared_stride_;
+        prepared_stride_ += align_up(t.prepared_nsamples);
+        max_prepared_ = std::max(max_prepared_, t.prepared_nsamples);
+      }
+      group_of[t.downsample_factor] = static_cast<std::uint32_t>(groups_.size());
+      groups_.push_back(g);
+    }
+  }
+  std::vector<std::vector<gaffa_hip_ffa_item>> by_depth;
+  for (std::size_t k = 0; k < plan.tasks.size(); ++k) {
+    const FfaSearchTask& t = plan.tasks[k];
+    const FfaDetectionPlan det = make_ffa_detection_plan(plan.width_trials, t.bins);
+    const float stdnoise = ffa_task_stdnoise(t);
+    check_arg(stdnoise > 0.0F && std::isfinite(stdnoise),
+              "FFA row detection stdnoise must be finite and > 0");
+    gaffa_hip_ffa_task d{};
+    d.group = group_of[t.downsample_factor];
+    d.bins = static_cast<std::uint32_t>(t.bins);
+    d.rows = static_cast<std::uint32_t>(t.rows);
+    d.rows_eval = static_cast<std::uint32_t>(t.rows_eval);
+    d.nwidths = static_cast<std::uint32_t>(det.boxcar_trials.size());
+    d.inv_stdnoise = 1.0F / stdnoise;
+    for (std::size_t w = 0; w < det.boxcar_trials.size(); ++w) {
+      d.width[w] = static_cast<std::uint32_t>(det.boxcar_trials[w].width);
+      d.height[w] = det.boxcar_trials[w].height;
+      d.baseline[w] = det.boxcar_trials[w].baseline;
+    }
+    check_arg(2 * det.max_width <= 2 * t.bins, "FFA detection width exceeds bins");
+    max_bins_ = std::max(max_bins_, d.bins);
+    const std::size_t lds_rows = kSubtreeLdsFloats / (2 * t.bins);
+    check_arg(lds_rows >= 2, "GPU FFA bins too large for an LDS subtree");
+    if (t.rows >= 2) {
+      std::uint32_t depth = 1;
+      while ((t.rows + (std::size_t{1} << depth) - 1) >> depth > lds_rows) ++depth;
+      d.depth = depth;
+      d.work_offset = work_stride_;
+      work_stride_ += align_up(t.rows * t.bins);
+      const std::uint32_t task_index = static_cast<std::uint32_t>(k);
+      const std::size_t before = subtree_.size();
+      collect_nodes(0, d.rows, 0, depth, task_index, subtree_);
+      for (std::size_t i = before; i < subtree_.size(); ++i) {
+        subtree_lds_bytes_ = std::max<std::uint32_t>(
+            subtree_lds_bytes_, 2u * subtree_[i].b * d.bins * 4u);
+      }
+      if (by_depth.size() < depth) by_depth.resize(depth);
+      for (std::uint32_t l = 1; l < depth; ++l)
+        for (std::uint32_t r = 0; r < d.rows; r += kLevelRows)
+          by_depth[l].push_back(
+              gaffa_hip_ffa_item{task_index, l, r, std::min(kLevelRows, d.rows - r)});
+      max_depth_ = std::max<std::size_t>(max_depth_, depth);
+    }
+    for (std::uint32_t r = 0; r < d.rows_eval; r += kTerminalRows)
+      terminal_.push_back(gaffa_hip_ffa_item{static_cast<std::uint32_t>(k), r,
+                                             std::min(kTerminalRows, d.rows_eval - r), 0});
+    tasks_.push_back(d);
+  }
+  level_offsets_.assign(max_depth_ + 1, 0);
+  for (std::size_t l = 0; l <= max_depth_; ++l) {
+    level_offsets_[l] = static_cast<std::uint32_t>(level_.size());
+    if (l < by_depth.size()) level_.insert(level_.end(), by_depth[l].begin(), by_depth[l].end());
+  }
+  level_offsets_.push_back(static_cast<std::uint32_t>(level_.size()));
+  check_arg(terminal_.size() < (1ULL << 31), "FFA plan has too many work items");
+}
+
+std::size_t GpuFfaExecutionPlan::workspace_bytes(std::size_t series) const noexcept {
+  return series * (prepared_stride_ + 2 * work_stride_) * sizeof(float);
+}
+
+GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptions& po,
+                             const GpuFfaExecutionOptions& eo)
+    : exec_plan_(plan), program_(po), exec_(eo) {
+  GpuDeviceScope scope(program_.device_id);
+  const std::size_t per_series = exec_plan_.workspace_bytes(1);
+  if (exec_.series_tile_size != 0) {
+    tile_ = exec_.series_tile_size;
+  } else {
+    tile_ = per_series == 0 ? 1024 : std::max<std::size_t>(1, exec_.workspace_bytes_limit / per_series);
+    tile_ = std::min<std::size_t>(tile_, 1024);
+  }
+  tile_ = std::min<std::size_t>(tile_, 65535);
+  if (exec_.workspace_bytes_limit != 0 && exec_plan_.workspace_bytes(tile_) > exec_.workspace_bytes_limit &&
+      exec_.series_tile_size != 0) {
+    throw std::invalid_argument("FFA workspace exceeds workspace_bytes_limit");
+  }
+  const auto& g = exec_plan_.groups();
+  const auto& t = exec_plan_.tasks();
+  d_groups_ = upload(g.data(), g.size());
+  d_tasks_ = upload(t.data(), t.size());
+  d_subtree_ = upload(exec_plan_.subtree_items().data(), exec_plan_.subtree_items().size());
+  d_level_ = upload(exec_plan_.level_items().data(), exec_plan_.level_items().size());
+  d_terminal_ = upload(exec_plan_.terminal_items().data(), exec_plan_.terminal_items().size());
+  prepared_ = GpuDeviceBuffer<float>(std::max<std::size_t>(1, tile_ * exec_plan_.prepared_stride()));
+  work_ = GpuDeviceBuffer<float>(std::max<std::size_t>(1, 2 * tile_ * exec_plan_.work_stride()));
+  const std::size_t cap =
+      std::max<std::size_t>(1, exec_.initial_peak_buffer_bytes / sizeof(gaffa_hip_ffa_peak));
+  peaks_ = GpuDeviceBuffer<gaffa_hip_ffa_peak>(cap);
+  peak_count_ = GpuDeviceBuffer<std::uint32_t>(1);
+  desc_.groups = d_groups_.data();
+  desc_.ngroups = static_cast<std::uint32_t>(g.size());
+  desc_.tasks = d_tasks_.data();
+  desc_.ntasks = static_cast<std::uint32_t>(t.size());
+  desc_.subtree_items = d_subtree_.data();
+  desc_.nsubtree_items = static_cast<std::uint32_t>(exec_plan_.subtree_items().size());
+  desc_.level_items = d_level_.data();
+  desc_.level_item_offsets = exec_plan_.level_offsets().data();
+  desc_.max_depth = static_cast<std::uint32_t>(exec_plan_.max_depth());
+  desc_.terminal_items = d_terminal_.data();
+  desc_.nterminal_items = static_cast<std::uint32_t>(exec_plan_.terminal_items().size());
+  desc_.max_bins = exec_plan_.max_bins();
+  desc_.subtree_lds_bytes = exec_plan_.subtree_lds_bytes();
+  desc_.max_prepared_nsamples = exec_plan_.max_prepared();
+  desc_.prepared_stride = exec_plan_.prepared_stride();
+  desc_.work_stride = exec_plan_.work_stride();
+}
+
+std::size_t GpuFfaProgram::workspace_bytes() const noexcept {
+  return prepared_.bytes() + work_.bytes() + peaks_.bytes();
+}
+
+gaffa_hip_ffa_search_args GpuFfaProgram::args(const float* series, std::size_t nseries,
+                                              std::size_t stride, std::uint32_t base, float thr) {
+  desc_.level_item_offsets = exec_plan_.level_offsets().data();
+  gaffa_hip_ffa_search_args a{};
+  a.plan = &desc_;
+  a.series = series;
+  a.nseries = nseries;
+  a.nsamples = exec_plan_.input_nsamples();
+  a.series_stride = stride;
+  a.series_index_base = base;
+  a.snr_threshold = thr;
+  a.prepared = prepared_.data();
+  a.work = work_.data();
+  a.peaks = peaks_.data();
+  a.peak_capacity = peaks_.size();
+  a.peak_count = peak_count_.data();
+  return a;
+}
+
+void GpuFfaProgram::enqueue_tile(const float* series, std::size_t nseries, std::size_t stride,
+                                 std::uint32_t base, float thr, bool reset_count) {
+  check_arg(nseries > 0 && nseries <= tile_, "FFA tile exceeds the program tile capacity");
+  GpuDeviceScope scope(program_.device_id);
+  if (reset_count) {
+    hip_ok(gaffa_hip_memset(peak_count_.data(), 0, sizeof(std::uint32_t), exec_.stream),
+           "hipMemset");
+  }
+  const gaffa_hip_ffa_search_args a = args(series, nseries, stride, base, thr);
+  hip_ok(gaffa_hip_ffa_search_batch(&a, exec_.stream), "ffa_search_batch");
+}
+
+std::uint32_t GpuFfaProgram::read_peak_count() {
+  std::uint32_t n = 0;
+  copy_to_host(&n, peak_count_.data(), sizeof(n), exec_.stream);
+  return n;
+}
+
+std::vector<gaffa_hip_ffa_peak> GpuFfaProgram::read_peaks(std::uint32_t count) {
+  std::vector<gaffa_hip_ffa_peak> v(std::min<std::size_t>(count, peaks_.size()));
+  if (!v.empty()) copy_to_host(v.data(), peaks_.data(), v.size() * sizeof(v[0]), exec_.stream);
+  return v;
+}
+
+void GpuFfaProgram::grow_peaks(std::size_t count) {
+  const std::size_t max_count = exec_.max_peak_buffer_bytes / sizeof(gaffa_hip_ffa_peak);
+  if (count > max_count) {
+    throw std::runtime_error("FFA compact peak buffer exceeded max_peak_buffer_bytes");
+  }
+  GpuDeviceScope scope(program_.device_id);
+  peaks_ = GpuDeviceBuffer<gaffa_hip_ffa_peak>(std::min(max_count, std::max(count, 2 * peaks_.size())));
+}
+
+void GpuFfaProgram::transform_rows(const float* series, std::size_t nseries, std::size_t stride,
+                                   std::size_t task, float* out) {
+  check_arg(nseries > 0 && nseries <= tile_, "FFA tile exceeds the program tile capacity");
+  check_arg(task < exec_plan_.tasks().size(), "FFA task index out of range");
+  GpuDeviceScope scope(program_.device_id);
+  const gaffa_hip_ffa_search_args a = args(series, nseries, stride, 0, 0.0F);
+  hip_ok(gaffa_hip_ffa_transform_rows(&a, static_cast<std::uint32_t>(task), out, exec_.stream),
+         "ffa_transform_rows");
+}
+
+std::vector<FfaBatchPeak> expand_ffa_peaks(const GpuFfaExecutionPlan& plan,
+                                           std::span<const gaffa_hip_ffa_peak> peaks) {
+  std::map<std::size_t, FfaDetectionPlan> det;
+  std::vector<FfaBatchPeak> out;
+  out.reserve(peaks.size());
+  for (const gaffa_hip_ffa_peak& p : peaks) {
+    const FfaSearchTask& task = plan.plan().tasks.at(p.task);
+    auto it = det.find(task.bins);
+    if (it == det.end())
+      it = det.emplace(task.bins, make_ffa_detection_plan(plan.plan().width_trials, task.bins)).first;
+    const std::size_t widx = p.phase_width >> 16;
+    const std::size_t phase = p.phase_width & 0xFFFFu;
+    out.push_back(FfaBatchPeak{p.series, make_ffa_peak(task, it->second.boxcar_trials.at(widx),
+                                                       p.shift, phase, p.snr)});
+  }
+  return out;
+}
+
+FfaBatchSearchResult run_ffa_batch_gpu(GpuFfaProgram& program, GpuTimeSeriesBatchView batch,
+                                       const FfaSearchOptions& options) {
+  check_arg(std::isfinite(options.snr_threshold), "FFA search S/N threshold must be finite");
+  check_arg(batch.data != nullptr, "FFA batch data must not be null");
+  check_arg(batch.nseries > 0, "FFA batch must contain at least one series");
+  check_arg(batch.nsamples == program.execution_plan().input_nsamples(),
+            "FFA executor task input_nsamples must match time series size");
+  const std::size_t stride = batch.stride == 0 ? batch.nsamples : batch.stride;
+  check_arg(stride >= batch.nsamples, "FFA batch stride must be >= nsamples");
+  FfaBatchSearchResult result;
+  const std::size_t tile = program.tile_capacity();
+  for (std::size_t s0 = 0; s0 < batch.nseries; s0 += tile) {
+    const std::size_t ns = std::min(tile, batch.nseries - s0);
+    const float* base = batch.data + s0 * stride;
+    std::uint32_t count = 0;
+    for (;;) {
+      program.enqueue_tile(base, ns, stride, static_cast<std::uint32_t>(s0), options.snr_threshold,
+                           true);
+      count = program.read_peak_count();
+      if (count <= program.peak_capacity()) break;
+      program.grow_peaks(count);  // repeat the tile with room for every peak
+    }
+    const std::vector<gaffa_hip_ffa_peak> compact = program.read_peaks(count);
+    std::vector<FfaBatchPeak> expanded = expand_ffa_peaks(program.execution_plan(), compact);
+    result.peaks.insert(result.peaks.end(), expanded.begin(), expanded.end());
+  }
+  if (options.max_peaks != 0) {
+    std::map<std::size_t, std::size_t> per_series;
+    for (const auto& p : result.peaks) {
+      if (++per_series[p.series_index] > options.max_peaks) {
+        throw std::runtime_error("FFA detection peak count exceeded max_peaks safety guard");
+      }
+    }
+  }
+  std::sort(result.peaks.begin(), result.peaks.end(),
+            [](const FfaBatchPeak& a, const FfaBatchPeak& b) {
+              if (a.series_index != b.series_index) return a.series_index < b.series_index;
+              return is_better_ffa_peak(a.peak, b.peak);
+            });
+  return result;
+}
+
+FfaSearchResult search_ffa_gpu(GpuFfaProgram& program, GpuSpan<const float> ts,
+                               const FfaSearchOptions& options) {
+  GpuTimeSeriesBatchView batch{ts.data, 1, ts.count, ts.count, ts.device_id};
+  FfaBatchSearchResult r = run_ffa_batch_gpu(program, batch, options);
+  FfaSearchResult out;
+  out.peaks.reserve(r.peaks.size());
+  for (const auto& p : r.peaks) out.peaks.push_back(p.peak);
+  return out;
+}
+
+FfaSearchResult search_ffa_gpu_host(std::span<const float> ts, const FfaSearchPlan& plan,
+                                    const FfaSearchOptions& options, int device_id) {
+  check_arg(std::isfinite(options.snr_threshold), "FFA search S/N threshold must be finite");
+  check_arg(!plan.width_trials.empty(), "FFA search plan width_trials must not be empty");
+  check_arg(!ts.empty(), "FFA executor time series must not be empty");
+  check_arg(!plan.tasks.empty(), "FFA executor plan must contain at least one task");
+  for (const FfaSearchTask& t : plan.tasks) validate_task(t, ts.size());
+  GpuDeviceScope scope(device_id);
+  GpuDeviceBuffer<float> d = upload(ts.data(), ts.size());
+  GpuFfaExecutionOptions eo;
+  eo.series_tile_size = 1;
+  GpuFfaProgram program(plan, GpuFfaProgramOptions{device_id}, eo);
+  return search_ffa_gpu(program, GpuSpan<const float>{d.data(), d.size(), device_id}, options);
+}
+
+bool is_better_dm_peak(const DmPeak& a, const DmPeak& b) {
+  if (is_better_ffa_peak(a.peak, b.peak)) return true;
+  if (is_better_ffa_peak(b.peak, a.peak)) return false;
+  if (a.dm_index != b.dm_index) return a.dm_index < b.dm_index;
+  return a.dm < b.dm;
+}
+
+DmSearchResult find_dm_peaks_gpu(GpuDedispersedView<std::uint32_t> plane,
+                                 std::span<const double> dms, double tsamp,
+                                 const DmSearchOptions& options, int device_id) {
+  check_arg(plane.shape.ndm != 0, "DM search dedispersed ndm must be > 0");
+  check_arg(plane.shape.nsamples != 0, "DM search dedispersed nsamples must be > 0");
+  check_arg(tsamp > 0.0 && std::isfinite(tsamp), "DM search tsamp must be finite and > 0");
+  check_arg(dms.size() == plane.shape.ndm, "DM search dms size must match ndm");
+  for (double dm : dms) check_arg(std::isfinite(dm), "DM search dms must be finite");
+  check_arg(std::isfinite(options.snr_threshold), "DM search S/N threshold must be finite");
+  const std::size_t stride = plane.stride == 0 ? plane.shape.nsamples : plane.stride;
+  const FfaSearchPlan plan = make_riptide_ffa_plan(plane.shape.nsamples, tsamp, options.plan);
+  GpuDeviceScope scope(device_id);
+  GpuPreprocessExecutionOptions pe;
+  pe.series_tile_size = std::min<std::size_t>(plane.shape.ndm, 1024);
+  PreprocessPlan pp = options.preprocess;
+  const bool convert_only = pp.steps.empty();
+  if (!convert_only) {
+    GpuPreprocessProgram prep(pp, GpuPreprocessProgramOptions{device_id}, pe);
+    prep.run(plane.data, plane.shape.ndm, plane.shape.nsamples, stride, true);
+    prep.synchronize();
+  } else {
+    gaffa_hip_preprocess_args a{};
+    a.data = plane.data;
+    a.nseries = plane.shape.ndm;
+    a.nsamples = plane.shape.nsamples;
+    a.stride = stride;
+    a.input_is_u32 = 1;
+    GpuDeviceBuffer<std::uint8_t> ws(
+        gaffa_hip_preprocess_workspace_bytes(plane.shape.ndm, plane.shape.nsamples, 1));
+    GpuDeviceBuffer<std::int32_t> st(plane.shape.ndm);
+    hip_ok(gaffa_hip_memset(st.data(), 0, st.bytes(), nullptr), "hipMemset");
+    a.workspace = ws.data();
+    a.workspace_bytes = ws.bytes();
+    a.status = st.data();
+    a.scrunch_factor = 1;
+    a.median_window = 1;
+    hip_ok(gaffa_hip_detrend_normalise_batch(&a, nullptr), "convert_uint32_to_float");
+    hip_ok(gaffa_hip_stream_synchronize(nullptr), "hipStreamSynchronize");
+  }
+  GpuFfaProgram program(plan, GpuFfaProgramOptions{device_id});
+  const FfaSearchOptions fo{options.snr_threshold, options.max_peaks};
+  GpuTimeSeriesBatchView batch{reinterpret_cast<const float*>(plane.data), plane.shape.ndm,
+                               plane.shape.nsamples, stride, device_id};
+  const FfaBatchSearchResult r = run_ffa_batch_gpu(program, batch, fo);
+  DmSearchResult out;
+  out.peaks.reserve(r.peaks.size());
+  for (const auto& p : r.peaks) out.peaks.push_back(DmPeak{dms[p.series_index], p.series_index, p.peak});
+  std::sort(out.peaks.begin(), out.peaks.end(), is_better_dm_peak);
+  return out;
+}
+
+}  // namespace gaffa

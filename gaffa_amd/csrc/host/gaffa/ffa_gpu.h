@@ -1,0 +1,170 @@
+// GPU FFA search program (the HIP counterpart of include/gaffa/ffa_cuda.h:
+// CudaFfaExecutionPlan / CudaFfaProgram / run_ffa_batch_cuda /
+// search_ffa_cuda) and the batched DM search built on it
+// (include/gaffa/dm_search.h's DmPeak and find_dm_peaks_cpu semantics).
+#pragma once
+
+#include "gaffa/dedispersion.h"
+#include "gaffa/ffa_plan.h"
+#include "gaffa/gpu_memory.h"
+#include "gaffa/preprocessing.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <span>
+#include <vector>
+
+namespace gaffa {
+
+struct GpuFfaProgramOptions {
+  int device_id = 0;
+};
+
+struct GpuFfaExecutionOptions {
+  // Series processed per launch sequence; 0 picks the largest tile whose
+  // workspace fits workspace_bytes_limit.
+  std::size_t series_tile_size = 0;
+  std::size_t workspace_bytes_limit = 16ULL << 30;
+  std::size_t initial_peak_buffer_bytes = 64ULL << 20;
+  std::size_t max_peak_buffer_bytes = 1ULL << 30;
+  gaffa_hip_stream stream = nullptr;
+};
+
+struct FfaBatchPeak {
+  std::size_t series_index = 0;
+  FfaPeak peak{};
+};
+
+struct FfaBatchSearchResult {
+  std::vector<FfaBatchPeak> peaks;
+};
+
+// Host-built device layout of one FfaSearchPlan: prepare groups keyed by
+// downsample factor, per-task LDS depth, and the work-item lists of the
+// subtree / level / terminal launches (see gaffa_hip.h).
+class GpuFfaExecutionPlan {
+ public:
+  explicit GpuFfaExecutionPlan(const FfaSearchPlan& plan);
+
+  const FfaSearchPlan& plan() const noexcept { return plan_; }
+  std::size_t input_nsamples() const noexcept { return input_nsamples_; }
+  std::size_t prepared_stride() const noexcept { return prepared_stride_; }
+  std::size_t work_stride() const noexcept { return work_stride_; }
+  std::size_t max_depth() const noexcept { return max_depth_; }
+  const std::vector<gaffa_hip_ffa_group>& groups() const noexcept { return groups_; }
+  const std::vector<gaffa_hip_ffa_task>& tasks() const noexcept { return tasks_; }
+  const std::vector<gaffa_hip_ffa_item>& subtree_items() const noexcept { return subtree_; }
+  const std::vector<gaffa_hip_ffa_item>& level_items() const noexcept { return level_; }
+  const std::vector<std::uint32_t>& level_offsets() const noexcept { return level_offsets_; }
+  const std::vector<gaffa_hip_ffa_item>& terminal_items() const noexcept { return terminal_; }
+  std::uint32_t subtree_lds_bytes() const noexcept { return subtree_lds_bytes_; }
+  std::uint32_t max_bins() const noexcept { return max_bins_; }
+  std::size_t max_prepared() const noexcept { return max_prepared_; }
+  // Bytes of device workspace for a tile of `series` series.
+  std::size_t workspace_bytes(std::size_t series) const noexcept;
+
+ private:
+  FfaSearchPlan plan_;
+  std::size_t input_nsamples_ = 0;
+  std::size_t prepared_stride_ = 0;
+  std::size_t work_stride_ = 0;
+  std::size_t max_depth_ = 0;
+  std::size_t max_prepared_ = 0;
+  std::uint32_t subtree_lds_bytes_ = 0;
+  std::uint32_t max_bins_ = 0;
+  std::vector<gaffa_hip_ffa_group> groups_;
+  std::vector<gaffa_hip_ffa_task> tasks_;
+  std::vector<gaffa_hip_ffa_item> subtree_, level_, terminal_;
+  std::vector<std::uint32_t> level_offsets_;
+};
+
+// Owns device metadata, workspace and the compact peak buffer for one plan on
+// one device.  Move-only, not thread-safe, one active run at a time
+// (ffa_cuda.h:141-182).
+class GpuFfaProgram {
+ public:
+  explicit GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptions& program_options = {},
+                         const GpuFfaExecutionOptions& execution_options = {});
+  GpuFfaProgram(GpuFfaProgram&&) noexcept = default;
+  GpuFfaProgram& operator=(GpuFfaProgram&&) noexcept = default;
+
+  const GpuFfaExecutionPlan& execution_plan() const noexcept { return exec_plan_; }
+  int device_id() const noexcept { return program_.device_id; }
+  std::size_t tile_capacity() const noexcept { return tile_; }
+  std::size_t workspace_bytes() const noexcept;
+
+  // Enqueue the whole search of one tile (<= tile_capacity series) without
+  // waiting; peaks land in the device peak buffer.  Used by the benchmark
+  // and the graph path; run_ffa_batch_gpu() wraps it with the host side.
+  void enqueue_tile(const float* series, std::size_t nseries, std::size_t stride,
+                    std::uint32_t series_index_base, float snr_threshold, bool reset_count);
+  std::uint32_t read_peak_count();
+  std::vector<gaffa_hip_ffa_peak> read_peaks(std::uint32_t count);
+  std::size_t peak_capacity() const noexcept { return peaks_.size(); }
+  void grow_peaks(std::size_t count);
+  gaffa_hip_stream stream() const noexcept { return exec_.stream; }
+  // Transform-only access for testing: rows_eval final rows of one task.
+  void transform_rows(const float* series, std::size_t nseries, std::size_t stride,
+                      std::size_t task, float* out);
+
+ private:
+  gaffa_hip_ffa_search_args args(const float* series, std::size_t nseries, std::size_t stride,
+                                 std::uint32_t base, float thr);
+  GpuFfaExecutionPlan exec_plan_;
+  GpuFfaProgramOptions program_;
+  GpuFfaExecutionOptions exec_;
+  std::size_t tile_ = 0;
+  GpuDeviceBuffer<gaffa_hip_ffa_group> d_groups_;
+  GpuDeviceBuffer<gaffa_hip_ffa_task> d_tasks_;
+  GpuDeviceBuffer<gaffa_hip_ffa_item> d_subtree_, d_level_, d_terminal_;
+  GpuDeviceBuffer<float> prepared_, work_;
+  GpuDeviceBuffer<gaffa_hip_ffa_peak> peaks_;
+  GpuDeviceBuffer<std::uint32_t> peak_count_;
+  gaffa_hip_ffa_plan_desc desc_{};
+};
+
+// Expand compact device peaks into FfaBatchPeak records (make_ffa_peak).
+std::vector<FfaBatchPeak> expand_ffa_peaks(const GpuFfaExecutionPlan& plan,
+                                           std::span<const gaffa_hip_ffa_peak> peaks);
+
+// Searches a dense device batch [nseries][stride] of preprocessed series
+// with the program's plan; returns peaks sorted by (series, peak order).
+FfaBatchSearchResult run_ffa_batch_gpu(GpuFfaProgram& program, GpuTimeSeriesBatchView batch,
+                                       const FfaSearchOptions& options = {});
+
+// Single-series counterpart of search_ffa_cpu (device input).
+FfaSearchResult search_ffa_gpu(GpuFfaProgram& program, GpuSpan<const float> time_series,
+                               const FfaSearchOptions& options = {});
+
+// One-shot host convenience: upload, search, return (ffa_bindings.cpp:124-163).
+FfaSearchResult search_ffa_gpu_host(std::span<const float> time_series, const FfaSearchPlan& plan,
+                                    const FfaSearchOptions& options = {}, int device_id = 0);
+
+// ------------------------------------------------------------ DM search --
+struct DmPeak {
+  double dm = 0.0;
+  std::size_t dm_index = 0;
+  FfaPeak peak{};
+};
+
+struct DmSearchOptions {
+  RiptideFfaPlanOptions plan{};
+  PreprocessPlan preprocess{};
+  float snr_threshold = 6.0F;
+  std::size_t max_peaks = 0;
+};
+
+struct DmSearchResult {
+  std::vector<DmPeak> peaks;
+};
+
+bool is_better_dm_peak(const DmPeak& lhs, const DmPeak& rhs);
+
+// GPU counterpart of find_dm_peaks_cpu (dm_search.cpp:142-199): the device
+// plane (uint32 sums) is converted, preprocessed in place and searched.
+// The plane's contents are consumed (it holds normalised floats afterwards).
+DmSearchResult find_dm_peaks_gpu(GpuDedispersedView<std::uint32_t> plane,
+                                 std::span<const double> dms, double tsamp,
+                                 const DmSearchOptions& options, int device_id = 0);
+
+}  // namespace gaffa

@@ -1,0 +1,101 @@
+// Device-resident DM search pipeline: filterbank -> (subband or brute-force)
+// dedispersion over one or more uniform DM segments -> in-place convert +
+// detrend + normalise -> batched FFA search -> DmPeak list.  This is the
+// composition gaffa assembles by hand in benchmarks/cuda_dm_search_benchmark
+// .cpp:179-294 and tests/cpp/test_cuda_search_pipeline.cpp:59-155, kept on
+// one stream with no host round trip between stages.
+#pragma once
+
+#include "gaffa/dedispersion.h"
+#include "gaffa/ffa_gpu.h"
+#include "gaffa/preprocessing.h"
+#include "gaffa/synth.h"
+
+#include <cstddef>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace gaffa {
+
+// One uniform run of DM trials; a plan may chain several (non-uniform steps).
+struct DmSegment {
+  double dm_low = 0.0;
+  double dm_step = 1.0;
+  std::size_t ndm = 0;
+};
+
+struct SearchPipelineConfig {
+  std::size_t nsamples = 0;
+  std::size_t nchans = 0;
+  double tsamp = 0.0;
+  std::vector<double> frequency_mhz;  // ascending, one per channel
+  std::vector<DmSegment> dm_segments;
+  bool subband = true;
+  SubbandDedispersionOptions subband_options{};
+  PreprocessPlan preprocess{};
+  RiptideFfaPlanOptions ffa{};
+  float snr_threshold = 6.0F;
+  std::size_t ffa_workspace_bytes_limit = 32ULL << 30;
+  std::size_t ffa_series_tile = 0;
+  int device_id = 0;
+};
+
+// GPU time per stage from HIP events on the pipeline stream: dedispersion
+// (incl. its table uploads), preprocessing, the FFA launches of every series
+// tile (kernels only; the peak readback follows), and first event to last.
+struct StageTimes {
+  float dedisperse_ms = 0.0F;
+  float preprocess_ms = 0.0F;
+  float ffa_ms = 0.0F;
+  float total_ms = 0.0F;
+};
+
+class GpuSearchPipeline {
+ public:
+  explicit GpuSearchPipeline(SearchPipelineConfig config);
+  ~GpuSearchPipeline();
+  GpuSearchPipeline(const GpuSearchPipeline&) = delete;
+  GpuSearchPipeline& operator=(const GpuSearchPipeline&) = delete;
+
+  void generate(const SynthFilterbankSpec& spec);
+  void upload(const std::uint8_t* samples, std::size_t count);
+  void download_filterbank(std::uint8_t* out) const;
+
+  // Stages, each enqueued on the pipeline stream.
+  void dedisperse();
+  void preprocess();
+  void search();
+  StageTimes run();  // all three, timed with HIP events
+
+  std::size_t ndm() const noexcept { return ndm_; }
+  std::size_t out_nsamples() const noexcept { return n_out_; }
+  std::size_t plane_stride() const noexcept { return stride_; }
+  const std::vector<double>& dms() const noexcept { return dms_; }
+  const FfaSearchPlan& ffa_plan() const noexcept { return ffa_plan_; }
+  const GpuFfaProgram& ffa_program() const noexcept { return *ffa_; }
+  GpuFfaProgram& ffa_program() noexcept { return *ffa_; }
+  const std::vector<DmPeak>& peaks() const noexcept { return peaks_; }
+  std::uint32_t* plane_u32() noexcept { return plane_.data(); }
+  const std::uint8_t* filterbank() const noexcept { return filterbank_.data(); }
+  gaffa_hip_stream stream() const noexcept { return stream_.get(); }
+  // Copies rows [0, ndm) x [0, out_nsamples) of the plane (uint32 or float
+  // bits, depending on the stage reached) into out.
+  void download_plane(void* out) const;
+
+ private:
+  SearchPipelineConfig cfg_;
+  GpuStream stream_;
+  std::size_t ndm_ = 0, n_out_ = 0, stride_ = 0;
+  std::vector<double> dms_;
+  std::vector<MultiDmDedispersionPlan> segment_plans_;
+  GpuDeviceBuffer<std::uint8_t> filterbank_;
+  GpuDeviceBuffer<std::uint32_t> plane_;
+  FfaSearchPlan ffa_plan_;
+  std::unique_ptr<GpuPreprocessProgram> prep_;
+  std::unique_ptr<GpuFfaProgram> ffa_;
+  std::vector<DmPeak> peaks_;
+  std::vector<gaffa_hip_event> events_;
+};
+
+}  // namespace gaffa

@@ -1,0 +1,178 @@
+// Device-resident DM search pipeline (see gaffa/pipeline.h).
+#include "gaffa/pipeline.h"
+
+#include <algorithm>
+#include <cmath>
+#include <stdexcept>
+
+namespace gaffa {
+namespace {
+
+void check_arg(bool ok, const char* message) {
+  if (!ok) throw std::invalid_argument(message);
+}
+
+}  // namespace
+
+GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::move(config)) {
+  check_arg(cfg_.nsamples > 0 && cfg_.nchans > 0, "pipeline filterbank must not be empty");
+  check_arg(cfg_.frequency_mhz.size() == cfg_.nchans,
+            "frequency table length must match sample channel count");
+  check_arg(!cfg_.dm_segments.empty(), "pipeline requires at least one DM segment");
+  GpuDeviceScope scope(cfg_.device_id);
+  stream_ = GpuStream::create();
+  const double ref = *std::max_element(cfg_.frequency_mhz.begin(), cfg_.frequency_mhz.end());
+  n_out_ = cfg_.nsamples;
+  for (const DmSegment& s : cfg_.dm_segments) {
+    check_arg(s.ndm > 0, "dedispersion requires at least one DM");
+    MultiDmDedispersionPlan p{s.dm_low, s.dm_step, s.ndm, ref, cfg_.tsamp, 0, cfg_.nchans};
+    n_out_ = std::min(n_out_, multi_dm_output_nsamples(cfg_.nsamples, cfg_.frequency_mhz, p));
+    segment_plans_.push_back(p);
+    for (std::size_t i = 0; i < s.ndm; ++i)
+      dms_.push_back(s.dm_low + static_cast<double>(i) * s.dm_step);
+    ndm_ += s.ndm;
+  }
+  stride_ = (n_out_ + 63) / 64 * 64;
+  filterbank_ = GpuDeviceBuffer<std::uint8_t>(cfg_.nsamples * cfg_.nchans);
+  plane_ = GpuDeviceBuffer<std::uint32_t>(ndm_ * stride_);
+  ffa_plan_ = make_riptide_ffa_plan(n_out_, cfg_.tsamp, cfg_.ffa);
+  if (!cfg_.preprocess.steps.empty()) {
+    GpuPreprocessExecutionOptions pe;
+    pe.series_tile_size = std::min<std::size_t>(ndm_, 1024);
+    pe.max_nsamples = n_out_;
+    pe.stream = stream_.get();
+    prep_ = std::make_unique<GpuPreprocessProgram>(cfg_.preprocess,
+                                                   GpuPreprocessProgramOptions{cfg_.device_id}, pe);
+  }
+  GpuFfaExecutionOptions fe;
+  fe.workspace_bytes_limit = cfg_.ffa_workspace_bytes_limit;
+  fe.series_tile_size = cfg_.ffa_series_tile;
+  fe.stream = stream_.get();
+  ffa_ = std::make_unique<GpuFfaProgram>(ffa_plan_, GpuFfaProgramOptions{cfg_.device_id}, fe);
+  events_.resize(5, nullptr);
+  for (auto& e : events_) hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
+}
+
+GpuSearchPipeline::~GpuSearchPipeline() {
+  for (auto e : events_)
+    if (e != nullptr) (void)gaffa_hip_event_destroy(e);
+}
+
+void GpuSearchPipeline::generate(const SynthFilterbankSpec& spec) {
+  check_arg(spec.nsamples == cfg_.nsamples && spec.nchans == cfg_.nchans,
+            "synthetic filterbank shape must match the pipeline");
+  GpuDeviceScope scope(cfg_.device_id);
+  synth_filterbank_gpu(spec, filterbank_.data(), 0, spec.nsamples, stream_.get());
+}
+
+void GpuSearchPipeline::upload(const std::uint8_t* samples, std::size_t count) {
+  check_arg(count == filterbank_.size(), "filterbank size does not match the pipeline");
+  GpuDeviceScope scope(cfg_.device_id);
+  copy_to_device(filterbank_.data(), samples, count, nullptr);
+}
+
+void GpuSearchPipeline::download_filterbank(std::uint8_t* out) const {
+  GpuDeviceScope scope(cfg_.device_id);
+  copy_to_host(out, filterbank_.data(), filterbank_.size(), stream_.get());
+}
+
+void GpuSearchPipeline::dedisperse() {
+  GpuDeviceScope scope(cfg_.device_id);
+  const GpuSampleView v{filterbank_.data(), SampleShape{cfg_.nsamples, 1, cfg_.nchans},
+                        cfg_.device_id};
+  GpuDedispersionOptions o;
+  o.device_id = cfg_.device_id;
+  o.stream = stream_.get();
+  std::size_t row = 0;
+  for (const MultiDmDedispersionPlan& p : segment_plans_) {
+    std::uint32_t* out = plane_.data() + row * stride_;
+    if (cfg_.subband) {
+      dedisperse_subband_gpu_into(v, cfg_.frequency_mhz, p, cfg_.subband_options, out, stride_, o,
+                                  n_out_);
+    } else {
+      dedisperse_multi_dm_gpu_into(v, cfg_.frequency_mhz, p, out, stride_, o, n_out_);
+    }
+    row += p.ndm;
+  }
+}
+
+void GpuSearchPipeline::preprocess() {
+  GpuDeviceScope scope(cfg_.device_id);
+  if (prep_) {
+    prep_->run(plane_.data(), ndm_, n_out_, stride_, true);
+  } else {
+    gaffa_hip_preprocess_args a{};
+    a.data = plane_.data();
+    a.nseries = ndm_;
+    a.nsamples = n_out_;
+    a.stride = stride_;
+    a.input_is_u32 = 1;
+    a.scrunch_factor = 1;
+    a.median_window = 1;
+    GpuDeviceBuffer<std::uint8_t> ws(gaffa_hip_preprocess_workspace_bytes(ndm_, n_out_, 1));
+    GpuDeviceBuffer<std::int32_t> st(ndm_);
+    hip_ok(gaffa_hip_memset(st.data(), 0, st.bytes(), stream_.get()), "hipMemset");
+    a.workspace = ws.data();
+    a.workspace_bytes = ws.bytes();
+    a.status = st.data();
+    hip_ok(gaffa_hip_detrend_normalise_batch(&a, stream_.get()), "convert_uint32_to_float");
+    stream_.synchronize();
+  }
+}
+
+void GpuSearchPipeline::search() {
+  GpuDeviceScope scope(cfg_.device_id);
+  GpuFfaProgram& prog = *ffa_;
+  const float* series = reinterpret_cast<const float*>(plane_.data());
+  const std::size_t tile = prog.tile_capacity();
+  std::uint32_t count = 0;
+  for (;;) {
+    hip_ok(gaffa_hip_event_record(events_[3], stream_.get()), "hipEventRecord");
+    for (std::size_t s0 = 0; s0 < ndm_; s0 += tile) {
+      const std::size_t ns = std::min(tile, ndm_ - s0);
+      prog.enqueue_tile(series + s0 * stride_, ns, stride_, static_cast<std::uint32_t>(s0),
+                        cfg_.snr_threshold, s0 == 0);
+    }
+    hip_ok(gaffa_hip_event_record(events_[4], stream_.get()), "hipEventRecord");
+    count = prog.read_peak_count();
+    if (count <= prog.peak_capacity()) break;
+    prog.grow_peaks(count);
+  }
+  const std::vector<gaffa_hip_ffa_peak> compact = prog.read_peaks(count);
+  const std::vector<FfaBatchPeak> expanded = expand_ffa_peaks(prog.execution_plan(), compact);
+  peaks_.clear();
+  peaks_.reserve(expanded.size());
+  for (const auto& p : expanded)
+    peaks_.push_back(DmPeak{dms_[p.series_index], p.series_index, p.peak});
+  std::sort(peaks_.begin(), peaks_.end(), is_better_dm_peak);
+}
+
+StageTimes GpuSearchPipeline::run() {
+  GpuDeviceScope scope(cfg_.device_id);
+  gaffa_hip_stream st = stream_.get();
+  hip_ok(gaffa_hip_event_record(events_[0], st), "hipEventRecord");
+  dedisperse();
+  hip_ok(gaffa_hip_event_record(events_[1], st), "hipEventRecord");
+  preprocess();
+  hip_ok(gaffa_hip_event_record(events_[2], st), "hipEventRecord");
+  if (prep_) prep_->synchronize();
+  search();  // records events_[3] / events_[4] around the FFA launches
+  stream_.synchronize();
+  StageTimes t;
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[0], events_[1], &t.dedisperse_ms), "elapsed");
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[1], events_[2], &t.preprocess_ms), "elapsed");
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[3], events_[4], &t.ffa_ms), "elapsed");
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[0], events_[4], &t.total_ms), "elapsed");
+  return t;
+}
+
+void GpuSearchPipeline::download_plane(void* out) const {
+  GpuDeviceScope scope(cfg_.device_id);
+  std::vector<std::uint32_t> tmp(ndm_ * stride_);
+  copy_to_host(tmp.data(), plane_.data(), tmp.size() * 4, stream_.get());
+  auto* o = static_cast<std::uint32_t*>(out);
+  for (std::size_t i = 0; i < ndm_; ++i)
+    std::copy_n(tmp.data() + i * stride_, n_out_, o + i * n_out_);
+}
+
+}  // namespace gaffa

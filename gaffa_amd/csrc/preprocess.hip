@@ -1,0 +1,371 @@
+// Batched in-place preprocessing of a DM x time plane on gfx950: uint32 ->
+// float conversion, riptide's fast running-median detrend and normalisation
+// (preprocessing.cpp:53-145, 212-231; time_series.cpp:50-104).
+//
+// Kernels (one launch each, all series at once, grid.y = series):
+//   scrunch   - low[k] = float(sum_j double(x[k*f + j]) / f), serial FP64 sum
+//               per output so the rounding equals the CPU loop
+//   median    - exact centred median of an edge-padded window: a workgroup
+//               bitonic-sorts (value, padded position) pairs of its tile's
+//               union window in LDS, then each output counts members of its
+//               own window in sorted order
+//   detrend   - FP64 linear interpolation of the low-resolution median back to
+//               full resolution (same operation order as the CPU), subtract,
+//               write float, per-block FP64 (sum, sum of squares) partials
+//   stats     - one workgroup per series: fixed-order (deterministic)
+//               reduction of the partials into mean and 1/stddev
+//   normalise - (x - mean) * (1/stddev) in FP64, rounded to float
+// Integer-valued inputs (dedispersed sums) make the scrunch sums exact; the
+// only difference from the CPU is the order of the FP64 statistics sums.
+#include "common.hpp"
+
+#include <cmath>
+
+namespace gaffa_hip {
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kDetrendItems = 8;  // samples per thread in detrend/normalise
+constexpr int kBlockSamples = kThreads * kDetrendItems;
+
+template <typename T>
+__device__ inline float load_as_float(const T* p, uint64_t i) {
+  return static_cast<float>(p[i]);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+scrunch_kernel(const T* __restrict__ data, uint64_t stride, uint64_t factor,
+               uint64_t m, float* __restrict__ low) {
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  const uint64_t series = blockIdx.y;
+  if (k >= m) return;
+  const T* x = data + series * stride + k * factor;
+  double sum = 0.0;
+  for (uint64_t j = 0; j < factor; ++j) sum += static_cast<double>(load_as_float(x, j));
+  low[series * m + k] = static_cast<float>(sum / static_cast<double>(factor));
+}
+
+// Median of every centred window of `window` (odd) points over an
+// edge-padded sequence of length m.  SORT >= kThreads + window - 1.
+template <int SORT, typename T>
+__global__ void __launch_bounds__(kThreads)
+median_kernel(const T* __restrict__ src, uint64_t src_stride, uint64_t m,
+              uint32_t window, float* __restrict__ med, uint64_t med_stride) {
+  __shared__ float key[SORT];
+  __shared__ int32_t pos[SORT];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t series = blockIdx.y;
+  const int64_t k0 = static_cast<int64_t>(blockIdx.x) * kThreads;
+  const int32_t half = static_cast<int32_t>(window / 2);
+  const int32_t span = kThreads + static_cast<int32_t>(window) - 1;
+  const T* s = src + series * src_stride;
+  for (int32_t e = tid; e < SORT; e += kThreads) {
+    if (e < span) {
+      const int64_t p = k0 - half + e;  // padded position
+      const int64_t idx = p < 0 ? 0 : (p >= static_cast<int64_t>(m) ? static_cast<int64_t>(m) - 1 : p);
+      key[e] = load_as_float(s, static_cast<uint64_t>(idx));
+      pos[e] = e;
+    } else {
+      key[e] = INFINITY;
+      pos[e] = 0x7fffffff;
+    }
+  }
+  __syncthreads();
+  // bitonic sort ascending by key
+  for (int32_t size = 2; size <= SORT; size <<= 1) {
+    for (int32_t stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int32_t e = tid; e < SORT / 2; e += kThreads) {
+        const int32_t lo = 2 * e - (e & (stride - 1));
+        const int32_t hi = lo + stride;
+        const bool ascending = (lo & size) == 0;
+        const float a = key[lo], b = key[hi];
+        if ((a > b) == ascending) {
+          key[lo] = b;
+          key[hi] = a;
+          const int32_t t = pos[lo];
+          pos[lo] = pos[hi];
+          pos[hi] = t;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  const int64_t k = k0 + tid;
+  if (k >= static_cast<int64_t>(m)) return;
+  // window of output tid covers padded offsets [tid, tid + window)
+  const uint32_t lo = tid;
+  int32_t count = 0;
+  float value = 0.0f;
+  for (int32_t e = 0; e < span; ++e) {
+    const uint32_t off = static_cast<uint32_t>(pos[e]) - lo;
+    if (off < window) {
+      if (count == half) {
+        value = key[e];
+        break;
+      }
+      ++count;
+    }
+  }
+  med[series * med_stride + static_cast<uint64_t>(k)] = value;
+}
+
+// x - interpolated median, written in place as float, plus FP64 partials.
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+detrend_kernel(T* __restrict__ data, uint64_t stride, uint64_t n,
+               int32_t detrend, uint64_t factor, const float* __restrict__ med,
+               uint64_t m, double2* __restrict__ partial, uint32_t nblocks,
+               int32_t* __restrict__ status) {
+  __shared__ double s1[kThreads], s2[kThreads];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t series = blockIdx.y;
+  T* x = data + series * stride;
+  float* out = reinterpret_cast<float*>(x);
+  const float* md = med + series * (factor == 1 ? n : m);
+  const double offset = 0.5 * static_cast<double>(factor - 1);
+  const double fd = static_cast<double>(factor);
+  double a1 = 0.0, a2 = 0.0;
+  bool bad = false;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlockSamples;
+#pragma unroll
+  for (int r = 0; r < kDetrendItems; ++r) {
+    const uint64_t i = base + static_cast<uint64_t>(r) * kThreads + tid;
+    if (i >= n) break;
+    const float v = load_as_float(x, i);
+    bad |= !isfinite(v);
+    float d = v;
+    if (detrend) {
+      float baseline;
+      if (factor == 1) {
+        baseline = md[i];
+      } else {
+        const double p = (static_cast<double>(i) - offset) / fd;
+        if (p <= 0.0) {
+          baseline = md[0];
+        } else {
+          const uint64_t lo = static_cast<uint64_t>(floor(p));
+          if (lo >= m - 1) {
+            baseline = md[m - 1];
+          } else {
+            const double fr = p - static_cast<double>(lo);
+            const double w0 = (1.0 - fr) * static_cast<double>(md[lo]);
+            const double w1 = fr * static_cast<double>(md[lo + 1]);
+            baseline = static_cast<float>(w0 + w1);
+          }
+        }
+      }
+      d = v - baseline;
+    }
+    out[i] = d;
+    const double dd = static_cast<double>(d);
+    a1 += dd;
+    a2 += dd * dd;
+  }
+  if (bad) atomicExch(&status[series], 1);
+  s1[tid] = a1;
+  s2[tid] = a2;
+  __syncthreads();
+  for (uint32_t h = kThreads / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+      s1[tid] += s1[tid + h];
+      s2[tid] += s2[tid + h];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) partial[series * nblocks + blockIdx.x] = make_double2(s1[0], s2[0]);
+}
+
+// One workgroup per series: fixed-order (deterministic) reduction of the
+// detrend partials into mean, 1/stddev and a status flag.
+__global__ void __launch_bounds__(kThreads)
+stats_kernel(const double2* __restrict__ partial, uint32_t nblocks, uint64_t n,
+             int32_t reject_constant, double2* __restrict__ stats,
+             int32_t* __restrict__ flags, int32_t* __restrict__ status) {
+  __shared__ double s1[kThreads], s2[kThreads];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t series = blockIdx.x;
+  const double2* p = partial + series * nblocks;
+  double a1 = 0.0, a2 = 0.0;
+  for (uint32_t b = tid; b < nblocks; b += kThreads) {
+    a1 += p[b].x;
+    a2 += p[b].y;
+  }
+  s1[tid] = a1;
+  s2[tid] = a2;
+  __syncthreads();
+  for (uint32_t h = kThreads / 2; h > 0; h >>= 1) {
+    if (tid < h) {
+      s1[tid] += s1[tid + h];
+      s2[tid] += s2[tid + h];
+    }
+    __syncthreads();
+  }
+  if (tid != 0) return;
+  const double count = static_cast<double>(n);
+  const double mean = s1[0] / count;
+  double var = s2[0] / count - mean * mean;
+  if (var < 0.0 && var > -1.0e-12) var = 0.0;
+  int32_t flag = 0;
+  if (var < 0.0 || !isfinite(var)) flag = 3;
+  const double sd = sqrt(var);
+  if (flag == 0 && (!(sd > 0.0) || !isfinite(sd))) flag = reject_constant ? 2 : 4;
+  stats[series] = make_double2(mean, flag == 0 ? 1.0 / sd : 0.0);
+  flags[series] = flag;
+  if (flag == 2 || flag == 3) atomicCAS(&status[series], 0, flag);
+}
+
+__global__ void __launch_bounds__(kThreads)
+normalise_kernel(float* __restrict__ data, uint64_t stride, uint64_t n,
+                 const double2* __restrict__ stats, const int32_t* __restrict__ flags) {
+  const uint32_t tid = threadIdx.x;
+  const uint64_t series = blockIdx.y;
+  const int32_t flag = flags[series];
+  if (flag == 2 || flag == 3) return;  // rejected: data left detrended, status set
+  const double mean = stats[series].x, scale = stats[series].y;
+  float* x = data + series * stride;
+  const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlockSamples;
+#pragma unroll
+  for (int r = 0; r < kDetrendItems; ++r) {
+    const uint64_t i = base + static_cast<uint64_t>(r) * kThreads + tid;
+    if (i >= n) break;
+    // flag 4: constant input accepted -> zeros (time_series.cpp:90-93)
+    x[i] = flag == 4 ? 0.0f
+                     : static_cast<float>((static_cast<double>(x[i]) - mean) * scale);
+  }
+}
+
+uint64_t median_points(uint64_t n, uint64_t factor) { return factor == 1 ? n : n / factor; }
+
+template <typename T>
+void launch_median(const T* src, uint64_t src_stride, uint64_t m, uint32_t window,
+                   float* med, uint64_t med_stride, uint64_t nseries, hipStream_t st) {
+  const dim3 grid(static_cast<uint32_t>(ceil_div(m, kThreads)), static_cast<uint32_t>(nseries));
+  const uint32_t span = kThreads + window - 1;
+  if (span <= 512) {
+    hipLaunchKernelGGL((median_kernel<512, T>), grid, dim3(kThreads), 0, st, src, src_stride, m,
+                       window, med, med_stride);
+  } else if (span <= 1024) {
+    hipLaunchKernelGGL((median_kernel<1024, T>), grid, dim3(kThreads), 0, st, src, src_stride, m,
+                       window, med, med_stride);
+  } else if (span <= 2048) {
+    hipLaunchKernelGGL((median_kernel<2048, T>), grid, dim3(kThreads), 0, st, src, src_stride, m,
+                       window, med, med_stride);
+  } else {
+    hipLaunchKernelGGL((median_kernel<4096, T>), grid, dim3(kThreads), 0, st, src, src_stride, m,
+                       window, med, med_stride);
+  }
+  launch_check("median_kernel");
+}
+
+struct Layout {
+  uint64_t m, low_floats, med_floats, nblocks, partial_bytes, partial_offset, stats_offset,
+      flags_offset, total;
+};
+
+Layout layout(uint64_t nseries, uint64_t n, uint64_t factor) {
+  Layout l{};
+  l.m = median_points(n, factor);
+  l.low_floats = factor == 1 ? 0 : nseries * l.m;
+  l.med_floats = nseries * l.m;
+  l.nblocks = ceil_div(n, kBlockSamples);
+  l.partial_bytes = nseries * l.nblocks * sizeof(double2);
+  l.partial_offset = ((l.low_floats + l.med_floats) * 4 + 255) / 256 * 256;
+  l.stats_offset = l.partial_offset + (l.partial_bytes + 255) / 256 * 256;
+  l.flags_offset = l.stats_offset + (nseries * sizeof(double2) + 255) / 256 * 256;
+  l.total = l.flags_offset + nseries * sizeof(int32_t);
+  return l;
+}
+
+template <typename T>
+void run(const gaffa_hip_preprocess_args* a, hipStream_t st) {
+  const Layout l = layout(a->nseries, a->nsamples, a->detrend ? a->scrunch_factor : 1);
+  float* low = static_cast<float*>(a->workspace);
+  float* med = low + l.low_floats;
+  char* ws = static_cast<char*>(a->workspace);
+  double2* partial = reinterpret_cast<double2*>(ws + l.partial_offset);
+  double2* stats = reinterpret_cast<double2*>(ws + l.stats_offset);
+  int32_t* flags = reinterpret_cast<int32_t*>(ws + l.flags_offset);
+  T* data = static_cast<T*>(a->data);
+  const uint64_t f = a->scrunch_factor;
+  if (a->detrend) {
+    if (f > 1) {
+      const dim3 g(static_cast<uint32_t>(ceil_div(l.m, kThreads)), static_cast<uint32_t>(a->nseries));
+      hipLaunchKernelGGL((scrunch_kernel<T>), g, dim3(kThreads), 0, st, data, a->stride, f, l.m,
+                         low);
+      launch_check("scrunch_kernel");
+      launch_median<float>(low, l.m, l.m, static_cast<uint32_t>(a->median_window), med, l.m,
+                           a->nseries, st);
+    } else {
+      launch_median<T>(data, a->stride, a->nsamples, static_cast<uint32_t>(a->median_window), med,
+                       a->nsamples, a->nseries, st);
+    }
+  }
+  const dim3 g(static_cast<uint32_t>(l.nblocks), static_cast<uint32_t>(a->nseries));
+  hipLaunchKernelGGL((detrend_kernel<T>), g, dim3(kThreads), 0, st, data, a->stride, a->nsamples,
+                     a->detrend, a->detrend ? f : 1, med, l.m, partial,
+                     static_cast<uint32_t>(l.nblocks), a->status);
+  launch_check("detrend_kernel");
+  if (a->normalise) {
+    hipLaunchKernelGGL(stats_kernel, dim3(static_cast<uint32_t>(a->nseries)), dim3(kThreads), 0,
+                       st, partial, static_cast<uint32_t>(l.nblocks), a->nsamples,
+                       a->reject_constant, stats, flags, a->status);
+    launch_check("stats_kernel");
+    hipLaunchKernelGGL(normalise_kernel, g, dim3(kThreads), 0, st,
+                       reinterpret_cast<float*>(data), a->stride, a->nsamples, stats, flags);
+    launch_check("normalise_kernel");
+  }
+}
+
+}  // namespace
+}  // namespace gaffa_hip
+
+using namespace gaffa_hip;
+
+extern "C" {
+
+uint64_t gaffa_hip_preprocess_workspace_bytes(uint64_t nseries, uint64_t nsamples,
+                                              uint64_t scrunch_factor) {
+  if (nseries == 0 || nsamples == 0 || scrunch_factor == 0) return 0;
+  return layout(nseries, nsamples, scrunch_factor).total;
+}
+
+int gaffa_hip_detrend_normalise_batch(const gaffa_hip_preprocess_args* a,
+                                      gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(a != nullptr, "preprocess arguments must not be null");
+    check(a->data != nullptr && a->status != nullptr, "preprocess pointers must not be null");
+    check(a->nseries > 0 && a->nsamples > 0, "preprocess batch must not be empty");
+    check(a->stride >= a->nsamples, "preprocess stride must be >= nsamples");
+    check(a->detrend || a->normalise || a->input_is_u32,
+          "preprocess plan must contain at least one step");
+    check(a->nseries < 65536, "preprocess batch has too many series for one launch");
+    const uint64_t f = a->detrend ? a->scrunch_factor : 1;
+    if (a->detrend) {
+      check(a->median_window > 0 && a->median_window % 2 == 1,
+            "running median detrend window must be odd and > 0");
+      check(f >= 1, "scrunch factor must be >= 1");
+      check(a->median_window + kThreads - 1 <= 4096,
+            "GPU running median supports windows up to 3841 points");
+      const uint64_t m = median_points(a->nsamples, f);
+      if (f == 1) {
+        check(a->median_window < a->nsamples,
+              "running median detrend window_samples must be < input size");
+      } else {
+        check(m > a->median_window,
+              "running median detrend low-resolution sample count must be > min_points");
+      }
+    }
+    const Layout l = layout(a->nseries, a->nsamples, f);
+    check(a->workspace != nullptr && a->workspace_bytes >= l.total,
+          "preprocess workspace is too small");
+    check_overflow(l.nblocks < 0x7FFFFFFFull, "preprocess series too long");
+    if (a->input_is_u32) {
+      run<uint32_t>(a, as_stream(stream));
+    } else {
+      run<float>(a, as_stream(stream));
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,75 @@
+// Seeded synthetic filterbank generator (no gaffa counterpart; it feeds the
+// benchmark and the tests).  Sample (t, c) of a [t][c] uint8 buffer is
+//   clip(pop - 384 + floor((dither + rfi_q[c] + sum_p profile_p[bin_p]) / 4096))
+// where pop is the popcount of 16 hashed 64-bit words (Binomial(1024, 1/2):
+// mean 512, sigma 16), dither is 12 hashed bits, and bin_p is the exact
+// integer pulse phase ((t - delay_p[c]) * 2^20 - phase0_q) mod period_q,
+// scaled to 4096 bins.  Only integer arithmetic runs on the device, so the
+// bytes equal the CPU oracle's for the same host tables.
+#include "common.hpp"
+
+namespace gaffa_hip {
+namespace {
+
+constexpr int kPhaseBins = 4096;
+constexpr int64_t kPeriodFrac = 1LL << 20;
+
+__device__ __forceinline__ uint64_t splitmix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+__global__ void __launch_bounds__(256)
+synth_kernel(gaffa_hip_synth_args a) {
+  const uint64_t total = a.t_count * a.nchans;
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= total) return;
+  const uint64_t tl = e / a.nchans;
+  const uint64_t c = e - tl * a.nchans;
+  const uint64_t t = a.t_begin + tl;
+  const uint64_t k = (t * a.nchans + c) * 17;
+  int64_t pop = 0;
+#pragma unroll
+  for (uint64_t j = 0; j < 16; ++j) pop += __popcll(splitmix(a.key + k + j));
+  int64_t acc = static_cast<int64_t>(splitmix(a.key + k + 16) & 4095u);
+  acc += a.rfi_q[c];
+  for (uint32_t p = 0; p < a.npulsars; ++p) {
+    const int64_t tau = static_cast<int64_t>(t) - a.delays[p * a.nchans + c];
+    const int64_t pq = a.period_q[p];
+    int64_t ph = (tau * kPeriodFrac - a.phase0_q[p]) % pq;
+    if (ph < 0) ph += pq;
+    const int64_t bin = ph * kPhaseBins / pq;
+    acc += a.profiles[static_cast<uint64_t>(p) * kPhaseBins + bin];
+  }
+  const int64_t q = acc >= 0 ? acc / 4096 : -((-acc + 4095) / 4096);
+  int64_t v = pop - 384 + q;
+  v = v < 0 ? 0 : (v > 255 ? 255 : v);
+  a.out[e] = static_cast<uint8_t>(v);
+}
+
+}  // namespace
+}  // namespace gaffa_hip
+
+using namespace gaffa_hip;
+
+extern "C" int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* a,
+                                          gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(a != nullptr && a->out != nullptr && a->rfi_q != nullptr,
+          "synth arguments must not be null");
+    check(a->nchans > 0, "synth requires at least one channel");
+    check(a->npulsars == 0 ||
+              (a->delays != nullptr && a->profiles != nullptr && a->period_q != nullptr &&
+               a->phase0_q != nullptr),
+          "synth pulsar tables must not be null");
+    const uint64_t total = a->t_count * a->nchans;
+    if (total == 0) return;
+    const uint64_t blocks = ceil_div(total, 256);
+    check_overflow(blocks < 0x7FFFFFFFull, "synth buffer too large for one launch");
+    hipLaunchKernelGGL(synth_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                       as_stream(stream), *a);
+    launch_check("synth_kernel");
+  });
+}

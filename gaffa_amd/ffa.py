@@ -1,0 +1,100 @@
+"""Fast-folding-algorithm search of preprocessed time series (gaffa's ``gaffa.ffa``).
+
+``make_riptide_plan`` and ``ffa_search`` keep gaffa's signatures, defaults
+and error messages (python/gaffa/ffa.py:12-133); the search runs on the GPU.
+``ffa_search_batch`` searches many equal-length series with one plan.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from ._core import (
+    FfaPeak,
+    FfaPlan,
+    _ffa_search_batch_gpu,
+    _ffa_search_gpu_host,
+    _ffa_transform_rows_gpu,
+    _make_riptide_ffa_plan,
+)
+
+_GPU_BACKENDS = ("gpu", "cuda")
+
+
+def make_riptide_plan(
+    *,
+    nsamples: int,
+    tsamp: float,
+    period_min: float,
+    period_max: float,
+    bins_min: int = 180,
+    bins_max: int = 256,
+    min_periods: int = 1,
+    duty_cycle_max: float = 0.20,
+    width_trial_spacing: float = 1.5,
+    max_tasks: int = 1_000_000,
+) -> FfaPlan:
+    """Build a reusable riptide-compatible FFA plan for series of ``nsamples``."""
+    return _make_riptide_ffa_plan(
+        nsamples=nsamples,
+        tsamp=tsamp,
+        period_min=period_min,
+        period_max=period_max,
+        bins_min=bins_min,
+        bins_max=bins_max,
+        min_periods=min_periods,
+        duty_cycle_max=duty_cycle_max,
+        width_trial_spacing=width_trial_spacing,
+        max_tasks=max_tasks,
+    )
+
+
+def _check_backend(backend: str, device_id: int) -> None:
+    if backend == "cpu":
+        raise ValueError("backend must be 'gpu' or 'cuda': gaffa_amd runs the FFA on the GPU")
+    if backend not in _GPU_BACKENDS:
+        raise ValueError("backend must be 'gpu' or 'cuda'")
+    if device_id < 0:
+        raise ValueError("device_id must be non-negative")
+
+
+def ffa_search(
+    time_series,
+    plan: FfaPlan,
+    *,
+    snr_threshold: float = 6.0,
+    max_peaks: int | None = None,
+    backend: str = "gpu",
+    device_id: int = 0,
+) -> list[FfaPeak]:
+    """Search one preprocessed float32 series; peaks in descending peak order."""
+    _check_backend(backend, device_id)
+    return _ffa_search_gpu_host(time_series, plan, device_id=device_id,
+                                snr_threshold=snr_threshold, max_peaks=max_peaks)
+
+
+def ffa_search_batch(
+    batch,
+    plan: FfaPlan,
+    *,
+    snr_threshold: float = 6.0,
+    max_peaks: int | None = None,
+    device_id: int = 0,
+    series_tile: int = 0,
+) -> list[tuple[int, FfaPeak]]:
+    """Search every row of a C-contiguous (nseries, nsamples) float32 array.
+
+    Returns ``(series_index, peak)`` pairs sorted by series, then peak order.
+    """
+    _check_backend("gpu", device_id)
+    return _ffa_search_batch_gpu(batch, plan, device_id=device_id, snr_threshold=snr_threshold,
+                                 max_peaks=max_peaks, series_tile=series_tile)
+
+
+def ffa_transform_rows(batch, plan: FfaPlan, task: int, *, device_id: int = 0) -> np.ndarray:
+    """The first ``rows_eval`` final FFA rows of ``task`` for every series."""
+    return _ffa_transform_rows_gpu(batch, plan, task, device_id=device_id)
+
+
+__all__ = ["FfaPeak", "FfaPlan", "ffa_search", "ffa_search_batch", "ffa_transform_rows",
+           "make_riptide_plan"]

@@ -1,0 +1,318 @@
+/* gaffa_hip.h - C ABI of the MI355X (gfx950) search hot path.
+ *
+ * This is the drop-in boundary: plain C, plain pointers and sizes, no C++ or
+ * torch types.  gaffa's own C++ host layer (plans, result types, programs)
+ * calls these entry points where gaffa's CUDA translation units launch their
+ * kernels; the functions each one replaces are cited beside it.  All device
+ * pointers are HIP device allocations on the current device; tables (delays,
+ * FFA task descriptors) are computed on the host by the caller and uploaded.
+ *
+ * Every entry point returns GAFFA_HIP_OK (0) or a status naming the C++
+ * exception class gaffa throws for the same condition; gaffa_hip_last_error()
+ * returns the message of the last failure on the calling thread.  Kernels are
+ * enqueued on the given stream (NULL = the default stream) and never
+ * synchronise unless the entry point says so.
+ */
+#ifndef GAFFA_HIP_H
+#define GAFFA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GAFFA_HIP_ABI_VERSION 1
+
+typedef enum gaffa_hip_status {
+  GAFFA_HIP_OK = 0,
+  GAFFA_HIP_INVALID_ARGUMENT = 1, /* std::invalid_argument */
+  GAFFA_HIP_OVERFLOW = 2,         /* std::overflow_error */
+  GAFFA_HIP_OUT_OF_RANGE = 3,     /* std::out_of_range */
+  GAFFA_HIP_RUNTIME = 4,          /* std::runtime_error (HIP / device failure) */
+  GAFFA_HIP_LOGIC = 5,            /* std::logic_error */
+  GAFFA_HIP_NOT_SUPPORTED = 6,
+} gaffa_hip_status;
+
+typedef struct gaffa_hip_stream_opaque* gaffa_hip_stream; /* == hipStream_t */
+typedef struct gaffa_hip_event_opaque* gaffa_hip_event;   /* == hipEvent_t */
+typedef struct gaffa_hip_graph_opaque* gaffa_hip_graph;   /* == hipGraphExec_t */
+
+const char* gaffa_hip_last_error(void);
+int gaffa_hip_abi_version(void);
+
+/* ------------------------------------------------------------- runtime ----
+ * Replace src/gaffa/cuda_memory.cu:21-116 (CudaDeviceMemory allocation and
+ * release), the cudaMemcpy calls of dedispersion_cuda.cu:619-626/718-728 and
+ * ffa_bindings.cpp:146-152, and vector_add.cu:64-89 (device queries). */
+typedef struct gaffa_hip_device_info {
+  char name[256];
+  char arch[64];           /* gcnArchName, e.g. "gfx950:sramecc+:xnack-" */
+  int32_t compute_units;
+  int32_t clock_khz;
+  int32_t memory_clock_khz;
+  int32_t memory_bus_width;
+  int64_t total_memory;
+  int32_t l2_bytes;
+  int32_t lds_bytes_per_block;
+  int32_t runtime_version;
+} gaffa_hip_device_info;
+
+int gaffa_hip_device_count(int32_t* count);
+int gaffa_hip_device_info_get(int32_t device, gaffa_hip_device_info* out);
+int gaffa_hip_set_device(int32_t device);
+int gaffa_hip_get_device(int32_t* device);
+int gaffa_hip_malloc(void** ptr, uint64_t bytes);
+int gaffa_hip_free(void* ptr);
+/* Copies in any direction (hipMemcpyDefault).  With a NULL stream the copy is
+ * synchronous; otherwise it is enqueued on the stream. */
+int gaffa_hip_memcpy(void* dst, const void* src, uint64_t bytes,
+                     gaffa_hip_stream stream);
+int gaffa_hip_memset(void* dst, int32_t value, uint64_t bytes,
+                     gaffa_hip_stream stream);
+int gaffa_hip_stream_create(gaffa_hip_stream* stream);
+int gaffa_hip_stream_destroy(gaffa_hip_stream stream);
+int gaffa_hip_stream_synchronize(gaffa_hip_stream stream);
+int gaffa_hip_device_synchronize(void);
+int gaffa_hip_event_create(gaffa_hip_event* event);
+int gaffa_hip_event_destroy(gaffa_hip_event event);
+int gaffa_hip_event_record(gaffa_hip_event event, gaffa_hip_stream stream);
+int gaffa_hip_event_elapsed_ms(gaffa_hip_event start, gaffa_hip_event stop,
+                               float* ms);
+/* Stream capture into a replayable graph (no counterpart in gaffa, whose
+ * CUDA path launches kernels one by one). */
+int gaffa_hip_graph_begin(gaffa_hip_stream stream);
+int gaffa_hip_graph_end(gaffa_hip_stream stream, gaffa_hip_graph* graph);
+int gaffa_hip_graph_launch(gaffa_hip_graph graph, gaffa_hip_stream stream);
+int gaffa_hip_graph_destroy(gaffa_hip_graph graph);
+
+/* -------------------------------------------------------- dedispersion ----
+ * Input: uint8 filterbank samples, [nsamples][nchans] (nifs == 1, channels
+ * ascending as gaffa's reader stores them).  Output: uint32 sums,
+ * out[dm * out_stride + t] for t < out_nsamples.  Samples at t >= nsamples
+ * read as zero (dedispersion.cpp:206-212, 367-372, 431-437). */
+
+/* Brute force, out[dm][t] = sum_c x[t + delays[dm][c]][chan_begin + c].
+ * Replaces multi_dm_kernel (dedispersion_cuda.cu:320-348) as launched by
+ * dedisperse_multi_dm_cuda_device_impl; also serves the single-DM case
+ * (single_dm_kernel, :269-293) with ndm == 1.  max_block_spread bounds
+ * max - min of the delays inside any 16-DM x 32-channel block (the host
+ * computes it from its table); it sizes the LDS slab. */
+typedef struct gaffa_hip_dedisperse_multi_args {
+  const uint8_t* samples;
+  uint64_t nsamples;
+  uint64_t nchans;
+  uint64_t chan_begin;
+  uint64_t chan_end;
+  const int32_t* delays; /* [ndm][chan_end - chan_begin], all >= 0 */
+  uint64_t ndm;
+  uint32_t* out;
+  uint64_t out_nsamples;
+  uint64_t out_stride;
+  int32_t max_block_spread;
+} gaffa_hip_dedisperse_multi_args;
+
+int gaffa_hip_dedisperse_multi_u8(const gaffa_hip_dedisperse_multi_args* args,
+                                  gaffa_hip_stream stream);
+
+/* Two-stage subband dedispersion fused in LDS: stage 1 sums each subband at
+ * the coarse delays of its nominal DM, stage 2 adds the subbands at the
+ * residual delays; the intermediate never reaches HBM.  Replaces
+ * subband_stage1_kernel + subband_stage2_kernel and their tile loop
+ * (dedispersion_cuda.cu:350-417, 659-707).  nsub = ceil(nchan_sel /
+ * subband_channels); n_nominal = ceil(ndm / ndm_per_nominal).
+ * max_residual bounds every residual delay; max_coarse_spread bounds
+ * max - min of the coarse delays inside any (nominal group, subband). */
+typedef struct gaffa_hip_dedisperse_subband_args {
+  const uint8_t* samples;
+  uint64_t nsamples;
+  uint64_t nchans;
+  uint64_t chan_begin;
+  uint64_t chan_end;
+  uint64_t subband_channels;
+  uint64_t ndm_per_nominal;
+  uint64_t ndm;
+  const int32_t* coarse_delays;   /* [n_nominal][chan_end - chan_begin] */
+  const int32_t* residual_delays; /* [ndm][nsub] */
+  uint32_t* out;
+  uint64_t out_nsamples;
+  uint64_t out_stride;
+  int32_t max_residual;
+  int32_t max_coarse_spread;
+} gaffa_hip_dedisperse_subband_args;
+
+int gaffa_hip_dedisperse_subband_u8(
+    const gaffa_hip_dedisperse_subband_args* args, gaffa_hip_stream stream);
+
+/* Synchronises `stream` and reports (then clears) a refused dedispersion
+ * block: a delay table whose spread exceeded the bound the caller passed.
+ * No counterpart in gaffa (its kernels read global memory directly). */
+int gaffa_hip_dedisperse_check(gaffa_hip_stream stream);
+
+/* ------------------------------------------------- detrend + normalise ----
+ * In-place batch preprocessing of a [nseries][stride] plane: optional
+ * uint32 -> float conversion (input_is_u32), riptide fast running-median
+ * detrend (mean scrunch by `scrunch_factor`, exact edge-padded median of
+ * `median_window` low-resolution points, linear interpolation back,
+ * subtract), and optional normalise to zero mean / unit population sigma.
+ * Replaces convert_uint32_to_float_kernel (time_series_cuda.cu:50-58) and the
+ * seven kernels of CudaPreprocessProgram (preprocessing_cuda.cu:274-516).
+ * scrunch_factor == 1 selects the exact running median at full resolution.
+ * `workspace` must hold gaffa_hip_preprocess_workspace_bytes(...) bytes.
+ * `status` (device, int32[nseries]) receives per-series deferred errors:
+ * 0 ok, 1 non-finite input, 2 constant input rejected, 3 non-finite
+ * variance.  The caller reads it after synchronising (preprocessing_cuda.cu:
+ * 781-817 reports the same conditions at synchronize()). */
+typedef struct gaffa_hip_preprocess_args {
+  void* data; /* uint32 (input_is_u32) or float, becomes float */
+  uint64_t nseries;
+  uint64_t nsamples;
+  uint64_t stride; /* elements between series starts */
+  int32_t input_is_u32;
+  int32_t detrend;
+  uint64_t scrunch_factor;
+  uint64_t median_window; /* odd */
+  int32_t normalise;
+  int32_t reject_constant;
+  void* workspace;
+  uint64_t workspace_bytes;
+  int32_t* status;
+} gaffa_hip_preprocess_args;
+
+uint64_t gaffa_hip_preprocess_workspace_bytes(uint64_t nseries,
+                                              uint64_t nsamples,
+                                              uint64_t scrunch_factor);
+int gaffa_hip_detrend_normalise_batch(const gaffa_hip_preprocess_args* args,
+                                      gaffa_hip_stream stream);
+
+/* ------------------------------------------------------------------ FFA ----
+ * Device-side layout of one FFA execution plan (host-built, see
+ * gaffa_amd/csrc/host/ffa_gpu.cpp).  A plan is shared by every series of a
+ * batch.  Per series, `prepared` holds each prepare group's downsampled copy
+ * at prep_offset[group] (factor-1 groups read the input series directly) and
+ * `work` holds two ping-pong regions of rows*bins floats per task. */
+typedef struct gaffa_hip_ffa_group {
+  double factor;            /* downsample factor, 1.0 = no copy */
+  uint64_t prepared_nsamples;
+  uint64_t prep_offset;     /* float offset inside a series' prepared region */
+} gaffa_hip_ffa_group;
+
+#define GAFFA_HIP_FFA_MAX_WIDTHS 16
+
+typedef struct gaffa_hip_ffa_task {
+  uint32_t group;
+  uint32_t bins;
+  uint32_t rows;
+  uint32_t rows_eval;
+  uint32_t depth;           /* depth of the LDS subtrees, >= 1 */
+  uint32_t nwidths;
+  uint64_t work_offset;     /* float offset of this task's rows*bins region */
+  float inv_stdnoise;       /* 1.0f / ffa_task_stdnoise(task) */
+  float pad_;
+  uint32_t width[GAFFA_HIP_FFA_MAX_WIDTHS];
+  float height[GAFFA_HIP_FFA_MAX_WIDTHS];
+  float baseline[GAFFA_HIP_FFA_MAX_WIDTHS];
+} gaffa_hip_ffa_task;
+
+typedef struct gaffa_hip_ffa_item {
+  uint32_t task;
+  uint32_t a; /* subtree: node start row; level: depth; terminal: first row */
+  uint32_t b; /* subtree: node rows;      level: first row; terminal: rows */
+  uint32_t c; /* level: rows in this item */
+} gaffa_hip_ffa_item;
+
+/* One significant (row, width) detection, 20 bytes (cf. FfaCudaPeak,
+ * ffa_cuda_kernels.cuh:70-79).  The host expands it with make_ffa_peak. */
+typedef struct gaffa_hip_ffa_peak {
+  uint32_t series;
+  uint32_t task;
+  uint32_t shift;
+  uint32_t phase_width; /* phase | (width_index << 16) */
+  float snr;
+} gaffa_hip_ffa_peak;
+
+typedef struct gaffa_hip_ffa_plan_desc {
+  const gaffa_hip_ffa_group* groups; /* device */
+  uint32_t ngroups;
+  const gaffa_hip_ffa_task* tasks;   /* device */
+  uint32_t ntasks;
+  const gaffa_hip_ffa_item* subtree_items; /* device */
+  uint32_t nsubtree_items;
+  const gaffa_hip_ffa_item* level_items;   /* device, sorted by depth desc */
+  const uint32_t* level_item_offsets;      /* host, [max_depth + 1] */
+  uint32_t max_depth;
+  const gaffa_hip_ffa_item* terminal_items; /* device */
+  uint32_t nterminal_items;
+  uint32_t max_bins;
+  uint32_t subtree_lds_bytes;
+  uint64_t max_prepared_nsamples;
+  uint64_t prepared_stride; /* floats per series in `prepared` */
+  uint64_t work_stride;     /* floats per series per work region */
+} gaffa_hip_ffa_plan_desc;
+
+typedef struct gaffa_hip_ffa_search_args {
+  const gaffa_hip_ffa_plan_desc* plan;
+  const float* series;  /* [nseries][series_stride], preprocessed */
+  uint64_t nseries;
+  uint64_t nsamples;
+  uint64_t series_stride;
+  uint32_t series_index_base; /* added to gaffa_hip_ffa_peak::series */
+  float snr_threshold;
+  float* prepared;      /* [nseries][plan->prepared_stride] */
+  float* work;          /* [2][nseries][plan->work_stride] */
+  gaffa_hip_ffa_peak* peaks;
+  uint64_t peak_capacity;
+  uint32_t* peak_count; /* device; incremented, never reset here */
+} gaffa_hip_ffa_search_args;
+
+/* Downsample every prepare group of every series (weighted-sum, serial FP32
+ * per output exactly as downsample_weighted_sum_cpu).  Replaces
+ * downsample_weighted_sum_float_kernel (time_series_cuda.cu:15-48) and
+ * prepare_copy_float_kernel (ffa_cuda_kernels.cuh:130-143). */
+int gaffa_hip_ffa_prepare_batch(const gaffa_hip_ffa_search_args* args,
+                                gaffa_hip_stream stream);
+
+/* Transform + boxcar detection of every task of every series; appends
+ * significant peaks to args->peaks (peak_count may exceed peak_capacity:
+ * the caller grows the buffer and repeats, as ffa_cuda_executor.cuh:491-531).
+ * Replaces the per-level copy/merge/subtree kernels and the detection
+ * kernels (ffa_cuda_kernels.cuh:145-737) and the tile runner that launches
+ * them (ffa_cuda_executor.cuh:385-531). */
+int gaffa_hip_ffa_search_batch(const gaffa_hip_ffa_search_args* args,
+                               gaffa_hip_stream stream);
+
+/* Transform only: writes the first rows_eval final rows of task `task` for
+ * every series, [nseries][rows_eval][bins] (testing and ffa_transform_block
+ * parity; replaces ffa_transform_block_cuda, ffa_cuda.h:240-263). */
+int gaffa_hip_ffa_transform_rows(const gaffa_hip_ffa_search_args* args,
+                                 uint32_t task, float* out,
+                                 gaffa_hip_stream stream);
+
+/* --------------------------------------------------- synthetic inputs ----
+ * Seeded, integer-exact synthetic filterbank (no gaffa counterpart): sample
+ * (t, c) = clip(pop1024 - 384 + floor((dither12 + signal_q + rfi_q)/4096)).
+ * The host builds the pulsar tables (delays, 4096-bin 1/4096-unit profiles,
+ * 2^-20-sample periods and phases). */
+typedef struct gaffa_hip_synth_args {
+  uint8_t* out; /* [t_count][nchans] */
+  uint64_t nchans;
+  uint64_t t_begin;
+  uint64_t t_count;
+  uint64_t key;            /* splitmix64(seed) */
+  uint32_t npulsars;
+  const int32_t* delays;   /* device [npulsars][nchans] */
+  const int32_t* profiles; /* device [npulsars][4096] */
+  const int64_t* period_q; /* device [npulsars] */
+  const int64_t* phase0_q; /* device [npulsars] */
+  const int32_t* rfi_q;    /* device [nchans] */
+} gaffa_hip_synth_args;
+
+int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* args,
+                               gaffa_hip_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GAFFA_HIP_H */

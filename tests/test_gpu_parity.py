@@ -1,0 +1,322 @@
+"""GPU parity: the HIP path (through the C ABI, via the C++ host layer or
+ctypes) against the CPU oracle on the same seeded inputs.
+
+Bars: dedispersed series bit-exact; detrended series bit-exact; normalised
+series within 1e-5 absolute (their FP64 statistics are summed in a different
+order); FFA rows and peak lists bit-exact on identical inputs (same FP32 add
+tree, serial FP32 prefixes); end-to-end top candidates identical and S/N
+within 1e-5 relative.
+"""
+
+import ctypes
+import hashlib
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from oracle.oracle import Plan, channel_frequencies
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# -------------------------------------------------------------- synth ----
+def test_synth_bytes_equal_oracle(gpu, port):
+    from gaffa_amd.pipeline import small_workload
+
+    say("synth: small workload slice")
+    w = small_workload(nsamples=1 << 15)
+    spec = w.synth_spec()
+    got = gpu._synth_filterbank_gpu(spec)
+    want = port.synth_filterbank(w.nsamples, w.nchans, w.fch1, w.foff, w.tsamp, w.seed,
+                                 pulsars=w.pulsars)
+    np.testing.assert_array_equal(got, want)
+    got = gpu._synth_filterbank_gpu(gpu.SynthFilterbankSpec(
+        nsamples=4096, nchans=64, fch1=1500.0, foff=-4.0, tsamp=256e-6, seed=1))
+    assert hashlib.sha256(got[:1024].tobytes()).hexdigest() == \
+        "c9f34b64d6ee966f3751a63fc3597e1b55c6fb7f8ef30f4f1bc45240ccecd2d5"
+
+
+# ------------------------------------------------------- dedispersion ----
+def test_basetest_dedispersion_bit_exact(gpu):
+    from gaffa_amd import dedispersion
+    from gaffa_amd.io import Filterbank
+
+    say("dedispersion: basetest.fil 50 DMs")
+    fb = Filterbank(os.path.join(GOLDEN, "basetest.fil"))
+    g = np.load(os.path.join(GOLDEN, "basetest_dedispersion.npz"))
+    multi = dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=50)
+    assert multi.backend == "gpu" and multi.data.dtype == np.uint32
+    np.testing.assert_array_equal(multi.data, g["multi"])
+    sub = dedispersion.dedisperse_subband(fb, dm_low=0.0, dm_step=1.0, ndm=50,
+                                          subband_channels=16, ndm_per_nominal=8)
+    np.testing.assert_array_equal(sub.data, g["subband"])
+    # gaffa's degenerate identity: subband(1, 1) == brute force
+    deg = dedispersion.dedisperse_subband(fb, dm_low=0.0, dm_step=1.0, ndm=4,
+                                          subband_channels=1, ndm_per_nominal=1)
+    np.testing.assert_array_equal(deg.data, g["multi"][:4, : deg.nsamples])
+    single = dedispersion.dedisperse_single_dm(fb, dm=3.0)
+    np.testing.assert_array_equal(single.data[0, : multi.nsamples], g["multi"][3])
+
+
+def test_small_brute_force_bit_exact(gpu, port):
+    from gaffa_amd import dedispersion
+    from gaffa_amd.io import Filterbank
+    from gaffa_amd.pipeline import small_workload
+
+    say("dedispersion: 64 ch x 2^17, 64 DMs brute force")
+    w = small_workload()
+    x = gpu._synth_filterbank_gpu(w.synth_spec())
+    fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
+    got = dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=64)
+    want = port.dedisperse(x, w.frequencies, dm_low=0.0, dm_step=1.0, ndm=64, tsamp=w.tsamp)
+    assert got.shape == want.shape == (64, 130870)
+    np.testing.assert_array_equal(got.data, want)
+
+
+def test_subband_slice_bit_exact(gpu):
+    from gaffa_amd import dedispersion
+    from gaffa_amd.io import Filterbank
+
+    say("dedispersion: 1024-ch slice, low- and high-DM subband segments")
+    g = np.load(os.path.join(GOLDEN, "subband_slice.npz"))
+    rng = np.random.default_rng(7)
+    x = rng.integers(96, 160, size=(20000, 1024), dtype=np.uint8)
+    fb = Filterbank.from_array(x, tsamp=64e-6, frequency_mhz=g["frequency"])
+    for i, (lo, step, ndm) in enumerate(g["segments"]):
+        r = dedispersion.dedisperse_subband(fb, dm_low=lo, dm_step=step, ndm=int(ndm),
+                                            subband_channels=32, ndm_per_nominal=32)
+        assert list(r.shape) == list(g[f"subband_{i}_shape"])
+        digests = [hashlib.sha256(row.tobytes()).hexdigest() for row in r.data]
+        assert digests == list(g[f"subband_{i}_digests"])
+        np.testing.assert_array_equal(r.data[[0, int(ndm) - 1]], g[f"subband_{i}_rows"])
+
+
+def test_subband_lband_slice_vs_oracle(gpu, port):
+    from gaffa_amd import dedispersion
+    from gaffa_amd.io import Filterbank
+    from gaffa_amd.pipeline import lband_workload
+
+    say("dedispersion: 1024-ch x 2^16 synthetic, 2 nominal groups per segment")
+    w = lband_workload(nsamples=1 << 16, npulsars=2)
+    x = gpu._synth_filterbank_gpu(w.synth_spec())
+    fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
+    for seg in (w.dm_segments[0], w.dm_segments[3]):
+        r = dedispersion.dedisperse_subband(fb, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64)
+        want = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
+                               tsamp=w.tsamp, subband=True, subband_channels=32,
+                               ndm_per_nominal=32)
+        np.testing.assert_array_equal(r.data, want)
+
+
+def test_c_abi_direct_call(gpu, port):
+    """Drive gaffa_hip_dedisperse_multi_u8 through ctypes with raw device memory."""
+    import gaffa_amd
+
+    say("C ABI: malloc / memcpy / dedisperse_multi_u8 via ctypes")
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    lib.gaffa_hip_last_error.restype = ctypes.c_char_p
+
+    class Args(ctypes.Structure):
+        _fields_ = [("samples", ctypes.c_void_p), ("nsamples", ctypes.c_uint64),
+                    ("nchans", ctypes.c_uint64), ("chan_begin", ctypes.c_uint64),
+                    ("chan_end", ctypes.c_uint64), ("delays", ctypes.c_void_p),
+                    ("ndm", ctypes.c_uint64), ("out", ctypes.c_void_p),
+                    ("out_nsamples", ctypes.c_uint64), ("out_stride", ctypes.c_uint64),
+                    ("max_block_spread", ctypes.c_int32)]
+
+    rng = np.random.default_rng(1)
+    n, nch, ndm = 3000, 40, 7
+    x = rng.integers(0, 256, size=(n, nch), dtype=np.uint8)
+    freq = channel_frequencies(nch, 1400.0, -2.0)
+    delays = port.multi_dm_delay_table(freq, 0.0, 5.0, ndm, freq.max(), 1e-3)
+    n_out = n - int(delays.max())
+    want = port.dedisperse(x, freq, dm_low=0.0, dm_step=5.0, ndm=ndm, tsamp=1e-3)
+
+    def dmalloc(nbytes):
+        p = ctypes.c_void_p()
+        assert lib.gaffa_hip_malloc(ctypes.byref(p), ctypes.c_uint64(nbytes)) == 0
+        return p
+
+    d_x, d_d, d_o = dmalloc(x.nbytes), dmalloc(delays.nbytes), dmalloc(ndm * n_out * 4)
+    try:
+        assert lib.gaffa_hip_memcpy(d_x, x.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_uint64(x.nbytes), None) == 0
+        dd = np.ascontiguousarray(delays, np.int32)
+        assert lib.gaffa_hip_memcpy(d_d, dd.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_uint64(dd.nbytes), None) == 0
+        a = Args(d_x, n, nch, 0, nch, d_d, ndm, d_o, n_out, n_out, int(delays.max()))
+        rc = lib.gaffa_hip_dedisperse_multi_u8(ctypes.byref(a), None)
+        assert rc == 0, lib.gaffa_hip_last_error()
+        assert lib.gaffa_hip_dedisperse_check(None) == 0, lib.gaffa_hip_last_error()
+        got = np.empty((ndm, n_out), np.uint32)
+        assert lib.gaffa_hip_memcpy(got.ctypes.data_as(ctypes.c_void_p), d_o,
+                                    ctypes.c_uint64(got.nbytes), None) == 0
+        np.testing.assert_array_equal(got, want)
+    finally:
+        for p in (d_x, d_d, d_o):
+            lib.gaffa_hip_free(p)
+
+
+# ------------------------------------------------------ preprocessing ----
+@pytest.mark.parametrize("name", ["small", "lband", "exact"])
+def test_preprocess_vs_golden(gpu, port, name):
+    say(f"preprocess: golden {name}")
+    g = np.load(os.path.join(GOLDEN, "preprocess.npz"))
+    window, min_points = (int(v) for v in g[f"{name}_window"])
+    x = g[f"{name}_input"]
+    u32 = x.astype(np.uint32)[None, :]
+    det = gpu._preprocess_batch_gpu(u32, window_samples=window, min_points=min_points,
+                                    normalise=False)
+    np.testing.assert_array_equal(det[0], g[f"{name}_detrend"])
+    full = gpu._preprocess_batch_gpu(u32, window_samples=window, min_points=min_points)
+    want = g[f"{name}_preprocess"]
+    err = np.abs(full[0].astype(np.float64) - want.astype(np.float64))
+    assert err.max() <= 1e-5, err.max()
+    # float32 input path
+    full32 = gpu._preprocess_batch_gpu(x[None, :].copy(), window_samples=window,
+                                       min_points=min_points)
+    np.testing.assert_array_equal(full32, full)
+
+
+def test_preprocess_batch_many_series(gpu, port):
+    say("preprocess: 24 series x 2^16, window 3907/101")
+    rng = np.random.default_rng(2)
+    plane = rng.integers(30000, 31000, size=(24, 1 << 16)).astype(np.uint32)
+    det = gpu._preprocess_batch_gpu(plane, window_samples=3907, min_points=101, normalise=False)
+    for i in (0, 11, 23):
+        np.testing.assert_array_equal(det[i], port.detrend(plane[i].astype(np.float32), 3907, 101))
+    conv = gpu._preprocess_batch_gpu(plane, normalise=False)
+    np.testing.assert_array_equal(conv, plane.astype(np.float32))
+
+
+def test_preprocess_rejects_constant(gpu):
+    say("preprocess: constant series rejected")
+    plane = np.full((2, 5000), 7, np.uint32)
+    with pytest.raises(ValueError, match="standard deviation"):
+        gpu._preprocess_batch_gpu(plane)
+
+
+# ---------------------------------------------------------------- FFA ----
+def test_ffa_rows_and_search_vs_golden(gpu):
+    from gaffa_amd import ffa
+
+    say("ffa: golden series, rows of 3 tasks and the full search")
+    g = np.load(os.path.join(GOLDEN, "ffa_search.npz"))
+    plan = gpu.FfaPlan(g["tasks"], list(g["widths"]))
+    series = g["series"]
+    sel = [0, 37, len(g["tasks"]) - 1]
+    got = np.concatenate([ffa.ffa_transform_rows(series[None, :], plan, k)[0].ravel()
+                          for k in sel])
+    np.testing.assert_array_equal(got, g["rows"])
+    peaks = ffa.ffa_search(series, plan, snr_threshold=6.0)
+    want = g["peaks"]
+    assert len(peaks) == len(want)
+    for p, q in zip(peaks, want):
+        assert (p.shift, p.phase, p.width, p.bins) == (q["shift"], q["phase"], q["width"], q["bins"])
+        assert np.float32(p.snr) == q["snr"]
+        assert p.period == q["period"] and p.frequency == q["frequency"]
+
+
+@pytest.mark.parametrize("rows_bins", [(1, 5), (2, 7), (3, 4), (5, 4), (163, 181), (700, 256),
+                                       (2679, 180)])
+def test_ffa_transform_shapes_bit_exact(gpu, port, rows_bins):
+    """Single-task plans over odd shapes: LDS-only trees, forced global levels."""
+    from oracle.oracle import TASK_DTYPE
+
+    rows, bins = rows_bins
+    say(f"ffa: transform rows={rows} bins={bins}")
+    n = rows * bins + 3
+    rng = np.random.default_rng(rows + bins)
+    x = rng.normal(size=n).astype(np.float32)
+    t = np.zeros(1, TASK_DTYPE)
+    t[0] = (1.0, 1e-3, n, n, bins, rows, rows, bins * 1e-3, (bins + 1) * 1e-3)
+    plan = gpu.FfaPlan(t, [1])
+    got = gpu._ffa_transform_rows_gpu(x[None, :], plan, 0)[0]
+    want = port.ffa_transform(x[: rows * bins], bins)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_ffa_batch_search_bit_exact(gpu, port):
+    """Several series, the small workload's plan (7 groups, 495 tasks)."""
+    say("ffa: 6 series x 130870, small plan, threshold 5")
+    rng = np.random.default_rng(8)
+    n, tsamp = 130870, 256e-6
+    batch = rng.normal(size=(6, n)).astype(np.float32)
+    ph = (np.arange(n) * tsamp / 0.714) % 1.0
+    batch[2] += (1.2 * np.exp(-0.5 * ((ph - 0.25) / 0.01) ** 2)).astype(np.float32)
+    plan = gpu._make_riptide_ffa_plan(nsamples=n, tsamp=tsamp, period_min=0.2, period_max=2.0,
+                                      duty_cycle_max=0.09, width_trial_spacing=2.0)
+    got = gpu._ffa_search_batch_gpu(batch, plan, snr_threshold=5.0, series_tile=4)
+    oplan = Plan(plan.tasks, np.array(plan.width_trials, np.uint64))
+    for s in range(6):
+        want = port.search_ffa(batch[s], oplan, 5.0)
+        mine = [p for i, p in got if i == s]
+        assert len(mine) == len(want), (s, len(mine), len(want))
+        for p, q in zip(mine, want):
+            assert (p.shift, p.phase, p.width, p.bins, np.float32(p.snr)) == \
+                (q["shift"], q["phase"], q["width"], q["bins"], q["snr"])
+    best = max((p for i, p in got if i == 2), key=lambda p: p.snr)
+    assert abs(best.period - 0.714) < 0.003
+
+
+def test_ffa_max_peaks_guard(gpu):
+    say("ffa: max_peaks guard")
+    from gaffa_amd import ffa
+
+    x = np.random.default_rng(1).normal(size=1 << 14).astype(np.float32)
+    plan = ffa.make_riptide_plan(nsamples=x.size, tsamp=1e-3, period_min=0.5, period_max=3.0)
+    with pytest.raises(RuntimeError, match="max_peaks"):
+        ffa.ffa_search(x, plan, snr_threshold=-1e6, max_peaks=3)
+
+
+# ------------------------------------------------------------ DM search ----
+def test_dm_search_vs_golden(gpu):
+    say("dm search: golden plane")
+    g = np.load(os.path.join(GOLDEN, "dm_search.npz"))
+    peaks = gpu._find_dm_peaks_gpu(g["plane"], list(g["dms"]), float(g["tsamp"][0]),
+                                   period_min=0.4, period_max=2.0, duty_cycle_max=0.09,
+                                   width_trial_spacing=2.0, window_samples=251, min_points=101,
+                                   snr_threshold=6.0)
+    want = g["peaks"]
+    assert len(peaks) == len(want)
+    for p, q in zip(peaks[:20], want[:20]):
+        assert (p.dm_index, p.peak.shift, p.peak.width, p.peak.bins) == \
+            (q["dm_index"], q["peak"]["shift"], q["peak"]["width"], q["peak"]["bins"])
+        assert abs(p.peak.snr - q["peak"]["snr"]) <= 1e-5 * abs(q["peak"]["snr"])
+
+
+def test_small_pipeline_end_to_end(gpu, port):
+    from gaffa_amd.pipeline import small_workload
+
+    say("pipeline: small workload end to end vs oracle")
+    w = small_workload()
+    pipe = w.pipeline()
+    pipe.generate(w.synth_spec())
+    pipe.dedisperse()
+    plane = pipe.download_plane_u32()
+    x = pipe.download_filterbank(w.nsamples, w.nchans)
+    want_plane = port.dedisperse(x, w.frequencies, dm_low=0.0, dm_step=1.0, ndm=64, tsamp=w.tsamp)
+    np.testing.assert_array_equal(plane, want_plane)
+    pipe.preprocess()
+    pipe.search()
+    got = pipe.peaks
+    opts = port.plan_options(w.period_min, w.period_max, w.bins_min, w.bins_max, 1,
+                             w.duty_cycle_max, w.width_trial_spacing)
+    pp = port.preprocess_plan(window_samples=w.window_samples(), min_points=w.min_points)
+    want = port.find_dm_peaks(plane, w.dms, w.tsamp, opts, pp, w.snr_threshold)
+    top = min(len(want), 50)
+    assert len(got) >= top > 0
+    for p, q in zip(got[:top], want[:top]):
+        assert (p.dm_index, p.peak.shift, p.peak.width, p.peak.bins) == \
+            (q["dm_index"], q["peak"]["shift"], q["peak"]["width"], q["peak"]["bins"])
+        assert abs(p.peak.snr - q["peak"]["snr"]) <= 1e-5 * abs(q["peak"]["snr"])
+    # the injected pulsar (P 0.714 s, DM 30) is the top candidate
+    assert abs(got[0].peak.period - 0.714) < 0.005 and abs(got[0].dm - 30.0) <= 3.0

@@ -1,0 +1,131 @@
+"""Host-side logic of the product, CPU only: the C ABI library exports every
+entry point include/gaffa_hip.h declares, gaffa's plan/delay arithmetic in
+the C++ host layer equals the oracle's, and the Python surface raises gaffa's
+errors before touching a device."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "gaffa_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(gaffa_hip_\w+)\s*\(", text, re.M)
+    assert len(names) > 20
+    return names
+
+
+def test_c_abi_exports_every_declared_symbol():
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert missing == []
+    lib.gaffa_hip_abi_version.restype = ctypes.c_int
+    assert lib.gaffa_hip_abi_version() == 1
+
+
+def test_c_abi_rejects_bad_arguments_without_a_device():
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    lib.gaffa_hip_last_error.restype = ctypes.c_char_p
+    assert lib.gaffa_hip_dedisperse_multi_u8(None, None) == 1
+    assert b"must not be null" in lib.gaffa_hip_last_error()
+    assert lib.gaffa_hip_ffa_search_batch(None, None) == 1
+    assert lib.gaffa_hip_detrend_normalise_batch(None, None) == 1
+    lib.gaffa_hip_preprocess_workspace_bytes.restype = ctypes.c_uint64
+    assert lib.gaffa_hip_preprocess_workspace_bytes(4, 1000, 10) > 0
+
+
+def test_product_plans_equal_oracle(core, port):
+    g = np.load(os.path.join(GOLDEN, "plans.npz"))
+    for name in ("small", "lband", "large", "riptide_default"):
+        n, ts, pmin, pmax, duty, spacing = g[f"{name}_args"]
+        plan = core._make_riptide_ffa_plan(nsamples=int(n), tsamp=ts, period_min=pmin,
+                                           period_max=pmax, duty_cycle_max=duty,
+                                           width_trial_spacing=spacing)
+        got = plan.tasks
+        want = g[f"{name}_tasks"]
+        for field in want.dtype.names:
+            np.testing.assert_array_equal(got[field], want[field])
+        assert list(plan.width_trials) == list(g[f"{name}_widths"])
+
+
+def test_product_delays_equal_oracle(core, port):
+    from gaffa_amd.pipeline import channel_frequencies
+
+    f = channel_frequencies(1024, 1520.0, -0.293)
+    for dm in (0.0, 0.1643, 30.0, 348.6, 498.8):
+        for c in (0, 1, 511, 1023):
+            assert core.dedispersion_delay_bins(dm, f[c], f.max(), 64e-6) == \
+                port.delay_bins(dm, f[c], f.max(), 64e-6)
+
+
+def test_python_surface_errors(core):
+    from gaffa_amd import ffa
+
+    plan = ffa.make_riptide_plan(nsamples=8, tsamp=1.0, period_min=2.0, period_max=4.0,
+                                 bins_min=2, bins_max=2)
+    assert plan.task_count > 0 and plan.width_trials == (1,)
+    with pytest.raises(TypeError, match="numpy.ndarray"):
+        ffa.ffa_search([0.0] * 8, plan)
+    with pytest.raises(TypeError, match="dtype float32"):
+        ffa.ffa_search(np.zeros(8, dtype=np.float64), plan)
+    with pytest.raises(ValueError, match="1D"):
+        ffa.ffa_search(np.zeros((1, 8), dtype=np.float32), plan)
+    with pytest.raises(ValueError, match="C-contiguous"):
+        ffa.ffa_search(np.zeros(16, dtype=np.float32)[::2], plan)
+    with pytest.raises(ValueError, match="input_nsamples"):
+        ffa.ffa_search(np.zeros(8, np.float32),
+                       ffa.make_riptide_plan(nsamples=9, tsamp=1.0, period_min=2.0,
+                                             period_max=4.0, bins_min=2, bins_max=2))
+    with pytest.raises(ValueError, match="positive or None"):
+        ffa.ffa_search(np.zeros(8, np.float32), plan, max_peaks=0)
+    with pytest.raises(ValueError, match="'gpu' or 'cuda'"):
+        ffa.ffa_search(np.zeros(8, np.float32), plan, backend="opencl")
+    with pytest.raises(ValueError, match="'gpu' or 'cuda'"):
+        ffa.ffa_search(np.zeros(8, np.float32), plan, backend="cpu")
+
+
+def test_dedispersion_surface_errors(core):
+    from gaffa_amd import dedispersion
+    from gaffa_amd.io import Filterbank
+
+    fb = Filterbank(os.path.join(GOLDEN, "basetest.fil"))
+    with pytest.raises(ValueError, match="unknown dedispersion backend"):
+        dedispersion.dedisperse_single_dm(fb, dm=0.0, backend="bad")
+    with pytest.raises(ValueError, match="backend 'cpu' is not provided"):
+        dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=2, backend="cpu")
+    with pytest.raises(NotImplementedError):
+        dedispersion.dedisperse_spectrum(fb, dm=0.0)
+
+
+def test_filterbank_from_array_roundtrip():
+    from gaffa_amd.io import Filterbank
+    from gaffa_amd.pipeline import channel_frequencies
+
+    x = np.arange(4 * 3, dtype=np.uint8).reshape(4, 3)
+    fb = Filterbank.from_array(x, tsamp=1e-3, frequency_mhz=channel_frequencies(3, 1500.0, -4.0))
+    assert fb.data.shape == (4, 1, 3) and fb.header.nbits == 8
+    assert fb.header.frequency_table == [1492.0, 1496.0, 1500.0]
+
+
+def test_workload_shapes():
+    from gaffa_amd.pipeline import lband_workload, small_workload
+
+    s = small_workload()
+    assert (s.nchans, s.ndm, s.window_samples()) == (64, 64, 3907)
+    w = lband_workload()
+    assert (w.nchans, w.ndm, w.window_samples()) == (1024, 1024, 31251)
+    dms = w.dms
+    assert dms[0] == 0.0 and 495.0 < dms[-1] <= 500.0
+    assert np.all(np.diff(dms) > 0)
+    steps = [s.dm_step for s in w.dm_segments]
+    assert steps == sorted(steps) and all(s.ndm % 32 == 0 for s in w.dm_segments)
