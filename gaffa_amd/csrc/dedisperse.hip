@@ -141,8 +141,9 @@ dedisperse_multi_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
 // Stage 1 builds inter[s][j], j in [0, tile + max_residual + pad), as u16 in
 // LDS, channel chunk by channel chunk through a byte slab; stage 2 sums the
 // subbands at the residual delays for the group's DMs.
-constexpr int kSbChunk = 32;   // channels per slab chunk
-constexpr int kSbQuads = 2;    // stage-1 quads per thread (4 samples each)
+constexpr int kSbChunk = 32;       // channels per slab chunk
+constexpr int kSbJ = 8;            // stage-1 samples per thread (jlen <= 2048)
+constexpr uint32_t kSlabStride = 36;  // slab row stride in bytes (9 dwords)
 
 __global__ void __launch_bounds__(kThreads)
 dedisperse_subband_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
@@ -160,7 +161,7 @@ dedisperse_subband_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
   // [nsub][inter_len] u16, then [kSbChunk][slab_len] u8, then residuals
   uint16_t* inter = reinterpret_cast<uint16_t*>(dyn_lds);
   uint8_t* slab = reinterpret_cast<uint8_t*>(inter + static_cast<size_t>(nsub) * inter_len);
-  int32_t* sres = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(kSbChunk) * slab_len);
+  int32_t* sres = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(slab_len) * kSlabStride);
   __shared__ int32_t sdel[kSbChunk];
   __shared__ int32_t sminmax[2];
 
@@ -189,12 +190,19 @@ dedisperse_subband_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
     return;
   }
 
+  // Stage 1.  The slab is row-major [row][channel] with a 36-byte (9-dword,
+  // odd) row stride: it is filled with 16-byte global loads, and lane j of a
+  // wave reads byte (j + delay_c - dmin, c), so the lanes of one read hit
+  // 64 distinct banks.  Thread tid owns samples j = tid + 256 q.
+  (void)nquads;
+  const bool vec_rows = (nchans % 16u) == 0 && (chan_begin % 16u) == 0 && (sbc % 32u) == 0 &&
+                        (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
   for (uint32_t s = 0; s < nsub; ++s) {
     const uint32_t cs = s * sbc;
     const uint32_t ce = min(cs + sbc, nsel);
-    uint32_t acc02[kSbQuads], acc13[kSbQuads];
+    uint32_t acc[kSbJ];
 #pragma unroll
-    for (int q = 0; q < kSbQuads; ++q) acc02[q] = acc13[q] = 0;
+    for (int q = 0; q < kSbJ; ++q) acc[q] = 0;
 
     for (uint32_t c0 = cs; c0 < ce; c0 += kSbChunk) {
       const uint32_t nc = min(static_cast<uint32_t>(kSbChunk), ce - c0);
@@ -216,41 +224,49 @@ dedisperse_subband_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
         if (tid == 0) atomicExch(error_flag, 1);
         return;
       }
-      for (uint32_t e = tid; e < rows * nc; e += kThreads) {
-        const uint32_t r = e / nc, c = e % nc;
-        const uint64_t t = t0 + static_cast<uint64_t>(dmin) + r;
-        uint8_t v = 0;
-        if (t < nsamples) v = x[t * nchans + chan_begin + c0 + c];
-        slab[c * slab_len + r] = v;
+      const uint64_t tbase = t0 + static_cast<uint64_t>(dmin);
+      if (vec_rows && nc == kSbChunk) {
+        uint32_t* slab32 = reinterpret_cast<uint32_t*>(slab);
+        for (uint32_t e = tid; e < rows * 2; e += kThreads) {
+          const uint32_t r = e >> 1, h = e & 1u;
+          const uint64_t t = tbase + r;
+          uint4 v = make_uint4(0, 0, 0, 0);
+          if (t < nsamples)
+            v = *reinterpret_cast<const uint4*>(x + t * nchans + chan_begin + c0 + 16u * h);
+          uint32_t* dst = slab32 + (r * kSlabStride + 16u * h) / 4u;
+          dst[0] = v.x;
+          dst[1] = v.y;
+          dst[2] = v.z;
+          dst[3] = v.w;
+        }
+      } else {
+        for (uint32_t e = tid; e < rows * nc; e += kThreads) {
+          const uint32_t r = e / nc, c = e % nc;
+          const uint64_t t = tbase + r;
+          uint8_t v = 0;
+          if (t < nsamples) v = x[t * nchans + chan_begin + c0 + c];
+          slab[r * kSlabStride + c] = v;
+        }
       }
       __syncthreads();
-      const uint32_t* slab32 = reinterpret_cast<const uint32_t*>(slab);
+      for (uint32_t c = 0; c < nc; ++c) {
+        const uint8_t* col = slab + static_cast<uint32_t>(sdel[c] - dmin) * kSlabStride + c;
 #pragma unroll
-      for (int q = 0; q < kSbQuads; ++q) {
-        const uint32_t jq = tid + static_cast<uint32_t>(q) * kThreads;
-        if (jq < nquads) {
-          for (uint32_t c = 0; c < nc; ++c) {
-            const uint32_t off =
-                c * slab_len + 4u * jq + static_cast<uint32_t>(sdel[c] - dmin);
-            const uint32_t w = read_bytes4(slab32, off);
-            acc02[q] += w & 0x00FF00FFu;
-            acc13[q] += (w >> 8) & 0x00FF00FFu;
-          }
+        for (int q = 0; q < kSbJ; ++q) {
+          const uint32_t j = tid + static_cast<uint32_t>(q) * kThreads;
+          if (j < jlen) acc[q] += col[j * kSlabStride];
         }
       }
       __syncthreads();
     }
-    // u16 pairs (j, j+1), (j+2, j+3) into inter[s]
-    uint32_t* inter32 = reinterpret_cast<uint32_t*>(inter + static_cast<size_t>(s) * inter_len);
+    uint16_t* row = inter + static_cast<size_t>(s) * inter_len;
 #pragma unroll
-    for (int q = 0; q < kSbQuads; ++q) {
-      const uint32_t jq = tid + static_cast<uint32_t>(q) * kThreads;
-      if (jq < nquads) {
-        inter32[2 * jq] = (acc02[q] & 0xFFFFu) | (acc13[q] << 16);
-        inter32[2 * jq + 1] = (acc02[q] >> 16) | (acc13[q] & 0xFFFF0000u);
-      } else if (jq < inter_len / 4) {
-        inter32[2 * jq] = 0;
-        inter32[2 * jq + 1] = 0;
+    for (int q = 0; q < kSbJ; ++q) {
+      const uint32_t j = tid + static_cast<uint32_t>(q) * kThreads;
+      if (j < jlen) {
+        row[j] = static_cast<uint16_t>(acc[q]);
+      } else if (j < inter_len) {
+        row[j] = 0;
       }
     }
   }
@@ -416,11 +432,11 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
     for (uint32_t cand = 2048; cand >= 64; cand /= 2) {
       const uint32_t jlen = cand + static_cast<uint32_t>(a->max_residual) + 4;
       const uint32_t jlen4 = (jlen + 3) & ~3u;
-      if (jlen4 / 4 > kThreads * kSbQuads) continue;
+      if (jlen4 > static_cast<uint32_t>(kThreads * kSbJ)) continue;
       const uint32_t il = jlen4 + 8;
       const uint32_t sl =
           (jlen4 + static_cast<uint32_t>(a->max_coarse_spread) + 8 + 3) & ~3u;
-      const uint64_t bytes = nsub * il * 2 + static_cast<uint64_t>(kSbChunk) * sl + res_bytes;
+      const uint64_t bytes = nsub * il * 2 + static_cast<uint64_t>(sl) * kSlabStride + res_bytes;
       if (bytes <= kMaxLds - 4096) {
         tile = cand;
         inter_len = il;

@@ -8,18 +8,23 @@
 // so every FP32 add matches the CPU's and the transform is bit-exact.
 //
 // Execution (all series of a batch share one host-built plan):
-//   prepare   one thread per prepared sample: riptide weighted-sum downsample
-//             with the CPU's serial FP32 order (time_series.cpp:140-174)
+//   prepare   riptide weighted-sum downsample (time_series.cpp:140-174): a
+//             workgroup stages a contiguous input span in LDS with coalesced
+//             loads, then each output is summed serially in the CPU's FP32
+//             order
 //   subtree   one workgroup per (node at the task's LDS depth D, series): the
-//             node's rows are staged in LDS and every level below it runs
-//             there, ping-ponging between two LDS buffers
-//   level     one launch per depth D-1 .. 1: global merges through L2
-//   terminal  one workgroup per (64 final rows, series): the root merge into
-//             LDS (rows padded with a circular extension of max_width bins),
-//             then one lane per (row, width) runs the serial FP32 prefix twice
-//             (P[phi] and P[phi+w]) so every boxcar sum equals the CPU's
-//             circular-prefix difference bit for bit (ffa_detection.cpp:
-//             126-239); significant peaks are appended with one atomic each.
+//             node's rows live in LDS and every level below it runs there,
+//             ping-ponging between two LDS buffers
+//   level     one launch per depth D-1 .. 1: merges through L2 / HBM
+//   terminal  one workgroup per (32 final rows, series): the root merge into
+//             LDS (rows padded with a circular extension), then one lane per
+//             (row, width) runs the serial FP32 prefix twice, P[phi] and
+//             P[phi + w], so every boxcar sum equals the CPU's circular-prefix
+//             difference bit for bit (ffa_detection.cpp:126-239); significant
+//             peaks are appended with one atomic each.
+// In every merge kernel the (head row, tail row, roll) descriptor of each
+// output row is computed once, by one lane, into LDS; the waves then stream
+// whole rows (lanes over phase bins) reading only that descriptor.
 #include "common.hpp"
 
 #include <cmath>
@@ -28,10 +33,15 @@ namespace gaffa_hip {
 namespace {
 
 constexpr uint32_t kLevelThreads = 256;
-constexpr uint32_t kSubtreeThreads = 512;
+constexpr uint32_t kLevelRows = 16;  // rows per level work item (host side too)
+constexpr uint32_t kSubtreeThreads = 1024;
+constexpr uint32_t kSubtreeMaxRows = 512;  // descriptor table capacity
 constexpr uint32_t kTerminalThreads = 256;
-constexpr uint32_t kTerminalRows = 64;
+constexpr uint32_t kTerminalRows = 32;  // rows per terminal work item (host side too)
+constexpr uint32_t kPrepThreads = 256;
+constexpr uint32_t kPrepStage = 8192;  // floats of input staged per prepare block
 constexpr uint32_t kMaxLdsBytes = 160 * 1024;
+constexpr uint32_t kCopyRow = 0xFFFFFFFFu;
 
 // ffa.cpp:133-143 in FP32, no contraction (the library is built with
 // -ffp-contract=off).
@@ -97,34 +107,138 @@ __device__ __forceinline__ Node find_node(uint32_t start, uint32_t size, uint32_
   return Node{start, size};
 }
 
-__device__ __forceinline__ const float* task_input(const KArgs& a,
-                                                   const gaffa_hip_ffa_group& g,
+// Merge descriptor of row r of node (start, size >= 2): the head and tail
+// child rows it reads and the roll applied to the tail row.
+struct Desc {
+  uint32_t h;
+  uint32_t t;
+  uint32_t roll;
+};
+
+__device__ __forceinline__ Desc merge_desc(Node nd, uint32_t r, uint32_t bins) {
+  const uint32_t s = r - nd.start;
+  const uint32_t hn = nd.size / 2;
+  const uint32_t tn = nd.size - hn;
+  const uint32_t h = pick_row(hn, nd.size, s);
+  const uint32_t t = pick_row(tn, nd.size, s);
+  return Desc{nd.start + h, nd.start + hn + t, (s - t) % bins};
+}
+
+__device__ __forceinline__ const float* task_input(const KArgs& a, const gaffa_hip_ffa_group& g,
                                                    uint64_t series) {
   if (g.factor == 1.0) return a.series + series * a.series_stride;
   return a.prepared + series * a.plan.prepared_stride + g.prep_offset;
 }
 
+// One wave merges RB rows i = first + k * step (k < RB; rows >= count are
+// skipped): out_i[p] = h_i[p] + t_i[(p + roll_i) mod b], lanes over p.  Every
+// load address is clamped in bounds and issued unconditionally, so all
+// 2 * RB * 4 loads of a batch are in flight together; only the stores are
+// predicated.  `row(i, h, t, roll)` supplies row i's operands and is only
+// called with i < count (skipped rows reuse row `first`'s operands).
+template <uint32_t RB, typename RowFn, typename OutFn>
+__device__ __forceinline__ void merge_batch(uint32_t first, uint32_t step, uint32_t count,
+                                            uint32_t b, uint32_t lane, RowFn&& row,
+                                            OutFn&& out) {
+  if (first >= count) return;  // wave-uniform: no row of this batch exists
+  const float* hp[RB];
+  const float* tp[RB];
+  uint32_t roll[RB];
+#pragma unroll
+  for (uint32_t k = 0; k < RB; ++k) {
+    const uint32_t i = first + k * step;
+    row(i < count ? i : first, hp[k], tp[k], roll[k]);
+  }
+  for (uint32_t p0 = lane; p0 < b; p0 += 256) {
+    float v[RB][4];
+#pragma unroll
+    for (uint32_t k = 0; k < RB; ++k) {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t p = min(p0 + 64u * j, b - 1);
+        uint32_t q = p + roll[k];
+        q = min(q, q - b);  // (p + roll) mod b for p, roll < b
+        v[k][j] = hp[k][p] + tp[k][q];
+      }
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < RB; ++k) {
+      const uint32_t i = first + k * step;
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t p = p0 + 64u * j;
+        if (i < count && p < b) out(i, p, v[k][j]);
+      }
+    }
+  }
+}
+
+// dst[e] = src[e] for e < total by `threads` threads, U loads in flight per
+// thread (addresses clamped, stores predicated).
+template <uint32_t U, typename D, typename S>
+__device__ __forceinline__ void copy_span(D* dst, const S* src, uint32_t total, uint32_t tid,
+                                          uint32_t threads) {
+  for (uint32_t e0 = 0; e0 < total; e0 += U * threads) {
+    float v[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) v[u] = src[min(e0 + u * threads + tid, total - 1)];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t e = e0 + u * threads + tid;
+      if (e < total) dst[e] = v[u];
+    }
+  }
+}
+
 // ------------------------------------------------------------- prepare ----
-__global__ void __launch_bounds__(256)
+// Outputs per block: K = min(256, 8192 / factor) so the staged input span
+// (about K * factor samples) fits kPrepStage floats.
+__device__ __host__ __forceinline__ uint32_t prep_outputs_per_block(double factor) {
+  const double k = static_cast<double>(kPrepStage - 8) / factor;
+  return k >= 256.0 ? 256u : (k < 1.0 ? 1u : static_cast<uint32_t>(k));
+}
+
+__global__ void __launch_bounds__(kPrepThreads)
 ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
                    uint64_t nsamples, const gaffa_hip_ffa_group* __restrict__ groups,
                    float* __restrict__ prepared, uint64_t prepared_stride) {
+  __shared__ float stage[kPrepStage];
   const gaffa_hip_ffa_group g = groups[blockIdx.y];
   if (g.factor == 1.0) return;
-  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (k >= g.prepared_nsamples) return;
+  const uint32_t K = prep_outputs_per_block(g.factor);
+  const uint64_t k0 = static_cast<uint64_t>(blockIdx.x) * K;
+  if (k0 >= g.prepared_nsamples) return;
+  const uint64_t k1 = min(k0 + K, g.prepared_nsamples);
   const uint64_t sidx = blockIdx.z;
   const float* x = series + sidx * series_stride;
   const double last_index = static_cast<double>(nsamples - 1);
+  // staged span: [floor(k0 f), min(floor(k1 f) + 2, n - 1)]
+  const uint64_t lo = static_cast<uint64_t>(floor(static_cast<double>(k0) * g.factor));
+  const uint64_t hi = static_cast<uint64_t>(
+      fmin(floor(static_cast<double>(k1) * g.factor) + 2.0, last_index));
+  // K * factor <= kPrepStage - 8 whenever K > 1; a single huge-factor output
+  // (factor > kPrepStage - 8) reads global memory directly.
+  const uint64_t span = hi - lo + 1;
+  const bool staged = span <= kPrepStage;
+  const float* s = x;
+  uint64_t base = 0;
+  if (staged) {
+    for (uint32_t e = threadIdx.x; e < span; e += kPrepThreads) stage[e] = x[lo + e];
+    __syncthreads();
+    s = stage;
+    base = lo;
+  }
+  const uint64_t k = k0 + threadIdx.x;
+  if (k >= k1) return;
   const double start = static_cast<double>(k) * g.factor;
   const double end = start + g.factor;
   const uint64_t first = static_cast<uint64_t>(floor(start));
   const uint64_t last = static_cast<uint64_t>(fmin(floor(end), last_index));
   const float w0 = static_cast<float>(static_cast<double>(first + 1) - start);
   const float w1 = static_cast<float>(end - static_cast<double>(last));
-  float sum = w0 * x[first];
-  for (uint64_t i = first + 1; i < last; ++i) sum += x[i];
-  sum += w1 * x[last];
+  float sum = w0 * s[first - base];
+  for (uint64_t i = first + 1; i < last; ++i) sum += s[i - base];
+  sum += w1 * s[last - base];
   prepared[sidx * prepared_stride + g.prep_offset + k] = sum;
 }
 
@@ -134,6 +248,7 @@ ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
 __global__ void __launch_bounds__(kSubtreeThreads)
 ffa_subtree_kernel(KArgs a) {
   extern __shared__ float lds[];
+  __shared__ uint32_t sh[kSubtreeMaxRows], st[kSubtreeMaxRows], sr[kSubtreeMaxRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.subtree_items[blockIdx.x];
   const gaffa_hip_ffa_task task = plan.tasks[item.task];
@@ -146,37 +261,43 @@ ffa_subtree_kernel(KArgs a) {
   const uint32_t wave = tid >> 6;
   constexpr uint32_t kWaves = kSubtreeThreads / 64;
 
-  float* buf0 = lds;
-  float* buf1 = lds + static_cast<size_t>(nrows) * b;
+  float* cur = lds;
+  float* nxt = lds + static_cast<size_t>(nrows) * b;
   const float* in = task_input(a, group, sidx) + static_cast<uint64_t>(item.a) * b;
-  for (uint32_t e = tid; e < nrows * b; e += kSubtreeThreads) buf0[e] = in[e];
-  __syncthreads();
+  copy_span<8>(cur, in, nrows * b, tid, kSubtreeThreads);
 
   uint32_t levels = 0;
   for (uint32_t sz = nrows; sz > 1; sz = sz - sz / 2) ++levels;  // ceil(log2(nrows))
-  float* cur = buf0;
-  float* nxt = buf1;
   for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 0; --d) {
-    for (uint32_t r = wave; r < nrows; r += kWaves) {
+    for (uint32_t r = tid; r < nrows; r += kSubtreeThreads) {
       const Node nd = find_node(0, nrows, static_cast<uint32_t>(d), r);
-      float* o = nxt + static_cast<size_t>(r) * b;
-      if (nd.size < 2) {
-        for (uint32_t p = lane; p < b; p += 64) o[p] = cur[static_cast<size_t>(r) * b + p];
-        continue;
-      }
-      const uint32_t s = r - nd.start;
-      const uint32_t hn = nd.size / 2;
-      const uint32_t tn = nd.size - hn;
-      const uint32_t h = pick_row(hn, nd.size, s);
-      const uint32_t t = pick_row(tn, nd.size, s);
-      const uint32_t roll = (s - t) % b;
-      const float* hr = cur + static_cast<size_t>(nd.start + h) * b;
-      const float* tr = cur + static_cast<size_t>(nd.start + hn + t) * b;
-      for (uint32_t p = lane; p < b; p += 64) {
-        uint32_t j = p + roll;
-        j = min(j, j - b);  // (p + roll) mod b for p, roll < b
-        o[p] = hr[p] + tr[j];
-      }
+      // a leaf above depth d is copied: merge row r with itself minus zero is
+      // not exact, so a copy is flagged explicitly
+      const Desc ds = nd.size < 2 ? Desc{r, kCopyRow, 0} : merge_desc(nd, r, b);
+      sh[r] = ds.h;
+      st[r] = ds.t;
+      sr[r] = ds.roll;
+    }
+    __syncthreads();  // descriptors ready; previous level's rows complete
+    for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
+      merge_batch<2>(
+          r0, kWaves, nrows, b, lane,
+          [&](uint32_t r, const float*& h, const float*& t, uint32_t& roll) {
+            const uint32_t tr = st[r];
+            h = cur + static_cast<size_t>(sh[r]) * b;
+            if (tr == kCopyRow) {
+              // copy rows read the row twice and keep the head value
+              t = h;
+              roll = 0;
+            } else {
+              t = cur + static_cast<size_t>(tr) * b;
+              roll = sr[r];
+            }
+          },
+          [&](uint32_t r, uint32_t p, float v) {
+            if (st[r] == kCopyRow) v = cur[static_cast<size_t>(r) * b + p];
+            nxt[static_cast<size_t>(r) * b + p] = v;
+          });
     }
     __syncthreads();
     float* tmp = cur;
@@ -185,7 +306,7 @@ ffa_subtree_kernel(KArgs a) {
   }
   float* out = a.work + static_cast<uint64_t>(task.depth & 1u) * a.nseries * plan.work_stride +
                sidx * plan.work_stride + task.work_offset + static_cast<uint64_t>(item.a) * b;
-  for (uint32_t e = tid; e < nrows * b; e += kSubtreeThreads) out[e] = cur[e];
+  copy_span<8>(out, cur, nrows * b, tid, kSubtreeThreads);
 }
 
 // --------------------------------------------------------------- level ----
@@ -193,6 +314,7 @@ ffa_subtree_kernel(KArgs a) {
 // from the depth+1 outputs in work[(depth+1) % 2] into work[depth % 2].
 __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
+  __shared__ Desc sdesc[kLevelRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
   const gaffa_hip_ffa_task task = plan.tasks[item.task];
@@ -201,37 +323,71 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
   const uint32_t depth = item.a;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
+  if (threadIdx.x < item.c) {
+    const uint32_t r = item.b + threadIdx.x;
+    sdesc[threadIdx.x] = merge_desc(find_node(0, task.rows, depth, r), r, b);
+  }
+  __syncthreads();
   const uint64_t region = a.nseries * plan.work_stride;
   const float* src = a.work + static_cast<uint64_t>((depth + 1) & 1u) * region +
                      sidx * plan.work_stride + task.work_offset;
   float* dst = a.work + static_cast<uint64_t>(depth & 1u) * region + sidx * plan.work_stride +
                task.work_offset;
-  for (uint32_t i = wave; i < item.c; i += kLevelThreads / 64) {
-    const uint32_t r = item.b + i;
-    const Node nd = find_node(0, task.rows, depth, r);
-    const uint32_t s = r - nd.start;
-    const uint32_t hn = nd.size / 2;
-    const uint32_t tn = nd.size - hn;
-    const uint32_t h = pick_row(hn, nd.size, s);
-    const uint32_t t = pick_row(tn, nd.size, s);
-    const uint32_t roll = (s - t) % b;
-    const float* hr = src + static_cast<uint64_t>(nd.start + h) * b;
-    const float* tr = src + static_cast<uint64_t>(nd.start + hn + t) * b;
-    float* o = dst + static_cast<uint64_t>(r) * b;
-    for (uint32_t p = lane; p < b; p += 64) {
-      uint32_t j = p + roll;
-      j = min(j, j - b);
-      o[p] = hr[p] + tr[j];
-    }
-  }
+  constexpr uint32_t kWaves = kLevelThreads / 64;
+  constexpr uint32_t kBatch = kLevelRows / kWaves;
+  merge_batch<kBatch>(
+      wave, kWaves, item.c, b, lane,
+      [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
+        const Desc ds = sdesc[i];
+        h = src + static_cast<uint64_t>(ds.h) * b;
+        t = src + static_cast<uint64_t>(ds.t) * b;
+        roll = ds.roll;
+      },
+      [&](uint32_t i, uint32_t p, float v) { dst[static_cast<uint64_t>(item.b + i) * b + p] = v; });
 }
 
 // ------------------------------------------------------------ terminal ----
+__device__ __host__ __forceinline__ uint32_t terminal_stride(uint32_t bins, uint32_t maxw) {
+  return (bins + maxw + 8) | 1u;  // odd: lane-per-row reads hit distinct banks
+}
+
+// max over phases of P[p + w] - P[p] for one row x (circularly extended by
+// at least w + 7 entries) with P the serial FP32 prefix; `total` = P[bins].
+__device__ __forceinline__ float boxcar_max(const float* x, uint32_t b, uint32_t w,
+                                            float& total) {
+  float A = 0.0f;  // P[p]
+  float B = 0.0f;  // P[p + w]
+  for (uint32_t k = 0; k < w; ++k) B += x[k];
+  float best = -INFINITY;
+  uint32_t p = 0;
+  for (; p + 8 <= b; p += 8) {
+    float xa[8], xb[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      xa[u] = x[p + u];
+      xb[u] = x[p + w + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      best = fmaxf(best, B - A);
+      A += xa[u];
+      B += xb[u];
+    }
+  }
+  for (; p < b; ++p) {
+    best = fmaxf(best, B - A);
+    A += x[p];
+    B += x[p + w];
+  }
+  total = A;
+  return best;
+}
+
 template <bool kDetect>
 __global__ void __launch_bounds__(kTerminalThreads)
-ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out,
-                    uint32_t only_task) {
+ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   extern __shared__ float lds[];
+  __shared__ Desc sdesc[kTerminalRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.terminal_items[blockIdx.x];
   if (!kDetect && item.task != only_task) return;
@@ -246,35 +402,34 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out,
   constexpr uint32_t kWaves = kTerminalThreads / 64;
   uint32_t maxw = 0;
   for (uint32_t w = 0; w < task.nwidths; ++w) maxw = max(maxw, task.width[w]);
-  const uint32_t sb = (b + maxw) | 1u;  // odd stride: lane-per-row reads hit distinct banks
+  const uint32_t sb = terminal_stride(b, maxw);
 
-  // Root merge of rows [s0, s0 + nrows) into LDS, with circular extension.
   const float* src;
-  uint32_t hn = 0, tn = 0;
   if (task.rows == 1) {
     src = task_input(a, group, sidx);
   } else {
     src = a.work + 1ull * a.nseries * plan.work_stride + sidx * plan.work_stride +
           task.work_offset;
-    hn = task.rows / 2;
-    tn = task.rows - hn;
+    if (threadIdx.x < nrows) {
+      const uint32_t s = s0 + threadIdx.x;
+      sdesc[threadIdx.x] = merge_desc(Node{0, task.rows}, s, b);
+    }
+    __syncthreads();
   }
-  for (uint32_t i = wave; i < nrows; i += kWaves) {
-    const uint32_t s = s0 + i;
-    float* o = lds + static_cast<size_t>(i) * sb;
-    if (task.rows == 1) {
-      for (uint32_t p = lane; p < b; p += 64) o[p] = src[p];
-    } else {
-      const uint32_t h = pick_row(hn, task.rows, s);
-      const uint32_t t = pick_row(tn, task.rows, s);
-      const uint32_t roll = (s - t) % b;
-      const float* hr = src + static_cast<uint64_t>(h) * b;
-      const float* tr = src + static_cast<uint64_t>(hn + t) * b;
-      for (uint32_t p = lane; p < b; p += 64) {
-        uint32_t j = p + roll;
-        j = min(j, j - b);
-        o[p] = hr[p] + tr[j];
-      }
+  if (task.rows == 1) {
+    for (uint32_t p = threadIdx.x; p < b; p += kTerminalThreads) lds[p] = src[p];
+  } else {
+    constexpr uint32_t kBatch = 4;
+    for (uint32_t i0 = wave; i0 < nrows; i0 += kBatch * kWaves) {
+      merge_batch<kBatch>(
+          i0, kWaves, nrows, b, lane,
+          [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
+            const Desc ds = sdesc[i];
+            h = src + static_cast<uint64_t>(ds.h) * b;
+            t = src + static_cast<uint64_t>(ds.t) * b;
+            roll = ds.roll;
+          },
+          [&](uint32_t i, uint32_t p, float v) { lds[static_cast<size_t>(i) * sb + p] = v; });
     }
   }
   __syncthreads();
@@ -284,9 +439,11 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out,
         rows_out[(sidx * task.rows_eval + s0 + i) * b + p] = lds[static_cast<size_t>(i) * sb + p];
     return;
   } else {
-    for (uint32_t i = wave; i < nrows; i += kWaves)
-      for (uint32_t p = lane; p < maxw; p += 64)
-        lds[static_cast<size_t>(i) * sb + b + p] = lds[static_cast<size_t>(i) * sb + p];
+    // circular extension [b, sb) = x[(q - b) mod b]
+    for (uint32_t i = wave; i < nrows; i += kWaves) {
+      float* row = lds + static_cast<size_t>(i) * sb;
+      for (uint32_t q = b + lane; q < sb; q += 64) row[q] = row[(q - b) % b];
+    }
     __syncthreads();
 
     // One lane per (row, width): wave-uniform width, lanes = rows.
@@ -297,17 +454,10 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out,
       if (i >= nrows) continue;
       const uint32_t w = task.width[widx];
       const float* x = lds + static_cast<size_t>(i) * sb;
-      float A = 0.0f;  // P[phi]
-      float B = 0.0f;  // P[phi + w]
-      for (uint32_t k = 0; k < w; ++k) B += x[k];
-      float best = -INFINITY;
-      for (uint32_t p = 0; p < b; ++p) {
-        best = fmaxf(best, B - A);
-        A += x[p];
-        B += x[p + w];
-      }
+      float total;
+      const float best = boxcar_max(x, b, w, total);
       const float hb = task.height[widx] + task.baseline[widx];
-      const float snr = (hb * best - task.baseline[widx] * A) * task.inv_stdnoise;
+      const float snr = (hb * best - task.baseline[widx] * total) * task.inv_stdnoise;
       if (snr >= a.snr_threshold) {
         // first phase attaining the maximum (ffa_detection.cpp:158-167)
         float A2 = 0.0f, B2 = 0.0f;
@@ -348,6 +498,8 @@ void validate(const gaffa_hip_ffa_search_args* a) {
   check(a->series_stride >= a->nsamples, "FFA series stride must be >= nsamples");
   check(a->work != nullptr, "FFA work buffer must not be null");
   check(p.max_bins > 1 && p.max_bins < 65536, "FFA bins must be in (1, 65536)");
+  check(p.terminal_lds_bytes <= kMaxLdsBytes, "FFA detection tile exceeds LDS");
+  check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");
 }
 
 }  // namespace
@@ -359,15 +511,13 @@ namespace {
 
 void launch_prepare(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   const gaffa_hip_ffa_plan_desc& p = *a->plan;
-  if (p.ngroups == 0) return;
-  // host-side maximum of prepared_nsamples is carried in prepared_stride
-  const uint64_t maxp = p.max_prepared_nsamples;
-  if (maxp == 0) return;
+  if (p.ngroups == 0 || p.max_prepared_nsamples == 0) return;
   check(a->prepared != nullptr, "FFA prepared buffer must not be null");
-  const dim3 grid(static_cast<uint32_t>(ceil_div(maxp, 256)), p.ngroups,
-                  static_cast<uint32_t>(a->nseries));
-  hipLaunchKernelGGL(ffa_prepare_kernel, grid, dim3(256), 0, st, a->series, a->series_stride,
-                     a->nsamples, p.groups, a->prepared, p.prepared_stride);
+  check(p.prep_blocks > 0 && p.prep_blocks < (1u << 31), "FFA prepare geometry missing");
+  // groups needing fewer blocks than the widest return at once
+  const dim3 grid(p.prep_blocks, p.ngroups, static_cast<uint32_t>(a->nseries));
+  hipLaunchKernelGGL(ffa_prepare_kernel, grid, dim3(kPrepThreads), 0, st, a->series,
+                     a->series_stride, a->nsamples, p.groups, a->prepared, p.prepared_stride);
   launch_check("ffa_prepare_kernel");
 }
 
@@ -375,7 +525,6 @@ void launch_transform(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   const gaffa_hip_ffa_plan_desc& p = *a->plan;
   const uint32_t ns = static_cast<uint32_t>(a->nseries);
   if (p.nsubtree_items > 0) {
-    check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");
     hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(ffa_subtree_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(p.subtree_lds_bytes)),
@@ -394,10 +543,18 @@ void launch_transform(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   }
 }
 
-uint32_t terminal_lds_bytes(const gaffa_hip_ffa_plan_desc& p) {
-  // rows of (max_bins + max width) floats, odd stride; widths < bins
-  const uint32_t sb = (2 * p.max_bins) | 1u;
-  return kTerminalRows * sb * 4;
+template <bool kDetect>
+void launch_terminal(const gaffa_hip_ffa_search_args* a, float* out, uint32_t task,
+                     hipStream_t st) {
+  const gaffa_hip_ffa_plan_desc& p = *a->plan;
+  hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(ffa_terminal_kernel<kDetect>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(p.terminal_lds_bytes)),
+            "hipFuncSetAttribute");
+  hipLaunchKernelGGL(ffa_terminal_kernel<kDetect>,
+                     dim3(p.nterminal_items, static_cast<uint32_t>(a->nseries)),
+                     dim3(kTerminalThreads), p.terminal_lds_bytes, st, kargs(a), out, task);
+  launch_check("ffa_terminal_kernel");
 }
 
 }  // namespace
@@ -419,18 +576,7 @@ int gaffa_hip_ffa_search_batch(const gaffa_hip_ffa_search_args* a, gaffa_hip_str
     const hipStream_t st = as_stream(stream);
     launch_prepare(a, st);
     launch_transform(a, st);
-    const gaffa_hip_ffa_plan_desc& p = *a->plan;
-    const uint32_t lds = terminal_lds_bytes(p);
-    check(lds <= kMaxLdsBytes, "FFA bins too large for the detection tile");
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(ffa_terminal_kernel<true>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)),
-              "hipFuncSetAttribute");
-    hipLaunchKernelGGL(ffa_terminal_kernel<true>,
-                       dim3(p.nterminal_items, static_cast<uint32_t>(a->nseries)),
-                       dim3(kTerminalThreads), lds, st, kargs(a), static_cast<float*>(nullptr),
-                       0u);
-    launch_check("ffa_terminal_kernel");
+    launch_terminal<true>(a, nullptr, 0, st);
   });
 }
 
@@ -443,17 +589,23 @@ int gaffa_hip_ffa_transform_rows(const gaffa_hip_ffa_search_args* a, uint32_t ta
     const hipStream_t st = as_stream(stream);
     launch_prepare(a, st);
     launch_transform(a, st);
-    const gaffa_hip_ffa_plan_desc& p = *a->plan;
-    const uint32_t lds = terminal_lds_bytes(p);
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(ffa_terminal_kernel<false>),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)),
-              "hipFuncSetAttribute");
-    hipLaunchKernelGGL(ffa_terminal_kernel<false>,
-                       dim3(p.nterminal_items, static_cast<uint32_t>(a->nseries)),
-                       dim3(kTerminalThreads), lds, st, kargs(a), out, task);
-    launch_check("ffa_terminal_kernel");
+    launch_terminal<false>(a, out, task, st);
   });
+}
+
+uint32_t gaffa_hip_ffa_prepare_outputs_per_block(double factor) {
+  return prep_outputs_per_block(factor);
+}
+
+uint32_t gaffa_hip_ffa_terminal_rows(void) { return kTerminalRows; }
+uint32_t gaffa_hip_ffa_level_rows(void) { return kLevelRows; }
+
+uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {
+  return kTerminalRows * terminal_stride(bins, max_width) * 4u;
+}
+
+uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins) {
+  return rows <= kSubtreeMaxRows ? 2u * rows * bins * 4u : 0xFFFFFFFFu;
 }
 
 }  // extern "C"

@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <parallel/algorithm>
 #include <cstring>
 #include <map>
 #include <stdexcept>
@@ -15,9 +16,8 @@ namespace gaffa {
 namespace {
 
 constexpr std::size_t kAlign = 64;               // floats
-constexpr std::size_t kSubtreeLdsFloats = 40960;  // 160 KiB: two ping-pong buffers
-constexpr std::uint32_t kLevelRows = 16;
-constexpr std::uint32_t kTerminalRows = 64;
+constexpr std::size_t kSubtreeLdsFloats = 38912;  // 152 KiB: two ping-pong buffers
+constexpr std::size_t kSubtreeMaxRows = 512;      // kernel's descriptor table
 
 std::size_t align_up(std::size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -73,6 +73,8 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
             "GPU FFA supports at most 16 width trials");
   input_nsamples_ = plan.tasks.front().input_nsamples;
   check_arg(input_nsamples_ > 0, "FFA executor time series must not be empty");
+  const std::uint32_t level_rows = gaffa_hip_ffa_level_rows();
+  const std::uint32_t terminal_rows = gaffa_hip_ffa_terminal_rows();
   std::map<double, std::uint32_t> group_of;
   for (const FfaSearchTask& t : plan.tasks) {
     validate_task(t, input_nsamples_);
@@ -84,6 +86,10 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
         g.prep_offset = prepared_stride_;
         prepared_stride_ += align_up(t.prepared_nsamples);
         max_prepared_ = std::max(max_prepared_, t.prepared_nsamples);
+        const std::size_t per_block = gaffa_hip_ffa_prepare_outputs_per_block(t.downsample_factor);
+        const std::size_t blocks = (t.prepared_nsamples + per_block - 1) / per_block;
+        check_arg(blocks < (1ULL << 31), "FFA prepare grid too large");
+        prep_blocks_ = std::max(prep_blocks_, static_cast<std::uint32_t>(blocks));
       }
       group_of[t.downsample_factor] = static_cast<std::uint32_t>(groups_.size());
       groups_.push_back(g);
@@ -108,9 +114,12 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
       d.height[w] = det.boxcar_trials[w].height;
       d.baseline[w] = det.boxcar_trials[w].baseline;
     }
-    check_arg(2 * det.max_width <= 2 * t.bins, "FFA detection width exceeds bins");
+    check_arg(det.max_width <= t.bins, "FFA detection width exceeds bins");
     max_bins_ = std::max(max_bins_, d.bins);
-    const std::size_t lds_rows = kSubtreeLdsFloats / (2 * t.bins);
+    terminal_lds_bytes_ = std::max(
+        terminal_lds_bytes_,
+        gaffa_hip_ffa_terminal_lds_bytes(d.bins, static_cast<std::uint32_t>(det.max_width)));
+    const std::size_t lds_rows = std::min(kSubtreeMaxRows, kSubtreeLdsFloats / (2 * t.bins));
     check_arg(lds_rows >= 2, "GPU FFA bins too large for an LDS subtree");
     if (t.rows >= 2) {
       std::uint32_t depth = 1;
@@ -123,18 +132,18 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
       collect_nodes(0, d.rows, 0, depth, task_index, subtree_);
       for (std::size_t i = before; i < subtree_.size(); ++i) {
         subtree_lds_bytes_ = std::max<std::uint32_t>(
-            subtree_lds_bytes_, 2u * subtree_[i].b * d.bins * 4u);
+            subtree_lds_bytes_, gaffa_hip_ffa_subtree_lds_bytes(subtree_[i].b, d.bins));
       }
       if (by_depth.size() < depth) by_depth.resize(depth);
       for (std::uint32_t l = 1; l < depth; ++l)
-        for (std::uint32_t r = 0; r < d.rows; r += kLevelRows)
+        for (std::uint32_t r = 0; r < d.rows; r += level_rows)
           by_depth[l].push_back(
-              gaffa_hip_ffa_item{task_index, l, r, std::min(kLevelRows, d.rows - r)});
+              gaffa_hip_ffa_item{task_index, l, r, std::min(level_rows, d.rows - r)});
       max_depth_ = std::max<std::size_t>(max_depth_, depth);
     }
-    for (std::uint32_t r = 0; r < d.rows_eval; r += kTerminalRows)
+    for (std::uint32_t r = 0; r < d.rows_eval; r += terminal_rows)
       terminal_.push_back(gaffa_hip_ffa_item{static_cast<std::uint32_t>(k), r,
-                                             std::min(kTerminalRows, d.rows_eval - r), 0});
+                                             std::min(terminal_rows, d.rows_eval - r), 0});
     tasks_.push_back(d);
   }
   level_offsets_.assign(max_depth_ + 1, 0);
@@ -192,6 +201,8 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.nterminal_items = static_cast<std::uint32_t>(exec_plan_.terminal_items().size());
   desc_.max_bins = exec_plan_.max_bins();
   desc_.subtree_lds_bytes = exec_plan_.subtree_lds_bytes();
+  desc_.terminal_lds_bytes = exec_plan_.terminal_lds_bytes();
+  desc_.prep_blocks = exec_plan_.prep_blocks();
   desc_.max_prepared_nsamples = exec_plan_.max_prepared();
   desc_.prepared_stride = exec_plan_.prepared_stride();
   desc_.work_stride = exec_plan_.work_stride();
@@ -265,18 +276,29 @@ void GpuFfaProgram::transform_rows(const float* series, std::size_t nseries, std
 
 std::vector<FfaBatchPeak> expand_ffa_peaks(const GpuFfaExecutionPlan& plan,
                                            std::span<const gaffa_hip_ffa_peak> peaks) {
-  std::map<std::size_t, FfaDetectionPlan> det;
-  std::vector<FfaBatchPeak> out;
-  out.reserve(peaks.size());
+  const auto& tasks = plan.plan().tasks;
+  // one detection plan per task (tasks index them directly in the loop)
+  std::map<std::size_t, FfaDetectionPlan> by_bins;
+  std::vector<const FfaDetectionPlan*> det(tasks.size());
+  for (std::size_t k = 0; k < tasks.size(); ++k) {
+    auto it = by_bins.find(tasks[k].bins);
+    if (it == by_bins.end())
+      it = by_bins.emplace(tasks[k].bins,
+                           make_ffa_detection_plan(plan.plan().width_trials, tasks[k].bins)).first;
+    det[k] = &it->second;
+  }
   for (const gaffa_hip_ffa_peak& p : peaks) {
-    const FfaSearchTask& task = plan.plan().tasks.at(p.task);
-    auto it = det.find(task.bins);
-    if (it == det.end())
-      it = det.emplace(task.bins, make_ffa_detection_plan(plan.plan().width_trials, task.bins)).first;
-    const std::size_t widx = p.phase_width >> 16;
-    const std::size_t phase = p.phase_width & 0xFFFFu;
-    out.push_back(FfaBatchPeak{p.series, make_ffa_peak(task, it->second.boxcar_trials.at(widx),
-                                                       p.shift, phase, p.snr)});
+    if (p.task >= tasks.size() || (p.phase_width >> 16) >= det[p.task]->boxcar_trials.size()) {
+      throw std::logic_error("FFA compact peak refers to an unknown task or width");
+    }
+  }
+  std::vector<FfaBatchPeak> out(peaks.size());
+#pragma omp parallel for schedule(static)
+  for (std::int64_t i = 0; i < static_cast<std::int64_t>(peaks.size()); ++i) {
+    const gaffa_hip_ffa_peak& p = peaks[static_cast<std::size_t>(i)];
+    out[static_cast<std::size_t>(i)] = FfaBatchPeak{
+        p.series, make_ffa_peak(tasks[p.task], det[p.task]->boxcar_trials[p.phase_width >> 16],
+                                p.shift, p.phase_width & 0xFFFFu, p.snr)};
   }
   return out;
 }
@@ -315,7 +337,7 @@ FfaBatchSearchResult run_ffa_batch_gpu(GpuFfaProgram& program, GpuTimeSeriesBatc
       }
     }
   }
-  std::sort(result.peaks.begin(), result.peaks.end(),
+  __gnu_parallel::sort(result.peaks.begin(), result.peaks.end(),
             [](const FfaBatchPeak& a, const FfaBatchPeak& b) {
               if (a.series_index != b.series_index) return a.series_index < b.series_index;
               return is_better_ffa_peak(a.peak, b.peak);
@@ -402,7 +424,7 @@ DmSearchResult find_dm_peaks_gpu(GpuDedispersedView<std::uint32_t> plane,
   DmSearchResult out;
   out.peaks.reserve(r.peaks.size());
   for (const auto& p : r.peaks) out.peaks.push_back(DmPeak{dms[p.series_index], p.series_index, p.peak});
-  std::sort(out.peaks.begin(), out.peaks.end(), is_better_dm_peak);
+  __gnu_parallel::sort(out.peaks.begin(), out.peaks.end(), is_better_dm_peak);
   return out;
 }
 

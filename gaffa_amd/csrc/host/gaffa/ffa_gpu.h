@@ -58,6 +58,8 @@ class GpuFfaExecutionPlan {
   const std::vector<std::uint32_t>& level_offsets() const noexcept { return level_offsets_; }
   const std::vector<gaffa_hip_ffa_item>& terminal_items() const noexcept { return terminal_; }
   std::uint32_t subtree_lds_bytes() const noexcept { return subtree_lds_bytes_; }
+  std::uint32_t terminal_lds_bytes() const noexcept { return terminal_lds_bytes_; }
+  std::uint32_t prep_blocks() const noexcept { return prep_blocks_; }
   std::uint32_t max_bins() const noexcept { return max_bins_; }
   std::size_t max_prepared() const noexcept { return max_prepared_; }
   // Bytes of device workspace for a tile of `series` series.
@@ -71,6 +73,8 @@ class GpuFfaExecutionPlan {
   std::size_t max_depth_ = 0;
   std::size_t max_prepared_ = 0;
   std::uint32_t subtree_lds_bytes_ = 0;
+  std::uint32_t terminal_lds_bytes_ = 0;
+  std::uint32_t prep_blocks_ = 0;
   std::uint32_t max_bins_ = 0;
   std::vector<gaffa_hip_ffa_group> groups_;
   std::vector<gaffa_hip_ffa_task> tasks_;

@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <parallel/algorithm>
 #include <stdexcept>
 
 namespace gaffa {
@@ -48,6 +49,9 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   fe.workspace_bytes_limit = cfg_.ffa_workspace_bytes_limit;
   fe.series_tile_size = cfg_.ffa_series_tile;
   fe.stream = stream_.get();
+  // a full DM plane easily yields millions of significant peaks: start large
+  // so the first run rarely repeats a tile
+  fe.initial_peak_buffer_bytes = std::max<std::size_t>(fe.initial_peak_buffer_bytes, 256u << 20);
   ffa_ = std::make_unique<GpuFfaProgram>(ffa_plan_, GpuFfaProgramOptions{cfg_.device_id}, fe);
   events_.resize(5, nullptr);
   for (auto& e : events_) hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
@@ -144,7 +148,7 @@ void GpuSearchPipeline::search() {
   peaks_.reserve(expanded.size());
   for (const auto& p : expanded)
     peaks_.push_back(DmPeak{dms_[p.series_index], p.series_index, p.peak});
-  std::sort(peaks_.begin(), peaks_.end(), is_better_dm_peak);
+  __gnu_parallel::sort(peaks_.begin(), peaks_.end(), is_better_dm_peak);
 }
 
 StageTimes GpuSearchPipeline::run() {

@@ -20,6 +20,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace gaffa_hip {
 namespace {
@@ -44,6 +45,27 @@ scrunch_kernel(const T* __restrict__ data, uint64_t stride, uint64_t factor,
   double sum = 0.0;
   for (uint64_t j = 0; j < factor; ++j) sum += static_cast<double>(load_as_float(x, j));
   low[series * m + k] = static_cast<float>(sum / static_cast<double>(factor));
+}
+
+// uint32 input: each sample is rounded to float (as gaffa converts a uint32
+// DM series), which leaves an integer of at most 2^32; the sum of f of them
+// is exact in a 64-bit integer, hence equal to the CPU's serial FP64 sum.  One
+// wave per output, coalesced loads, shuffle reduction.
+__global__ void __launch_bounds__(kThreads)
+scrunch_u32_kernel(const uint32_t* __restrict__ data, uint64_t stride, uint64_t factor,
+                   uint64_t m, float* __restrict__ low) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t k = static_cast<uint64_t>(blockIdx.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  const uint64_t series = blockIdx.y;
+  if (k >= m) return;  // wave-uniform
+  const uint32_t* x = data + series * stride + k * factor;
+  uint64_t sum = 0;
+  for (uint64_t j = lane; j < factor; j += 64)
+    sum += static_cast<uint64_t>(static_cast<float>(x[j]));
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if (lane == 0)
+    low[series * m + k] = static_cast<float>(static_cast<double>(sum) / static_cast<double>(factor));
 }
 
 // Median of every centred window of `window` (odd) points over an
@@ -290,9 +312,17 @@ void run(const gaffa_hip_preprocess_args* a, hipStream_t st) {
   const uint64_t f = a->scrunch_factor;
   if (a->detrend) {
     if (f > 1) {
-      const dim3 g(static_cast<uint32_t>(ceil_div(l.m, kThreads)), static_cast<uint32_t>(a->nseries));
-      hipLaunchKernelGGL((scrunch_kernel<T>), g, dim3(kThreads), 0, st, data, a->stride, f, l.m,
-                         low);
+      if constexpr (std::is_same_v<T, uint32_t>) {
+        const dim3 g(static_cast<uint32_t>(ceil_div(l.m, kThreads / 64)),
+                     static_cast<uint32_t>(a->nseries));
+        hipLaunchKernelGGL(scrunch_u32_kernel, g, dim3(kThreads), 0, st, data, a->stride, f, l.m,
+                           low);
+      } else {
+        const dim3 g(static_cast<uint32_t>(ceil_div(l.m, kThreads)),
+                     static_cast<uint32_t>(a->nseries));
+        hipLaunchKernelGGL((scrunch_kernel<T>), g, dim3(kThreads), 0, st, data, a->stride, f,
+                           l.m, low);
+      }
       launch_check("scrunch_kernel");
       launch_median<float>(low, l.m, l.m, static_cast<uint32_t>(a->median_window), med, l.m,
                            a->nseries, st);

@@ -64,12 +64,20 @@ extern "C" int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* a,
               (a->delays != nullptr && a->profiles != nullptr && a->period_q != nullptr &&
                a->phase0_q != nullptr),
           "synth pulsar tables must not be null");
-    const uint64_t total = a->t_count * a->nchans;
-    if (total == 0) return;
-    const uint64_t blocks = ceil_div(total, 256);
-    check_overflow(blocks < 0x7FFFFFFFull, "synth buffer too large for one launch");
-    hipLaunchKernelGGL(synth_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
-                       as_stream(stream), *a);
-    launch_check("synth_kernel");
+    if (a->t_count * a->nchans == 0) return;
+    // one launch covers at most 2^30 samples (the grid's total work-item
+    // count must stay below 2^32)
+    const uint64_t rows_per_launch = std::max<uint64_t>(1, (1ull << 30) / a->nchans);
+    for (uint64_t t0 = 0; t0 < a->t_count; t0 += rows_per_launch) {
+      gaffa_hip_synth_args part = *a;
+      part.t_begin = a->t_begin + t0;
+      part.t_count = std::min(rows_per_launch, a->t_count - t0);
+      part.out = a->out + t0 * a->nchans;
+      const uint64_t blocks = ceil_div(part.t_count * part.nchans, 256);
+      check_overflow(blocks <= (1ull << 22), "synth launch too large");
+      hipLaunchKernelGGL(synth_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0,
+                         as_stream(stream), part);
+      launch_check("synth_kernel");
+    }
   });
 }

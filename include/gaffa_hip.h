@@ -245,11 +245,25 @@ typedef struct gaffa_hip_ffa_plan_desc {
   const gaffa_hip_ffa_item* terminal_items; /* device */
   uint32_t nterminal_items;
   uint32_t max_bins;
-  uint32_t subtree_lds_bytes;
+  uint32_t subtree_lds_bytes;  /* max over subtree items */
+  uint32_t terminal_lds_bytes; /* max over tasks of ..._terminal_lds_bytes() */
+  uint32_t prep_blocks;        /* max over downsampled groups of
+                                  ceil(prepared_nsamples / outputs per block) */
   uint64_t max_prepared_nsamples;
   uint64_t prepared_stride; /* floats per series in `prepared` */
   uint64_t work_stride;     /* floats per series per work region */
 } gaffa_hip_ffa_plan_desc;
+
+/* Geometry the host needs to build a plan descriptor: rows per terminal and
+ * level work item, prepared outputs per prepare workgroup for a factor, and
+ * the LDS bytes of a terminal tile / subtree node (0xFFFFFFFF = too many
+ * rows for a subtree).  No gaffa counterpart (its launch geometry is
+ * compiled into ffa_cuda_executor.cuh). */
+uint32_t gaffa_hip_ffa_terminal_rows(void);
+uint32_t gaffa_hip_ffa_level_rows(void);
+uint32_t gaffa_hip_ffa_prepare_outputs_per_block(double factor);
+uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width);
+uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins);
 
 typedef struct gaffa_hip_ffa_search_args {
   const gaffa_hip_ffa_plan_desc* plan;

@@ -61,7 +61,8 @@ def test_basetest_dedispersion_bit_exact(gpu):
     # gaffa's degenerate identity: subband(1, 1) == brute force
     deg = dedispersion.dedisperse_subband(fb, dm_low=0.0, dm_step=1.0, ndm=4,
                                           subband_channels=1, ndm_per_nominal=1)
-    np.testing.assert_array_equal(deg.data, g["multi"][:4, : deg.nsamples])
+    # (4 DMs have a longer valid range than 50: compare the common prefix)
+    np.testing.assert_array_equal(deg.data[:, : multi.nsamples], g["multi"][:4])
     single = dedispersion.dedisperse_single_dm(fb, dm=3.0)
     np.testing.assert_array_equal(single.data[0, : multi.nsamples], g["multi"][3])
 
@@ -226,9 +227,10 @@ def test_ffa_rows_and_search_vs_golden(gpu):
 
 
 @pytest.mark.parametrize("rows_bins", [(1, 5), (2, 7), (3, 4), (5, 4), (163, 181), (700, 256),
-                                       (2679, 180)])
+                                       (2679, 180), (2657, 180), (1203, 200), (1090, 256)])
 def test_ffa_transform_shapes_bit_exact(gpu, port, rows_bins):
-    """Single-task plans over odd shapes: LDS-only trees, forced global levels."""
+    """Single-task plans over odd shapes: LDS-only trees, forced global levels,
+    and level work items of 1, 2 and 3 rows (rows % 16 in {1, 2, 3})."""
     from oracle.oracle import TASK_DTYPE
 
     rows, bins = rows_bins
