@@ -8,9 +8,12 @@ every DM trial with peak collection.  The filterbank is generated on the GPU
 before timing (seeded, 8 injected pulsars).
 
 Prints one JSON line: DM-trial*samples/s over the whole step (wall clock with
-device synchronisation on both sides), the FFA stage's algorithmic-byte
-roofline from HIP events recorded on the pipeline stream, and a CPU baseline
-(gaffa's own CPU code, compiled from its sources, on a bounded sample).
+device synchronisation on both sides); the roofline of the dominant kernel
+(largest share of the step): its algorithmic bytes per launch over its
+average launch time, from HIP events recorded between the launch stages on
+the pipeline stream; every hot kernel's figures under "kernels"; and a CPU
+baseline (gaffa's own CPU code, compiled from its sources, on a bounded
+sample).
 """
 
 from __future__ import annotations
@@ -29,20 +32,103 @@ UNIT = "DM-trial·samples/s"
 HBM_PEAK_GBS = 8000.0
 
 
-def ffa_algorithmic_bytes(plan, n_out: int, nseries: int, npeaks: int) -> float:
-    """Per launch sequence over `nseries` series: each prepare group reads the
-    series and writes its prepared copy (factor-1 groups read in place), each
-    task reads its rows*bins input once, each peak is written once (20 B)."""
+def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
+    """Algorithmic HBM bytes per launch of each hot kernel (the minimum any
+    implementation must move), keyed like the stage times:
+
+    dedisperse  one launch per DM segment: the uint8 filterbank once
+                (nsamples * nchans) + the segment's uint32 rows (4 * ndm_seg * n_out)
+    prepare     per series: the float series once (4 n) + every downsampled copy
+                (4 P_g per downsample group)
+    subtree     per series and task with rows >= 2: read + write its rows x bins
+                floats once (8 rows bins)
+    levels      per series and task: 8 rows bins for each level done through
+                global memory (depth_t - 1 of them), averaged over the launches
+    detect      per series and task: the depth-1 rows once (4 rows bins; the input
+                series row for 1-row tasks) + 20 B per peak
+    """
     import numpy as np
 
-    tasks = plan.tasks
-    per_series = 0.0
-    for f in np.unique(tasks["downsample_factor"]):
+    t = pipe.ffa_plan.tasks
+    lay = pipe.ffa_layout
+    n_out, ndm = pipe.out_nsamples, pipe.ndm
+    rb = t["rows"].astype(np.float64) * t["bins"]
+    depth = lay["task_depth"].astype(np.float64)
+    series_per_tile = ndm / ntiles
+    prep = 4.0 * n_out
+    for f in np.unique(t["downsample_factor"]):
         if f != 1.0:
-            p = int(tasks["prepared_nsamples"][tasks["downsample_factor"] == f][0])
-            per_series += 4.0 * n_out + 4.0 * p
-    per_series += 4.0 * float((tasks["rows"].astype(np.float64) * tasks["bins"]).sum())
-    return per_series * nseries + 20.0 * npeaks
+            prep += 4.0 * float(t["prepared_nsamples"][t["downsample_factor"] == f][0])
+    subtree = 8.0 * float(rb[t["rows"] >= 2].sum())
+    levels = 8.0 * float((rb * np.maximum(depth - 1.0, 0.0)).sum())
+    detect = 4.0 * float(rb.sum())
+    nlev = max(1, lay["level_launches"])
+    nseg = len(workload.dm_segments)
+    return {
+        "dedisperse": (nseg, workload.nsamples * workload.nchans + 4.0 * (ndm / nseg) * n_out),
+        "ffa_prepare": (ntiles, prep * series_per_tile),
+        "ffa_subtree": (ntiles, subtree * series_per_tile),
+        "ffa_levels": (ntiles * nlev, levels * series_per_tile / nlev),
+        "ffa_detect": (ntiles, detect * series_per_tile + 20.0 * npeaks / ntiles),
+    }
+
+
+KERNEL_NAMES = {
+    "dedisperse": "dedisperse_subband_cm_kernel",
+    "ffa_prepare": "ffa_prepare_kernel",
+    "ffa_subtree": "ffa_subtree_kernel",
+    "ffa_levels": "ffa_level_kernel",
+    "ffa_detect": "ffa_terminal_kernel<true>",
+}
+
+
+def read_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by a copy kernel of
+    known size), or None when no summary covers this kernel and workload."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    with open(path) as fh:
+        for line in fh:
+            if "=" in line and line.startswith("    "):
+                k, v = line.strip().split("=", 1)
+                vals[k.strip()] = v.strip()
+    try:
+        return float(vals["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def calibrate_copy(mib: int) -> None:
+    """One device-to-device copy of `mib` MiB through the C ABI (the plain copy
+    kernel whose FETCH_SIZE / WRITE_SIZE scale the pmc counters)."""
+    import ctypes
+
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    lib.gaffa_hip_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint64]
+    lib.gaffa_hip_free.argtypes = [ctypes.c_void_p]
+    lib.gaffa_hip_memcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint64,
+                                     ctypes.c_void_p]
+    lib.gaffa_hip_memset.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_uint64,
+                                     ctypes.c_void_p]
+    nbytes = mib << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (a, b):
+        if lib.gaffa_hip_malloc(ctypes.byref(ptr), nbytes) != 0:
+            raise RuntimeError("copy calibration: allocation failed")
+    try:
+        if lib.gaffa_hip_memset(a, 1, nbytes, None) != 0 or \
+                lib.gaffa_hip_memcpy(b, a, nbytes, None) != 0 or \
+                lib.gaffa_hip_device_synchronize() != 0:
+            raise RuntimeError("copy calibration: copy failed")
+    finally:
+        lib.gaffa_hip_free(a)
+        lib.gaffa_hip_free(b)
+    print(f"[bench] copy calibration: {mib} MiB device-to-device", file=sys.stderr, flush=True)
 
 
 def cpu_baseline(workload, filterbank, threads: int) -> dict:
@@ -93,6 +179,8 @@ def main() -> int:
     ap.add_argument("--workload", choices=["lband", "small"], default="lband")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="after the timed steps, copy MIB MiB device-to-device (pmc scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the DM axis would "
@@ -111,6 +199,7 @@ def main() -> int:
     print(f"[bench] workload {w.name}: {w.nchans} ch x {w.nsamples} samples, {w.ndm} DMs",
           file=sys.stderr, flush=True)
     pipe = w.pipeline()
+    pipe.set_kernel_timing(True)
     pipe.generate(w.synth_spec())
     ndm, n_out = pipe.ndm, pipe.out_nsamples
     print(f"[bench] n_out {n_out}, FFA tasks {pipe.ffa_plan.task_count}, "
@@ -120,12 +209,12 @@ def main() -> int:
         t = pipe.run()
         print(f"[bench] warmup {i}: {json.dumps(t)}", file=sys.stderr, flush=True)
 
-    stage_sum = {"dedisperse_ms": 0.0, "preprocess_ms": 0.0, "ffa_ms": 0.0, "total_ms": 0.0}
+    stage_sum: dict = {}
     t0 = time.perf_counter()
     for i in range(args.steps):
         t = pipe.run()  # ends with a stream synchronisation
-        for k in stage_sum:
-            stage_sum[k] += t[k]
+        for k, v in t.items():
+            stage_sum[k] = stage_sum.get(k, 0.0) + v
         print(f"[bench] step {i}: {json.dumps(t)}", file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
     ms_per_step = 1000.0 * wall / args.steps
@@ -133,13 +222,24 @@ def main() -> int:
     value = units / (wall / args.steps)
     stages = {k: v / args.steps for k, v in stage_sum.items()}
     npeaks = len(pipe.peaks)
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
 
-    ffa_bytes = ffa_algorithmic_bytes(pipe.ffa_plan, n_out, ndm, npeaks)
-    ffa_gbs = ffa_bytes / (stages["ffa_ms"] * 1e-3) / 1e9
-    dd_bytes = w.nsamples * w.nchans + 4.0 * ndm * n_out
-    dd_gbs = dd_bytes / (stages["dedisperse_ms"] * 1e-3) / 1e9
-    pp_bytes = 8.0 * ndm * n_out
-    pp_gbs = pp_bytes / (stages["preprocess_ms"] * 1e-3) / 1e9
+    # per-kernel rooflines: algorithmic bytes per launch / average launch time
+    ntiles = int(stages.get("ffa_tiles", 0)) or -(-ndm // pipe.ffa_tile)
+    kernels = {}
+    for key, (launches, nbytes) in kernel_bytes(w, pipe, ntiles, npeaks).items():
+        ms = stages.get(f"{key}_ms")
+        if ms is None or launches == 0:
+            continue
+        launch_ms = ms / launches
+        gbs = nbytes / (launch_ms * 1e-3) / 1e9
+        kernels[key] = {"kernel": KERNEL_NAMES[key], "launches_per_step": launches,
+                        "launch_ms": launch_ms, "bytes_per_launch": nbytes,
+                        "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                        "share_of_step": ms / stages["total_ms"]}
+    dom_key = max(kernels, key=lambda k: kernels[k]["launch_ms"] * kernels[k]["launches_per_step"])
+    dom = kernels[dom_key]
 
     result = {
         "metric": METRIC,
@@ -168,19 +268,17 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "FFA stage (prepare + subtree + level + terminal/detect launches)",
-            "achieved": ffa_gbs,
+            "kernel": dom["kernel"],
+            "achieved": dom["achieved_gbs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": ffa_gbs / HBM_PEAK_GBS,
-            "traffic": None,
+            "frac": dom["frac"],
+            "traffic": read_traffic(dom["kernel"].split("<")[0], w.name),
+            "launch_ms": dom["launch_ms"],
+            "bytes_per_launch": dom["bytes_per_launch"],
         },
-        "stages_ms": stages,
-        "stage_roofline_frac": {
-            "dedisperse": dd_gbs / HBM_PEAK_GBS,
-            "preprocess": pp_gbs / HBM_PEAK_GBS,
-            "ffa": ffa_gbs / HBM_PEAK_GBS,
-        },
+        "stages_ms": {k: v for k, v in stages.items() if k.endswith("_ms")},
+        "kernels": kernels,
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))

@@ -1,11 +1,13 @@
 // Incoherent dedispersion of uint8 filterbanks on gfx950.
 //
-// Both kernels stage delay-shifted input through LDS: a workgroup copies a
+// Brute force stages delay-shifted input through LDS: a workgroup copies a
 // time slab of a channel block from HBM (row segments of contiguous channel
 // bytes, coalesced), transposes it to channel-major bytes in LDS, then every
 // thread reads 4 consecutive time samples of a channel with one aligned dword
-// pair + v_alignbyte and accumulates them as packed 16-bit pairs (<= 32 or
-// subband_channels channels per flush, so no 16-bit lane can overflow).
+// pair + v_alignbyte and accumulates them as packed 16-bit pairs (<= 32
+// channels per flush, so no 16-bit lane can overflow).  Subband dedispersion
+// reads a channel-major copy of the filterbank (transposed once, shared by
+// every DM segment) directly from global memory in the same 4-sample form.
 // Integer sums are order-independent, so results equal gaffa's CPU sums bit
 // for bit (dedispersion.cpp:197-215, 291-329, 351-453).
 #include "common.hpp"
@@ -136,49 +138,109 @@ dedisperse_multi_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
   }
 }
 
-// ------------------------------------------------------------- subband ----
-// One workgroup = (time tile of `tile` outputs, nominal DM group k).
-// Stage 1 builds inter[s][j], j in [0, tile + max_residual + pad), as u16 in
-// LDS, channel chunk by channel chunk through a byte slab; stage 2 sums the
-// subbands at the residual delays for the group's DMs.
-constexpr int kSbChunk = 32;       // channels per slab chunk
-constexpr int kSbJ = 8;            // stage-1 samples per thread (jlen <= 2048)
-constexpr uint32_t kSlabStride = 36;  // slab row stride in bytes (9 dwords)
+// ---------------------------------------------------------- transpose ----
+// out[c][t] = x[t][chan_begin + c] for t < nsamples, 0 for nsamples <= t <
+// out_stride: one workgroup moves a 256-sample x 64-channel tile through LDS
+// (16-byte loads along channels, byte writes into [channel][time] rows,
+// 16-byte stores along time).
+constexpr uint32_t kTrTime = 256;
+constexpr uint32_t kTrChans = 64;
+constexpr uint32_t kTrStride = kTrTime + 16;  // LDS row bytes (16-aligned)
 
 __global__ void __launch_bounds__(kThreads)
-dedisperse_subband_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
-                             uint64_t nchans, uint64_t chan_begin,
-                             uint32_t nsel, uint32_t sbc, uint32_t nsub,
-                             uint32_t npn, uint32_t ndm,
+transpose_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples, uint64_t nchans,
+                    uint64_t chan_begin, uint32_t nsel, uint8_t* __restrict__ out,
+                    uint64_t out_stride, uint32_t vec_in) {
+  __shared__ __align__(16) uint8_t tile[kTrChans * kTrStride];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kTrTime;
+  const uint32_t c0 = blockIdx.y * kTrChans;
+  const uint32_t nc = min(kTrChans, nsel - c0);
+  // 256 rows x 4 chunks of 16 channels
+#pragma unroll
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t row = (tid >> 2) + 64u * r;
+    const uint32_t chunk = tid & 3u;
+    const uint64_t t = t0 + row;
+    uint8_t v[16];
+    if (vec_in && 16u * chunk + 16u <= nc) {
+      uint4 q = make_uint4(0, 0, 0, 0);
+      if (t < nsamples)
+        q = *reinterpret_cast<const uint4*>(x + t * nchans + chan_begin + c0 + 16u * chunk);
+      const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] = static_cast<uint8_t>(w[u >> 2] >> (8 * (u & 3)));
+    } else {
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        const uint32_t c = 16u * chunk + u;
+        v[u] = (t < nsamples && c < nc) ? x[t * nchans + chan_begin + c0 + c] : 0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) tile[(16u * chunk + u) * kTrStride + row] = v[u];
+  }
+  __syncthreads();
+  // 64 channel rows x 16 chunks of 16 samples
+#pragma unroll
+  for (uint32_t r = 0; r < 4; ++r) {
+    const uint32_t c = (tid >> 4) + 16u * r;
+    const uint32_t part = tid & 15u;
+    const uint64_t t = t0 + 16u * part;
+    if (c < nc && t < out_stride) {
+      const uint4 q = *reinterpret_cast<const uint4*>(tile + c * kTrStride + 16u * part);
+      *reinterpret_cast<uint4*>(out + static_cast<uint64_t>(c0 + c) * out_stride + t) = q;
+    }
+  }
+}
+
+// ------------------------------------------------------------- subband ----
+// One workgroup = (time tile of `tile` outputs, nominal DM group k), blocks
+// ordered nominal-fastest so the groups of one tile share the channel rows
+// in L2.  Stage 1 builds inter[s][j] for j < jlen4 (tile + max_residual +
+// pad) as u16 in LDS straight from the channel-major rows: lane w sums the
+// 4 samples [4w, 4w + 4) of every channel of the subband with one dword pair
+// + v_alignbyte (the channel's delay shift is wave-uniform) into two packed
+// 16-bit pairs (<= 256 channels x 255 fits 16 bits).  Stage 2 sums the
+// subbands at the residual delays for the group's DMs.
+constexpr int kSbQ = 4;  // stage-1 dword columns per thread (jlen4 <= 4096)
+constexpr int kSbUnroll = 8;  // channels whose loads are in flight together
+constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, one L2 each
+
+__global__ void __launch_bounds__(kThreads)
+dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
+                             uint32_t nsel, uint32_t sbc, uint32_t nsub, uint32_t npn,
+                             uint32_t ndm, uint32_t nnom, uint32_t nblocks,
                              const int32_t* __restrict__ coarse,
                              const int32_t* __restrict__ resid,
                              uint32_t* __restrict__ out, uint64_t out_nsamples,
-                             uint64_t out_stride, uint32_t tile,
-                             uint32_t inter_len, uint32_t slab_len,
-                             uint32_t flush_every,
-                             int32_t* __restrict__ error_flag) {
+                             uint64_t out_stride, uint32_t tile, uint32_t inter_len,
+                             uint32_t flush_every, int32_t* __restrict__ error_flag) {
   extern __shared__ uint32_t dyn_lds[];
-  // [nsub][inter_len] u16, then [kSbChunk][slab_len] u8, then residuals
+  // [nsub][inter_len] u16, then the group's residuals [npn][nsub] i32
   uint16_t* inter = reinterpret_cast<uint16_t*>(dyn_lds);
-  uint8_t* slab = reinterpret_cast<uint8_t*>(inter + static_cast<size_t>(nsub) * inter_len);
-  int32_t* sres = reinterpret_cast<int32_t*>(slab + static_cast<size_t>(slab_len) * kSlabStride);
-  __shared__ int32_t sdel[kSbChunk];
-  __shared__ int32_t sminmax[2];
+  int32_t* sres = reinterpret_cast<int32_t*>(inter + static_cast<size_t>(nsub) * inter_len);
 
   const uint32_t tid = threadIdx.x;
-  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * tile;
-  const uint32_t k = blockIdx.y;
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+  // so XCD x takes the contiguous work range [x * per_xcd, (x + 1) * per_xcd)
+  // and the nominal groups of one tile, and neighbouring tiles, share its L2.
+  const uint32_t nwork = nblocks;
+  const uint32_t per_xcd = (nwork + kXcds - 1) / kXcds;
+  const uint32_t w_id = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
+  if (w_id >= nwork) return;
+  const uint32_t k = w_id % nnom;
+  const uint64_t t0 = static_cast<uint64_t>(w_id / nnom) * tile;
   const uint32_t dm0 = k * npn;
   const uint32_t nd = min(npn, ndm - dm0);
-  const uint32_t jlen = inter_len - 8;  // stage-1 samples computed
-  const uint32_t nquads = jlen / 4;
+  const uint32_t jlen4 = inter_len - 8;  // stage-1 samples computed, multiple of 4
 
   __shared__ int32_t sbad;
   if (tid == 0) sbad = 0;
   __syncthreads();
   // stage 2 reads inter[j + r .. j + r + 5] with j <= tile - 4: r must stay
-  // within the staged halo or the block is refused (uniformly)
-  const int32_t rmax = static_cast<int32_t>(jlen - tile - 4);
+  // within the computed samples or the block is refused (uniformly)
+  const int32_t rmax = static_cast<int32_t>(jlen4 - tile - 4);
   for (uint32_t e = tid; e < nd * nsub; e += kThreads) {
     const int32_t r = resid[static_cast<uint64_t>(dm0) * nsub + e];
     if (r < 0 || r > rmax) sbad = 1;
@@ -190,124 +252,101 @@ dedisperse_subband_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples,
     return;
   }
 
-  // Stage 1.  The slab is row-major [row][channel] with a 36-byte (9-dword,
-  // odd) row stride: it is filled with 16-byte global loads, and lane j of a
-  // wave reads byte (j + delay_c - dmin, c), so the lanes of one read hit
-  // 64 distinct banks.  Thread tid owns samples j = tid + 256 q.
-  (void)nquads;
-  const bool vec_rows = (nchans % 16u) == 0 && (chan_begin % 16u) == 0 && (sbc % 32u) == 0 &&
-                        (reinterpret_cast<uintptr_t>(x) & 15u) == 0;
+  // Stage 1: the loads of kSbUnroll channels are issued together.
+  const int32_t* kdel = coarse + static_cast<uint64_t>(k) * nsel;
+  uint32_t* inter32 = reinterpret_cast<uint32_t*>(inter);
+  const uint32_t nw = jlen4 / 4;                        // dword columns
+  const uint32_t nq = (nw + kThreads - 1) / kThreads;  // <= kSbQ (host)
   for (uint32_t s = 0; s < nsub; ++s) {
     const uint32_t cs = s * sbc;
     const uint32_t ce = min(cs + sbc, nsel);
-    uint32_t acc[kSbJ];
+    uint32_t lo[kSbQ], hi[kSbQ];
 #pragma unroll
-    for (int q = 0; q < kSbJ; ++q) acc[q] = 0;
-
-    for (uint32_t c0 = cs; c0 < ce; c0 += kSbChunk) {
-      const uint32_t nc = min(static_cast<uint32_t>(kSbChunk), ce - c0);
-      if (tid == 0) {
-        sminmax[0] = 0x7fffffff;
-        sminmax[1] = 0;
+    for (int q = 0; q < kSbQ; ++q) lo[q] = hi[q] = 0;
+    for (uint32_t c0 = cs; c0 < ce; c0 += kSbUnroll) {
+      const uint32_t nu = min(static_cast<uint32_t>(kSbUnroll), ce - c0);
+      const uint32_t* rows[kSbUnroll];
+      uint32_t sh[kSbUnroll];
+      bool bad = false;
+#pragma unroll
+      for (int u = 0; u < kSbUnroll; ++u) {
+        const uint32_t c = min(c0 + u, ce - 1);  // spare slots repeat a valid channel
+        const int32_t d = kdel[c];
+        const uint64_t base = t0 + static_cast<uint64_t>(d);
+        // rows are zero-padded to cm_stride; a delay beyond the padding
+        // would read past the row
+        bad |= d < 0 || base + jlen4 + 4 > cm_stride;
+        rows[u] = reinterpret_cast<const uint32_t*>(cm + static_cast<uint64_t>(c) * cm_stride) +
+                  (base >> 2);
+        sh[u] = static_cast<uint32_t>(base & 3u);
       }
-      __syncthreads();
-      if (tid < nc) {
-        const int32_t d = coarse[static_cast<uint64_t>(k) * nsel + c0 + tid];
-        sdel[tid] = d;
-        atomicMin(&sminmax[0], d);
-        atomicMax(&sminmax[1], d);
-      }
-      __syncthreads();
-      const int32_t dmin = sminmax[0];
-      const uint32_t rows = jlen + static_cast<uint32_t>(sminmax[1] - dmin) + 8;
-      if (rows > slab_len) {
+      if (bad) {  // uniform: the delays are the same for every lane
         if (tid == 0) atomicExch(error_flag, 1);
         return;
       }
-      const uint64_t tbase = t0 + static_cast<uint64_t>(dmin);
-      if (vec_rows && nc == kSbChunk) {
-        uint32_t* slab32 = reinterpret_cast<uint32_t*>(slab);
-        for (uint32_t e = tid; e < rows * 2; e += kThreads) {
-          const uint32_t r = e >> 1, h = e & 1u;
-          const uint64_t t = tbase + r;
-          uint4 v = make_uint4(0, 0, 0, 0);
-          if (t < nsamples)
-            v = *reinterpret_cast<const uint4*>(x + t * nchans + chan_begin + c0 + 16u * h);
-          uint32_t* dst = slab32 + (r * kSlabStride + 16u * h) / 4u;
-          dst[0] = v.x;
-          dst[1] = v.y;
-          dst[2] = v.z;
-          dst[3] = v.w;
-        }
-      } else {
-        for (uint32_t e = tid; e < rows * nc; e += kThreads) {
-          const uint32_t r = e / nc, c = e % nc;
-          const uint64_t t = tbase + r;
-          uint8_t v = 0;
-          if (t < nsamples) v = x[t * nchans + chan_begin + c0 + c];
-          slab[r * kSlabStride + c] = v;
-        }
-      }
-      __syncthreads();
-      for (uint32_t c = 0; c < nc; ++c) {
-        const uint8_t* col = slab + static_cast<uint32_t>(sdel[c] - dmin) * kSlabStride + c;
 #pragma unroll
-        for (int q = 0; q < kSbJ; ++q) {
-          const uint32_t j = tid + static_cast<uint32_t>(q) * kThreads;
-          if (j < jlen) acc[q] += col[j * kSlabStride];
+      for (int q = 0; q < kSbQ; ++q) {
+        if (static_cast<uint32_t>(q) >= nq) break;  // uniform
+        const uint32_t w = min(tid + static_cast<uint32_t>(q) * kThreads, nw - 1);
+        uint32_t x0[kSbUnroll], x1[kSbUnroll];
+#pragma unroll
+        for (int u = 0; u < kSbUnroll; ++u) {
+          x0[u] = rows[u][w];
+          x1[u] = rows[u][w + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kSbUnroll; ++u) {
+          if (static_cast<uint32_t>(u) < nu) {
+            const uint32_t v = __builtin_amdgcn_alignbyte(x1[u], x0[u], sh[u]);
+            lo[q] += v & 0x00FF00FFu;         // samples 4w, 4w + 2
+            hi[q] += (v >> 8) & 0x00FF00FFu;  // samples 4w + 1, 4w + 3
+          }
         }
       }
-      __syncthreads();
     }
-    uint16_t* row = inter + static_cast<size_t>(s) * inter_len;
+    uint32_t* row = inter32 + static_cast<size_t>(s) * (inter_len / 2);
 #pragma unroll
-    for (int q = 0; q < kSbJ; ++q) {
-      const uint32_t j = tid + static_cast<uint32_t>(q) * kThreads;
-      if (j < jlen) {
-        row[j] = static_cast<uint16_t>(acc[q]);
-      } else if (j < inter_len) {
-        row[j] = 0;
+    for (int q = 0; q < kSbQ; ++q) {
+      const uint32_t w = tid + static_cast<uint32_t>(q) * kThreads;
+      if (4u * w < jlen4) {
+        row[2 * w] = (lo[q] & 0xFFFFu) | (hi[q] << 16);
+        row[2 * w + 1] = (lo[q] >> 16) | (hi[q] & 0xFFFF0000u);
+      } else if (4u * w < inter_len) {
+        row[2 * w] = 0;
+        row[2 * w + 1] = 0;
       }
     }
   }
   __syncthreads();
 
-  // Stage 2: thread -> (dm, quad) pairs.
+  // Stage 2: thread -> (dm, quad) pairs; packed 16-bit partial sums over
+  // flush_every subbands at a time (no lane can overflow), read 4 subbands
+  // ahead.  The three dwords around sample j are read unconditionally and
+  // v_alignbyte picks the pair (shift 0 or 2 bytes).
   const uint32_t tquads = tile / 4;
+  const uint32_t half_len = inter_len / 2;
   for (uint32_t e = tid; e < nd * tquads; e += kThreads) {
     const uint32_t i = e / tquads;
     const uint32_t q = e % tquads;
+    const int32_t* ri = sres + i * nsub;
     uint32_t tot0 = 0, tot1 = 0, tot2 = 0, tot3 = 0;
-    uint32_t p01 = 0, p23 = 0, pending = 0;
-    for (uint32_t s = 0; s < nsub; ++s) {
-      const uint32_t j = 4u * q + static_cast<uint32_t>(sres[i * nsub + s]);
-      const uint32_t* row32 =
-          reinterpret_cast<const uint32_t*>(inter + static_cast<size_t>(s) * inter_len);
-      const uint32_t a = j >> 1;
-      uint32_t w01, w23;
-      if ((j & 1u) == 0) {
-        w01 = row32[a];
-        w23 = row32[a + 1];
-      } else {
-        const uint32_t d0 = row32[a], d1 = row32[a + 1], d2 = row32[a + 2];
-        w01 = __builtin_amdgcn_alignbyte(d1, d0, 2);
-        w23 = __builtin_amdgcn_alignbyte(d2, d1, 2);
+    for (uint32_t s0 = 0; s0 < nsub; s0 += flush_every) {
+      const uint32_t s1 = min(s0 + flush_every, nsub);
+      uint32_t p01 = 0, p23 = 0;
+#pragma unroll 4
+      for (uint32_t s = s0; s < s1; ++s) {
+        const uint32_t j = 4u * q + static_cast<uint32_t>(ri[s]);
+        const uint32_t* r32 = inter32 + static_cast<size_t>(s) * half_len + (j >> 1);
+        const uint32_t d0 = r32[0], d1 = r32[1], d2 = r32[2];
+        const uint32_t shb = (j & 1u) * 2u;
+        p01 += __builtin_amdgcn_alignbyte(d1, d0, shb);
+        p23 += __builtin_amdgcn_alignbyte(d2, d1, shb);
       }
-      p01 += w01;
-      p23 += w23;
-      if (++pending == flush_every) {
-        tot0 += p01 & 0xFFFFu;
-        tot1 += p01 >> 16;
-        tot2 += p23 & 0xFFFFu;
-        tot3 += p23 >> 16;
-        p01 = p23 = 0;
-        pending = 0;
-      }
+      tot0 += p01 & 0xFFFFu;
+      tot1 += p01 >> 16;
+      tot2 += p23 & 0xFFFFu;
+      tot3 += p23 >> 16;
     }
-    tot0 += p01 & 0xFFFFu;
-    tot1 += p01 >> 16;
-    tot2 += p23 & 0xFFFFu;
-    tot3 += p23 >> 16;
     uint32_t* o = out + static_cast<uint64_t>(dm0 + i) * out_stride;
     const uint64_t t = t0 + 4u * q;
     if (t + 3 < out_nsamples && (out_stride & 3u) == 0 &&
@@ -398,17 +437,48 @@ int gaffa_hip_dedisperse_multi_u8(const gaffa_hip_dedisperse_multi_args* a,
   });
 }
 
+int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* a,
+                                      gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(a != nullptr, "transpose arguments must not be null");
+    check(a->samples != nullptr && a->out != nullptr, "transpose pointers must not be null");
+    check(a->nsamples > 0 && a->nchans > 0, "transpose requires a non-empty filterbank");
+    check(a->chan_begin < a->chan_end && a->chan_end <= a->nchans,
+          "channel range exceeds sample channel count");
+    check(a->out_stride >= a->nsamples && a->out_stride % 16 == 0,
+          "channel-major stride must be >= nsamples and a multiple of 16");
+    check((reinterpret_cast<uintptr_t>(a->out) & 15u) == 0,
+          "channel-major output must be 16-byte aligned");
+    const uint64_t nsel = a->chan_end - a->chan_begin;
+    check_overflow(nsel <= 0xFFFFFFFFull, "transpose channel range too large");
+    const uint64_t tblocks = ceil_div(a->out_stride, kTrTime);
+    const uint64_t cblocks = ceil_div(nsel, kTrChans);
+    check_overflow(tblocks < 0x7FFFFFFFull && cblocks < 65536, "transpose grid too large");
+    const uint32_t vec_in = (a->nchans % 16 == 0 && a->chan_begin % 16 == 0 &&
+                             (reinterpret_cast<uintptr_t>(a->samples) & 15u) == 0)
+                                ? 1u
+                                : 0u;
+    hipLaunchKernelGGL(transpose_u8_kernel,
+                       dim3(static_cast<uint32_t>(tblocks), static_cast<uint32_t>(cblocks)),
+                       dim3(kThreads), 0, as_stream(stream), a->samples, a->nsamples, a->nchans,
+                       a->chan_begin, static_cast<uint32_t>(nsel), a->out, a->out_stride,
+                       vec_in);
+    launch_check("transpose_u8_kernel");
+  });
+}
+
 int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
                                     gaffa_hip_stream stream) {
   return guarded([&] {
     check(a != nullptr, "dedispersion arguments must not be null");
-    check(a->samples != nullptr && a->coarse_delays != nullptr &&
+    check(a->channels != nullptr && a->coarse_delays != nullptr &&
               a->residual_delays != nullptr && a->out != nullptr,
           "dedispersion device pointers must not be null");
     check(a->nsamples > 0, "dedispersion requires at least one sample");
-    check(a->nchans > 0, "dedispersion requires at least one channel");
-    check(a->chan_begin < a->chan_end, "channel range must be non-empty");
-    check(a->chan_end <= a->nchans, "channel range exceeds sample channel count");
+    check(a->nsel > 0, "channel range must be non-empty");
+    check(a->channel_stride >= a->nsamples && a->channel_stride % 4 == 0 &&
+              (reinterpret_cast<uintptr_t>(a->channels) & 3u) == 0,
+          "channel rows must be dword aligned and hold nsamples");
     check(a->ndm > 0, "dedispersion requires at least one DM");
     check(a->subband_channels > 0, "subband_channels must be positive");
     check(a->ndm_per_nominal > 0, "ndm_per_nominal must be positive");
@@ -417,53 +487,69 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
     check(a->out_nsamples > 0 && a->out_nsamples <= a->nsamples,
           "dedispersion valid output range is empty");
     check(a->out_stride >= a->out_nsamples, "output stride must be >= out_nsamples");
-    check(a->max_residual >= 0 && a->max_coarse_spread >= 0,
+    check(a->max_residual >= 0 && a->max_coarse >= 0,
           "dedispersion requires non-negative delays");
-    const uint64_t nsel = a->chan_end - a->chan_begin;
+    const uint64_t nsel = a->nsel;
+    check_overflow(nsel <= 0xFFFFFFFFull, "dedispersion channel range too large");
     const uint64_t nsub = ceil_div(nsel, a->subband_channels);
     const uint64_t nnom = ceil_div(a->ndm, a->ndm_per_nominal);
     check(a->ndm_per_nominal <= 1024, "ndm_per_nominal must be <= 1024 on the GPU");
     const uint32_t flush = static_cast<uint32_t>(
         std::max<uint64_t>(1, 65535 / (255 * a->subband_channels)));
-    // Pick the largest time tile whose LDS footprint fits.
-    uint32_t tile = 0, inter_len = 0, slab_len = 0;
-    uint64_t lds = 0;
+    // Largest time tile whose LDS fits two workgroups per CU, else one.
     const uint64_t res_bytes = a->ndm_per_nominal * nsub * 4;
-    for (uint32_t cand = 2048; cand >= 64; cand /= 2) {
-      const uint32_t jlen = cand + static_cast<uint32_t>(a->max_residual) + 4;
-      const uint32_t jlen4 = (jlen + 3) & ~3u;
-      if (jlen4 > static_cast<uint32_t>(kThreads * kSbJ)) continue;
-      const uint32_t il = jlen4 + 8;
-      const uint32_t sl =
-          (jlen4 + static_cast<uint32_t>(a->max_coarse_spread) + 8 + 3) & ~3u;
-      const uint64_t bytes = nsub * il * 2 + static_cast<uint64_t>(sl) * kSlabStride + res_bytes;
-      if (bytes <= kMaxLds - 4096) {
-        tile = cand;
-        inter_len = il;
-        slab_len = sl;
-        lds = bytes;
-        break;
+    uint32_t tile = 0, inter_len = 0;
+    uint64_t lds = 0;
+    for (const uint64_t budget : {kMaxLds / 2 - 2048, kMaxLds - 4096}) {
+      for (uint32_t cand = 2048; cand >= 64 && tile == 0; cand /= 2) {
+        const uint64_t jlen4 = (cand + static_cast<uint64_t>(a->max_residual) + 4 + 3) & ~3ull;
+        const uint64_t il = jlen4 + 8;
+        // stage 1 writes (computes or zeroes) inter[0, il) in kSbQ dword
+        // pairs per thread
+        if (il > static_cast<uint64_t>(kThreads) * kSbQ * 4) continue;
+        const uint64_t bytes = nsub * il * 2 + res_bytes;
+        if (bytes <= budget) {
+          tile = cand;
+          inter_len = static_cast<uint32_t>(il);
+          lds = bytes;
+        }
       }
+      if (tile != 0) break;
     }
     check_overflow(tile != 0, "subband dedispersion delays too large for LDS");
     const uint64_t tiles = ceil_div(a->out_nsamples, tile);
-    check_overflow(tiles < 0x7FFFFFFFull && nnom < 65536, "dedispersion grid too large");
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(dedisperse_subband_u8_kernel),
+    check_overflow(tiles * nnom < 0x7FFFFFF0ull, "dedispersion grid too large");
+    // every stage-1 read of the last tile stays inside the padded rows
+    check((tiles - 1) * tile + static_cast<uint64_t>(a->max_coarse) + (inter_len - 8) + 4 <=
+              a->channel_stride,
+          "channel rows are not padded far enough for the delays (see "
+          "gaffa_hip_subband_channel_stride)");
+    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(dedisperse_subband_cm_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)),
               "hipFuncSetAttribute");
-    const dim3 grid(static_cast<uint32_t>(tiles), static_cast<uint32_t>(nnom));
-    hipLaunchKernelGGL(dedisperse_subband_u8_kernel, grid, dim3(kThreads),
-                       static_cast<uint32_t>(lds), as_stream(stream), a->samples,
-                       a->nsamples, a->nchans, a->chan_begin, static_cast<uint32_t>(nsel),
-                       static_cast<uint32_t>(a->subband_channels),
-                       static_cast<uint32_t>(nsub),
-                       static_cast<uint32_t>(a->ndm_per_nominal),
-                       static_cast<uint32_t>(a->ndm), a->coarse_delays,
-                       a->residual_delays, a->out, a->out_nsamples, a->out_stride, tile,
-                       inter_len, slab_len, flush, error_flag_for_device());
-    launch_check("dedisperse_subband_u8_kernel");
+    const uint64_t nblocks = tiles * nnom;
+    const uint64_t grid = ceil_div(nblocks, kXcds) * kXcds;  // whole rounds over the XCDs
+    hipLaunchKernelGGL(dedisperse_subband_cm_kernel, dim3(static_cast<uint32_t>(grid)),
+                       dim3(kThreads), static_cast<uint32_t>(lds), as_stream(stream),
+                       a->channels, a->channel_stride, static_cast<uint32_t>(nsel),
+                       static_cast<uint32_t>(a->subband_channels), static_cast<uint32_t>(nsub),
+                       static_cast<uint32_t>(a->ndm_per_nominal), static_cast<uint32_t>(a->ndm),
+                       static_cast<uint32_t>(nnom), static_cast<uint32_t>(nblocks),
+                       a->coarse_delays, a->residual_delays, a->out,
+                       a->out_nsamples, a->out_stride, tile, inter_len, flush,
+                       error_flag_for_device());
+    launch_check("dedisperse_subband_cm_kernel");
   });
+}
+
+uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,
+                                          int32_t max_residual) {
+  // the last tile starts below nsamples; it reads tile (<= 2048) + residual
+  // + pad samples past its coarse delay
+  const uint64_t need = nsamples + static_cast<uint64_t>(std::max(max_coarse, 0)) +
+                        static_cast<uint64_t>(std::max(max_residual, 0)) + 2048 + 64;
+  return (need + 255) / 256 * 256;
 }
 
 int gaffa_hip_dedisperse_check(gaffa_hip_stream stream) {

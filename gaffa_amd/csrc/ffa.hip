@@ -35,13 +35,12 @@ namespace {
 constexpr uint32_t kLevelThreads = 256;
 constexpr uint32_t kLevelRows = 16;  // rows per level work item (host side too)
 constexpr uint32_t kSubtreeThreads = 1024;
-constexpr uint32_t kSubtreeMaxRows = 512;  // descriptor table capacity
+constexpr uint32_t kSubtreeMaxRows = 512;  // 16-bit row fields in the descriptors
 constexpr uint32_t kTerminalThreads = 256;
 constexpr uint32_t kTerminalRows = 32;  // rows per terminal work item (host side too)
 constexpr uint32_t kPrepThreads = 256;
 constexpr uint32_t kPrepStage = 8192;  // floats of input staged per prepare block
 constexpr uint32_t kMaxLdsBytes = 160 * 1024;
-constexpr uint32_t kCopyRow = 0xFFFFFFFFu;
 
 // ffa.cpp:133-143 in FP32, no contraction (the library is built with
 // -ffp-contract=off).
@@ -243,70 +242,113 @@ ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
 }
 
 // ------------------------------------------------------------- subtree ----
-// One node at the task's LDS depth: load, run every level below it in LDS,
-// store the node's transform into work[depth % 2].
+// Persistent: one workgroup per CU walks the (node, series) units of the
+// launch, unit u = (subtree item u % nitems, series u / nitems), so that the
+// units in flight together share a series' prepared data in L2.  For each
+// unit: the node's rows, prefetched into registers while the previous unit
+// computed, are written to LDS; the merge descriptors of every level are
+// built in one pass; the levels run in LDS (ping-pong), the top level writing
+// straight to work[depth % 2].  LDS = [2][half] floats + descriptor table.
+constexpr uint32_t kSubtreePrefetch = 20;  // floats per thread: node <= 20480 floats
+constexpr uint32_t kCopyTail = 0xFFFFu;    // descriptor tail of a copied row
+
+// Packed merge descriptor: x = head | tail << 16 (tail kCopyTail = copy the
+// row), y = roll.
+__device__ __forceinline__ uint2 pack_desc(Node nd, uint32_t r, uint32_t b) {
+  if (nd.size < 2) return make_uint2(r | (kCopyTail << 16), 0u);
+  const Desc d = merge_desc(nd, r, b);
+  return make_uint2(d.h | (d.t << 16), d.roll);
+}
+
+__device__ __forceinline__ uint32_t tree_levels(uint32_t nrows) {
+  uint32_t levels = 0;
+  for (uint32_t sz = nrows; sz > 1; sz = sz - sz / 2) ++levels;  // ceil(log2(nrows))
+  return levels;
+}
+
 __global__ void __launch_bounds__(kSubtreeThreads)
-ffa_subtree_kernel(KArgs a) {
+ffa_subtree_kernel(KArgs a, uint32_t nunits) {
   extern __shared__ float lds[];
-  __shared__ uint32_t sh[kSubtreeMaxRows], st[kSubtreeMaxRows], sr[kSubtreeMaxRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
-  const gaffa_hip_ffa_item item = plan.subtree_items[blockIdx.x];
-  const gaffa_hip_ffa_task task = plan.tasks[item.task];
-  const gaffa_hip_ffa_group group = plan.groups[task.group];
-  const uint64_t sidx = blockIdx.y;
-  const uint32_t b = task.bins;
-  const uint32_t nrows = item.b;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = tid >> 6;
   constexpr uint32_t kWaves = kSubtreeThreads / 64;
+  const uint32_t half = plan.subtree_half_floats;
+  uint2* desc = reinterpret_cast<uint2*>(lds + 2 * static_cast<size_t>(half));
+  const uint32_t nitems = plan.nsubtree_items;
 
-  float* cur = lds;
-  float* nxt = lds + static_cast<size_t>(nrows) * b;
-  const float* in = task_input(a, group, sidx) + static_cast<uint64_t>(item.a) * b;
-  copy_span<8>(cur, in, nrows * b, tid, kSubtreeThreads);
+  float pf[kSubtreePrefetch];
+  auto fetch = [&](uint32_t u) {
+    const gaffa_hip_ffa_item it = plan.subtree_items[u % nitems];
+    const gaffa_hip_ffa_task& t = plan.tasks[it.task];
+    const float* in = task_input(a, plan.groups[t.group], u / nitems) +
+                      static_cast<uint64_t>(it.a) * t.bins;
+    const uint32_t total = it.b * t.bins;  // >= 1
+#pragma unroll
+    for (uint32_t k = 0; k < kSubtreePrefetch; ++k)
+      pf[k] = in[min(tid + k * kSubtreeThreads, total - 1)];
+  };
 
-  uint32_t levels = 0;
-  for (uint32_t sz = nrows; sz > 1; sz = sz - sz / 2) ++levels;  // ceil(log2(nrows))
-  for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 0; --d) {
-    for (uint32_t r = tid; r < nrows; r += kSubtreeThreads) {
-      const Node nd = find_node(0, nrows, static_cast<uint32_t>(d), r);
-      // a leaf above depth d is copied: merge row r with itself minus zero is
-      // not exact, so a copy is flagged explicitly
-      const Desc ds = nd.size < 2 ? Desc{r, kCopyRow, 0} : merge_desc(nd, r, b);
-      sh[r] = ds.h;
-      st[r] = ds.t;
-      sr[r] = ds.roll;
+  uint32_t u = blockIdx.x;
+  if (u >= nunits) return;
+  fetch(u);
+  for (; u < nunits; u += gridDim.x) {
+    const gaffa_hip_ffa_item item = plan.subtree_items[u % nitems];
+    const gaffa_hip_ffa_task& task = plan.tasks[item.task];
+    const uint64_t sidx = u / nitems;
+    const uint32_t b = task.bins;
+    const uint32_t nrows = item.b;
+    const uint32_t total = nrows * b;
+    float* cur = lds;
+    float* nxt = lds + half;
+#pragma unroll
+    for (uint32_t k = 0; k < kSubtreePrefetch; ++k) {
+      const uint32_t e = tid + k * kSubtreeThreads;
+      if (e < total) cur[e] = pf[k];
     }
-    __syncthreads();  // descriptors ready; previous level's rows complete
-    for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
-      merge_batch<2>(
-          r0, kWaves, nrows, b, lane,
-          [&](uint32_t r, const float*& h, const float*& t, uint32_t& roll) {
-            const uint32_t tr = st[r];
-            h = cur + static_cast<size_t>(sh[r]) * b;
-            if (tr == kCopyRow) {
-              // copy rows read the row twice and keep the head value
-              t = h;
-              roll = 0;
-            } else {
-              t = cur + static_cast<size_t>(tr) * b;
-              roll = sr[r];
-            }
-          },
-          [&](uint32_t r, uint32_t p, float v) {
-            if (st[r] == kCopyRow) v = cur[static_cast<size_t>(r) * b + p];
-            nxt[static_cast<size_t>(r) * b + p] = v;
-          });
+    const uint32_t levels = tree_levels(nrows);
+    for (uint32_t e = tid; e < levels * nrows; e += kSubtreeThreads) {
+      const uint32_t d = e / nrows, r = e - d * nrows;
+      desc[e] = pack_desc(find_node(0, nrows, d, r), r, b);
     }
     __syncthreads();
-    float* tmp = cur;
-    cur = nxt;
-    nxt = tmp;
+    // the next unit's rows travel while this one computes
+    if (u + gridDim.x < nunits) fetch(u + gridDim.x);
+
+    float* out = a.work + static_cast<uint64_t>(task.depth & 1u) * a.nseries * plan.work_stride +
+                 sidx * plan.work_stride + task.work_offset + static_cast<uint64_t>(item.a) * b;
+    if (levels == 0) {
+      for (uint32_t e = tid; e < total; e += kSubtreeThreads) out[e] = cur[e];
+    }
+    for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 0; --d) {
+      const uint2* dd = desc + static_cast<uint32_t>(d) * nrows;
+      float* dst = d == 0 ? out : nxt;
+      for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
+        merge_batch<2>(
+            r0, kWaves, nrows, b, lane,
+            [&](uint32_t r, const float*& h, const float*& t, uint32_t& roll) {
+              const uint2 ds = dd[r];
+              const uint32_t tr = ds.x >> 16;
+              h = cur + static_cast<size_t>(ds.x & 0xFFFFu) * b;
+              // copied rows read their own row twice; the store keeps it
+              t = tr == kCopyTail ? h : cur + static_cast<size_t>(tr) * b;
+              roll = tr == kCopyTail ? 0u : ds.y;
+            },
+            [&](uint32_t r, uint32_t p, float v) {
+              if ((dd[r].x >> 16) == kCopyTail) v = cur[static_cast<size_t>(r) * b + p];
+              dst[static_cast<size_t>(r) * b + p] = v;
+            });
+      }
+      if (d > 0) {
+        __syncthreads();
+        float* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+      }
+    }
+    __syncthreads();  // LDS rows and descriptors are rewritten by the next unit
   }
-  float* out = a.work + static_cast<uint64_t>(task.depth & 1u) * a.nseries * plan.work_stride +
-               sidx * plan.work_stride + task.work_offset + static_cast<uint64_t>(item.a) * b;
-  copy_span<8>(out, cur, nrows * b, tid, kSubtreeThreads);
 }
 
 // --------------------------------------------------------------- level ----
@@ -391,7 +433,9 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.terminal_items[blockIdx.x];
   if (!kDetect && item.task != only_task) return;
-  const gaffa_hip_ffa_task task = plan.tasks[item.task];
+  // by reference: the per-width tables are indexed at run time, and a
+  // private copy of the 240-byte descriptor would live in scratch memory
+  const gaffa_hip_ffa_task& task = plan.tasks[item.task];
   const gaffa_hip_ffa_group group = plan.groups[task.group];
   const uint64_t sidx = blockIdx.y;
   const uint32_t b = task.bins;
@@ -500,6 +544,11 @@ void validate(const gaffa_hip_ffa_search_args* a) {
   check(p.max_bins > 1 && p.max_bins < 65536, "FFA bins must be in (1, 65536)");
   check(p.terminal_lds_bytes <= kMaxLdsBytes, "FFA detection tile exceeds LDS");
   check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");
+  check(p.nsubtree_items == 0 ||
+            (p.subtree_half_floats > 0 &&
+             p.subtree_half_floats <= kSubtreePrefetch * kSubtreeThreads &&
+             8ull * p.subtree_half_floats + 8ull * p.max_subtree_desc <= p.subtree_lds_bytes),
+        "FFA subtree geometry inconsistent");
 }
 
 }  // namespace
@@ -508,6 +557,21 @@ void validate(const gaffa_hip_ffa_search_args* a) {
 using namespace gaffa_hip;
 
 namespace {
+
+int compute_units() {
+  // per device, queried once
+  static int cus[64] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  check(dev >= 0 && dev < 64, "device ordinal out of range");
+  if (cus[dev] == 0) {
+    int n = 0;
+    hip_check(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev),
+              "hipDeviceGetAttribute");
+    cus[dev] = n > 0 ? n : 1;
+  }
+  return cus[dev];
+}
 
 void launch_prepare(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   const gaffa_hip_ffa_plan_desc& p = *a->plan;
@@ -521,7 +585,7 @@ void launch_prepare(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   launch_check("ffa_prepare_kernel");
 }
 
-void launch_transform(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
+void launch_subtrees(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   const gaffa_hip_ffa_plan_desc& p = *a->plan;
   const uint32_t ns = static_cast<uint32_t>(a->nseries);
   if (p.nsubtree_items > 0) {
@@ -529,10 +593,20 @@ void launch_transform(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(p.subtree_lds_bytes)),
               "hipFuncSetAttribute");
-    hipLaunchKernelGGL(ffa_subtree_kernel, dim3(p.nsubtree_items, ns), dim3(kSubtreeThreads),
-                       p.subtree_lds_bytes, st, kargs(a));
+    const uint64_t nunits = static_cast<uint64_t>(p.nsubtree_items) * ns;
+    check_overflow(nunits < (1ull << 32), "FFA subtree work too large");
+    // one resident workgroup per CU (the LDS footprint allows no more)
+    const uint32_t grid = static_cast<uint32_t>(
+        std::min<uint64_t>(nunits, static_cast<uint64_t>(compute_units())));
+    hipLaunchKernelGGL(ffa_subtree_kernel, dim3(grid), dim3(kSubtreeThreads),
+                       p.subtree_lds_bytes, st, kargs(a), static_cast<uint32_t>(nunits));
     launch_check("ffa_subtree_kernel");
   }
+}
+
+void launch_levels(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
+  const gaffa_hip_ffa_plan_desc& p = *a->plan;
+  const uint32_t ns = static_cast<uint32_t>(a->nseries);
   for (int32_t d = static_cast<int32_t>(p.max_depth) - 1; d >= 1; --d) {
     const uint32_t lo = p.level_item_offsets[d];
     const uint32_t hi = p.level_item_offsets[d + 1];
@@ -541,6 +615,17 @@ void launch_transform(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
                        lo);
     launch_check("ffa_level_kernel");
   }
+}
+
+void launch_transform(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
+  launch_subtrees(a, st);
+  launch_levels(a, st);
+}
+
+void validate_search(const gaffa_hip_ffa_search_args* a) {
+  validate(a);
+  check(a->peaks != nullptr && a->peak_count != nullptr, "FFA peak buffers must not be null");
+  check(std::isfinite(a->snr_threshold), "FFA search S/N threshold must be finite");
 }
 
 template <bool kDetect>
@@ -570,13 +655,32 @@ int gaffa_hip_ffa_prepare_batch(const gaffa_hip_ffa_search_args* a, gaffa_hip_st
 
 int gaffa_hip_ffa_search_batch(const gaffa_hip_ffa_search_args* a, gaffa_hip_stream stream) {
   return guarded([&] {
-    validate(a);
-    check(a->peaks != nullptr && a->peak_count != nullptr, "FFA peak buffers must not be null");
-    check(std::isfinite(a->snr_threshold), "FFA search S/N threshold must be finite");
+    validate_search(a);
     const hipStream_t st = as_stream(stream);
     launch_prepare(a, st);
     launch_transform(a, st);
     launch_terminal<true>(a, nullptr, 0, st);
+  });
+}
+
+int gaffa_hip_ffa_subtree_batch(const gaffa_hip_ffa_search_args* a, gaffa_hip_stream stream) {
+  return guarded([&] {
+    validate(a);
+    launch_subtrees(a, as_stream(stream));
+  });
+}
+
+int gaffa_hip_ffa_levels_batch(const gaffa_hip_ffa_search_args* a, gaffa_hip_stream stream) {
+  return guarded([&] {
+    validate(a);
+    launch_levels(a, as_stream(stream));
+  });
+}
+
+int gaffa_hip_ffa_detect_batch(const gaffa_hip_ffa_search_args* a, gaffa_hip_stream stream) {
+  return guarded([&] {
+    validate_search(a);
+    launch_terminal<true>(a, nullptr, 0, as_stream(stream));
   });
 }
 
@@ -605,7 +709,13 @@ uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {
 }
 
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins) {
-  return rows <= kSubtreeMaxRows ? 2u * rows * bins * 4u : 0xFFFFFFFFu;
+  // two row buffers + the descriptor table of every level
+  if (rows == 0 || rows > kSubtreeMaxRows ||
+      static_cast<uint64_t>(rows) * bins > kSubtreePrefetch * kSubtreeThreads)
+    return 0xFFFFFFFFu;
+  uint32_t levels = 0;
+  for (uint32_t sz = rows; sz > 1; sz = sz - sz / 2) ++levels;
+  return 8u * rows * bins + 8u * levels * rows;
 }
 
 }  // extern "C"

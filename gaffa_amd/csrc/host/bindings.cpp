@@ -385,6 +385,12 @@ py::dict stage_dict(const StageTimes& t) {
   d["preprocess_ms"] = t.preprocess_ms;
   d["ffa_ms"] = t.ffa_ms;
   d["total_ms"] = t.total_ms;
+  if (t.ffa_tiles != 0) {
+    static constexpr const char* names[FfaStageMarks::kStages] = {
+        "ffa_prepare_ms", "ffa_subtree_ms", "ffa_levels_ms", "ffa_detect_ms"};
+    for (int k = 0; k < FfaStageMarks::kStages; ++k) d[names[k]] = t.ffa_stage_ms[k];
+    d["ffa_tiles"] = t.ffa_tiles;
+  }
   return d;
 }
 
@@ -653,6 +659,8 @@ PYBIND11_MODULE(_core, m) {
         py::gil_scoped_release release;
         p.search();
       })
+      .def("set_kernel_timing", &GpuSearchPipeline::set_kernel_timing, py::arg("on"),
+           "Record HIP events between the FFA launch stages of every tile")
       .def("run", [](GpuSearchPipeline& p) {
         StageTimes t;
         {
@@ -689,5 +697,28 @@ PYBIND11_MODULE(_core, m) {
       .def_property_readonly("peaks", &GpuSearchPipeline::peaks)
       .def_property_readonly("ffa_tile", [](const GpuSearchPipeline& p) {
         return p.ffa_program().tile_capacity();
-      });
+      })
+      .def_property_readonly(
+          "ffa_layout",
+          [](const GpuSearchPipeline& p) {
+            const GpuFfaExecutionPlan& e = p.ffa_program().execution_plan();
+            std::vector<std::uint32_t> depth;
+            for (const auto& t : e.tasks()) depth.push_back(t.depth);
+            std::size_t level_launches = 0;
+            const auto& off = e.level_offsets();
+            for (std::size_t d = 1; d + 1 < off.size(); ++d)
+              if (off[d + 1] > off[d]) ++level_launches;
+            py::dict d;
+            d["task_depth"] = owning_array(std::move(depth),
+                                           {static_cast<py::ssize_t>(e.tasks().size())});
+            d["max_depth"] = e.max_depth();
+            d["level_launches"] = level_launches;
+            d["subtree_items"] = e.subtree_items().size();
+            d["level_items"] = e.level_items().size();
+            d["terminal_items"] = e.terminal_items().size();
+            d["subtree_lds_bytes"] = e.subtree_lds_bytes();
+            d["terminal_lds_bytes"] = e.terminal_lds_bytes();
+            return d;
+          },
+          "Device layout of the FFA plan: LDS depth per task, launches and items");
 }

@@ -146,18 +146,7 @@ SubbandTables make_subband_tables(std::span<const double> f, const MultiDmDedisp
     for (std::size_t c = 0; c < nc; ++c)
       t.coarse[k * nc + c] =
           dedispersion_delay_bins(nominal, f[p.chan_begin + c], p.ref_frequency_mhz, p.tsamp);
-    for (std::size_t s = 0; s < t.nsub; ++s) {
-      const std::size_t cs = s * o.subband_channels;
-      const std::size_t ce = std::min(cs + o.subband_channels, nc);
-      for (std::size_t c0 = cs; c0 < ce; c0 += 32) {
-        std::int32_t lo = std::numeric_limits<std::int32_t>::max(), hi = 0;
-        for (std::size_t c = c0; c < std::min(ce, c0 + 32); ++c) {
-          lo = std::min(lo, t.coarse[k * nc + c]);
-          hi = std::max(hi, t.coarse[k * nc + c]);
-        }
-        t.max_coarse_spread = std::max(t.max_coarse_spread, hi - lo);
-      }
-    }
+    for (std::size_t c = 0; c < nc; ++c) t.max_coarse = std::max(t.max_coarse, t.coarse[k * nc + c]);
   }
   t.residual.resize(p.ndm * t.nsub);
   for (std::size_t i = 0; i < p.ndm; ++i) {
@@ -206,15 +195,42 @@ void dedisperse_multi_dm_gpu_into(GpuSampleView samples, std::span<const double>
   hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_multi_u8");
 }
 
-void dedisperse_subband_gpu_into(GpuSampleView samples, std::span<const double> f,
-                                 const MultiDmDedispersionPlan& p,
-                                 const SubbandDedispersionOptions& so, std::uint32_t* out,
-                                 std::size_t out_stride, const GpuDedispersionOptions& o,
-    std::size_t out_nsamples) {
-  validate_multi(samples.shape, samples.data != nullptr, f, p);
+std::size_t subband_channel_stride(std::size_t nsamples, std::span<const double> f,
+                                   const MultiDmDedispersionPlan& p,
+                                   const SubbandDedispersionOptions& so) {
+  const SubbandTables t = make_subband_tables(f, p, so);
+  return static_cast<std::size_t>(
+      gaffa_hip_subband_channel_stride(nsamples, t.max_coarse, t.max_residual));
+}
+
+void transpose_filterbank_gpu_into(GpuSampleView samples, std::size_t chan_begin,
+                                   std::size_t chan_end, std::uint8_t* out, std::size_t stride,
+                                   const GpuDedispersionOptions& o) {
+  check_arg(samples.data != nullptr && out != nullptr, "transpose pointers must not be null");
+  check_arg(samples.shape.nifs == 1, "GPU dedispersion supports nifs == 1");
+  GpuDeviceScope scope(o.device_id);
+  gaffa_hip_transpose_args a{};
+  a.samples = samples.data;
+  a.nsamples = samples.shape.nsamples;
+  a.nchans = samples.shape.nchans;
+  a.chan_begin = chan_begin;
+  a.chan_end = chan_end;
+  a.out = out;
+  a.out_stride = stride;
+  hip_ok(gaffa_hip_filterbank_transpose_u8(&a, o.stream), "filterbank_transpose_u8");
+}
+
+void dedisperse_subband_gpu_cm_into(const GpuChannelMajorView& cm, std::span<const double> f,
+                                    const MultiDmDedispersionPlan& p,
+                                    const SubbandDedispersionOptions& so, std::uint32_t* out,
+                                    std::size_t out_stride, const GpuDedispersionOptions& o,
+                                    std::size_t out_nsamples) {
+  validate_multi(SampleShape{cm.nsamples, 1, f.size()}, cm.data != nullptr, f, p);
+  check_arg(p.chan_begin >= cm.chan_begin && p.chan_end <= cm.chan_begin + cm.nchans,
+            "plan channel range lies outside the channel-major copy");
   check_arg(so.subband_channels != 0, "subband_channels must be positive");
   check_arg(so.ndm_per_nominal != 0, "ndm_per_nominal must be positive");
-  std::size_t n_out = multi_dm_output_nsamples(samples.shape.nsamples, f, p);
+  std::size_t n_out = multi_dm_output_nsamples(cm.nsamples, f, p);
   if (out_nsamples != 0) {
     check_arg(out_nsamples <= n_out, "requested output length exceeds the valid range");
     n_out = out_nsamples;
@@ -226,11 +242,10 @@ void dedisperse_subband_gpu_into(GpuSampleView samples, std::span<const double> 
   GpuDeviceBuffer<std::int32_t> d_coarse = upload(t.coarse.data(), t.coarse.size());
   GpuDeviceBuffer<std::int32_t> d_resid = upload(t.residual.data(), t.residual.size());
   gaffa_hip_dedisperse_subband_args a{};
-  a.samples = samples.data;
-  a.nsamples = samples.shape.nsamples;
-  a.nchans = samples.shape.nchans;
-  a.chan_begin = p.chan_begin;
-  a.chan_end = p.chan_end;
+  a.channels = cm.data + (p.chan_begin - cm.chan_begin) * cm.stride;
+  a.channel_stride = cm.stride;
+  a.nsamples = cm.nsamples;
+  a.nsel = p.chan_end - p.chan_begin;
   a.subband_channels = so.subband_channels;
   a.ndm_per_nominal = so.ndm_per_nominal;
   a.ndm = p.ndm;
@@ -240,9 +255,25 @@ void dedisperse_subband_gpu_into(GpuSampleView samples, std::span<const double> 
   a.out_nsamples = n_out;
   a.out_stride = out_stride;
   a.max_residual = t.max_residual;
-  a.max_coarse_spread = t.max_coarse_spread;
+  a.max_coarse = t.max_coarse;
   hip_ok(gaffa_hip_dedisperse_subband_u8(&a, o.stream), "dedisperse_subband_u8");
   hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_subband_u8");
+}
+
+void dedisperse_subband_gpu_into(GpuSampleView samples, std::span<const double> f,
+                                 const MultiDmDedispersionPlan& p,
+                                 const SubbandDedispersionOptions& so, std::uint32_t* out,
+                                 std::size_t out_stride, const GpuDedispersionOptions& o,
+                                 std::size_t out_nsamples) {
+  validate_multi(samples.shape, samples.data != nullptr, f, p);
+  const std::size_t stride = subband_channel_stride(samples.shape.nsamples, f, p, so);
+  GpuDeviceScope scope(o.device_id);
+  GpuDeviceBuffer<std::uint8_t> cm((p.chan_end - p.chan_begin) * stride);
+  transpose_filterbank_gpu_into(samples, p.chan_begin, p.chan_end, cm.data(), stride, o);
+  const GpuChannelMajorView v{cm.data(), samples.shape.nsamples, p.chan_begin,
+                              p.chan_end - p.chan_begin, stride, o.device_id};
+  // synchronises the stream (the delay-bound check), so `cm` may be freed after
+  dedisperse_subband_gpu_cm_into(v, f, p, so, out, out_stride, o, out_nsamples);
 }
 
 GpuDedispersedResult<std::uint32_t> dedisperse_multi_dm_gpu_device(

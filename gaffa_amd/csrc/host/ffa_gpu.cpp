@@ -16,8 +16,15 @@ namespace gaffa {
 namespace {
 
 constexpr std::size_t kAlign = 64;               // floats
-constexpr std::size_t kSubtreeLdsFloats = 38912;  // 152 KiB: two ping-pong buffers
-constexpr std::size_t kSubtreeMaxRows = 512;      // kernel's descriptor table
+constexpr std::size_t kSubtreeNodeFloats = 19456;  // per ping-pong buffer (2 x 76 KiB)
+constexpr std::size_t kSubtreeMaxDesc = 1024;      // levels x rows descriptors (8 KiB)
+constexpr std::size_t kSubtreeMaxRows = 512;
+
+std::size_t tree_levels(std::size_t rows) {
+  std::size_t levels = 0;
+  for (std::size_t sz = rows; sz > 1; sz = sz - sz / 2) ++levels;
+  return levels;
+}
 
 std::size_t align_up(std::size_t v) { return (v + kAlign - 1) / kAlign * kAlign; }
 
@@ -119,7 +126,8 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
     terminal_lds_bytes_ = std::max(
         terminal_lds_bytes_,
         gaffa_hip_ffa_terminal_lds_bytes(d.bins, static_cast<std::uint32_t>(det.max_width)));
-    const std::size_t lds_rows = std::min(kSubtreeMaxRows, kSubtreeLdsFloats / (2 * t.bins));
+    std::size_t lds_rows = std::min(kSubtreeMaxRows, kSubtreeNodeFloats / t.bins);
+    while (lds_rows > 2 && tree_levels(lds_rows) * lds_rows > kSubtreeMaxDesc) --lds_rows;
     check_arg(lds_rows >= 2, "GPU FFA bins too large for an LDS subtree");
     if (t.rows >= 2) {
       std::uint32_t depth = 1;
@@ -131,9 +139,13 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
       const std::size_t before = subtree_.size();
       collect_nodes(0, d.rows, 0, depth, task_index, subtree_);
       for (std::size_t i = before; i < subtree_.size(); ++i) {
-        subtree_lds_bytes_ = std::max<std::uint32_t>(
-            subtree_lds_bytes_, gaffa_hip_ffa_subtree_lds_bytes(subtree_[i].b, d.bins));
+        check_arg(gaffa_hip_ffa_subtree_lds_bytes(subtree_[i].b, d.bins) != 0xFFFFFFFFu,
+                  "GPU FFA subtree node exceeds the kernel limits");
+        subtree_half_ = std::max<std::uint32_t>(subtree_half_, subtree_[i].b * d.bins);
+        subtree_desc_ = std::max<std::uint32_t>(
+            subtree_desc_, static_cast<std::uint32_t>(tree_levels(subtree_[i].b) * subtree_[i].b));
       }
+      subtree_lds_bytes_ = 8u * subtree_half_ + 8u * subtree_desc_;
       if (by_depth.size() < depth) by_depth.resize(depth);
       for (std::uint32_t l = 1; l < depth; ++l)
         for (std::uint32_t r = 0; r < d.rows; r += level_rows)
@@ -201,6 +213,8 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.nterminal_items = static_cast<std::uint32_t>(exec_plan_.terminal_items().size());
   desc_.max_bins = exec_plan_.max_bins();
   desc_.subtree_lds_bytes = exec_plan_.subtree_lds_bytes();
+  desc_.subtree_half_floats = exec_plan_.subtree_half_floats();
+  desc_.max_subtree_desc = exec_plan_.max_subtree_desc();
   desc_.terminal_lds_bytes = exec_plan_.terminal_lds_bytes();
   desc_.prep_blocks = exec_plan_.prep_blocks();
   desc_.max_prepared_nsamples = exec_plan_.max_prepared();
@@ -232,7 +246,8 @@ gaffa_hip_ffa_search_args GpuFfaProgram::args(const float* series, std::size_t n
 }
 
 void GpuFfaProgram::enqueue_tile(const float* series, std::size_t nseries, std::size_t stride,
-                                 std::uint32_t base, float thr, bool reset_count) {
+                                 std::uint32_t base, float thr, bool reset_count,
+                                 const FfaStageMarks* marks) {
   check_arg(nseries > 0 && nseries <= tile_, "FFA tile exceeds the program tile capacity");
   GpuDeviceScope scope(program_.device_id);
   if (reset_count) {
@@ -240,7 +255,19 @@ void GpuFfaProgram::enqueue_tile(const float* series, std::size_t nseries, std::
            "hipMemset");
   }
   const gaffa_hip_ffa_search_args a = args(series, nseries, stride, base, thr);
-  hip_ok(gaffa_hip_ffa_search_batch(&a, exec_.stream), "ffa_search_batch");
+  if (marks == nullptr) {
+    hip_ok(gaffa_hip_ffa_search_batch(&a, exec_.stream), "ffa_search_batch");
+    return;
+  }
+  using Stage = int (*)(const gaffa_hip_ffa_search_args*, gaffa_hip_stream);
+  static constexpr Stage stages[FfaStageMarks::kStages] = {
+      gaffa_hip_ffa_prepare_batch, gaffa_hip_ffa_subtree_batch, gaffa_hip_ffa_levels_batch,
+      gaffa_hip_ffa_detect_batch};
+  hip_ok(gaffa_hip_event_record(marks->at[0], exec_.stream), "hipEventRecord");
+  for (int k = 0; k < FfaStageMarks::kStages; ++k) {
+    hip_ok(stages[k](&a, exec_.stream), "ffa_search_batch");
+    hip_ok(gaffa_hip_event_record(marks->at[k + 1], exec_.stream), "hipEventRecord");
+  }
 }
 
 std::uint32_t GpuFfaProgram::read_peak_count() {

@@ -102,7 +102,7 @@ struct SubbandTables {
   std::vector<std::int32_t> coarse;    // [n_nominal][nchan_sel]
   std::vector<std::int32_t> residual;  // [ndm][nsub]
   std::int32_t max_residual = 0;
-  std::int32_t max_coarse_spread = 0;  // within any (nominal, <=32-channel chunk of a subband)
+  std::int32_t max_coarse = 0;
 };
 
 SubbandTables make_subband_tables(std::span<const double> frequency_mhz,
@@ -162,6 +162,39 @@ GpuDedispersedResult<std::uint32_t> dedisperse_single_dm_gpu_device(
 // Writes into caller storage: ndm rows of `out_stride` elements, the first
 // `out_nsamples` of each (0 = the plan's own valid length; a smaller value
 // truncates, as a multi-segment DM plan needs a common length).
+// Channel-major device copy of a filterbank's channels [chan_begin,
+// chan_begin + nchans): row c holds channel chan_begin + c over time, zero
+// padded from nsamples to stride (the layout subband dedispersion reads).
+struct GpuChannelMajorView {
+  const std::uint8_t* data = nullptr;
+  std::size_t nsamples = 0;
+  std::size_t chan_begin = 0;
+  std::size_t nchans = 0;
+  std::size_t stride = 0;
+  int device_id = 0;
+};
+
+// Row length a channel-major copy needs for subband dedispersion of `plan`.
+std::size_t subband_channel_stride(std::size_t nsamples, std::span<const double> frequency_mhz,
+                                   const MultiDmDedispersionPlan& plan,
+                                   const SubbandDedispersionOptions& subband_options);
+
+// Transposes channels [chan_begin, chan_end) of `samples` into `out`
+// ([chan_end - chan_begin][stride] bytes), enqueued on options.stream.
+void transpose_filterbank_gpu_into(GpuSampleView samples, std::size_t chan_begin,
+                                   std::size_t chan_end, std::uint8_t* out, std::size_t stride,
+                                   const GpuDedispersionOptions& options);
+
+// Subband dedispersion from a channel-major copy (the plan's channel range
+// must lie inside the copy's).
+void dedisperse_subband_gpu_cm_into(const GpuChannelMajorView& channels,
+                                    std::span<const double> frequency_mhz,
+                                    const MultiDmDedispersionPlan& plan,
+                                    const SubbandDedispersionOptions& subband_options,
+                                    std::uint32_t* out, std::size_t out_stride,
+                                    const GpuDedispersionOptions& options,
+                                    std::size_t out_nsamples = 0);
+
 void dedisperse_subband_gpu_into(GpuSampleView samples, std::span<const double> frequency_mhz,
                                  const MultiDmDedispersionPlan& plan,
                                  const SubbandDedispersionOptions& subband_options,

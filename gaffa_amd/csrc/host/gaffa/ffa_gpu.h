@@ -39,6 +39,12 @@ struct FfaBatchSearchResult {
   std::vector<FfaBatchPeak> peaks;
 };
 
+// Events bracketing the four launch stages of one FFA tile.
+struct FfaStageMarks {
+  static constexpr int kStages = 4;  // prepare, subtree, levels, detect
+  gaffa_hip_event at[kStages + 1] = {};
+};
+
 // Host-built device layout of one FfaSearchPlan: prepare groups keyed by
 // downsample factor, per-task LDS depth, and the work-item lists of the
 // subtree / level / terminal launches (see gaffa_hip.h).
@@ -58,6 +64,8 @@ class GpuFfaExecutionPlan {
   const std::vector<std::uint32_t>& level_offsets() const noexcept { return level_offsets_; }
   const std::vector<gaffa_hip_ffa_item>& terminal_items() const noexcept { return terminal_; }
   std::uint32_t subtree_lds_bytes() const noexcept { return subtree_lds_bytes_; }
+  std::uint32_t subtree_half_floats() const noexcept { return subtree_half_; }
+  std::uint32_t max_subtree_desc() const noexcept { return subtree_desc_; }
   std::uint32_t terminal_lds_bytes() const noexcept { return terminal_lds_bytes_; }
   std::uint32_t prep_blocks() const noexcept { return prep_blocks_; }
   std::uint32_t max_bins() const noexcept { return max_bins_; }
@@ -73,6 +81,8 @@ class GpuFfaExecutionPlan {
   std::size_t max_depth_ = 0;
   std::size_t max_prepared_ = 0;
   std::uint32_t subtree_lds_bytes_ = 0;
+  std::uint32_t subtree_half_ = 0;
+  std::uint32_t subtree_desc_ = 0;
   std::uint32_t terminal_lds_bytes_ = 0;
   std::uint32_t prep_blocks_ = 0;
   std::uint32_t max_bins_ = 0;
@@ -100,8 +110,12 @@ class GpuFfaProgram {
   // Enqueue the whole search of one tile (<= tile_capacity series) without
   // waiting; peaks land in the device peak buffer.  Used by the benchmark
   // and the graph path; run_ffa_batch_gpu() wraps it with the host side.
+  // With `marks`, its five events are recorded on the program stream before
+  // the prepare launch and after the prepare, subtree, level and detection
+  // stages.
   void enqueue_tile(const float* series, std::size_t nseries, std::size_t stride,
-                    std::uint32_t series_index_base, float snr_threshold, bool reset_count);
+                    std::uint32_t series_index_base, float snr_threshold, bool reset_count,
+                    const FfaStageMarks* marks = nullptr);
   std::uint32_t read_peak_count();
   std::vector<gaffa_hip_ffa_peak> read_peaks(std::uint32_t count);
   std::size_t peak_capacity() const noexcept { return peaks_.size(); }

@@ -49,6 +49,10 @@ struct StageTimes {
   float preprocess_ms = 0.0F;
   float ffa_ms = 0.0F;
   float total_ms = 0.0F;
+  // With kernel timing on: each FFA launch stage summed over the tiles of
+  // the last pass (prepare, subtree, levels, detect), and the tile count.
+  float ffa_stage_ms[FfaStageMarks::kStages] = {};
+  std::uint32_t ffa_tiles = 0;
 };
 
 class GpuSearchPipeline {
@@ -67,6 +71,9 @@ class GpuSearchPipeline {
   void preprocess();
   void search();
   StageTimes run();  // all three, timed with HIP events
+  // Record events between the FFA launch stages of every tile (off by
+  // default: the marks split one enqueue into four).
+  void set_kernel_timing(bool on) noexcept { kernel_timing_ = on; }
 
   std::size_t ndm() const noexcept { return ndm_; }
   std::size_t out_nsamples() const noexcept { return n_out_; }
@@ -90,12 +97,17 @@ class GpuSearchPipeline {
   std::vector<double> dms_;
   std::vector<MultiDmDedispersionPlan> segment_plans_;
   GpuDeviceBuffer<std::uint8_t> filterbank_;
+  GpuDeviceBuffer<std::uint8_t> channels_;  // channel-major copy (subband path)
+  std::size_t channel_stride_ = 0;
   GpuDeviceBuffer<std::uint32_t> plane_;
   FfaSearchPlan ffa_plan_;
   std::unique_ptr<GpuPreprocessProgram> prep_;
   std::unique_ptr<GpuFfaProgram> ffa_;
   std::vector<DmPeak> peaks_;
   std::vector<gaffa_hip_event> events_;
+  bool kernel_timing_ = false;
+  std::vector<FfaStageMarks> marks_;  // one per tile, events created on demand
+  std::size_t marks_used_ = 0;
 };
 
 }  // namespace gaffa

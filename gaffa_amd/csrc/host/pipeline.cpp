@@ -35,6 +35,13 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   }
   stride_ = (n_out_ + 63) / 64 * 64;
   filterbank_ = GpuDeviceBuffer<std::uint8_t>(cfg_.nsamples * cfg_.nchans);
+  if (cfg_.subband) {
+    for (const MultiDmDedispersionPlan& p : segment_plans_)
+      channel_stride_ = std::max(channel_stride_,
+                                 subband_channel_stride(cfg_.nsamples, cfg_.frequency_mhz, p,
+                                                        cfg_.subband_options));
+    channels_ = GpuDeviceBuffer<std::uint8_t>(cfg_.nchans * channel_stride_);
+  }
   plane_ = GpuDeviceBuffer<std::uint32_t>(ndm_ * stride_);
   ffa_plan_ = make_riptide_ffa_plan(n_out_, cfg_.tsamp, cfg_.ffa);
   if (!cfg_.preprocess.steps.empty()) {
@@ -60,6 +67,9 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
 GpuSearchPipeline::~GpuSearchPipeline() {
   for (auto e : events_)
     if (e != nullptr) (void)gaffa_hip_event_destroy(e);
+  for (auto& m : marks_)
+    for (auto e : m.at)
+      if (e != nullptr) (void)gaffa_hip_event_destroy(e);
 }
 
 void GpuSearchPipeline::generate(const SynthFilterbankSpec& spec) {
@@ -87,12 +97,18 @@ void GpuSearchPipeline::dedisperse() {
   GpuDedispersionOptions o;
   o.device_id = cfg_.device_id;
   o.stream = stream_.get();
+  if (cfg_.subband) {
+    // one channel-major copy serves every DM segment
+    transpose_filterbank_gpu_into(v, 0, cfg_.nchans, channels_.data(), channel_stride_, o);
+  }
+  const GpuChannelMajorView cm{channels_.data(), cfg_.nsamples, 0, cfg_.nchans, channel_stride_,
+                               cfg_.device_id};
   std::size_t row = 0;
   for (const MultiDmDedispersionPlan& p : segment_plans_) {
     std::uint32_t* out = plane_.data() + row * stride_;
     if (cfg_.subband) {
-      dedisperse_subband_gpu_into(v, cfg_.frequency_mhz, p, cfg_.subband_options, out, stride_, o,
-                                  n_out_);
+      dedisperse_subband_gpu_cm_into(cm, cfg_.frequency_mhz, p, cfg_.subband_options, out,
+                                     stride_, o, n_out_);
     } else {
       dedisperse_multi_dm_gpu_into(v, cfg_.frequency_mhz, p, out, stride_, o, n_out_);
     }
@@ -129,14 +145,24 @@ void GpuSearchPipeline::search() {
   GpuFfaProgram& prog = *ffa_;
   const float* series = reinterpret_cast<const float*>(plane_.data());
   const std::size_t tile = prog.tile_capacity();
+  const std::size_t ntiles = (ndm_ + tile - 1) / tile;
+  if (kernel_timing_) {
+    while (marks_.size() < ntiles) {
+      FfaStageMarks m;
+      for (auto& e : m.at) hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
+      marks_.push_back(m);
+    }
+  }
   std::uint32_t count = 0;
   for (;;) {
     hip_ok(gaffa_hip_event_record(events_[3], stream_.get()), "hipEventRecord");
-    for (std::size_t s0 = 0; s0 < ndm_; s0 += tile) {
+    for (std::size_t k = 0; k < ntiles; ++k) {
+      const std::size_t s0 = k * tile;
       const std::size_t ns = std::min(tile, ndm_ - s0);
       prog.enqueue_tile(series + s0 * stride_, ns, stride_, static_cast<std::uint32_t>(s0),
-                        cfg_.snr_threshold, s0 == 0);
+                        cfg_.snr_threshold, s0 == 0, kernel_timing_ ? &marks_[k] : nullptr);
     }
+    marks_used_ = kernel_timing_ ? ntiles : 0;
     hip_ok(gaffa_hip_event_record(events_[4], stream_.get()), "hipEventRecord");
     count = prog.read_peak_count();
     if (count <= prog.peak_capacity()) break;
@@ -167,6 +193,14 @@ StageTimes GpuSearchPipeline::run() {
   hip_ok(gaffa_hip_event_elapsed_ms(events_[1], events_[2], &t.preprocess_ms), "elapsed");
   hip_ok(gaffa_hip_event_elapsed_ms(events_[3], events_[4], &t.ffa_ms), "elapsed");
   hip_ok(gaffa_hip_event_elapsed_ms(events_[0], events_[4], &t.total_ms), "elapsed");
+  for (std::size_t k = 0; k < marks_used_; ++k) {
+    for (int j = 0; j < FfaStageMarks::kStages; ++j) {
+      float ms = 0.0F;
+      hip_ok(gaffa_hip_event_elapsed_ms(marks_[k].at[j], marks_[k].at[j + 1], &ms), "elapsed");
+      t.ffa_stage_ms[j] += ms;
+    }
+  }
+  t.ffa_tiles = static_cast<std::uint32_t>(marks_used_);
   return t;
 }
 

@@ -116,30 +116,53 @@ typedef struct gaffa_hip_dedisperse_multi_args {
 int gaffa_hip_dedisperse_multi_u8(const gaffa_hip_dedisperse_multi_args* args,
                                   gaffa_hip_stream stream);
 
-/* Two-stage subband dedispersion fused in LDS: stage 1 sums each subband at
- * the coarse delays of its nominal DM, stage 2 adds the subbands at the
- * residual delays; the intermediate never reaches HBM.  Replaces
- * subband_stage1_kernel + subband_stage2_kernel and their tile loop
- * (dedispersion_cuda.cu:350-417, 659-707).  nsub = ceil(nchan_sel /
- * subband_channels); n_nominal = ceil(ndm / ndm_per_nominal).
- * max_residual bounds every residual delay; max_coarse_spread bounds
- * max - min of the coarse delays inside any (nominal group, subband). */
-typedef struct gaffa_hip_dedisperse_subband_args {
-  const uint8_t* samples;
+/* Channel-major copy of a filterbank for subband dedispersion:
+ * out[c * out_stride + t] = samples[t * nchans + chan_begin + c] for
+ * t < nsamples and 0 for nsamples <= t < out_stride (the zero padding stands
+ * for gaffa's out-of-range reads).  out_stride: multiple of 16, at least
+ * gaffa_hip_subband_channel_stride(); out 16-byte aligned.  No gaffa
+ * counterpart (its kernels read the sample-major filterbank). */
+typedef struct gaffa_hip_transpose_args {
+  const uint8_t* samples; /* [nsamples][nchans] */
   uint64_t nsamples;
   uint64_t nchans;
   uint64_t chan_begin;
   uint64_t chan_end;
+  uint8_t* out;           /* [chan_end - chan_begin][out_stride] */
+  uint64_t out_stride;
+} gaffa_hip_transpose_args;
+
+int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* args,
+                                      gaffa_hip_stream stream);
+
+/* Row length (bytes) a channel-major copy needs so that every stage-1 read
+ * of gaffa_hip_dedisperse_subband_u8 stays in the row. */
+uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,
+                                          int32_t max_residual);
+
+/* Two-stage subband dedispersion fused in LDS: stage 1 sums each subband at
+ * the coarse delays of its nominal DM, reading the channel-major rows made by
+ * gaffa_hip_filterbank_transpose_u8; stage 2 adds the subbands at the residual
+ * delays; the intermediate never reaches HBM.  Replaces subband_stage1_kernel
+ * + subband_stage2_kernel and their tile loop (dedispersion_cuda.cu:350-417,
+ * 659-707).  nsub = ceil(nsel / subband_channels); n_nominal = ceil(ndm /
+ * ndm_per_nominal).  max_residual bounds every residual delay, max_coarse
+ * every coarse delay. */
+typedef struct gaffa_hip_dedisperse_subband_args {
+  const uint8_t* channels;  /* channel-major [nsel][channel_stride] */
+  uint64_t channel_stride;  /* bytes per channel row, multiple of 4 */
+  uint64_t nsamples;        /* valid samples per row */
+  uint64_t nsel;            /* channels (the selected range) */
   uint64_t subband_channels;
   uint64_t ndm_per_nominal;
   uint64_t ndm;
-  const int32_t* coarse_delays;   /* [n_nominal][chan_end - chan_begin] */
+  const int32_t* coarse_delays;   /* [n_nominal][nsel] */
   const int32_t* residual_delays; /* [ndm][nsub] */
   uint32_t* out;
   uint64_t out_nsamples;
   uint64_t out_stride;
   int32_t max_residual;
-  int32_t max_coarse_spread;
+  int32_t max_coarse;
 } gaffa_hip_dedisperse_subband_args;
 
 int gaffa_hip_dedisperse_subband_u8(
@@ -245,7 +268,9 @@ typedef struct gaffa_hip_ffa_plan_desc {
   const gaffa_hip_ffa_item* terminal_items; /* device */
   uint32_t nterminal_items;
   uint32_t max_bins;
-  uint32_t subtree_lds_bytes;  /* max over subtree items */
+  uint32_t subtree_lds_bytes;  /* 8 * subtree_half_floats + 8 * max_subtree_desc */
+  uint32_t subtree_half_floats; /* max over subtree items of rows * bins */
+  uint32_t max_subtree_desc;    /* max over subtree items of levels * rows */
   uint32_t terminal_lds_bytes; /* max over tasks of ..._terminal_lds_bytes() */
   uint32_t prep_blocks;        /* max over downsampled groups of
                                   ceil(prepared_nsamples / outputs per block) */
@@ -294,6 +319,20 @@ int gaffa_hip_ffa_prepare_batch(const gaffa_hip_ffa_search_args* args,
  * kernels (ffa_cuda_kernels.cuh:145-737) and the tile runner that launches
  * them (ffa_cuda_executor.cuh:385-531). */
 int gaffa_hip_ffa_search_batch(const gaffa_hip_ffa_search_args* args,
+                               gaffa_hip_stream stream);
+
+/* The four launch stages of gaffa_hip_ffa_search_batch, enqueued in this
+ * order on one stream: prepare_batch, subtree_batch (every level of the
+ * task trees that fits in LDS), levels_batch (the remaining levels below the
+ * root, through global memory), detect_batch (root merge + boxcar S/N +
+ * peak append).  Separate entry points let a caller time each stage with
+ * events; the sequence is equivalent to search_batch.  No gaffa counterpart
+ * (its executor launches per-level kernels itself). */
+int gaffa_hip_ffa_subtree_batch(const gaffa_hip_ffa_search_args* args,
+                                gaffa_hip_stream stream);
+int gaffa_hip_ffa_levels_batch(const gaffa_hip_ffa_search_args* args,
+                               gaffa_hip_stream stream);
+int gaffa_hip_ffa_detect_batch(const gaffa_hip_ffa_search_args* args,
                                gaffa_hip_stream stream);
 
 /* Transform only: writes the first rows_eval final rows of task `task` for

@@ -246,6 +246,26 @@ def test_ffa_transform_shapes_bit_exact(gpu, port, rows_bins):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("rows_bins", [(167, 180), (2679, 256)])
+def test_ffa_error_vs_fp64(gpu, port, rows_bins):
+    """The FP32 transform (bit-exact to the CPU's) against the FP64 FFA on
+    N(0,1) input: every output row is a sum of `rows` terms, and the error is
+    stated relative to the largest output magnitude (tolerance 1e-5)."""
+    from oracle.oracle import TASK_DTYPE
+
+    rows, bins = rows_bins
+    say(f"ffa: fp32 vs fp64 rows={rows} bins={bins}")
+    n = rows * bins
+    x = np.random.default_rng(rows).normal(size=n).astype(np.float32)
+    t = np.zeros(1, TASK_DTYPE)
+    t[0] = (1.0, 1e-3, n, n, bins, rows, rows, bins * 1e-3, (bins + 1) * 1e-3)
+    got = gpu._ffa_transform_rows_gpu(x[None, :], gpu.FfaPlan(t, [1]), 0)[0].astype(np.float64)
+    want = port.ffa_transform(x.astype(np.float64), bins)
+    rel = float(np.abs(got - want).max() / np.abs(want).max())
+    say(f"ffa: max |fp32 - fp64| / max |fp64| = {rel:.3e}")
+    assert rel < 1e-5
+
+
 def test_ffa_batch_search_bit_exact(gpu, port):
     """Several series, the small workload's plan (7 groups, 495 tasks)."""
     say("ffa: 6 series x 130870, small plan, threshold 5")
