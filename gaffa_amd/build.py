@@ -29,6 +29,7 @@ CXX = os.environ.get("CXX", "g++")
 HIP_SOURCES = ["runtime.hip", "dedisperse.hip", "preprocess.hip", "ffa.hip", "synth.hip"]
 HOST_SOURCES = ["host/dedispersion.cpp", "host/preprocessing.cpp", "host/ffa_plan.cpp",
                 "host/ffa_gpu.cpp", "host/hip_runtime.cpp", "host/synth.cpp", "host/pipeline.cpp",
+                "host/ffa_batch.cpp", "host/candidate.cpp",
                 "host/bindings.cpp"]
 
 # gfx950 only.  -ffp-contract=off keeps every FP32/FP64 operation separately

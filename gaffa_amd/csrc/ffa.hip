@@ -125,7 +125,7 @@ __device__ __forceinline__ Desc merge_desc(Node nd, uint32_t r, uint32_t bins) {
 
 __device__ __forceinline__ const float* task_input(const KArgs& a, const gaffa_hip_ffa_group& g,
                                                    uint64_t series) {
-  if (g.factor == 1.0) return a.series + series * a.series_stride;
+  if (g.factor == 1.0) return a.series + series * a.series_stride + g.input_offset;
   return a.prepared + series * a.plan.prepared_stride + g.prep_offset;
 }
 
@@ -199,7 +199,7 @@ __device__ __host__ __forceinline__ uint32_t prep_outputs_per_block(double facto
 
 __global__ void __launch_bounds__(kPrepThreads)
 ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
-                   uint64_t nsamples, const gaffa_hip_ffa_group* __restrict__ groups,
+                   const gaffa_hip_ffa_group* __restrict__ groups,
                    float* __restrict__ prepared, uint64_t prepared_stride) {
   __shared__ float stage[kPrepStage];
   const gaffa_hip_ffa_group g = groups[blockIdx.y];
@@ -209,8 +209,8 @@ ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
   if (k0 >= g.prepared_nsamples) return;
   const uint64_t k1 = min(k0 + K, g.prepared_nsamples);
   const uint64_t sidx = blockIdx.z;
-  const float* x = series + sidx * series_stride;
-  const double last_index = static_cast<double>(nsamples - 1);
+  const float* x = series + sidx * series_stride + g.input_offset;
+  const double last_index = static_cast<double>(g.input_nsamples - 1);
   // staged span: [floor(k0 f), min(floor(k1 f) + 2, n - 1)]
   const uint64_t lo = static_cast<uint64_t>(floor(static_cast<double>(k0) * g.factor));
   const uint64_t hi = static_cast<uint64_t>(
@@ -219,13 +219,9 @@ ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
   // (factor > kPrepStage - 8) reads global memory directly.
   const uint64_t span = hi - lo + 1;
   const bool staged = span <= kPrepStage;
-  const float* s = x;
-  uint64_t base = 0;
   if (staged) {
     for (uint32_t e = threadIdx.x; e < span; e += kPrepThreads) stage[e] = x[lo + e];
     __syncthreads();
-    s = stage;
-    base = lo;
   }
   const uint64_t k = k0 + threadIdx.x;
   if (k >= k1) return;
@@ -235,9 +231,13 @@ ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
   const uint64_t last = static_cast<uint64_t>(fmin(floor(end), last_index));
   const float w0 = static_cast<float>(static_cast<double>(first + 1) - start);
   const float w1 = static_cast<float>(end - static_cast<double>(last));
-  float sum = w0 * s[first - base];
-  for (uint64_t i = first + 1; i < last; ++i) sum += s[i - base];
-  sum += w1 * s[last - base];
+  // the serial sum, written once per source so each keeps its address space
+  auto weighted = [&](const float* src, uint64_t base) {
+    float sum = w0 * src[first - base];
+    for (uint64_t i = first + 1; i < last; ++i) sum += src[i - base];
+    return sum + w1 * src[last - base];
+  };
+  const float sum = staged ? weighted(stage, lo) : weighted(x, 0);
   prepared[sidx * prepared_stride + g.prep_offset + k] = sum;
 }
 
@@ -321,32 +321,66 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     if (levels == 0) {
       for (uint32_t e = tid; e < total; e += kSubtreeThreads) out[e] = cur[e];
     }
-    for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 0; --d) {
-      const uint2* dd = desc + static_cast<uint32_t>(d) * nrows;
-      float* dst = d == 0 ? out : nxt;
-      for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
-        merge_batch<2>(
-            r0, kWaves, nrows, b, lane,
-            [&](uint32_t r, const float*& h, const float*& t, uint32_t& roll) {
-              const uint2 ds = dd[r];
-              const uint32_t tr = ds.x >> 16;
-              h = cur + static_cast<size_t>(ds.x & 0xFFFFu) * b;
-              // copied rows read their own row twice; the store keeps it
-              t = tr == kCopyTail ? h : cur + static_cast<size_t>(tr) * b;
-              roll = tr == kCopyTail ? 0u : ds.y;
-            },
-            [&](uint32_t r, uint32_t p, float v) {
-              if ((dd[r].x >> 16) == kCopyTail) v = cur[static_cast<size_t>(r) * b + p];
-              dst[static_cast<size_t>(r) * b + p] = v;
-            });
+    // One level: rows of `src` (LDS) merged into `dst`, written once per
+    // destination kind so every access keeps its address space (LDS rows
+    // are ds_* operations, the top level's stores global_*).  Lanes cover
+    // 4 phases each per 256-phase pass; each wave merges two rows at a time,
+    // decoding each row's descriptor once into 32-bit LDS indices.
+    auto level = [&](const uint2* dd, const float* src, auto&& store) {
+      for (uint32_t p0 = 0; p0 < b; p0 += 256) {
+        uint32_t pc[4];
+        bool ok[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t p = p0 + lane + 64u * j;
+          ok[j] = p < b;
+          pc[j] = min(p, b - 1);  // clamped reads, predicated stores
+        }
+        for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
+          const uint32_t r1 = min(r0 + kWaves, nrows - 1);  // duplicate of a valid row
+          const bool has1 = r0 + kWaves < nrows;
+          // descriptors are wave-uniform: keep the row math on the scalar unit
+          const uint2 e0 = dd[r0], e1 = dd[r1];
+          const uint2 d0 = make_uint2(__builtin_amdgcn_readfirstlane(e0.x),
+                                      __builtin_amdgcn_readfirstlane(e0.y));
+          const uint2 d1 = make_uint2(__builtin_amdgcn_readfirstlane(e1.x),
+                                      __builtin_amdgcn_readfirstlane(e1.y));
+          const uint32_t hb0 = (d0.x & 0xFFFFu) * b, hb1 = (d1.x & 0xFFFFu) * b;
+          const bool cp0 = (d0.x >> 16) == kCopyTail, cp1 = (d1.x >> 16) == kCopyTail;
+          const uint32_t tb0 = cp0 ? hb0 : (d0.x >> 16) * b, tb1 = cp1 ? hb1 : (d1.x >> 16) * b;
+          const uint32_t ro0 = cp0 ? 0u : d0.y, ro1 = cp1 ? 0u : d1.y;
+          float v0[4], v1[4];
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            uint32_t q0 = pc[j] + ro0, q1 = pc[j] + ro1;
+            q0 = min(q0, q0 - b);  // (p + roll) mod b for p, roll < b
+            q1 = min(q1, q1 - b);
+            const float h0 = src[hb0 + pc[j]], t0 = src[tb0 + q0];
+            const float h1 = src[hb1 + pc[j]], t1 = src[tb1 + q1];
+            // a copied row (a leaf above this depth) keeps its own value
+            v0[j] = cp0 ? h0 : h0 + t0;
+            v1[j] = cp1 ? h1 : h1 + t1;
+          }
+          const uint32_t ob0 = r0 * b, ob1 = r1 * b;
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            if (ok[j]) {
+              store(ob0 + pc[j], v0[j]);
+              if (has1) store(ob1 + pc[j], v1[j]);
+            }
+          }
+        }
       }
-      if (d > 0) {
-        __syncthreads();
-        float* tmp = cur;
-        cur = nxt;
-        nxt = tmp;
-      }
+    };
+    for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 1; --d) {
+      float* dst = nxt;
+      level(desc + static_cast<uint32_t>(d) * nrows, cur,
+            [&](uint32_t i, float v) { dst[i] = v; });
+      __syncthreads();
+      nxt = cur;
+      cur = dst;
     }
+    if (levels > 0) level(desc, cur, [&](uint32_t i, float v) { out[i] = v; });
     __syncthreads();  // LDS rows and descriptors are rewritten by the next unit
   }
 }
@@ -518,7 +552,7 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
         const uint32_t slot = atomicAdd(a.peak_count, 1u);
         if (slot < a.peak_capacity) {
           gaffa_hip_ffa_peak pk;
-          pk.series = a.series_index_base + static_cast<uint32_t>(sidx);
+          pk.series = a.series_index_base + static_cast<uint32_t>(sidx) + task.series;
           pk.task = item.task;
           pk.shift = s0 + i;
           pk.phase_width = phase | (widx << 16);
@@ -558,6 +592,21 @@ using namespace gaffa_hip;
 
 namespace {
 
+// Raises a kernel's dynamic-LDS limit to at least `bytes`, never lowering
+// it: launches captured into a graph replay under the largest limit any of
+// them needed.  Per device, for the two LDS-heavy kernels.
+void ensure_dynamic_lds(const void* kernel, int slot, uint32_t bytes) {
+  static uint32_t set[64][3] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  check(dev >= 0 && dev < 64, "device ordinal out of range");
+  if (bytes <= set[dev][slot]) return;
+  hip_check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                static_cast<int>(bytes)),
+            "hipFuncSetAttribute");
+  set[dev][slot] = bytes;
+}
+
 int compute_units() {
   // per device, queried once
   static int cus[64] = {};
@@ -581,7 +630,7 @@ void launch_prepare(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   // groups needing fewer blocks than the widest return at once
   const dim3 grid(p.prep_blocks, p.ngroups, static_cast<uint32_t>(a->nseries));
   hipLaunchKernelGGL(ffa_prepare_kernel, grid, dim3(kPrepThreads), 0, st, a->series,
-                     a->series_stride, a->nsamples, p.groups, a->prepared, p.prepared_stride);
+                     a->series_stride, p.groups, a->prepared, p.prepared_stride);
   launch_check("ffa_prepare_kernel");
 }
 
@@ -589,10 +638,7 @@ void launch_subtrees(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   const gaffa_hip_ffa_plan_desc& p = *a->plan;
   const uint32_t ns = static_cast<uint32_t>(a->nseries);
   if (p.nsubtree_items > 0) {
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(ffa_subtree_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(p.subtree_lds_bytes)),
-              "hipFuncSetAttribute");
+    ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_subtree_kernel), 0, p.subtree_lds_bytes);
     const uint64_t nunits = static_cast<uint64_t>(p.nsubtree_items) * ns;
     check_overflow(nunits < (1ull << 32), "FFA subtree work too large");
     // one resident workgroup per CU (the LDS footprint allows no more)
@@ -632,10 +678,8 @@ template <bool kDetect>
 void launch_terminal(const gaffa_hip_ffa_search_args* a, float* out, uint32_t task,
                      hipStream_t st) {
   const gaffa_hip_ffa_plan_desc& p = *a->plan;
-  hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(ffa_terminal_kernel<kDetect>),
-                                hipFuncAttributeMaxDynamicSharedMemorySize,
-                                static_cast<int>(p.terminal_lds_bytes)),
-            "hipFuncSetAttribute");
+  ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_terminal_kernel<kDetect>), kDetect ? 1 : 2,
+                     p.terminal_lds_bytes);
   hipLaunchKernelGGL(ffa_terminal_kernel<kDetect>,
                      dim3(p.nterminal_items, static_cast<uint32_t>(a->nseries)),
                      dim3(kTerminalThreads), p.terminal_lds_bytes, st, kargs(a), out, task);
@@ -694,6 +738,20 @@ int gaffa_hip_ffa_transform_rows(const gaffa_hip_ffa_search_args* a, uint32_t ta
     launch_prepare(a, st);
     launch_transform(a, st);
     launch_terminal<false>(a, out, task, st);
+  });
+}
+
+int gaffa_hip_ffa_reserve_launch(const gaffa_hip_ffa_plan_desc* p) {
+  return guarded([&] {
+    check(p != nullptr, "FFA plan must not be null");
+    check(p->subtree_lds_bytes <= kMaxLdsBytes && p->terminal_lds_bytes <= kMaxLdsBytes,
+          "FFA plan exceeds LDS");
+    if (p->nsubtree_items > 0)
+      ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_subtree_kernel), 0,
+                         p->subtree_lds_bytes);
+    ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_terminal_kernel<true>), 1,
+                       p->terminal_lds_bytes);
+    (void)compute_units();
   });
 }
 

@@ -3,7 +3,9 @@
 // ffa_bindings.cpp:169-217), running on the GPU through the C++ host layer.
 // Numpy inputs are zero-copy views that must be C-contiguous; outputs own
 // their memory through a capsule; the GIL is released around device work.
+#include "gaffa/candidate.h"
 #include "gaffa/dedispersion.h"
+#include "gaffa/ffa_batch.h"
 #include "gaffa/ffa_gpu.h"
 #include "gaffa/pipeline.h"
 #include "gaffa/preprocessing.h"
@@ -498,6 +500,203 @@ PYBIND11_MODULE(_core, m) {
         py::arg("task"), py::kw_only(), py::arg("device_id") = 0);
   m.def("ffa_task_stdnoise", &ffa_task_stdnoise);
 
+  // candidate selection and harmonic flagging (host code over the peak list)
+  py::class_<Candidate>(m, "Candidate")
+      .def_readonly("dm", &Candidate::dm)
+      .def_readonly("dm_index", &Candidate::dm_index)
+      .def_readonly("period", &Candidate::period)
+      .def_readonly("frequency", &Candidate::frequency)
+      .def_readonly("width", &Candidate::width)
+      .def_readonly("duty_cycle", &Candidate::duty_cycle)
+      .def_readonly("snr", &Candidate::snr)
+      .def_readonly("peak_count", &Candidate::peak_count)
+      .def_readonly("dm_index_min", &Candidate::dm_index_min)
+      .def_readonly("dm_index_max", &Candidate::dm_index_max)
+      .def_readonly("frequency_min", &Candidate::frequency_min)
+      .def_readonly("frequency_max", &Candidate::frequency_max)
+      .def_readonly("best_peak", &Candidate::best_peak)
+      .def("__repr__", [](const Candidate& c) {
+        return "<Candidate dm=" + std::to_string(c.dm) + " period=" + std::to_string(c.period) +
+               " snr=" + std::to_string(c.snr) + " peaks=" + std::to_string(c.peak_count) + ">";
+      });
+  py::class_<HarmonicRelation>(m, "HarmonicRelation")
+      .def_readonly("is_harmonic", &HarmonicRelation::is_harmonic)
+      .def_readonly("parent_index", &HarmonicRelation::parent_index)
+      .def_readonly("numerator", &HarmonicRelation::numerator)
+      .def_readonly("denominator", &HarmonicRelation::denominator)
+      .def_readonly("frequency_error_bins", &HarmonicRelation::frequency_error_bins)
+      .def_readonly("phase_distance", &HarmonicRelation::phase_distance)
+      .def_readonly("dm_distance", &HarmonicRelation::dm_distance)
+      .def_readonly("expected_snr", &HarmonicRelation::expected_snr)
+      .def_readonly("snr_distance", &HarmonicRelation::snr_distance);
+  py::class_<HarmonicCandidate>(m, "HarmonicCandidate")
+      .def_readonly("candidate", &HarmonicCandidate::candidate)
+      .def_readonly("harmonic", &HarmonicCandidate::harmonic);
+  m.def(
+      "_select_candidates_records",
+      [](const py::array& peaks, double obs, double radius, std::size_t dm_radius, bool across,
+         std::size_t max_candidates) {
+        if (peaks.itemsize() != static_cast<py::ssize_t>(sizeof(DmPeak)))
+          throw std::invalid_argument("peak records must be gaffa DmPeak records");
+        const py::array c = py::array::ensure(peaks, py::array::c_style);
+        std::vector<DmPeak> v(static_cast<std::size_t>(c.size()));
+        if (!v.empty()) std::memcpy(v.data(), c.data(), v.size() * sizeof(DmPeak));
+        std::vector<Candidate> r;
+        {
+          py::gil_scoped_release release;
+          r = select_candidates_cpu(v, obs,
+                                    CandidateSelectionOptions{radius, dm_radius, across,
+                                                              max_candidates});
+        }
+        return py::bytes(reinterpret_cast<const char*>(r.data()), r.size() * sizeof(Candidate));
+      },
+      py::arg("peaks"), py::arg("observation_seconds"), py::arg("frequency_cluster_radius"),
+      py::arg("dm_cluster_radius"), py::arg("cluster_across_widths"), py::arg("max_candidates"));
+  m.def(
+      "_flag_harmonics_records",
+      [](const py::array& cands, double obs, double flo, double fhi, std::size_t max_harmonic,
+         std::size_t denominator_max, double ftol, double phase_max, double dm_max,
+         bool use_snr, double snr_max) {
+        if (cands.itemsize() != static_cast<py::ssize_t>(sizeof(Candidate)))
+          throw std::invalid_argument("candidate records must be gaffa Candidate records");
+        const py::array c = py::array::ensure(cands, py::array::c_style);
+        std::vector<Candidate> v(static_cast<std::size_t>(c.size()));
+        if (!v.empty()) std::memcpy(v.data(), c.data(), v.size() * sizeof(Candidate));
+        const auto r = flag_harmonics_cpu(
+            v, HarmonicContext{obs, flo, fhi},
+            HarmonicOptions{max_harmonic, denominator_max, ftol, phase_max, dm_max, use_snr,
+                            snr_max});
+        std::vector<Candidate> out;
+        py::list rel;
+        for (const auto& h : r) {
+          out.push_back(h.candidate);
+          rel.append(h.harmonic);
+        }
+        return py::make_tuple(
+            py::bytes(reinterpret_cast<const char*>(out.data()), out.size() * sizeof(Candidate)),
+            rel);
+      });
+  m.def(
+      "select_candidates",
+      [](const std::vector<DmPeak>& peaks, double obs, double radius, std::size_t dm_radius,
+         bool across, std::size_t max_candidates) {
+        py::gil_scoped_release release;
+        return select_candidates_cpu(
+            peaks, obs, CandidateSelectionOptions{radius, dm_radius, across, max_candidates});
+      },
+      py::arg("peaks"), py::arg("observation_seconds"), py::kw_only(),
+      py::arg("frequency_cluster_radius") = 0.1, py::arg("dm_cluster_radius") = 1,
+      py::arg("cluster_across_widths") = true, py::arg("max_candidates") = 0);
+  m.def(
+      "flag_harmonics",
+      [](const std::vector<Candidate>& c, double obs, double flo, double fhi,
+         std::size_t max_harmonic, std::size_t denominator_max, double ftol, double phase_max,
+         double dm_max, bool use_snr, double snr_max) {
+        return flag_harmonics_cpu(c, HarmonicContext{obs, flo, fhi},
+                                  HarmonicOptions{max_harmonic, denominator_max, ftol, phase_max,
+                                                  dm_max, use_snr, snr_max});
+      },
+      py::arg("candidates"), py::arg("observation_seconds"), py::arg("frequency_low_mhz"),
+      py::arg("frequency_high_mhz"), py::kw_only(), py::arg("max_harmonic") = 16,
+      py::arg("denominator_max") = 5, py::arg("frequency_tolerance_bins") = 1.5,
+      py::arg("phase_distance_max") = 1.0, py::arg("dm_distance_max") = 3.0,
+      py::arg("use_snr_consistency") = false, py::arg("snr_distance_max") = 3.0);
+  m.def("remove_harmonics", &remove_harmonics_cpu, py::arg("candidates"),
+        py::arg("max_candidates") = 0);
+
+  // batched FFA over series with their own plans
+  py::class_<GpuFfaBatchedSearch>(m, "FfaBatchedSearch")
+      .def(py::init([](std::vector<FfaSearchPlan> plans, std::size_t series_per_tile,
+                       std::size_t workspace_bytes, bool use_graph, int device_id) {
+             GpuFfaBatchedOptions o;
+             o.series_per_tile = series_per_tile;
+             o.workspace_bytes_limit = workspace_bytes;
+             o.use_graph = use_graph;
+             o.device_id = device_id;
+             py::gil_scoped_release release;
+             return std::make_unique<GpuFfaBatchedSearch>(std::move(plans), o);
+           }),
+           py::arg("plans"), py::kw_only(), py::arg("series_per_tile") = 16,
+           py::arg("workspace_bytes") = std::size_t{32} << 30, py::arg("use_graph") = true,
+           py::arg("device_id") = 0)
+      .def_property_readonly("nseries", &GpuFfaBatchedSearch::nseries)
+      .def_property_readonly("row_length", &GpuFfaBatchedSearch::row_length)
+      .def_property_readonly("series_per_tile", &GpuFfaBatchedSearch::series_per_tile)
+      .def_property_readonly("offsets", &GpuFfaBatchedSearch::offsets)
+      .def_property_readonly("lengths", &GpuFfaBatchedSearch::lengths)
+      .def("upload",
+           [](GpuFfaBatchedSearch& b, const py::array_t<float, py::array::c_style>& row) {
+             if (static_cast<std::size_t>(row.size()) != b.row_length())
+               throw std::invalid_argument("batch row length does not match");
+             py::gil_scoped_release release;
+             b.upload(row.data(), b.row_length());
+           })
+      .def("download_row",
+           [](GpuFfaBatchedSearch& b) {
+             std::vector<float> host(b.row_length());
+             copy_to_host(host.data(), b.row(), host.size() * sizeof(float), nullptr);
+             return owning_array(std::move(host), {static_cast<py::ssize_t>(b.row_length())});
+           })
+      .def(
+          "generate",
+          [](GpuFfaBatchedSearch& b, std::uint64_t seed, double tsamp,
+             const std::vector<double>& period, const std::vector<double>& phase0,
+             const std::vector<float>& sigma, const std::vector<float>& amplitude) {
+            const std::size_t n = b.nseries();
+            if (period.size() != n || phase0.size() != n || sigma.size() != n ||
+                amplitude.size() != n)
+              throw std::invalid_argument("one injection entry per series is required");
+            py::gil_scoped_release release;
+            GpuDeviceBuffer<std::uint64_t> off = upload(b.offsets().data(), n);
+            GpuDeviceBuffer<std::uint64_t> len = upload(b.lengths().data(), n);
+            GpuDeviceBuffer<double> per = upload(period.data(), n);
+            GpuDeviceBuffer<double> ph = upload(phase0.data(), n);
+            GpuDeviceBuffer<float> sg = upload(sigma.data(), n);
+            GpuDeviceBuffer<float> amp = upload(amplitude.data(), n);
+            gaffa_hip_synth_series_args a{};
+            a.out = b.row();
+            a.row_length = b.row_length();
+            a.nseries = static_cast<std::uint32_t>(n);
+            a.key = seed;
+            a.tsamp = tsamp;
+            a.offsets = off.data();
+            a.lengths = len.data();
+            a.period = per.data();
+            a.phase0 = ph.data();
+            a.sigma = sg.data();
+            a.amplitude = amp.data();
+            hip_ok(gaffa_hip_synth_series(&a, b.stream()), "synth_series");
+            hip_ok(gaffa_hip_stream_synchronize(b.stream()), "hipStreamSynchronize");
+          },
+          py::arg("seed"), py::arg("tsamp"), py::arg("period"), py::arg("phase0"),
+          py::arg("sigma"), py::arg("amplitude"),
+          "Seeded N(0,1) series (+ Gaussian pulse trains where amplitude != 0) in the row")
+      .def("set_series_per_tile", &GpuFfaBatchedSearch::set_series_per_tile)
+      .def("autotune",
+           [](GpuFfaBatchedSearch& b, const std::vector<std::size_t>& candidates) {
+             py::gil_scoped_release release;
+             return b.autotune(candidates);
+           })
+      .def(
+          "run",
+          [](GpuFfaBatchedSearch& b, float snr_threshold, bool per_series) {
+            FfaBatchTiming t;
+            FfaBatchSearchResult r;
+            {
+              py::gil_scoped_release release;
+              r = per_series ? b.run_per_series(snr_threshold, &t) : b.run(snr_threshold, &t);
+            }
+            py::list peaks;
+            for (const auto& p : r.peaks) peaks.append(py::make_tuple(p.series_index, p.peak));
+            py::dict timing;
+            timing["gpu_ms"] = t.gpu_ms;
+            timing["tiles"] = t.tiles;
+            timing["launches"] = t.launches;
+            return py::make_tuple(peaks, timing);
+          },
+          py::arg("snr_threshold") = 6.0F, py::kw_only(), py::arg("per_series") = false,
+          "Search every series; returns ([(series, FfaPeak)] in (series, peak) order, timing)");
+
   // preprocessing
   m.def("running_median_window_samples", [](double seconds, double tsamp) {
     return running_median_window_seconds(seconds, tsamp).window_samples;
@@ -515,6 +714,33 @@ PYBIND11_MODULE(_core, m) {
         return "<DmPeak dm=" + std::to_string(p.dm) + " period=" + std::to_string(p.peak.period) +
                " snr=" + std::to_string(p.peak.snr) + ">";
       });
+  m.def(
+      "_expand_sorted_dm_peaks",
+      [](const FfaSearchPlan& plan, const py::array_t<std::uint32_t, py::array::c_style>& series,
+         const py::array_t<std::uint32_t, py::array::c_style>& task,
+         const py::array_t<std::uint32_t, py::array::c_style>& shift,
+         const py::array_t<std::uint32_t, py::array::c_style>& phase,
+         const py::array_t<std::uint32_t, py::array::c_style>& width_index,
+         const py::array_t<float, py::array::c_style>& snr, const std::vector<double>& dms) {
+        const std::size_t n = static_cast<std::size_t>(series.size());
+        if (task.size() != series.size() || shift.size() != series.size() ||
+            phase.size() != series.size() || width_index.size() != series.size() ||
+            snr.size() != series.size())
+          throw std::invalid_argument("compact peak fields must have equal lengths");
+        std::vector<gaffa_hip_ffa_peak> c(n);
+        for (std::size_t i = 0; i < n; ++i) {
+          if (phase.data()[i] > 0xFFFFu || width_index.data()[i] > 0xFFFFu)
+            throw std::invalid_argument("phase and width index must fit 16 bits");
+          c[i] = gaffa_hip_ffa_peak{series.data()[i], task.data()[i], shift.data()[i],
+                                    phase.data()[i] | (width_index.data()[i] << 16),
+                                    snr.data()[i]};
+        }
+        const GpuFfaExecutionPlan e(plan);
+        return expand_sorted_dm_peaks(e, c, dms);
+      },
+      py::arg("plan"), py::arg("series"), py::arg("task"), py::arg("shift"), py::arg("phase"),
+      py::arg("width_index"), py::arg("snr"), py::arg("dms"),
+      "Host-side expansion + is_better_dm_peak ordering of compact GPU peaks (no device use)");
   m.def(
       "_find_dm_peaks_gpu",
       [](const py::array& plane, const std::vector<double>& dms, double tsamp, double period_min,
@@ -659,6 +885,20 @@ PYBIND11_MODULE(_core, m) {
         py::gil_scoped_release release;
         p.search();
       })
+      .def(
+          "select_candidates",
+          [](const GpuSearchPipeline& p, double radius, std::size_t dm_radius, bool across,
+             std::size_t max_candidates) {
+            py::gil_scoped_release release;
+            const double obs = static_cast<double>(p.out_nsamples()) * p.tsamp();
+            return select_candidates_cpu(
+                p.peaks(), obs, CandidateSelectionOptions{radius, dm_radius, across,
+                                                          max_candidates});
+          },
+          py::kw_only(), py::arg("frequency_cluster_radius") = 0.1,
+          py::arg("dm_cluster_radius") = 1, py::arg("cluster_across_widths") = true,
+          py::arg("max_candidates") = 0,
+          "Candidates from the last search's peaks (observation = out_nsamples * tsamp)")
       .def("set_kernel_timing", &GpuSearchPipeline::set_kernel_timing, py::arg("on"),
            "Record HIP events between the FFA launch stages of every tile")
       .def("run", [](GpuSearchPipeline& p) {

@@ -75,20 +75,47 @@ void collect_nodes(std::uint32_t start, std::uint32_t size, std::uint32_t depth,
 
 GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan) {
   check_arg(!plan.tasks.empty(), "FFA executor plan must contain at least one task");
+  input_nsamples_ = plan.tasks.front().input_nsamples;
+  check_arg(input_nsamples_ > 0, "FFA executor time series must not be empty");
+  append(plan, 0, 0);
+  finish();
+}
+
+GpuFfaExecutionPlan::GpuFfaExecutionPlan(std::span<const FfaBatchMember> members) {
+  check_arg(!members.empty(), "FFA batch must contain at least one series");
+  plan_.width_trials = members.front().plan->width_trials;
+  for (const FfaBatchMember& m : members) {
+    check_arg(m.plan != nullptr && !m.plan->tasks.empty(),
+              "FFA executor plan must contain at least one task");
+    check_arg(m.plan->width_trials == plan_.width_trials,
+              "FFA batch members must share one width-trial list");
+    check_arg(m.input_offset % 64 == 0, "FFA batch member offsets must be 64-float aligned");
+    const std::size_t n = m.plan->tasks.front().input_nsamples;
+    check_arg(n > 0, "FFA executor time series must not be empty");
+    input_nsamples_ = std::max<std::size_t>(input_nsamples_, m.input_offset + n);
+    append(*m.plan, m.input_offset, m.series);
+    plan_.tasks.insert(plan_.tasks.end(), m.plan->tasks.begin(), m.plan->tasks.end());
+  }
+  finish();
+}
+
+void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_offset,
+                                 std::uint32_t series) {
   check_arg(!plan.width_trials.empty(), "FFA search plan width_trials must not be empty");
   check_arg(plan.width_trials.size() <= GAFFA_HIP_FFA_MAX_WIDTHS,
             "GPU FFA supports at most 16 width trials");
-  input_nsamples_ = plan.tasks.front().input_nsamples;
-  check_arg(input_nsamples_ > 0, "FFA executor time series must not be empty");
+  const std::size_t n = plan.tasks.front().input_nsamples;
   const std::uint32_t level_rows = gaffa_hip_ffa_level_rows();
   const std::uint32_t terminal_rows = gaffa_hip_ffa_terminal_rows();
   std::map<double, std::uint32_t> group_of;
   for (const FfaSearchTask& t : plan.tasks) {
-    validate_task(t, input_nsamples_);
+    validate_task(t, n);
     if (group_of.count(t.downsample_factor) == 0) {
       gaffa_hip_ffa_group g{};
       g.factor = t.downsample_factor;
       g.prepared_nsamples = t.prepared_nsamples;
+      g.input_offset = input_offset;
+      g.input_nsamples = n;
       if (t.downsample_factor != 1.0) {
         g.prep_offset = prepared_stride_;
         prepared_stride_ += align_up(t.prepared_nsamples);
@@ -102,9 +129,7 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
       groups_.push_back(g);
     }
   }
-  std::vector<std::vector<gaffa_hip_ffa_item>> by_depth;
-  for (std::size_t k = 0; k < plan.tasks.size(); ++k) {
-    const FfaSearchTask& t = plan.tasks[k];
+  for (const FfaSearchTask& t : plan.tasks) {
     const FfaDetectionPlan det = make_ffa_detection_plan(plan.width_trials, t.bins);
     const float stdnoise = ffa_task_stdnoise(t);
     check_arg(stdnoise > 0.0F && std::isfinite(stdnoise),
@@ -116,6 +141,7 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
     d.rows_eval = static_cast<std::uint32_t>(t.rows_eval);
     d.nwidths = static_cast<std::uint32_t>(det.boxcar_trials.size());
     d.inv_stdnoise = 1.0F / stdnoise;
+    d.series = series;
     for (std::size_t w = 0; w < det.boxcar_trials.size(); ++w) {
       d.width[w] = static_cast<std::uint32_t>(det.boxcar_trials[w].width);
       d.height[w] = det.boxcar_trials[w].height;
@@ -129,13 +155,13 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
     std::size_t lds_rows = std::min(kSubtreeMaxRows, kSubtreeNodeFloats / t.bins);
     while (lds_rows > 2 && tree_levels(lds_rows) * lds_rows > kSubtreeMaxDesc) --lds_rows;
     check_arg(lds_rows >= 2, "GPU FFA bins too large for an LDS subtree");
+    const std::uint32_t task_index = static_cast<std::uint32_t>(tasks_.size());
     if (t.rows >= 2) {
       std::uint32_t depth = 1;
       while ((t.rows + (std::size_t{1} << depth) - 1) >> depth > lds_rows) ++depth;
       d.depth = depth;
       d.work_offset = work_stride_;
       work_stride_ += align_up(t.rows * t.bins);
-      const std::uint32_t task_index = static_cast<std::uint32_t>(k);
       const std::size_t before = subtree_.size();
       collect_nodes(0, d.rows, 0, depth, task_index, subtree_);
       for (std::size_t i = before; i < subtree_.size(); ++i) {
@@ -146,25 +172,31 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
             subtree_desc_, static_cast<std::uint32_t>(tree_levels(subtree_[i].b) * subtree_[i].b));
       }
       subtree_lds_bytes_ = 8u * subtree_half_ + 8u * subtree_desc_;
-      if (by_depth.size() < depth) by_depth.resize(depth);
+      if (by_depth_.size() < depth) by_depth_.resize(depth);
       for (std::uint32_t l = 1; l < depth; ++l)
         for (std::uint32_t r = 0; r < d.rows; r += level_rows)
-          by_depth[l].push_back(
+          by_depth_[l].push_back(
               gaffa_hip_ffa_item{task_index, l, r, std::min(level_rows, d.rows - r)});
       max_depth_ = std::max<std::size_t>(max_depth_, depth);
     }
     for (std::uint32_t r = 0; r < d.rows_eval; r += terminal_rows)
-      terminal_.push_back(gaffa_hip_ffa_item{static_cast<std::uint32_t>(k), r,
-                                             std::min(terminal_rows, d.rows_eval - r), 0});
+      terminal_.push_back(
+          gaffa_hip_ffa_item{task_index, r, std::min(terminal_rows, d.rows_eval - r), 0});
     tasks_.push_back(d);
   }
+}
+
+void GpuFfaExecutionPlan::finish() {
   level_offsets_.assign(max_depth_ + 1, 0);
   for (std::size_t l = 0; l <= max_depth_; ++l) {
     level_offsets_[l] = static_cast<std::uint32_t>(level_.size());
-    if (l < by_depth.size()) level_.insert(level_.end(), by_depth[l].begin(), by_depth[l].end());
+    if (l < by_depth_.size())
+      level_.insert(level_.end(), by_depth_[l].begin(), by_depth_[l].end());
   }
   level_offsets_.push_back(static_cast<std::uint32_t>(level_.size()));
-  check_arg(terminal_.size() < (1ULL << 31), "FFA plan has too many work items");
+  by_depth_.clear();
+  check_arg(terminal_.size() < (1ULL << 31) && tasks_.size() < (1ULL << 31),
+            "FFA plan has too many work items");
 }
 
 std::size_t GpuFfaExecutionPlan::workspace_bytes(std::size_t series) const noexcept {
@@ -327,6 +359,82 @@ std::vector<FfaBatchPeak> expand_ffa_peaks(const GpuFfaExecutionPlan& plan,
         p.series, make_ffa_peak(tasks[p.task], det[p.task]->boxcar_trials[p.phase_width >> 16],
                                 p.shift, p.phase_width & 0xFFFFu, p.snr)};
   }
+  return out;
+}
+
+std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
+                                           std::span<const gaffa_hip_ffa_peak> peaks,
+                                           std::span<const double> dms) {
+  const auto& tasks = plan.plan().tasks;
+  std::map<std::size_t, FfaDetectionPlan> by_bins;
+  std::vector<const FfaDetectionPlan*> det(tasks.size());
+  for (std::size_t k = 0; k < tasks.size(); ++k) {
+    auto it = by_bins.find(tasks[k].bins);
+    if (it == by_bins.end())
+      it = by_bins.emplace(tasks[k].bins,
+                           make_ffa_detection_plan(plan.plan().width_trials, tasks[k].bins)).first;
+    det[k] = &it->second;
+  }
+  for (const gaffa_hip_ffa_peak& p : peaks) {
+    if (p.task >= tasks.size() || (p.phase_width >> 16) >= det[p.task]->boxcar_trials.size() ||
+        p.series >= dms.size()) {
+      throw std::logic_error("FFA compact peak refers to an unknown task, width or series");
+    }
+  }
+  auto expand = [&](const gaffa_hip_ffa_peak& p) {
+    return DmPeak{dms[p.series], p.series,
+                  make_ffa_peak(tasks[p.task], det[p.task]->boxcar_trials[p.phase_width >> 16],
+                                p.shift, p.phase_width & 0xFFFFu, p.snr)};
+  };
+  // Sort (key, index) pairs instead of 96-byte records.  The key orders by
+  // is_better_dm_peak's first two criteria (finite peaks first, then S/N
+  // descending; -0 and +0 share a key because they compare equal); runs of
+  // equal keys are then ordered with the full comparator, so the result is
+  // exactly std::sort(..., is_better_dm_peak) of the expanded peaks.
+  struct Key {
+    std::uint64_t key;
+    std::uint32_t index;
+  };
+  const std::int64_t n = static_cast<std::int64_t>(peaks.size());
+  std::vector<Key> keys(peaks.size());
+#pragma omp parallel for schedule(static)
+  for (std::int64_t i = 0; i < n; ++i) {
+    const gaffa_hip_ffa_peak& c = peaks[static_cast<std::size_t>(i)];
+    const FfaPeak e = expand(c).peak;
+    const bool finite =
+        std::isfinite(e.snr) && std::isfinite(e.period) && std::isfinite(e.frequency);
+    std::uint64_t k = finite ? 0 : (1ULL << 33);
+    if (std::isnan(c.snr)) {
+      k |= 0xFFFFFFFFULL;  // one run, ordered by the comparator below
+    } else {
+      const float snr = c.snr == 0.0F ? 0.0F : c.snr;  // canonical zero
+      std::uint32_t bits = 0;
+      std::memcpy(&bits, &snr, 4);
+      const std::uint32_t ascending = (bits & 0x80000000U) ? ~bits : (bits | 0x80000000U);
+      k |= static_cast<std::uint64_t>(~ascending);  // descending S/N
+    }
+    keys[static_cast<std::size_t>(i)] = Key{k, static_cast<std::uint32_t>(i)};
+  }
+  __gnu_parallel::sort(keys.begin(), keys.end(),
+                       [](const Key& a, const Key& b) { return a.key < b.key; });
+  for (std::size_t i = 0; i < keys.size();) {
+    std::size_t j = i + 1;
+    while (j < keys.size() && keys[j].key == keys[i].key) ++j;
+    if (j - i > 1) {
+      std::vector<std::pair<DmPeak, std::uint32_t>> run;
+      run.reserve(j - i);
+      for (std::size_t k = i; k < j; ++k) run.emplace_back(expand(peaks[keys[k].index]), keys[k].index);
+      std::sort(run.begin(), run.end(), [](const auto& a, const auto& b) {
+        return is_better_dm_peak(a.first, b.first);
+      });
+      for (std::size_t k = i; k < j; ++k) keys[k].index = run[k - i].second;
+    }
+    i = j;
+  }
+  std::vector<DmPeak> out(peaks.size());
+#pragma omp parallel for schedule(static)
+  for (std::int64_t i = 0; i < n; ++i)
+    out[static_cast<std::size_t>(i)] = expand(peaks[keys[static_cast<std::size_t>(i)].index]);
   return out;
 }
 

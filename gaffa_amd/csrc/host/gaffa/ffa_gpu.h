@@ -48,9 +48,22 @@ struct FfaStageMarks {
 // Host-built device layout of one FfaSearchPlan: prepare groups keyed by
 // downsample factor, per-task LDS depth, and the work-item lists of the
 // subtree / level / terminal launches (see gaffa_hip.h).
+// One series of a heterogeneous batch: its plan, the float offset of its
+// samples in the batch row (64-float aligned) and its series index.
+struct FfaBatchMember {
+  const FfaSearchPlan* plan = nullptr;
+  std::uint64_t input_offset = 0;
+  std::uint32_t series = 0;
+};
+
 class GpuFfaExecutionPlan {
  public:
   explicit GpuFfaExecutionPlan(const FfaSearchPlan& plan);
+  // Several series with their own lengths and plans laid end to end in one
+  // row: every group reads its member's offset, every task reports its
+  // member's series index, and plan().tasks is the concatenation of the
+  // members' tasks (indexed by the compact peaks' task field).
+  explicit GpuFfaExecutionPlan(std::span<const FfaBatchMember> members);
 
   const FfaSearchPlan& plan() const noexcept { return plan_; }
   std::size_t input_nsamples() const noexcept { return input_nsamples_; }
@@ -74,6 +87,9 @@ class GpuFfaExecutionPlan {
   std::size_t workspace_bytes(std::size_t series) const noexcept;
 
  private:
+  void append(const FfaSearchPlan& plan, std::uint64_t input_offset, std::uint32_t series);
+  void finish();
+  std::vector<std::vector<gaffa_hip_ffa_item>> by_depth_;
   FfaSearchPlan plan_;
   std::size_t input_nsamples_ = 0;
   std::size_t prepared_stride_ = 0;
@@ -175,6 +191,13 @@ struct DmSearchOptions {
 struct DmSearchResult {
   std::vector<DmPeak> peaks;
 };
+
+// Expands compact peaks (series = DM index into `dms`) and returns them in
+// is_better_dm_peak order (dm_search.cpp:142-199), sorting keys rather than
+// records.
+std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
+                                           std::span<const gaffa_hip_ffa_peak> peaks,
+                                           std::span<const double> dms);
 
 bool is_better_dm_peak(const DmPeak& lhs, const DmPeak& rhs);
 

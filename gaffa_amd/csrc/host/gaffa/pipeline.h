@@ -78,6 +78,7 @@ class GpuSearchPipeline {
   std::size_t ndm() const noexcept { return ndm_; }
   std::size_t out_nsamples() const noexcept { return n_out_; }
   std::size_t plane_stride() const noexcept { return stride_; }
+  double tsamp() const noexcept { return cfg_.tsamp; }
   const std::vector<double>& dms() const noexcept { return dms_; }
   const FfaSearchPlan& ffa_plan() const noexcept { return ffa_plan_; }
   const GpuFfaProgram& ffa_program() const noexcept { return *ffa_; }

@@ -169,12 +169,7 @@ void GpuSearchPipeline::search() {
     prog.grow_peaks(count);
   }
   const std::vector<gaffa_hip_ffa_peak> compact = prog.read_peaks(count);
-  const std::vector<FfaBatchPeak> expanded = expand_ffa_peaks(prog.execution_plan(), compact);
-  peaks_.clear();
-  peaks_.reserve(expanded.size());
-  for (const auto& p : expanded)
-    peaks_.push_back(DmPeak{dms_[p.series_index], p.series_index, p.peak});
-  __gnu_parallel::sort(peaks_.begin(), peaks_.end(), is_better_dm_peak);
+  peaks_ = expand_sorted_dm_peaks(prog.execution_plan(), compact, dms_);
 }
 
 StageTimes GpuSearchPipeline::run() {

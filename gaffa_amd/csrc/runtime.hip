@@ -13,6 +13,16 @@ void set_error(const std::string& message) { g_last_error = message; }
 
 using namespace gaffa_hip;
 
+namespace gaffa_hip {
+// A kernel fill, for memory that a captured graph resets on every replay
+// (memset nodes are avoided in graphs).
+__global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t* dst, uint32_t value,
+                                                       uint64_t count) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (i < count) dst[i] = value;
+}
+}  // namespace gaffa_hip
+
 extern "C" {
 
 const char* gaffa_hip_last_error(void) { return g_last_error.c_str(); }
@@ -150,6 +160,17 @@ int gaffa_hip_event_elapsed_ms(gaffa_hip_event start, gaffa_hip_event stop, floa
     hip_check(hipEventElapsedTime(ms, reinterpret_cast<hipEvent_t>(start),
                                   reinterpret_cast<hipEvent_t>(stop)),
               "hipEventElapsedTime");
+  });
+}
+
+int gaffa_hip_fill_u32(uint32_t* dst, uint32_t value, uint64_t count, gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(dst != nullptr || count == 0, "fill destination must not be null");
+    if (count == 0) return;
+    check_overflow(count < (1ull << 32), "fill too large");
+    hipLaunchKernelGGL(gaffa_hip::fill_u32_kernel, dim3(static_cast<uint32_t>(ceil_div(count, 256))),
+                       dim3(256), 0, as_stream(stream), dst, value, count);
+    launch_check("fill_u32_kernel");
   });
 }
 

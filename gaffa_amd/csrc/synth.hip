@@ -8,6 +8,8 @@
 // bytes equal the CPU oracle's for the same host tables.
 #include "common.hpp"
 
+#include <cmath>
+
 namespace gaffa_hip {
 namespace {
 
@@ -49,10 +51,58 @@ synth_kernel(gaffa_hip_synth_args a) {
   a.out[e] = static_cast<uint8_t>(v);
 }
 
+// One sample of a packed batch of float series: N(0,1) from two hashed
+// uniforms (Box-Muller), plus a Gaussian pulse train for injected series.
+__global__ void __launch_bounds__(256)
+synth_series_kernel(gaffa_hip_synth_series_args a, uint64_t e0) {
+  const uint64_t e = e0 + static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= a.row_length) return;
+  // series s owns [offsets[s], offsets[s] + lengths[s]); the tail of its
+  // 64-float padding is written as 0
+  uint32_t lo = 0, hi = a.nseries;  // last series with offsets[s] <= e
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) / 2;
+    if (a.offsets[mid] <= e) lo = mid; else hi = mid;
+  }
+  const uint64_t t = e - a.offsets[lo];
+  if (t >= a.lengths[lo]) {
+    a.out[e] = 0.0f;
+    return;
+  }
+  const uint64_t h = splitmix(a.key ^ (static_cast<uint64_t>(lo) << 40) ^ t);
+  const float u1 = (static_cast<float>(h >> 40) + 1.0f) * (1.0f / 16777217.0f);  // (0, 1]
+  const float u2 = static_cast<float>((h >> 16) & 0xFFFFFFu) * (1.0f / 16777216.0f);
+  float v = sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+  const float amp = a.amplitude[lo];
+  if (amp != 0.0f) {
+    const double ph = fmod(static_cast<double>(t) * a.tsamp / a.period[lo] + a.phase0[lo], 1.0);
+    const float d = static_cast<float>(ph - 0.5) / a.sigma[lo];
+    v += amp * expf(-0.5f * d * d);
+  }
+  a.out[e] = v;
+}
+
 }  // namespace
 }  // namespace gaffa_hip
 
 using namespace gaffa_hip;
+
+extern "C" int gaffa_hip_synth_series(const gaffa_hip_synth_series_args* a,
+                                      gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(a != nullptr && a->out != nullptr && a->offsets != nullptr && a->lengths != nullptr &&
+              a->period != nullptr && a->phase0 != nullptr && a->sigma != nullptr &&
+              a->amplitude != nullptr,
+          "synth arguments must not be null");
+    check(a->nseries > 0 && a->tsamp > 0.0, "synth batch must be non-empty with tsamp > 0");
+    for (uint64_t e0 = 0; e0 < a->row_length; e0 += (1ull << 30)) {
+      const uint64_t count = std::min<uint64_t>(1ull << 30, a->row_length - e0);
+      hipLaunchKernelGGL(synth_series_kernel, dim3(static_cast<uint32_t>(ceil_div(count, 256))),
+                         dim3(256), 0, as_stream(stream), *a, e0);
+      launch_check("synth_series_kernel");
+    }
+  });
+}
 
 extern "C" int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* a,
                                           gaffa_hip_stream stream) {

@@ -10,6 +10,7 @@ from __future__ import annotations
 import numpy as np
 
 from ._core import (
+    FfaBatchedSearch,
     FfaPeak,
     FfaPlan,
     _ffa_search_batch_gpu,
@@ -47,6 +48,26 @@ def make_riptide_plan(
         width_trial_spacing=width_trial_spacing,
         max_tasks=max_tasks,
     )
+
+
+def make_octave_plan(
+    *,
+    nsamples: int,
+    tsamp: float,
+    period_min: float = 0.1,
+    period_max: float = 10.0,
+    bins: int = 128,
+    duty_cycle_max: float = 0.20,
+    width_trial_spacing: float = 1.5,
+) -> FfaPlan:
+    """Period-octave plan: downsample factors f0 * 2^k with f0 = period_min /
+    (tsamp * bins) and bins in [bins, 2 * bins) per octave.  It is gaffa's
+    riptide plan generator with bins_max = 2 * bins - 1, whose cycle growth
+    (bins_max + 1) / bins_min is then exactly 2 (ffa_plan.cpp:90-171)."""
+    return make_riptide_plan(nsamples=nsamples, tsamp=tsamp, period_min=period_min,
+                             period_max=period_max, bins_min=bins, bins_max=2 * bins - 1,
+                             duty_cycle_max=duty_cycle_max,
+                             width_trial_spacing=width_trial_spacing)
 
 
 def _check_backend(backend: str, device_id: int) -> None:

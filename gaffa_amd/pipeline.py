@@ -177,6 +177,71 @@ def lband_workload(nsamples: int = 1 << 22, seed: int = 20260202, ndm: int = 102
         duty_cycle_max=0.18, width_trial_spacing=2.0)
 
 
-__all__ = ["DmSegment", "GpuSearchPipeline", "SynthFilterbankSpec", "SynthPulsar", "Workload",
-           "channel_frequencies", "lband_workload", "pulse_amplitude", "small_workload",
+@dataclass
+class BatchedWorkload:
+    """Many float series of different lengths, each searched with its own
+    octave plan (one batch, heterogeneous row counts)."""
+    nseries: int
+    lengths: np.ndarray
+    tsamp: float
+    seed: int
+    period: np.ndarray     # injected period (s) per series
+    phase0: np.ndarray
+    sigma: np.ndarray      # pulse sigma in phase units
+    amplitude: np.ndarray  # per-sample peak amplitude, 0 = noise only
+    period_min: float = 0.1
+    period_max: float = 10.0
+    bins: int = 128
+    duty_cycle_max: float = 0.20
+    width_trial_spacing: float = 1.5
+    snr_threshold: float = 6.0
+
+    def plans(self) -> list:
+        from .ffa import make_octave_plan
+
+        cache: dict = {}
+        out = []
+        for n in self.lengths:
+            n = int(n)
+            if n not in cache:
+                cache[n] = make_octave_plan(nsamples=n, tsamp=self.tsamp,
+                                            period_min=self.period_min,
+                                            period_max=self.period_max, bins=self.bins,
+                                            duty_cycle_max=self.duty_cycle_max,
+                                            width_trial_spacing=self.width_trial_spacing)
+            out.append(cache[n])
+        return out
+
+    def search(self, **kw):
+        from ._core import FfaBatchedSearch
+
+        b = FfaBatchedSearch(self.plans(), **kw)
+        b.generate(self.seed, self.tsamp, list(self.period), list(self.phase0),
+                   list(self.sigma), list(self.amplitude))
+        return b
+
+
+def batched_workload(nseries: int = 2048, seed: int = 20260303, *, tsamp: float = 64e-6,
+                     log2_min: int = 20, log2_max: int = 22, injected: float = 0.05,
+                     snr: tuple = (8.0, 30.0)) -> BatchedWorkload:
+    """Float N(0,1) series with lengths uniform in [2^log2_min, 2^log2_max]
+    samples; a fraction `injected` carries a Gaussian pulse train (period
+    log-uniform 0.1-10 s, duty 1-10 %, integrated S/N uniform in `snr`)."""
+    rng = np.random.default_rng(seed)
+    lengths = rng.integers(1 << log2_min, (1 << log2_max) + 1, nseries)
+    period = np.exp(rng.uniform(np.log(0.1), np.log(10.0), nseries))
+    phase0 = rng.uniform(0.0, 1.0, nseries)
+    duty = rng.uniform(0.01, 0.10, nseries)
+    sigma = (duty / 2.3548).astype(np.float32)
+    target = rng.uniform(snr[0], snr[1], nseries)
+    # integrated S/N of a Gaussian pulse train on unit noise: A * sqrt(N sigma sqrt(pi))
+    amp = target / np.sqrt(lengths * sigma * np.sqrt(np.pi))
+    amp[rng.random(nseries) >= injected] = 0.0
+    return BatchedWorkload(nseries=nseries, lengths=lengths, tsamp=tsamp, seed=seed,
+                           period=period, phase0=phase0, sigma=sigma,
+                           amplitude=amp.astype(np.float32))
+
+
+__all__ = ["BatchedWorkload", "DmSegment", "GpuSearchPipeline", "SynthFilterbankSpec", "SynthPulsar", "Workload",
+           "batched_workload", "channel_frequencies", "lband_workload", "pulse_amplitude", "small_workload",
            "smearing_dm_segments"]

@@ -80,6 +80,10 @@ int gaffa_hip_event_destroy(gaffa_hip_event event);
 int gaffa_hip_event_record(gaffa_hip_event event, gaffa_hip_stream stream);
 int gaffa_hip_event_elapsed_ms(gaffa_hip_event start, gaffa_hip_event stop,
                                float* ms);
+/* dst[0 .. count) = value with a kernel (graph-safe reset of counters; no
+ * gaffa counterpart). */
+int gaffa_hip_fill_u32(uint32_t* dst, uint32_t value, uint64_t count,
+                       gaffa_hip_stream stream);
 /* Stream capture into a replayable graph (no counterpart in gaffa, whose
  * CUDA path launches kernels one by one). */
 int gaffa_hip_graph_begin(gaffa_hip_stream stream);
@@ -212,13 +216,18 @@ int gaffa_hip_detrend_normalise_batch(const gaffa_hip_preprocess_args* args,
 /* ------------------------------------------------------------------ FFA ----
  * Device-side layout of one FFA execution plan (host-built, see
  * gaffa_amd/csrc/host/ffa_gpu.cpp).  A plan is shared by every series of a
- * batch.  Per series, `prepared` holds each prepare group's downsampled copy
+ * batch; a batch of series with different lengths and plans is run as one
+ * "series" row holding all of them, each group naming its input's offset and
+ * length and each task the series its peaks belong to.  Per series, `prepared` holds each prepare group's downsampled copy
  * at prep_offset[group] (factor-1 groups read the input series directly) and
  * `work` holds two ping-pong regions of rows*bins floats per task. */
 typedef struct gaffa_hip_ffa_group {
   double factor;            /* downsample factor, 1.0 = no copy */
   uint64_t prepared_nsamples;
   uint64_t prep_offset;     /* float offset inside a series' prepared region */
+  uint64_t input_offset;    /* float offset of this group's input series in a
+                               series row (0 when every series has one plan) */
+  uint64_t input_nsamples;  /* length of that input series */
 } gaffa_hip_ffa_group;
 
 #define GAFFA_HIP_FFA_MAX_WIDTHS 16
@@ -232,7 +241,8 @@ typedef struct gaffa_hip_ffa_task {
   uint32_t nwidths;
   uint64_t work_offset;     /* float offset of this task's rows*bins region */
   float inv_stdnoise;       /* 1.0f / ffa_task_stdnoise(task) */
-  float pad_;
+  uint32_t series;          /* added to the peak's series index (batches of
+                               series with their own plans in one row) */
   uint32_t width[GAFFA_HIP_FFA_MAX_WIDTHS];
   float height[GAFFA_HIP_FFA_MAX_WIDTHS];
   float baseline[GAFFA_HIP_FFA_MAX_WIDTHS];
@@ -278,6 +288,11 @@ typedef struct gaffa_hip_ffa_plan_desc {
   uint64_t prepared_stride; /* floats per series in `prepared` */
   uint64_t work_stride;     /* floats per series per work region */
 } gaffa_hip_ffa_plan_desc;
+
+/* Raises the launch limits (dynamic LDS) the plan's kernels need, so that a
+ * later stream capture of its launches records kernels only.  No gaffa
+ * counterpart. */
+int gaffa_hip_ffa_reserve_launch(const gaffa_hip_ffa_plan_desc* plan);
 
 /* Geometry the host needs to build a plan descriptor: rows per terminal and
  * level work item, prepared outputs per prepare workgroup for a factor, and
@@ -360,6 +375,29 @@ typedef struct gaffa_hip_synth_args {
   const int64_t* phase0_q; /* device [npulsars] */
   const int32_t* rfi_q;    /* device [nchans] */
 } gaffa_hip_synth_args;
+
+/* Seeded batch of float time series packed in one row (no gaffa
+ * counterpart; feeds the batched FFA): series s occupies
+ * out[offsets[s] .. offsets[s] + lengths[s]) with N(0,1) noise (hashed
+ * Box-Muller) plus, where amplitude[s] != 0, a Gaussian pulse train of the
+ * given period (s), start phase and phase sigma; the rest of the row up to
+ * row_length (padding) is 0.  offsets ascending, offsets[0] == 0. */
+typedef struct gaffa_hip_synth_series_args {
+  float* out;
+  uint64_t row_length;
+  uint32_t nseries;
+  uint64_t key;
+  double tsamp;
+  const uint64_t* offsets;  /* device [nseries] */
+  const uint64_t* lengths;  /* device [nseries] */
+  const double* period;     /* device [nseries] */
+  const double* phase0;     /* device [nseries] */
+  const float* sigma;       /* device [nseries], phase units */
+  const float* amplitude;   /* device [nseries], 0 = noise only */
+} gaffa_hip_synth_series_args;
+
+int gaffa_hip_synth_series(const gaffa_hip_synth_series_args* args,
+                           gaffa_hip_stream stream);
 
 int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* args,
                                gaffa_hip_stream stream);

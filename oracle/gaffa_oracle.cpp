@@ -1249,3 +1249,18 @@ int orc_synth_filterbank(uint64_t nsamples, uint64_t nchans, double fch1,
 }
 
 }  // extern "C"
+
+// Candidate selection and harmonic flagging are host code in the product
+// itself; they are checked against the reference build, not restated here.
+int orc_select_candidates(const orc_dm_peak*, uint64_t, double, double, uint64_t, int32_t,
+                          uint64_t, orc_candidate*, uint64_t, uint64_t*) {
+  g_error = "candidate selection is checked against the reference build";
+  return ORC_OTHER;
+}
+
+int orc_flag_harmonics(const orc_candidate*, uint64_t, double, double, double,
+                       const orc_harmonic_options*, orc_candidate*, orc_harmonic*, uint64_t,
+                       uint64_t*) {
+  g_error = "harmonic flagging is checked against the reference build";
+  return ORC_OTHER;
+}

@@ -119,6 +119,32 @@ PEAK_DTYPE = np.dtype(
 assert PEAK_DTYPE.itemsize == C.sizeof(Peak)
 DM_PEAK_DTYPE = np.dtype([("dm", "f8"), ("dm_index", "u8"), ("peak", PEAK_DTYPE)], align=True)
 assert DM_PEAK_DTYPE.itemsize == C.sizeof(DmPeak)
+CANDIDATE_DTYPE = np.dtype(
+    [
+        ("dm", "f8"), ("dm_index", "u8"), ("period", "f8"), ("frequency", "f8"),
+        ("width", "u8"), ("duty_cycle", "f8"), ("snr", "f4"), ("peak_count", "u8"),
+        ("dm_index_min", "u8"), ("dm_index_max", "u8"), ("frequency_min", "f8"),
+        ("frequency_max", "f8"), ("best_peak", DM_PEAK_DTYPE),
+    ],
+    align=True,
+)
+HARMONIC_DTYPE = np.dtype(
+    [
+        ("is_harmonic", "i4"), ("parent_index", "u8"), ("numerator", "i4"),
+        ("denominator", "i4"), ("frequency_error_bins", "f8"), ("phase_distance", "f8"),
+        ("dm_distance", "f8"), ("expected_snr", "f8"), ("snr_distance", "f8"),
+    ],
+    align=True,
+)
+
+
+class HarmonicOptions(C.Structure):
+    _fields_ = [("max_harmonic", C.c_uint64), ("denominator_max", C.c_uint64),
+                ("frequency_tolerance_bins", C.c_double), ("phase_distance_max", C.c_double),
+                ("dm_distance_max", C.c_double), ("use_snr_consistency", C.c_int32),
+                ("snr_distance_max", C.c_double)]
+
+
 TASK_DTYPE = np.dtype(
     [
         ("downsample_factor", "f8"),
@@ -411,6 +437,35 @@ class Oracle:
         if rc != 0:
             raise ERRORS.get(rc, RuntimeError)(self.lib.orc_last_error().decode())
         return out[: count.value].copy()
+
+    # -- candidates (reference build only) -------------------------------
+    def select_candidates(self, peaks, observation_seconds, frequency_cluster_radius=0.1,
+                          dm_cluster_radius=1, cluster_across_widths=True, max_candidates=0):
+        peaks = np.ascontiguousarray(peaks, dtype=DM_PEAK_DTYPE)
+        out = np.zeros(max(1, peaks.size), dtype=CANDIDATE_DTYPE)
+        count = C.c_uint64()
+        self._call(self.lib.orc_select_candidates, _ptr(peaks), C.c_uint64(peaks.size),
+                   C.c_double(observation_seconds), C.c_double(frequency_cluster_radius),
+                   C.c_uint64(dm_cluster_radius), C.c_int32(1 if cluster_across_widths else 0),
+                   C.c_uint64(max_candidates), _ptr(out), C.c_uint64(out.size), C.byref(count))
+        return out[: count.value].copy()
+
+    def flag_harmonics(self, candidates, observation_seconds, frequency_low_mhz,
+                       frequency_high_mhz, max_harmonic=16, denominator_max=5,
+                       frequency_tolerance_bins=1.5, phase_distance_max=1.0,
+                       dm_distance_max=3.0, use_snr_consistency=False, snr_distance_max=3.0):
+        cands = np.ascontiguousarray(candidates, dtype=CANDIDATE_DTYPE)
+        opts = HarmonicOptions(max_harmonic, denominator_max, frequency_tolerance_bins,
+                               phase_distance_max, dm_distance_max,
+                               1 if use_snr_consistency else 0, snr_distance_max)
+        out = np.zeros(max(1, cands.size), dtype=CANDIDATE_DTYPE)
+        rel = np.zeros(max(1, cands.size), dtype=HARMONIC_DTYPE)
+        count = C.c_uint64()
+        self._call(self.lib.orc_flag_harmonics, _ptr(cands), C.c_uint64(cands.size),
+                   C.c_double(observation_seconds), C.c_double(frequency_low_mhz),
+                   C.c_double(frequency_high_mhz), C.byref(opts), _ptr(out), _ptr(rel),
+                   C.c_uint64(out.size), C.byref(count))
+        return out[: count.value].copy(), rel[: count.value].copy()
 
     # -- synthetic data (port only) ---------------------------------------
     def synth_filterbank(self, nsamples, nchans, fch1, foff, tsamp, seed, pulsars=(),

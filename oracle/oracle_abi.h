@@ -71,6 +71,46 @@ typedef struct orc_dm_peak {
   orc_peak peak;
 } orc_dm_peak;
 
+/* gaffa::Candidate (candidate.h:12-28). */
+typedef struct orc_candidate {
+  double dm;
+  uint64_t dm_index;
+  double period;
+  double frequency;
+  uint64_t width;
+  double duty_cycle;
+  float snr;
+  uint64_t peak_count;
+  uint64_t dm_index_min;
+  uint64_t dm_index_max;
+  double frequency_min;
+  double frequency_max;
+  orc_dm_peak best_peak;
+} orc_candidate;
+
+/* gaffa::HarmonicOptions and HarmonicRelation (harmonic.h:19-48). */
+typedef struct orc_harmonic_options {
+  uint64_t max_harmonic;
+  uint64_t denominator_max;
+  double frequency_tolerance_bins;
+  double phase_distance_max;
+  double dm_distance_max;
+  int32_t use_snr_consistency;
+  double snr_distance_max;
+} orc_harmonic_options;
+
+typedef struct orc_harmonic {
+  int32_t is_harmonic;
+  uint64_t parent_index;
+  int32_t numerator;
+  int32_t denominator;
+  double frequency_error_bins;
+  double phase_distance;
+  double dm_distance;
+  double expected_snr;
+  double snr_distance;
+} orc_harmonic;
+
 /* gaffa::RiptideFfaPlanOptions (ffa_plan.h:34-47). */
 typedef struct orc_plan_options {
   double period_min;
@@ -163,6 +203,21 @@ int orc_search_ffa(const float* series, uint64_t n, const orc_task* tasks,
                    float snr_threshold, uint64_t max_peaks, orc_peak* peaks,
                    uint64_t capacity, uint64_t* count);
 int orc_sort_peaks(orc_peak* peaks, uint64_t count);
+
+/* ---- candidates (candidate.cpp:175-212, harmonic.cpp:245-311) ----
+ * Only the reference build implements these (the port returns ORC_OTHER):
+ * the product's own host code is the restatement, checked against it. */
+int orc_select_candidates(const orc_dm_peak* peaks, uint64_t npeaks,
+                          double observation_seconds, double frequency_radius,
+                          uint64_t dm_radius, int32_t across_widths,
+                          uint64_t max_candidates, orc_candidate* out,
+                          uint64_t capacity, uint64_t* count);
+int orc_flag_harmonics(const orc_candidate* candidates, uint64_t n,
+                       double observation_seconds, double frequency_low_mhz,
+                       double frequency_high_mhz,
+                       const orc_harmonic_options* options,
+                       orc_candidate* out, orc_harmonic* relations,
+                       uint64_t capacity, uint64_t* count);
 
 /* ---- DM search (dm_search.cpp) ---- */
 int orc_find_dm_peaks(const uint32_t* plane, uint64_t ndm, uint64_t nsamples,

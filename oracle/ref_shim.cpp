@@ -6,7 +6,9 @@
 
 #include "oracle_abi.h"
 
+#include "gaffa/candidate.h"
 #include "gaffa/dedispersion.h"
+#include "gaffa/harmonic.h"
 #include "gaffa/dm_search.h"
 #include "gaffa/ffa.h"
 #include "gaffa/ffa_detection.h"
@@ -427,6 +429,91 @@ int orc_synth_filterbank(uint64_t, uint64_t, double, double, double, uint64_t,
                          int32_t, uint64_t, uint64_t, uint8_t*) {
   g_error = "no reference counterpart";
   return ORC_OTHER;
+}
+
+namespace {
+
+gaffa::DmPeak to_dm_peak(const orc_dm_peak& p) {
+  gaffa::DmPeak d;
+  d.dm = p.dm;
+  d.dm_index = p.dm_index;
+  d.peak = gaffa::FfaPeak{p.peak.period,       p.peak.frequency,    p.peak.width,
+                          p.peak.duty_cycle,   p.peak.width_index,  p.peak.period_index,
+                          p.peak.phase,        p.peak.shift,        p.peak.bins,
+                          p.peak.snr};
+  return d;
+}
+
+orc_candidate from_candidate(const gaffa::Candidate& c) {
+  return orc_candidate{c.dm, c.dm_index, c.period, c.frequency, c.width, c.duty_cycle, c.snr,
+                       c.peak_count, c.dm_index_min, c.dm_index_max, c.frequency_min,
+                       c.frequency_max,
+                       orc_dm_peak{c.best_peak.dm, c.best_peak.dm_index,
+                                   from_peak(c.best_peak.peak)}};
+}
+
+gaffa::Candidate to_candidate(const orc_candidate& c) {
+  gaffa::Candidate d;
+  d.dm = c.dm;
+  d.dm_index = c.dm_index;
+  d.period = c.period;
+  d.frequency = c.frequency;
+  d.width = c.width;
+  d.duty_cycle = c.duty_cycle;
+  d.snr = c.snr;
+  d.peak_count = c.peak_count;
+  d.dm_index_min = c.dm_index_min;
+  d.dm_index_max = c.dm_index_max;
+  d.frequency_min = c.frequency_min;
+  d.frequency_max = c.frequency_max;
+  d.best_peak = to_dm_peak(c.best_peak);
+  return d;
+}
+
+}  // namespace
+
+int orc_select_candidates(const orc_dm_peak* peaks, uint64_t n, double obs, double radius,
+                          uint64_t dm_radius, int32_t across, uint64_t max_candidates,
+                          orc_candidate* out, uint64_t cap, uint64_t* count) {
+  return guarded([&] {
+    std::vector<gaffa::DmPeak> v;
+    for (uint64_t i = 0; i < n; ++i) v.push_back(to_dm_peak(peaks[i]));
+    gaffa::CandidateSelectionOptions o;
+    o.frequency_cluster_radius = radius;
+    o.dm_cluster_radius = dm_radius;
+    o.cluster_across_widths = across != 0;
+    o.max_candidates = max_candidates;
+    std::vector<orc_candidate> r;
+    for (const auto& c : gaffa::select_candidates_cpu(v, obs, o)) r.push_back(from_candidate(c));
+    emit(r, out, cap, count);
+  });
+}
+
+int orc_flag_harmonics(const orc_candidate* cands, uint64_t n, double obs, double flo,
+                       double fhi, const orc_harmonic_options* ho, orc_candidate* out,
+                       orc_harmonic* rel, uint64_t cap, uint64_t* count) {
+  return guarded([&] {
+    std::vector<gaffa::Candidate> v;
+    for (uint64_t i = 0; i < n; ++i) v.push_back(to_candidate(cands[i]));
+    gaffa::HarmonicOptions o;
+    o.max_harmonic = ho->max_harmonic;
+    o.denominator_max = ho->denominator_max;
+    o.frequency_tolerance_bins = ho->frequency_tolerance_bins;
+    o.phase_distance_max = ho->phase_distance_max;
+    o.dm_distance_max = ho->dm_distance_max;
+    o.use_snr_consistency = ho->use_snr_consistency != 0;
+    o.snr_distance_max = ho->snr_distance_max;
+    const auto flagged = gaffa::flag_harmonics_cpu(v, gaffa::HarmonicContext{obs, flo, fhi}, o);
+    *count = flagged.size();
+    if (flagged.size() > cap) throw too_small{};
+    for (std::size_t i = 0; i < flagged.size(); ++i) {
+      out[i] = from_candidate(flagged[i].candidate);
+      const auto& h = flagged[i].harmonic;
+      rel[i] = orc_harmonic{h.is_harmonic ? 1 : 0, h.parent_index, h.numerator, h.denominator,
+                            h.frequency_error_bins, h.phase_distance, h.dm_distance,
+                            h.expected_snr, h.snr_distance};
+    }
+  });
 }
 
 }  // extern "C"

@@ -342,3 +342,59 @@ def test_small_pipeline_end_to_end(gpu, port):
         assert abs(p.peak.snr - q["peak"]["snr"]) <= 1e-5 * abs(q["peak"]["snr"])
     # the injected pulsar (P 0.714 s, DM 30) is the top candidate
     assert abs(got[0].peak.period - 0.714) < 0.005 and abs(got[0].dm - 30.0) <= 3.0
+
+
+def _check_series_peaks(got, want, s):
+    mine = [p for i, p in got if i == s]
+    assert len(mine) == len(want), (s, len(mine), len(want))
+    for p, q in zip(mine, want):
+        assert (p.shift, p.phase, p.width, p.bins, np.float32(p.snr)) == \
+            (q["shift"], q["phase"], q["width"], q["bins"], q["snr"])
+
+
+def test_ffa_batched_heterogeneous_bit_exact(gpu, port):
+    """Seven series of different lengths, each with its own octave plan, in
+    uneven tiles of 3: one launch sequence per tile, replayed from a HIP graph,
+    and the per-series launch loop, all equal to the oracle's search of the
+    same bytes."""
+    from gaffa_amd.pipeline import batched_workload
+
+    say("ffa batch: 7 heterogeneous series, tiles of 3, graph + loop")
+    w = batched_workload(7, seed=5, log2_min=15, log2_max=17, tsamp=256e-6, injected=0.5)
+    plans = w.plans()
+    b = gpu.FfaBatchedSearch(plans, series_per_tile=3, workspace_bytes=1 << 30)
+    b.generate(w.seed, w.tsamp, list(w.period), list(w.phase0), list(w.sigma),
+               list(w.amplitude))
+    row = b.download_row()
+    offs, lens = list(b.offsets), list(b.lengths)
+    for s in range(7):
+        x = row[offs[s]: offs[s] + lens[s]]
+        assert abs(float(x.mean())) < 0.05 and abs(float(x.std()) - 1.0) < 0.2
+        pad_end = offs[s + 1] if s + 1 < 7 else b.row_length
+        assert not row[offs[s] + lens[s]: pad_end].any()
+    first, timing = b.run(6.0)
+    assert timing["tiles"] == 3
+    replay, _ = b.run(6.0)
+    loop, t_loop = b.run(6.0, per_series=True)
+    assert t_loop["tiles"] == 7
+    for s in range(7):
+        x = row[offs[s]: offs[s] + lens[s]]
+        want = port.search_ffa(x, Plan(plans[s].tasks, np.array(plans[s].width_trials,
+                                                              np.uint64)), 6.0)
+        for got in (first, replay, loop):
+            _check_series_peaks(got, want, s)
+    say(f"ffa batch: batched {timing['gpu_ms']:.2f} ms, loop {t_loop['gpu_ms']:.2f} ms")
+
+
+def test_ffa_batched_autotune_keeps_results(gpu):
+    from gaffa_amd.pipeline import batched_workload
+
+    say("ffa batch: autotune over 1/2/4 series per tile")
+    w = batched_workload(5, seed=9, log2_min=15, log2_max=16, tsamp=256e-6, injected=1.0)
+    b = w.search(series_per_tile=5, workspace_bytes=1 << 30)
+    before, _ = b.run(6.0)
+    best = b.autotune([1, 2, 4])
+    assert best in (1, 2, 4) and b.series_per_tile == best
+    after, _ = b.run(6.0)
+    assert [(i, p.shift, p.phase, p.width, p.bins, p.snr) for i, p in before] == \
+        [(i, p.shift, p.phase, p.width, p.bins, p.snr) for i, p in after]
