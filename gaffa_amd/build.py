@@ -26,10 +26,11 @@ INCLUDE = os.path.join(ROOT, "include")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
-HIP_SOURCES = ["runtime.hip", "dedisperse.hip", "preprocess.hip", "ffa.hip", "synth.hip"]
+HIP_SOURCES = ["runtime.hip", "dedisperse.hip", "preprocess.hip", "ffa.hip", "synth.hip",
+               "fold.hip"]
 HOST_SOURCES = ["host/dedispersion.cpp", "host/preprocessing.cpp", "host/ffa_plan.cpp",
                 "host/ffa_gpu.cpp", "host/hip_runtime.cpp", "host/synth.cpp", "host/pipeline.cpp",
-                "host/ffa_batch.cpp", "host/candidate.cpp",
+                "host/ffa_batch.cpp", "host/candidate.cpp", "host/folding.cpp",
                 "host/bindings.cpp"]
 
 # gfx950 only.  -ffp-contract=off keeps every FP32/FP64 operation separately

@@ -7,6 +7,7 @@
 #include "gaffa/dedispersion.h"
 #include "gaffa/ffa_batch.h"
 #include "gaffa/ffa_gpu.h"
+#include "gaffa/folding.h"
 #include "gaffa/pipeline.h"
 #include "gaffa/preprocessing.h"
 #include "gaffa/synth.h"
@@ -398,6 +399,107 @@ py::dict stage_dict(const StageTimes& t) {
 
 }  // namespace
 
+struct PyFoldedProfile {
+  py::array profile, exposure, phase;
+  std::size_t nbin = 0;
+  double period = 0.0, tsamp = 0.0;
+};
+
+struct PyFoldResult {
+  py::array cube, exposure, profile, freq_phase, time_phase, phase, time;
+  std::size_t nsubint = 0, nchans = 0, nbin = 0;
+  double period = 0.0, tsamp = 0.0, tsubint = 0.0;
+};
+
+std::vector<float> phase_axis(std::size_t nbin) {
+  std::vector<float> p(nbin);
+  for (std::size_t i = 0; i < nbin; ++i)
+    p[i] = static_cast<float>(static_cast<double>(i) / static_cast<double>(nbin));
+  return p;
+}
+
+FoldDtype fold_dtype(const py::array& a) {
+  if (py::isinstance<py::array_t<std::uint8_t>>(a)) return FoldDtype::U8;
+  if (py::isinstance<py::array_t<std::uint16_t>>(a)) return FoldDtype::U16;
+  if (py::isinstance<py::array_t<std::uint32_t>>(a)) return FoldDtype::U32;
+  if (py::isinstance<py::array_t<float>>(a)) return FoldDtype::F32;
+  throw py::type_error("fold input dtype must be uint8, uint16, uint32 or float32");
+}
+
+PyFoldedProfile py_fold_profile(const py::array& data, double tsamp, double period,
+                                std::size_t nbin, std::size_t dm_index, int device_id) {
+  const py::array c = py::array::ensure(data, py::array::c_style);
+  if (!c || (c.ndim() != 1 && c.ndim() != 2))
+    throw std::invalid_argument("fold input must be a C-contiguous 1D or 2D array");
+  const FoldDtype dt = fold_dtype(c);
+  const std::size_t rows = c.ndim() == 2 ? static_cast<std::size_t>(c.shape(0)) : 1;
+  const std::size_t n = static_cast<std::size_t>(c.shape(c.ndim() - 1));
+  if (dm_index >= rows) throw std::out_of_range("dm_index out of range");
+  if (n == 0) throw std::invalid_argument("fold input must not be empty");
+  const std::size_t esize = static_cast<std::size_t>(c.itemsize());
+  const std::uint8_t* src = static_cast<const std::uint8_t*>(c.data()) + dm_index * n * esize;
+  FoldedProfile r;
+  {
+    py::gil_scoped_release release;  // no Python object is touched below
+    GpuDeviceScope scope(device_id);
+    GpuDeviceBuffer<std::uint8_t> d(n * esize);
+    copy_to_device(d.data(), src, n * esize, nullptr);
+    r = fold_time_series_gpu(GpuFoldInput{d.data(), dt, n, 1, 1, device_id},
+                             FoldOptions{period, tsamp, nbin, 0});
+  }
+  PyFoldedProfile p;
+  p.profile = owning_array(std::move(r.profile), {static_cast<py::ssize_t>(nbin)});
+  p.exposure = owning_array(std::move(r.exposure), {static_cast<py::ssize_t>(nbin)});
+  p.phase = owning_array(phase_axis(nbin), {static_cast<py::ssize_t>(nbin)});
+  p.nbin = nbin;
+  p.period = period;
+  p.tsamp = tsamp;
+  return p;
+}
+
+PyFoldResult py_fold_spectrum(const py::array& data, double tsamp, double period,
+                              std::size_t nbin, double tsubint, std::size_t output_channels,
+                              int device_id) {
+  const py::array c = py::array::ensure(data, py::array::c_style);
+  if (!c || c.ndim() != 2)
+    throw std::invalid_argument("fold spectrum must be a C-contiguous (nsamples, nchans) array");
+  const FoldDtype dt = fold_dtype(c);
+  if (dt == FoldDtype::U32) throw py::type_error("fold spectrum dtype must be uint8, uint16 or float32");
+  const std::size_t ns = static_cast<std::size_t>(c.shape(0));
+  const std::size_t nc = static_cast<std::size_t>(c.shape(1));
+  const std::size_t nbytes = static_cast<std::size_t>(c.nbytes());
+  const void* src = c.data();
+  FoldDedispersedSpectrumResult r;
+  {
+    py::gil_scoped_release release;  // no Python object is touched below
+    GpuDeviceScope scope(device_id);
+    GpuDeviceBuffer<std::uint8_t> d(nbytes);
+    copy_to_device(d.data(), src, nbytes, nullptr);
+    r = fold_dedispersed_spectrum_gpu(GpuFoldInput{d.data(), dt, ns, nc, nc, device_id}, tsamp,
+                                      FoldDedispersedSpectrumOptions{period, nbin, tsubint,
+                                                                     output_channels});
+  }
+  PyFoldResult p;
+  const auto S = static_cast<py::ssize_t>(r.nsubint), C = static_cast<py::ssize_t>(r.nchans),
+             B = static_cast<py::ssize_t>(r.nbin);
+  p.cube = owning_array(std::move(r.cube.data), {S, C, B});
+  p.exposure = owning_array(std::move(r.cube.exposure), {S, B});
+  p.profile = owning_array(std::move(r.profile), {B});
+  p.freq_phase = owning_array(std::move(r.freq_phase), {C, B});
+  p.time_phase = owning_array(std::move(r.time_phase), {S, B});
+  p.phase = owning_array(phase_axis(r.nbin), {B});
+  std::vector<double> t(r.nsubint);
+  for (std::size_t i = 0; i < r.nsubint; ++i) t[i] = (static_cast<double>(i) + 0.5) * r.tsubint;
+  p.time = owning_array(std::move(t), {S});
+  p.nsubint = r.nsubint;
+  p.nchans = r.nchans;
+  p.nbin = r.nbin;
+  p.period = r.period;
+  p.tsamp = r.tsamp;
+  p.tsubint = r.tsubint;
+  return p;
+}
+
 PYBIND11_MODULE(_core, m) {
   m.doc() = "gaffa search hot path on AMD Instinct MI355X (HIP, gfx950)";
   PYBIND11_NUMPY_DTYPE(FfaSearchTask, downsample_factor, effective_tsamp, input_nsamples,
@@ -499,6 +601,35 @@ PYBIND11_MODULE(_core, m) {
   m.def("_ffa_transform_rows_gpu", &py_ffa_transform_rows, py::arg("batch"), py::arg("plan"),
         py::arg("task"), py::kw_only(), py::arg("device_id") = 0);
   m.def("ffa_task_stdnoise", &ffa_task_stdnoise);
+
+  // folding
+  py::class_<PyFoldedProfile>(m, "FoldedProfile")
+      .def_readonly("profile", &PyFoldedProfile::profile)
+      .def_readonly("exposure", &PyFoldedProfile::exposure)
+      .def_readonly("phase", &PyFoldedProfile::phase)
+      .def_readonly("nbin", &PyFoldedProfile::nbin)
+      .def_readonly("period", &PyFoldedProfile::period)
+      .def_readonly("tsamp", &PyFoldedProfile::tsamp);
+  py::class_<PyFoldResult>(m, "FoldResult")
+      .def_readonly("cube", &PyFoldResult::cube)
+      .def_readonly("exposure", &PyFoldResult::exposure)
+      .def_readonly("profile", &PyFoldResult::profile)
+      .def_readonly("freq_phase", &PyFoldResult::freq_phase)
+      .def_readonly("time_phase", &PyFoldResult::time_phase)
+      .def_readonly("phase", &PyFoldResult::phase)
+      .def_readonly("time", &PyFoldResult::time)
+      .def_readonly("nsubint", &PyFoldResult::nsubint)
+      .def_readonly("nchans", &PyFoldResult::nchans)
+      .def_readonly("nbin", &PyFoldResult::nbin)
+      .def_readonly("period", &PyFoldResult::period)
+      .def_readonly("tsamp", &PyFoldResult::tsamp)
+      .def_readonly("tsubint", &PyFoldResult::tsubint);
+  m.def("_fold_dedispersed_profile", &py_fold_profile, py::arg("data"), py::kw_only(),
+        py::arg("tsamp"), py::arg("period"), py::arg("nbin"), py::arg("dm_index") = 0,
+        py::arg("device_id") = 0);
+  m.def("_fold_dedispersed_spectrum", &py_fold_spectrum, py::arg("data"), py::kw_only(),
+        py::arg("tsamp"), py::arg("period"), py::arg("nbin"), py::arg("tsubint"),
+        py::arg("output_channels"), py::arg("device_id") = 0);
 
   // candidate selection and harmonic flagging (host code over the peak list)
   py::class_<Candidate>(m, "Candidate")
@@ -628,8 +759,9 @@ PYBIND11_MODULE(_core, m) {
            [](GpuFfaBatchedSearch& b, const py::array_t<float, py::array::c_style>& row) {
              if (static_cast<std::size_t>(row.size()) != b.row_length())
                throw std::invalid_argument("batch row length does not match");
+             const float* src = row.data();
              py::gil_scoped_release release;
-             b.upload(row.data(), b.row_length());
+             b.upload(src, b.row_length());
            })
       .def("download_row",
            [](GpuFfaBatchedSearch& b) {
@@ -899,6 +1031,34 @@ PYBIND11_MODULE(_core, m) {
           py::arg("dm_cluster_radius") = 1, py::arg("cluster_across_widths") = true,
           py::arg("max_candidates") = 0,
           "Candidates from the last search's peaks (observation = out_nsamples * tsamp)")
+      .def(
+          "fold_candidates",
+          [](GpuSearchPipeline& p, const std::vector<std::size_t>& dm_index,
+             const std::vector<double>& period, std::size_t nbin, std::size_t nsubint) {
+            if (dm_index.size() != period.size() || dm_index.empty())
+              throw std::invalid_argument("one DM index per period is required");
+            if (nsubint == 0) throw std::invalid_argument("fold nsubint must be positive");
+            std::vector<FoldJob> jobs;
+            for (std::size_t i = 0; i < dm_index.size(); ++i) {
+              if (dm_index[i] >= p.ndm()) throw std::out_of_range("dm_index out of range");
+              jobs.push_back(FoldJob{dm_index[i] * p.plane_stride(), period[i], 0});
+            }
+            const std::size_t n = p.out_nsamples();
+            const std::size_t sub = (n + nsubint - 1) / nsubint;
+            FoldedCube c;
+            {
+              py::gil_scoped_release release;
+              c = fold_batch_gpu(GpuFoldInput{p.plane_u32(), FoldDtype::F32, n, 1, 1, 0}, jobs,
+                                 p.tsamp(), nbin, sub, 1, nullptr, p.stream());
+            }
+            const auto J = static_cast<py::ssize_t>(jobs.size()),
+                       S = static_cast<py::ssize_t>(c.nsubint), B = static_cast<py::ssize_t>(nbin);
+            return py::make_tuple(owning_array(std::move(c.data), {J, S, B}),
+                                  owning_array(std::move(c.exposure), {J, S, B}));
+          },
+          py::arg("dm_index"), py::arg("period"), py::kw_only(), py::arg("nbin") = 64,
+          py::arg("nsubint") = 32,
+          "Subint x phase cubes of the preprocessed plane rows (after run())")
       .def("set_kernel_timing", &GpuSearchPipeline::set_kernel_timing, py::arg("on"),
            "Record HIP events between the FFA launch stages of every tile")
       .def("run", [](GpuSearchPipeline& p) {

@@ -357,6 +357,51 @@ int gaffa_hip_ffa_transform_rows(const gaffa_hip_ffa_search_args* args,
                                  uint32_t task, float* out,
                                  gaffa_hip_stream stream);
 
+/* ------------------------------------------------------------- folding ----
+ * Fractional-phase-overlap folds (folding.cpp:300-420): sample t of a job's
+ * input (data + input_offset + t * row_stride, channels contiguous) covers
+ * phase [i s, i s + s) bins, i = start_sample + t, s = tsamp / period * nbin,
+ * and adds value * overlap (FP64) to each overlapped bin; out = sum /
+ * exposure where exposure > 0, else 0.  Adjacent input channels are averaged
+ * into output_channels (nchans % output_channels == 0).  Layouts:
+ * out [njobs][nsubint][output_channels][nbin] float, exposure
+ * [njobs][nsubint][nbin] double (may be NULL).  nsubint = ceil(nsamples /
+ * subint_samples).  Every FP64 sum is formed in the CPU's order, so results
+ * equal gaffa's fold_spectrum_cpu bit for bit when period > tsamp.
+ * Replaces fold_spectrum_cpu / fold_time_series_cpu for device data (gaffa
+ * folds on the host only). */
+typedef enum gaffa_hip_fold_dtype {
+  GAFFA_HIP_FOLD_U8 = 0,
+  GAFFA_HIP_FOLD_U16 = 1,
+  GAFFA_HIP_FOLD_U32 = 2,
+  GAFFA_HIP_FOLD_F32 = 3,
+} gaffa_hip_fold_dtype;
+
+typedef struct gaffa_hip_fold_job {
+  uint64_t input_offset; /* elements from data to this job's sample 0 */
+  double period;         /* seconds */
+  uint64_t start_sample; /* absolute index of sample 0 (phase reference) */
+} gaffa_hip_fold_job;
+
+typedef struct gaffa_hip_fold_args {
+  const void* data;
+  int32_t dtype;         /* gaffa_hip_fold_dtype */
+  uint64_t nsamples;     /* samples per job */
+  uint32_t nchans;
+  uint64_t row_stride;   /* elements between consecutive samples */
+  double tsamp;
+  uint32_t nbin;
+  uint64_t subint_samples;
+  uint64_t nsubint;
+  uint32_t output_channels;
+  const gaffa_hip_fold_job* jobs; /* device [njobs] */
+  uint32_t njobs;
+  float* out;
+  double* exposure;
+} gaffa_hip_fold_args;
+
+int gaffa_hip_fold_batch(const gaffa_hip_fold_args* args, gaffa_hip_stream stream);
+
 /* --------------------------------------------------- synthetic inputs ----
  * Seeded, integer-exact synthetic filterbank (no gaffa counterpart): sample
  * (t, c) = clip(pop1024 - 384 + floor((dither12 + signal_q + rfi_q)/4096)).

@@ -1264,3 +1264,16 @@ int orc_flag_harmonics(const orc_candidate*, uint64_t, double, double, double,
   g_error = "harmonic flagging is checked against the reference build";
   return ORC_OTHER;
 }
+
+// Folding on device data is checked against the reference build.
+int orc_fold_time_series(const void*, int32_t, uint64_t, double, double, uint64_t, uint64_t,
+                         float*, double*) {
+  g_error = "folding is checked against the reference build";
+  return ORC_OTHER;
+}
+
+int orc_fold_spectrum(const void*, int32_t, uint64_t, uint64_t, double, double, uint64_t,
+                      uint64_t, uint64_t, uint64_t, float*, double*, uint64_t, uint64_t*) {
+  g_error = "folding is checked against the reference build";
+  return ORC_OTHER;
+}

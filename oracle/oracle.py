@@ -438,6 +438,35 @@ class Oracle:
             raise ERRORS.get(rc, RuntimeError)(self.lib.orc_last_error().decode())
         return out[: count.value].copy()
 
+    # -- folding (reference build only) ----------------------------------
+    _FOLD_DTYPES = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2,
+                    np.dtype(np.float32): 3}
+
+    def fold_time_series(self, x, period, tsamp, nbin, start_sample=0):
+        x = np.ascontiguousarray(x)
+        prof = np.zeros(nbin, np.float32)
+        expo = np.zeros(nbin, np.float64)
+        self._call(self.lib.orc_fold_time_series, _ptr(x), C.c_int32(self._FOLD_DTYPES[x.dtype]),
+                   C.c_uint64(x.size), C.c_double(period), C.c_double(tsamp), C.c_uint64(nbin),
+                   C.c_uint64(start_sample), _ptr(prof), _ptr(expo))
+        return prof, expo
+
+    def fold_spectrum(self, x, period, tsamp, nbin, subint_samples, start_sample=0,
+                      output_channels=0):
+        x = np.ascontiguousarray(x)
+        ns, nc = x.shape
+        nsub = -(-ns // subint_samples)
+        nout = output_channels or nc
+        cube = np.zeros(nsub * nout * nbin, np.float32)
+        expo = np.zeros(nsub * nbin, np.float64)
+        count = C.c_uint64()
+        self._call(self.lib.orc_fold_spectrum, _ptr(x), C.c_int32(self._FOLD_DTYPES[x.dtype]),
+                   C.c_uint64(ns), C.c_uint64(nc), C.c_double(period), C.c_double(tsamp),
+                   C.c_uint64(nbin), C.c_uint64(subint_samples), C.c_uint64(start_sample),
+                   C.c_uint64(output_channels), _ptr(cube), _ptr(expo), C.c_uint64(cube.size),
+                   C.byref(count))
+        return cube.reshape(nsub, nout, nbin), expo.reshape(nsub, nbin)
+
     # -- candidates (reference build only) -------------------------------
     def select_candidates(self, peaks, observation_seconds, frequency_cluster_radius=0.1,
                           dm_cluster_radius=1, cluster_across_widths=True, max_candidates=0):

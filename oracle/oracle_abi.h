@@ -219,6 +219,18 @@ int orc_flag_harmonics(const orc_candidate* candidates, uint64_t n,
                        orc_candidate* out, orc_harmonic* relations,
                        uint64_t capacity, uint64_t* count);
 
+/* ---- folding (folding.cpp:300-470; reference build only) ----
+ * dtype: 0 u8, 1 u16, 2 u32 (time series only), 3 f32.  Time series:
+ * profile/exposure [nbin].  Spectrum: data [nsamples][nchans], cube
+ * [nsubint][out_chans][nbin], exposure [nsubint][nbin]. */
+int orc_fold_time_series(const void* data, int32_t dtype, uint64_t n, double period,
+                         double tsamp, uint64_t nbin, uint64_t start_sample,
+                         float* profile, double* exposure);
+int orc_fold_spectrum(const void* data, int32_t dtype, uint64_t nsamples, uint64_t nchans,
+                      double period, double tsamp, uint64_t nbin, uint64_t subint_samples,
+                      uint64_t start_sample, uint64_t output_channels, float* cube,
+                      double* exposure, uint64_t cube_capacity, uint64_t* nsubint);
+
 /* ---- DM search (dm_search.cpp) ---- */
 int orc_find_dm_peaks(const uint32_t* plane, uint64_t ndm, uint64_t nsamples,
                       const double* dms, double tsamp,

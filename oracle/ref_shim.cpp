@@ -11,6 +11,7 @@
 #include "gaffa/harmonic.h"
 #include "gaffa/dm_search.h"
 #include "gaffa/ffa.h"
+#include "gaffa/folding.h"
 #include "gaffa/ffa_detection.h"
 #include "gaffa/ffa_executor.h"
 #include "gaffa/ffa_peak.h"
@@ -471,6 +472,52 @@ gaffa::Candidate to_candidate(const orc_candidate& c) {
 }
 
 }  // namespace
+
+int orc_fold_time_series(const void* data, int32_t dtype, uint64_t n, double period,
+                         double tsamp, uint64_t nbin, uint64_t start, float* profile,
+                         double* exposure) {
+  return guarded([&] {
+    const gaffa::FoldOptions o{period, tsamp, nbin, start};
+    gaffa::FoldedProfile r;
+    if (dtype == 2) {
+      r = gaffa::fold_time_series_cpu(
+          std::span<const uint32_t>(static_cast<const uint32_t*>(data), n), o);
+    } else if (dtype == 3) {
+      r = gaffa::fold_time_series_cpu(std::span<const float>(static_cast<const float*>(data), n),
+                                      o);
+    } else {
+      throw std::invalid_argument("fold time series dtype must be u32 or f32");
+    }
+    std::copy(r.profile.begin(), r.profile.end(), profile);
+    std::copy(r.exposure.begin(), r.exposure.end(), exposure);
+  });
+}
+
+int orc_fold_spectrum(const void* data, int32_t dtype, uint64_t ns, uint64_t nc, double period,
+                      double tsamp, uint64_t nbin, uint64_t sub, uint64_t start, uint64_t outc,
+                      float* cube, double* exposure, uint64_t cap, uint64_t* nsubint) {
+  return guarded([&] {
+    const gaffa::FoldSpectrumOptions o{period, tsamp, nbin, sub, start, outc};
+    const gaffa::SampleShape shape{ns, 1, nc};
+    gaffa::FoldedCube r;
+    if (dtype == 0) {
+      r = gaffa::fold_spectrum_cpu(gaffa::HostSampleView<uint8_t>{
+          std::span<const uint8_t>(static_cast<const uint8_t*>(data), ns * nc), shape}, o);
+    } else if (dtype == 1) {
+      r = gaffa::fold_spectrum_cpu(gaffa::HostSampleView<uint16_t>{
+          std::span<const uint16_t>(static_cast<const uint16_t*>(data), ns * nc), shape}, o);
+    } else if (dtype == 3) {
+      r = gaffa::fold_spectrum_cpu(gaffa::HostSampleView<float>{
+          std::span<const float>(static_cast<const float*>(data), ns * nc), shape}, o);
+    } else {
+      throw std::invalid_argument("fold spectrum dtype must be u8, u16 or f32");
+    }
+    *nsubint = r.nsubint;
+    if (r.data.size() > cap) throw too_small{};
+    std::copy(r.data.begin(), r.data.end(), cube);
+    std::copy(r.exposure.begin(), r.exposure.end(), exposure);
+  });
+}
 
 int orc_select_candidates(const orc_dm_peak* peaks, uint64_t n, double obs, double radius,
                           uint64_t dm_radius, int32_t across, uint64_t max_candidates,
