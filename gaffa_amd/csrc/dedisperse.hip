@@ -557,3 +557,136 @@ int gaffa_hip_dedisperse_check(gaffa_hip_stream stream) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------ spectrum ----
+// Aligned dynamic spectrum (dedispersion.cpp:249-289): out[t][c] =
+// x[t + delay[c]][chan_begin + c] for t < out_nsamples, any element size.  One
+// workgroup per (time tile, channel tile): the tile's input rows
+// [t0 + dmin, t0 + tile + dmax) x tile channels are staged in LDS with loads
+// along channels, then every output row of the tile is written with stores
+// along channels.
+namespace gaffa_hip {
+namespace {
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+spectrum_kernel(const T* __restrict__ x, uint64_t nsamples, uint64_t nchans, uint64_t chan_begin,
+                uint32_t nsel, const int32_t* __restrict__ delays, T* __restrict__ out,
+                uint64_t out_nsamples, uint32_t tile, uint32_t ctile, uint32_t slab_rows,
+                int32_t* __restrict__ error_flag) {
+  extern __shared__ uint32_t dyn_lds[];
+  T* slab = reinterpret_cast<T*>(dyn_lds);  // [slab_rows][ctile]
+  __shared__ int32_t sdel[256];
+  __shared__ int32_t smin, smax;
+  const uint32_t tid = threadIdx.x;
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * tile;
+  const uint32_t c0 = blockIdx.y * ctile;
+  const uint32_t nc = min(ctile, nsel - c0);
+  if (tid == 0) {
+    smin = 0x7fffffff;
+    smax = 0;
+  }
+  __syncthreads();
+  if (tid < nc) {
+    const int32_t d = delays[c0 + tid];
+    sdel[tid] = d;
+    atomicMin(&smin, d);
+    atomicMax(&smax, d);
+  }
+  __syncthreads();
+  const int32_t dmin = smin;
+  const uint32_t rows = tile + static_cast<uint32_t>(smax - dmin);
+  if (rows > slab_rows) {  // host/kernel table mismatch: refuse the block
+    if (tid == 0) atomicExch(error_flag, 1);
+    return;
+  }
+  for (uint32_t e = tid; e < rows * nc; e += kThreads) {
+    const uint32_t r = e / nc, c = e - r * nc;
+    const uint64_t t = t0 + static_cast<uint64_t>(dmin) + r;
+    slab[r * ctile + c] = t < nsamples ? x[t * nchans + chan_begin + c0 + c] : T(0);
+  }
+  __syncthreads();
+  const uint32_t tcount = static_cast<uint32_t>(min<uint64_t>(tile, out_nsamples - t0));
+  for (uint32_t e = tid; e < tcount * nc; e += kThreads) {
+    const uint32_t tl = e / nc, c = e - tl * nc;
+    out[(t0 + tl) * nsel + c0 + c] =
+        slab[(tl + static_cast<uint32_t>(sdel[c] - dmin)) * ctile + c];
+  }
+}
+
+}  // namespace
+}  // namespace gaffa_hip
+
+extern "C" int gaffa_hip_dedisperse_spectrum(const gaffa_hip_spectrum_args* a,
+                                             gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(a != nullptr && a->samples != nullptr && a->delays != nullptr && a->out != nullptr,
+          "dedispersion device pointers must not be null");
+    check(a->nsamples > 0 && a->nchans > 0, "dedispersion requires a non-empty filterbank");
+    check(a->chan_begin < a->chan_end && a->chan_end <= a->nchans,
+          "channel range exceeds sample channel count");
+    check(a->out_nsamples > 0 && a->out_nsamples <= a->nsamples,
+          "dedispersion valid output range is empty");
+    check(a->element_size == 1 || a->element_size == 2 || a->element_size == 4,
+          "spectrum element size must be 1, 2 or 4 bytes");
+    check(a->max_tile_spread >= 0, "dedispersion requires non-negative delays");
+    const uint64_t nsel = a->chan_end - a->chan_begin;
+    check_overflow(nsel <= 0xFFFFFFFFull, "dedispersion channel range too large");
+    // widest tile (time x channels) whose slab fits 96 KiB
+    const uint32_t ctile = a->channel_tile;
+    check(ctile == 8 || ctile == 16 || ctile == 32 || ctile == 64 || ctile == 128 || ctile == 256,
+          "spectrum channel tile must be a power of two in [8, 256]");
+    uint32_t tile = 0, slab_rows = 0;
+    for (uint32_t cand = 1024; cand >= 16 && tile == 0; cand /= 2) {
+      const uint64_t rows = cand + static_cast<uint64_t>(a->max_tile_spread);
+      if (rows * ctile * a->element_size <= 96 * 1024) {
+        tile = cand;
+        slab_rows = static_cast<uint32_t>(rows);
+      }
+    }
+    check_overflow(tile != 0, "spectrum delay spread too large for LDS; use a smaller channel tile");
+    const uint64_t tiles = ceil_div(a->out_nsamples, tile);
+    const uint64_t ctiles = ceil_div(nsel, ctile);
+    check_overflow(tiles < 0x7FFFFFFFull && ctiles < 65536, "spectrum grid too large");
+    const uint32_t lds = slab_rows * ctile * a->element_size;
+    const dim3 grid(static_cast<uint32_t>(tiles), static_cast<uint32_t>(ctiles));
+    const hipStream_t st = as_stream(stream);
+    int32_t* flag = error_flag_for_device();
+    switch (a->element_size) {
+      case 1:
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spectrum_kernel<uint8_t>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds)),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(spectrum_kernel<uint8_t>, grid, dim3(kThreads), lds, st,
+                           static_cast<const uint8_t*>(a->samples), a->nsamples, a->nchans,
+                           a->chan_begin, static_cast<uint32_t>(nsel), a->delays,
+                           static_cast<uint8_t*>(a->out), a->out_nsamples, tile, ctile, slab_rows,
+                           flag);
+        break;
+      case 2:
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spectrum_kernel<uint16_t>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds)),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(spectrum_kernel<uint16_t>, grid, dim3(kThreads), lds, st,
+                           static_cast<const uint16_t*>(a->samples), a->nsamples, a->nchans,
+                           a->chan_begin, static_cast<uint32_t>(nsel), a->delays,
+                           static_cast<uint16_t*>(a->out), a->out_nsamples, tile, ctile,
+                           slab_rows, flag);
+        break;
+      default:
+        hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(spectrum_kernel<uint32_t>),
+                                      hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      static_cast<int>(lds)),
+                  "hipFuncSetAttribute");
+        hipLaunchKernelGGL(spectrum_kernel<uint32_t>, grid, dim3(kThreads), lds, st,
+                           static_cast<const uint32_t*>(a->samples), a->nsamples, a->nchans,
+                           a->chan_begin, static_cast<uint32_t>(nsel), a->delays,
+                           static_cast<uint32_t*>(a->out), a->out_nsamples, tile, ctile,
+                           slab_rows, flag);
+        break;
+    }
+    launch_check("spectrum_kernel");
+  });
+}

@@ -399,6 +399,56 @@ py::dict stage_dict(const StageTimes& t) {
 
 }  // namespace
 
+struct PyDedispersedSpectrum {
+  py::array data;
+  std::string backend = "external";
+  double dm = 0.0, tsamp = 0.0;
+  std::size_t nsamples = 0, nchans = 0, chan_begin = 0, chan_end = 0;
+};
+
+PyDedispersedSpectrum py_spectrum_array(const py::array& data, const std::vector<double>& freq,
+                                        double tsamp, double dm, std::size_t chan_begin,
+                                        std::optional<std::size_t> chan_end, int device_id) {
+  const py::array c = py::array::ensure(data, py::array::c_style);
+  if (!c || c.ndim() != 2)
+    throw std::invalid_argument("spectrum input must be a C-contiguous (nsamples, nchans) array");
+  const std::size_t esize = static_cast<std::size_t>(c.itemsize());
+  const bool ok = py::isinstance<py::array_t<std::uint8_t>>(c) ||
+                  py::isinstance<py::array_t<std::uint16_t>>(c) ||
+                  py::isinstance<py::array_t<float>>(c);
+  if (!ok) throw py::type_error("spectrum input dtype must be uint8, uint16 or float32");
+  const SampleShape shape{static_cast<std::size_t>(c.shape(0)), 1,
+                          static_cast<std::size_t>(c.shape(1))};
+  const std::size_t ce = chan_end.value_or(shape.nchans);
+  const SingleDmDedispersionPlan plan{dm, reference_frequency(freq), tsamp, chan_begin, ce};
+  const std::size_t n_out = spectrum_output_nsamples(shape.nsamples, freq, plan);
+  const std::size_t nc = ce - chan_begin;
+  const void* src = c.data();
+  const std::size_t nbytes = static_cast<std::size_t>(c.nbytes());
+  py::array out(c.dtype(), {static_cast<py::ssize_t>(n_out), static_cast<py::ssize_t>(nc)});
+  void* dst = out.mutable_data();
+  {
+    py::gil_scoped_release release;  // no Python object is touched below
+    GpuDeviceScope scope(device_id);
+    GpuDeviceBuffer<std::uint8_t> d_in(nbytes);
+    copy_to_device(d_in.data(), src, nbytes, nullptr);
+    GpuDeviceBuffer<std::uint8_t> d_out(std::max<std::size_t>(1, n_out * nc * esize));
+    dedisperse_spectrum_gpu_raw(d_in.data(), esize, shape, freq, plan, d_out.data(), n_out,
+                                GpuDedispersionOptions{device_id, 256, 81920, nullptr});
+    copy_to_host(dst, d_out.data(), n_out * nc * esize, nullptr);
+  }
+  PyDedispersedSpectrum r;
+  r.data = out;
+  r.backend = "gpu";
+  r.dm = dm;
+  r.tsamp = tsamp;
+  r.nsamples = n_out;
+  r.nchans = nc;
+  r.chan_begin = chan_begin;
+  r.chan_end = ce;
+  return r;
+}
+
 struct PyFoldedProfile {
   py::array profile, exposure, phase;
   std::size_t nbin = 0;
@@ -601,6 +651,61 @@ PYBIND11_MODULE(_core, m) {
   m.def("_ffa_transform_rows_gpu", &py_ffa_transform_rows, py::arg("batch"), py::arg("plan"),
         py::arg("task"), py::kw_only(), py::arg("device_id") = 0);
   m.def("ffa_task_stdnoise", &ffa_task_stdnoise);
+
+  // aligned dynamic spectrum
+  py::class_<PyDedispersedSpectrum>(m, "DedispersedSpectrum")
+      .def(py::init([](const py::array& data, double tsamp, double dm, std::size_t chan_begin,
+                       std::optional<std::size_t> chan_end, std::string backend) {
+             const py::array c = py::array::ensure(data, py::array::c_style);
+             if (!c || c.ndim() != 2)
+               throw std::invalid_argument("spectrum data must be a C-contiguous 2D array");
+             if (!(tsamp > 0.0) || !std::isfinite(tsamp))
+               throw std::invalid_argument("spectrum tsamp must be finite and positive");
+             if (!(dm >= 0.0)) throw std::invalid_argument("spectrum dm must be non-negative");
+             PyDedispersedSpectrum r;
+             r.data = c;
+             r.backend = std::move(backend);
+             r.dm = dm;
+             r.tsamp = tsamp;
+             r.nsamples = static_cast<std::size_t>(c.shape(0));
+             r.nchans = static_cast<std::size_t>(c.shape(1));
+             r.chan_begin = chan_begin;
+             r.chan_end = chan_end.value_or(chan_begin + r.nchans);
+             if (r.chan_end - r.chan_begin != r.nchans)
+               throw std::invalid_argument("spectrum channel range must match nchans");
+             return r;
+           }),
+           py::arg("data"), py::kw_only(), py::arg("tsamp"), py::arg("dm") = 0.0,
+           py::arg("chan_begin") = 0, py::arg("chan_end") = py::none(),
+           py::arg("backend") = "external")
+      .def_readonly("data", &PyDedispersedSpectrum::data)
+      .def_readonly("backend", &PyDedispersedSpectrum::backend)
+      .def_readonly("dm", &PyDedispersedSpectrum::dm)
+      .def_readonly("tsamp", &PyDedispersedSpectrum::tsamp)
+      .def_readonly("nsamples", &PyDedispersedSpectrum::nsamples)
+      .def_readonly("nchans", &PyDedispersedSpectrum::nchans)
+      .def_readonly("chan_begin", &PyDedispersedSpectrum::chan_begin)
+      .def_readonly("chan_end", &PyDedispersedSpectrum::chan_end)
+      .def_property_readonly("shape", [](const PyDedispersedSpectrum& r) {
+        return py::make_tuple(r.nsamples, r.nchans);
+      });
+  m.def("_dedisperse_spectrum_array", &py_spectrum_array, py::arg("data"), py::kw_only(),
+        py::arg("frequency_mhz"), py::arg("tsamp"), py::arg("dm"), py::arg("chan_begin") = 0,
+        py::arg("chan_end") = py::none(), py::arg("device_id") = 0);
+  m.def(
+      "dedisperse_spectrum",
+      [](const py::object& fb, double dm, const std::string& backend, int device_id, std::size_t,
+         std::size_t) {
+        parse_backend(backend, "dedispersion");
+        FilterbankInput in = filterbank_input(fb);
+        const py::array flat = in.keep.reshape({static_cast<py::ssize_t>(in.samples.shape.nsamples),
+                                                static_cast<py::ssize_t>(in.samples.shape.nchans)});
+        return py_spectrum_array(flat, in.freq, in.tsamp, dm, 0, std::nullopt, device_id);
+      },
+      py::arg("filterbank"), py::kw_only(), py::arg("dm"), py::arg("backend") = "gpu",
+      py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
+      py::arg("time_tile_samples") = 81920,
+      "Aligned single-DM dynamic spectrum (valid samples only), on the GPU");
 
   // folding
   py::class_<PyFoldedProfile>(m, "FoldedProfile")

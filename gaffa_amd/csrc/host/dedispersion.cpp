@@ -195,6 +195,57 @@ void dedisperse_multi_dm_gpu_into(GpuSampleView samples, std::span<const double>
   hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_multi_u8");
 }
 
+std::size_t spectrum_output_nsamples(std::size_t nsamples, std::span<const double> f,
+                                     const SingleDmDedispersionPlan& p) {
+  const MultiDmDedispersionPlan m{p.dm, 1.0, 1, p.ref_frequency_mhz, p.tsamp, p.chan_begin,
+                                  p.chan_end};
+  return multi_dm_output_nsamples(nsamples, f, m);
+}
+
+void dedisperse_spectrum_gpu_raw(const void* samples, std::size_t esize, const SampleShape& shape,
+                                 std::span<const double> f, const SingleDmDedispersionPlan& p,
+                                 void* out, std::size_t out_nsamples,
+                                 const GpuDedispersionOptions& o) {
+  const MultiDmDedispersionPlan m{p.dm, 1.0, 1, p.ref_frequency_mhz, p.tsamp, p.chan_begin,
+                                  p.chan_end};
+  validate_multi(shape, samples != nullptr, f, m);
+  check_arg(shape.nifs == 1, "GPU dedispersion supports nifs == 1");
+  check_arg(out_nsamples == multi_dm_output_nsamples(shape.nsamples, f, m),
+            "spectrum output length must be the valid range");
+  const std::vector<std::int32_t> delays = make_multi_dm_delay_table(f, m);
+  for (std::int32_t d : delays) check_arg(d >= 0, "dedispersion requires non-negative delays");
+  // largest channel tile whose delay spread keeps a 64-sample tile in LDS
+  const std::size_t nc = delays.size();
+  std::uint32_t tile = 256;
+  std::int32_t spread = 0;
+  for (; tile >= 8; tile /= 2) {
+    spread = 0;
+    for (std::size_t c0 = 0; c0 < nc; c0 += tile) {
+      const auto b = delays.begin() + static_cast<std::ptrdiff_t>(c0);
+      const auto e = delays.begin() + static_cast<std::ptrdiff_t>(std::min(nc, c0 + tile));
+      spread = std::max(spread, *std::max_element(b, e) - *std::min_element(b, e));
+    }
+    if ((64 + static_cast<std::size_t>(spread)) * tile * esize <= 96 * 1024) break;
+  }
+  check_arg(tile >= 8, "spectrum delay spread too large for the GPU tile");
+  GpuDeviceScope scope(o.device_id);
+  GpuDeviceBuffer<std::int32_t> d_delays = upload(delays.data(), delays.size());
+  gaffa_hip_spectrum_args a{};
+  a.samples = samples;
+  a.nsamples = shape.nsamples;
+  a.nchans = shape.nchans;
+  a.chan_begin = p.chan_begin;
+  a.chan_end = p.chan_end;
+  a.element_size = static_cast<std::uint32_t>(esize);
+  a.delays = d_delays.data();
+  a.out = out;
+  a.out_nsamples = out_nsamples;
+  a.channel_tile = tile;
+  a.max_tile_spread = spread;
+  hip_ok(gaffa_hip_dedisperse_spectrum(&a, o.stream), "dedisperse_spectrum");
+  hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_spectrum");
+}
+
 std::size_t subband_channel_stride(std::size_t nsamples, std::span<const double> f,
                                    const MultiDmDedispersionPlan& p,
                                    const SubbandDedispersionOptions& so) {

@@ -162,6 +162,50 @@ GpuDedispersedResult<std::uint32_t> dedisperse_single_dm_gpu_device(
 // Writes into caller storage: ndm rows of `out_stride` elements, the first
 // `out_nsamples` of each (0 = the plan's own valid length; a smaller value
 // truncates, as a multi-segment DM plan needs a common length).
+// Aligned single-DM dynamic spectrum (dedispersion.h:117-142 in gaffa):
+// data [nsamples][nchans] of the input element type, valid samples only.
+template <typename T>
+struct DedispersedSpectrum {
+  std::vector<T> data;
+  SampleShape shape{};
+  double dm = 0.0;
+  double tsamp = 0.0;
+  std::size_t chan_begin = 0;
+  std::size_t chan_end = 0;
+};
+
+// out[t][c] = samples[t + delay(c)][chan_begin + c] for the plan's DM
+// (dedispersion.cpp:249-289), on the device.  element_size 1, 2 or 4.
+void dedisperse_spectrum_gpu_raw(const void* samples, std::size_t element_size,
+                                 const SampleShape& shape, std::span<const double> frequency_mhz,
+                                 const SingleDmDedispersionPlan& plan, void* out,
+                                 std::size_t out_nsamples, const GpuDedispersionOptions& options);
+std::size_t spectrum_output_nsamples(std::size_t nsamples, std::span<const double> frequency_mhz,
+                                     const SingleDmDedispersionPlan& plan);
+
+template <typename T>
+DedispersedSpectrum<T> dedisperse_spectrum_gpu(HostSampleView<T> samples,
+                                               std::span<const double> frequency_mhz,
+                                               const SingleDmDedispersionPlan& plan,
+                                               const GpuDedispersionOptions& options = {}) {
+  DedispersedSpectrum<T> r;
+  const std::size_t n_out = spectrum_output_nsamples(samples.shape.nsamples, frequency_mhz, plan);
+  const std::size_t nc = plan.chan_end - plan.chan_begin;
+  r.shape = SampleShape{n_out, 1, nc};
+  r.dm = plan.dm;
+  r.tsamp = plan.tsamp;
+  r.chan_begin = plan.chan_begin;
+  r.chan_end = plan.chan_end;
+  GpuDeviceScope scope(options.device_id);
+  GpuDeviceBuffer<T> in = upload(samples.data.data(), samples.data.size());
+  GpuDeviceBuffer<T> out(n_out * nc);
+  dedisperse_spectrum_gpu_raw(in.data(), sizeof(T), samples.shape, frequency_mhz, plan, out.data(),
+                              n_out, options);
+  r.data.resize(n_out * nc);
+  copy_to_host(r.data.data(), out.data(), r.data.size() * sizeof(T), options.stream);
+  return r;
+}
+
 // Channel-major device copy of a filterbank's channels [chan_begin,
 // chan_begin + nchans): row c holds channel chan_begin + c over time, zero
 // padded from nsamples to stride (the layout subband dedispersion reads).

@@ -172,6 +172,29 @@ typedef struct gaffa_hip_dedisperse_subband_args {
 int gaffa_hip_dedisperse_subband_u8(
     const gaffa_hip_dedisperse_subband_args* args, gaffa_hip_stream stream);
 
+/* Aligned dynamic spectrum for one DM: out[t][c] = samples[t + delays[c]]
+ * [chan_begin + c] for t < out_nsamples (out [out_nsamples][chan_end -
+ * chan_begin], element_size 1, 2 or 4 bytes, copied bit for bit).
+ * max_tile_spread bounds max - min of the delays inside any channel_tile
+ * consecutive channels (sizes the LDS slab).  Replaces
+ * dedisperse_spectrum_kernel (dedispersion_cuda.cu:295-318). */
+typedef struct gaffa_hip_spectrum_args {
+  const void* samples;   /* [nsamples][nchans] */
+  uint64_t nsamples;
+  uint64_t nchans;
+  uint64_t chan_begin;
+  uint64_t chan_end;
+  uint32_t element_size;
+  const int32_t* delays; /* [chan_end - chan_begin], >= 0 */
+  void* out;
+  uint64_t out_nsamples;
+  uint32_t channel_tile; /* 8..256, power of two */
+  int32_t max_tile_spread;
+} gaffa_hip_spectrum_args;
+
+int gaffa_hip_dedisperse_spectrum(const gaffa_hip_spectrum_args* args,
+                                  gaffa_hip_stream stream);
+
 /* Synchronises `stream` and reports (then clears) a refused dedispersion
  * block: a delay table whose spread exceeded the bound the caller passed.
  * No counterpart in gaffa (its kernels read global memory directly). */

@@ -34,6 +34,7 @@
 #include <limits>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -966,6 +967,35 @@ int orc_dedisperse_single(int32_t dtype, const void* samples, uint64_t nsamples,
       if (out == nullptr) return;
       if (capacity < r.size()) throw too_small();
       std::memcpy(out, r.data(), r.size() * sizeof(r[0]));
+    };
+    if (dtype == 0) run(static_cast<const uint8_t*>(samples));
+    else if (dtype == 1) run(static_cast<const uint16_t*>(samples));
+    else if (dtype == 2) run(static_cast<const float*>(samples));
+    else throw std::invalid_argument("unsupported sample dtype");
+  });
+}
+
+// dedispersion.cpp:249-289: each channel read at its own delay, valid range
+// only (every read is in range there).
+int orc_dedisperse_spectrum(int32_t dtype, const void* samples, uint64_t nsamples,
+                            uint64_t nchans, const double* freq, double dm, double ref,
+                            double tsamp, uint64_t cb, uint64_t ce, void* out,
+                            uint64_t capacity, uint64_t* out_nsamples) {
+  return guarded([&] {
+    validate_common(samples, nsamples, nchans, freq, ref, tsamp, cb, ce);
+    require(std::isfinite(dm) && dm >= 0.0, "dm must be finite and non-negative");
+    const uint64_t nc = ce - cb;
+    std::vector<int32_t> d(nc);
+    for (uint64_t c = 0; c < nc; ++c) d[c] = delay_bins(dm, freq[cb + c], ref, tsamp);
+    const uint64_t no = valid_nout(nsamples, delay_bins(dm, min_freq(freq, cb, ce), ref, tsamp));
+    *out_nsamples = no;
+    if (out == nullptr) return;
+    if (capacity < no * nc) throw too_small();
+    auto run = [&](const auto* x) {
+      using T = std::remove_const_t<std::remove_pointer_t<decltype(x)>>;
+      T* o = static_cast<T*>(out);
+      for (uint64_t t = 0; t < no; ++t)
+        for (uint64_t c = 0; c < nc; ++c) o[t * nc + c] = x[(t + d[c]) * nchans + cb + c];
     };
     if (dtype == 0) run(static_cast<const uint8_t*>(samples));
     else if (dtype == 1) run(static_cast<const uint16_t*>(samples));

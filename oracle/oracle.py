@@ -438,6 +438,24 @@ class Oracle:
             raise ERRORS.get(rc, RuntimeError)(self.lib.orc_last_error().decode())
         return out[: count.value].copy()
 
+    def dedisperse_spectrum(self, samples, freq, dm, tsamp, ref=None, cb=0, ce=None):
+        samples = np.ascontiguousarray(samples)
+        ns, nc = samples.shape
+        freq = np.ascontiguousarray(freq, dtype=np.float64)
+        ref = float(freq.max()) if ref is None else ref
+        ce = nc if ce is None else ce
+        dtype = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1,
+                 np.dtype(np.float32): 2}[samples.dtype]
+        args = (C.c_int32(dtype), _ptr(samples), C.c_uint64(ns), C.c_uint64(nc), _ptr(freq),
+                C.c_double(dm), C.c_double(ref), C.c_double(tsamp), C.c_uint64(cb),
+                C.c_uint64(ce))
+        no = C.c_uint64()
+        self._call(self.lib.orc_dedisperse_spectrum, *args, None, C.c_uint64(0), C.byref(no))
+        out = np.zeros((no.value, ce - cb), dtype=samples.dtype)
+        self._call(self.lib.orc_dedisperse_spectrum, *args, _ptr(out), C.c_uint64(out.size),
+                   C.byref(no))
+        return out
+
     # -- folding (reference build only) ----------------------------------
     _FOLD_DTYPES = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2,
                     np.dtype(np.float32): 3}

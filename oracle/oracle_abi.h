@@ -219,6 +219,13 @@ int orc_flag_harmonics(const orc_candidate* candidates, uint64_t n,
                        orc_candidate* out, orc_harmonic* relations,
                        uint64_t capacity, uint64_t* count);
 
+/* Aligned single-DM spectrum (dedispersion.cpp:249-289): out [n_out][ce - cb]
+ * of the input element type (dtype 0 u8, 1 u16, 2 f32). */
+int orc_dedisperse_spectrum(int32_t dtype, const void* samples, uint64_t nsamples,
+                            uint64_t nchans, const double* freq, double dm, double ref,
+                            double tsamp, uint64_t cb, uint64_t ce, void* out,
+                            uint64_t capacity, uint64_t* out_nsamples);
+
 /* ---- folding (folding.cpp:300-470; reference build only) ----
  * dtype: 0 u8, 1 u16, 2 u32 (time series only), 3 f32.  Time series:
  * profile/exposure [nbin].  Spectrum: data [nsamples][nchans], cube

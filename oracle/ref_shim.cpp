@@ -235,6 +235,28 @@ int orc_dedisperse_single(int32_t dtype, const void* samples, uint64_t nsamples,
   });
 }
 
+int orc_dedisperse_spectrum(int32_t dtype, const void* samples, uint64_t nsamples,
+                            uint64_t nchans, const double* freq, double dm, double ref,
+                            double tsamp, uint64_t cb, uint64_t ce, void* out, uint64_t cap,
+                            uint64_t* out_nsamples) {
+  return guarded([&] {
+    const gaffa::SingleDmDedispersionPlan p{dm, ref, tsamp, cb, ce};
+    const std::span<const double> f(freq, nchans);
+    auto run = [&](auto tag) {
+      using T = decltype(tag);
+      auto r = gaffa::dedisperse_spectrum_cpu(view<T>(samples, nsamples, nchans), f, p);
+      *out_nsamples = r.shape.nsamples;
+      if (out == nullptr) return;
+      if (cap < r.data.size()) throw too_small();
+      std::memcpy(out, r.data.data(), r.data.size() * sizeof(r.data[0]));
+    };
+    if (dtype == 0) run(uint8_t{});
+    else if (dtype == 1) run(uint16_t{});
+    else if (dtype == 2) run(float{});
+    else throw std::invalid_argument("unsupported sample dtype");
+  });
+}
+
 int orc_time_series_stats(const float* x, uint64_t n, double* mean,
                           double* variance, double* stddev) {
   return guarded([&] {

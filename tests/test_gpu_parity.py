@@ -462,3 +462,40 @@ def test_fold_profile_and_candidates(gpu, ref):
     # the injected pulsar (P 0.714 s, DM 30) folds into a clear peak
     prof = cubes[0].mean(axis=0)
     assert prof.max() - np.median(prof) > 5 * prof.std() / 3
+
+
+# ------------------------------------------------------------ spectrum ----
+@pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
+def test_spectrum_bit_exact(gpu, port, dtype):
+    say(f"spectrum: 1024 ch x 12000, DM 250, {np.dtype(dtype).name}")
+    rng = np.random.default_rng(2)
+    freq = channel_frequencies(1024, 1520.0, -0.293)
+    x = (rng.integers(0, 250, size=(12000, 1024)) if dtype != np.float32
+         else rng.normal(size=(12000, 1024))).astype(dtype)
+    got = gpu._dedisperse_spectrum_array(x, frequency_mhz=list(freq), tsamp=64e-6, dm=250.0)
+    want = port.dedisperse_spectrum(x, freq, 250.0, 64e-6)
+    assert got.data.shape == want.shape and got.data.dtype == want.dtype
+    np.testing.assert_array_equal(got.data, want)
+    sub = gpu._dedisperse_spectrum_array(x, frequency_mhz=list(freq), tsamp=64e-6, dm=250.0,
+                                         chan_begin=100, chan_end=612)
+    np.testing.assert_array_equal(
+        sub.data, port.dedisperse_spectrum(x, freq, 250.0, 64e-6, ref=freq.max(), cb=100,
+                                           ce=612))
+
+
+def test_spectrum_filterbank_and_fold(gpu, ref):
+    from gaffa_amd import dedispersion, pfold
+    from gaffa_amd.io import Filterbank
+
+    say("spectrum: basetest.fil DM 10 + fold")
+    fb = Filterbank(os.path.join(GOLDEN, "basetest.fil"))
+    spec = dedispersion.dedisperse_spectrum(fb, dm=10.0)
+    want = ref.dedisperse_spectrum(fb.data[:, 0, :], fb.header.frequency_table, 10.0,
+                                   fb.header.tsamp)
+    np.testing.assert_array_equal(spec.data, want)
+    assert spec.backend == "gpu" and spec.dm == 10.0
+    r = pfold.fold_spectrum(spec, period=0.25, nbin=32, tsubint=0.5, nsubband=4)
+    cube, expo = ref.fold_spectrum(want, 0.25, fb.header.tsamp, 32,
+                                   round(0.5 / fb.header.tsamp), output_channels=4)
+    np.testing.assert_array_equal(r.cube, cube)
+    np.testing.assert_array_equal(r.exposure, expo)

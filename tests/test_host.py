@@ -103,8 +103,8 @@ def test_dedispersion_surface_errors(core):
         dedispersion.dedisperse_single_dm(fb, dm=0.0, backend="bad")
     with pytest.raises(ValueError, match="backend 'cpu' is not provided"):
         dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=2, backend="cpu")
-    with pytest.raises(NotImplementedError):
-        dedispersion.dedisperse_spectrum(fb, dm=0.0)
+    with pytest.raises(ValueError, match="backend 'cpu' is not provided"):
+        dedispersion.dedisperse_spectrum(fb, dm=0.0, backend="cpu")
 
 
 def test_filterbank_from_array_roundtrip():
