@@ -150,10 +150,10 @@ constexpr uint32_t kTrStride = kTrTime + 16;  // LDS row bytes (16-aligned)
 __global__ void __launch_bounds__(kThreads)
 transpose_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples, uint64_t nchans,
                     uint64_t chan_begin, uint32_t nsel, uint8_t* __restrict__ out,
-                    uint64_t out_stride, uint32_t vec_in) {
+                    uint64_t out_stride, uint32_t vec_in, uint64_t t_begin) {
   __shared__ __align__(16) uint8_t tile[kTrChans * kTrStride];
   const uint32_t tid = threadIdx.x;
-  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kTrTime;
+  const uint64_t t0 = t_begin + static_cast<uint64_t>(blockIdx.x) * kTrTime;
   const uint32_t c0 = blockIdx.y * kTrChans;
   const uint32_t nc = min(kTrChans, nsel - c0);
   // 256 rows x 4 chunks of 16 channels
@@ -209,7 +209,7 @@ constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, one L2 each
 
 __global__ void __launch_bounds__(kThreads)
 dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
-                             uint32_t nsel, uint32_t sbc, uint32_t nsub, uint32_t npn,
+                             uint64_t cm_length, uint32_t nsel, uint32_t sbc, uint32_t nsub, uint32_t npn,
                              uint32_t ndm, uint32_t nnom, uint32_t nblocks,
                              const int32_t* __restrict__ coarse,
                              const int32_t* __restrict__ resid,
@@ -273,9 +273,9 @@ dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
         const uint32_t c = min(c0 + u, ce - 1);  // spare slots repeat a valid channel
         const int32_t d = kdel[c];
         const uint64_t base = t0 + static_cast<uint64_t>(d);
-        // rows are zero-padded to cm_stride; a delay beyond the padding
+        // rows are zero-padded to cm_length; a delay beyond the padding
         // would read past the row
-        bad |= d < 0 || base + jlen4 + 4 > cm_stride;
+        bad |= d < 0 || base + jlen4 + 4 > cm_length;
         rows[u] = reinterpret_cast<const uint32_t*>(cm + static_cast<uint64_t>(c) * cm_stride) +
                   (base >> 2);
         sh[u] = static_cast<uint32_t>(base & 3u);
@@ -451,7 +451,12 @@ int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* a,
           "channel-major output must be 16-byte aligned");
     const uint64_t nsel = a->chan_end - a->chan_begin;
     check_overflow(nsel <= 0xFFFFFFFFull, "transpose channel range too large");
-    const uint64_t tblocks = ceil_div(a->out_stride, kTrTime);
+    const uint64_t t_count = a->t_count == 0 ? a->out_stride - a->t_begin : a->t_count;
+    check(a->t_begin % kTrTime == 0 && a->t_begin < a->out_stride &&
+              t_count <= a->out_stride - a->t_begin &&
+              (t_count % kTrTime == 0 || a->t_begin + t_count == a->out_stride),
+          "transpose time range must be 256-aligned and inside the rows");
+    const uint64_t tblocks = ceil_div(t_count, kTrTime);
     const uint64_t cblocks = ceil_div(nsel, kTrChans);
     check_overflow(tblocks < 0x7FFFFFFFull && cblocks < 65536, "transpose grid too large");
     const uint32_t vec_in = (a->nchans % 16 == 0 && a->chan_begin % 16 == 0 &&
@@ -462,7 +467,7 @@ int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* a,
                        dim3(static_cast<uint32_t>(tblocks), static_cast<uint32_t>(cblocks)),
                        dim3(kThreads), 0, as_stream(stream), a->samples, a->nsamples, a->nchans,
                        a->chan_begin, static_cast<uint32_t>(nsel), a->out, a->out_stride,
-                       vec_in);
+                       vec_in, a->t_begin);
     launch_check("transpose_u8_kernel");
   });
 }
@@ -476,7 +481,9 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
           "dedispersion device pointers must not be null");
     check(a->nsamples > 0, "dedispersion requires at least one sample");
     check(a->nsel > 0, "channel range must be non-empty");
-    check(a->channel_stride >= a->nsamples && a->channel_stride % 4 == 0 &&
+    const uint64_t row_len = a->channel_length == 0 ? a->channel_stride : a->channel_length;
+    check(row_len <= a->channel_stride || a->nsel == 1, "channel_length exceeds the row pitch");
+    check(row_len >= a->nsamples && a->channel_stride % 4 == 0 &&
               (reinterpret_cast<uintptr_t>(a->channels) & 3u) == 0,
           "channel rows must be dword aligned and hold nsamples");
     check(a->ndm > 0, "dedispersion requires at least one DM");
@@ -521,7 +528,7 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
     check_overflow(tiles * nnom < 0x7FFFFFF0ull, "dedispersion grid too large");
     // every stage-1 read of the last tile stays inside the padded rows
     check((tiles - 1) * tile + static_cast<uint64_t>(a->max_coarse) + (inter_len - 8) + 4 <=
-              a->channel_stride,
+              row_len,
           "channel rows are not padded far enough for the delays (see "
           "gaffa_hip_subband_channel_stride)");
     hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(dedisperse_subband_cm_kernel),
@@ -532,7 +539,7 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
     const uint64_t grid = ceil_div(nblocks, kXcds) * kXcds;  // whole rounds over the XCDs
     hipLaunchKernelGGL(dedisperse_subband_cm_kernel, dim3(static_cast<uint32_t>(grid)),
                        dim3(kThreads), static_cast<uint32_t>(lds), as_stream(stream),
-                       a->channels, a->channel_stride, static_cast<uint32_t>(nsel),
+                       a->channels, a->channel_stride, row_len, static_cast<uint32_t>(nsel),
                        static_cast<uint32_t>(a->subband_channels), static_cast<uint32_t>(nsub),
                        static_cast<uint32_t>(a->ndm_per_nominal), static_cast<uint32_t>(a->ndm),
                        static_cast<uint32_t>(nnom), static_cast<uint32_t>(nblocks),
@@ -688,5 +695,35 @@ extern "C" int gaffa_hip_dedisperse_spectrum(const gaffa_hip_spectrum_args* a,
         break;
     }
     launch_check("spectrum_kernel");
+  });
+}
+
+// ------------------------------------------------------------- row copy ----
+namespace gaffa_hip {
+namespace {
+
+__global__ void __launch_bounds__(kThreads)
+copy_rows_kernel(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst, uint64_t total,
+                 uint64_t nchans, int32_t reverse) {
+  const uint64_t e = static_cast<uint64_t>(blockIdx.x) * kThreads + threadIdx.x;
+  if (e >= total) return;
+  const uint64_t r = e / nchans, c = e - r * nchans;
+  dst[e] = src[r * nchans + (reverse ? nchans - 1 - c : c)];
+}
+
+}  // namespace
+}  // namespace gaffa_hip
+
+extern "C" int gaffa_hip_copy_rows_u8(const uint8_t* src, uint8_t* dst, uint64_t rows,
+                                      uint64_t nchans, int32_t reverse, gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(src != nullptr && dst != nullptr, "row copy pointers must not be null");
+    check(nchans > 0, "row copy requires channels");
+    const uint64_t total = rows * nchans;
+    if (total == 0) return;
+    check_overflow(total / kThreads < (1ull << 31), "row copy too large");
+    hipLaunchKernelGGL(copy_rows_kernel, dim3(static_cast<uint32_t>(ceil_div(total, kThreads))),
+                       dim3(kThreads), 0, as_stream(stream), src, dst, total, nchans, reverse);
+    launch_check("copy_rows_kernel");
   });
 }

@@ -652,6 +652,48 @@ PYBIND11_MODULE(_core, m) {
         py::arg("task"), py::kw_only(), py::arg("device_id") = 0);
   m.def("ffa_task_stdnoise", &ffa_task_stdnoise);
 
+  // SIGPROC files (native reader / writer)
+  m.def(
+      "_read_filterbank_header",
+      [](const std::string& path) {
+        const FilterbankHeader h = read_filterbank_header(path);
+        py::dict d;
+        d["header_size"] = h.header_size;
+        d["nsamples"] = h.nsamples;
+        d["nbits"] = h.nbits;
+        d["nifs"] = h.nifs;
+        d["nchans"] = h.nchans;
+        d["tsamp"] = h.tsamp;
+        d["tstart"] = h.tstart;
+        d["fch1"] = h.fch1;
+        d["foff"] = h.foff;
+        d["telescope_id"] = h.telescope_id;
+        d["machine_id"] = h.machine_id;
+        d["source_name"] = h.source_name;
+        d["frequency_table"] = h.frequency_table;
+        d["uses_frequency_table"] = h.uses_frequency_table;
+        d["reversed"] = h.reversed;
+        return d;
+      },
+      py::arg("path"));
+  m.def(
+      "_write_filterbank_u8",
+      [](const std::string& path, const py::array_t<std::uint8_t, py::array::c_style>& samples,
+         double tsamp, double fch1, double foff, const std::string& source_name, double tstart) {
+        if (samples.ndim() != 2) throw std::invalid_argument("samples must be (nsamples, nchans)");
+        FilterbankHeader h;
+        h.nchans = static_cast<int>(samples.shape(1));
+        h.nbits = 8;
+        h.tsamp = tsamp;
+        h.fch1 = fch1;
+        h.foff = foff;
+        h.source_name = source_name;
+        h.tstart = tstart;
+        write_filterbank_u8(path, h, samples.data(), static_cast<std::size_t>(samples.shape(0)));
+      },
+      py::arg("path"), py::arg("samples"), py::kw_only(), py::arg("tsamp"), py::arg("fch1"),
+      py::arg("foff"), py::arg("source_name") = "", py::arg("tstart") = 0.0);
+
   // aligned dynamic spectrum
   py::class_<PyDedispersedSpectrum>(m, "DedispersedSpectrum")
       .def(py::init([](const py::array& data, double tsamp, double dm, std::size_t chan_begin,
@@ -1164,6 +1206,24 @@ PYBIND11_MODULE(_core, m) {
           py::arg("dm_index"), py::arg("period"), py::kw_only(), py::arg("nbin") = 64,
           py::arg("nsubint") = 32,
           "Subint x phase cubes of the preprocessed plane rows (after run())")
+      .def(
+          "run_file",
+          [](GpuSearchPipeline& p, const std::string& path, std::size_t chunk_samples) {
+            StageTimes t;
+            {
+              py::gil_scoped_release release;
+              t = p.run_file(path, chunk_samples);
+            }
+            py::dict d = stage_dict(t);
+            const FilterbankStreamStats& s = p.stream_stats();
+            d["stream_bytes"] = s.bytes;
+            d["stream_chunks"] = s.chunks;
+            d["stream_read_ms"] = s.read_ms;
+            d["stream_wall_ms"] = s.wall_ms;
+            return d;
+          },
+          py::arg("path"), py::arg("chunk_samples") = std::size_t{1} << 18,
+          "Stream an 8-bit SIGPROC file in and search it, dedispersing while it arrives")
       .def("set_kernel_timing", &GpuSearchPipeline::set_kernel_timing, py::arg("on"),
            "Record HIP events between the FFA launch stages of every tile")
       .def("run", [](GpuSearchPipeline& p) {

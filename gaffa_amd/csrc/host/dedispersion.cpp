@@ -195,6 +195,77 @@ void dedisperse_multi_dm_gpu_into(GpuSampleView samples, std::span<const double>
   hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_multi_u8");
 }
 
+GpuSegmentTables make_gpu_segment_tables(std::span<const double> f,
+                                         const MultiDmDedispersionPlan& p, bool subband,
+                                         const SubbandDedispersionOptions& so) {
+  GpuSegmentTables t;
+  t.subband = subband;
+  t.ndm = p.ndm;
+  t.chan_begin = p.chan_begin;
+  t.chan_end = p.chan_end;
+  t.subband_channels = so.subband_channels;
+  t.ndm_per_nominal = so.ndm_per_nominal;
+  if (subband) {
+    const SubbandTables st = make_subband_tables(f, p, so);
+    for (std::int32_t d : st.coarse) check_arg(d >= 0, "dedispersion requires non-negative delays");
+    t.first = upload(st.coarse.data(), st.coarse.size());
+    t.second = upload(st.residual.data(), st.residual.size());
+    t.bound_first = st.max_coarse;
+    t.bound_second = st.max_residual;
+    // stage 1 reads up to tile (<= 2048) + residual + pad past the coarse delay
+    t.lookahead = static_cast<std::size_t>(st.max_coarse) + st.max_residual + 2048 + 64;
+  } else {
+    const std::vector<std::int32_t> delays = make_multi_dm_delay_table(f, p);
+    for (std::int32_t d : delays) check_arg(d >= 0, "dedispersion requires non-negative delays");
+    t.first = upload(delays.data(), delays.size());
+    t.bound_first = block_spread(delays, p.ndm, p.chan_end - p.chan_begin);
+    std::int32_t dmax = 0;
+    for (std::int32_t d : delays) dmax = std::max(dmax, d);
+    t.lookahead = static_cast<std::size_t>(dmax) + 512;  // slab of 256 + spread + pad
+  }
+  return t;
+}
+
+void enqueue_segment_range(const GpuSegmentTables& t, const std::uint8_t* samples,
+                           const GpuChannelMajorView& cm, std::size_t nsamples,
+                           std::size_t nchans, std::uint32_t* out, std::size_t out_stride,
+                           std::size_t begin, std::size_t end, gaffa_hip_stream stream) {
+  check_arg(begin < end && end <= nsamples, "dedispersion output range is empty");
+  if (t.subband) {
+    gaffa_hip_dedisperse_subband_args a{};
+    a.channels = cm.data + (t.chan_begin - cm.chan_begin) * cm.stride + begin;
+    a.channel_stride = cm.stride;
+    a.channel_length = cm.stride - begin;
+    a.nsamples = nsamples - begin;
+    a.nsel = t.chan_end - t.chan_begin;
+    a.subband_channels = t.subband_channels;
+    a.ndm_per_nominal = t.ndm_per_nominal;
+    a.ndm = t.ndm;
+    a.coarse_delays = t.first.data();
+    a.residual_delays = t.second.data();
+    a.out = out + begin;
+    a.out_nsamples = end - begin;
+    a.out_stride = out_stride;
+    a.max_residual = t.bound_second;
+    a.max_coarse = t.bound_first;
+    hip_ok(gaffa_hip_dedisperse_subband_u8(&a, stream), "dedisperse_subband_u8");
+  } else {
+    gaffa_hip_dedisperse_multi_args a{};
+    a.samples = samples + begin * nchans;
+    a.nsamples = nsamples - begin;
+    a.nchans = nchans;
+    a.chan_begin = t.chan_begin;
+    a.chan_end = t.chan_end;
+    a.delays = t.first.data();
+    a.ndm = t.ndm;
+    a.out = out + begin;
+    a.out_nsamples = end - begin;
+    a.out_stride = out_stride;
+    a.max_block_spread = t.bound_first;
+    hip_ok(gaffa_hip_dedisperse_multi_u8(&a, stream), "dedisperse_multi_u8");
+  }
+}
+
 std::size_t spectrum_output_nsamples(std::size_t nsamples, std::span<const double> f,
                                      const SingleDmDedispersionPlan& p) {
   const MultiDmDedispersionPlan m{p.dm, 1.0, 1, p.ref_frequency_mhz, p.tsamp, p.chan_begin,
@@ -295,6 +366,7 @@ void dedisperse_subband_gpu_cm_into(const GpuChannelMajorView& cm, std::span<con
   gaffa_hip_dedisperse_subband_args a{};
   a.channels = cm.data + (p.chan_begin - cm.chan_begin) * cm.stride;
   a.channel_stride = cm.stride;
+  a.channel_length = cm.length;
   a.nsamples = cm.nsamples;
   a.nsel = p.chan_end - p.chan_begin;
   a.subband_channels = so.subband_channels;

@@ -206,6 +206,32 @@ DedispersedSpectrum<T> dedisperse_spectrum_gpu(HostSampleView<T> samples,
   return r;
 }
 
+struct GpuChannelMajorView;
+
+// Device-resident delay tables of one DM segment, built once and reused by
+// every launch over it: launches then need no host synchronisation.
+struct GpuSegmentTables {
+  bool subband = false;
+  GpuDeviceBuffer<std::int32_t> first, second;  // brute: delays; subband: coarse, residual
+  std::int32_t bound_first = 0, bound_second = 0;  // brute: block spread; subband: maxima
+  std::size_t ndm = 0, chan_begin = 0, chan_end = 0;
+  std::size_t subband_channels = 0, ndm_per_nominal = 0;
+  std::size_t lookahead = 0;  // input samples past an output range its kernel reads
+};
+
+GpuSegmentTables make_gpu_segment_tables(std::span<const double> frequency_mhz,
+                                         const MultiDmDedispersionPlan& plan, bool subband,
+                                         const SubbandDedispersionOptions& subband_options);
+
+// Enqueues output samples [begin, end) of one segment on `stream` (no
+// synchronisation; the caller checks gaffa_hip_dedisperse_check once).
+// Brute force reads `samples` ([nsamples][nchans]); subband reads `channels`
+// (the full channel-major copy, rows of channels.stride bytes).
+void enqueue_segment_range(const GpuSegmentTables& tables, const std::uint8_t* samples,
+                           const GpuChannelMajorView& channels, std::size_t nsamples,
+                           std::size_t nchans, std::uint32_t* out, std::size_t out_stride,
+                           std::size_t begin, std::size_t end, gaffa_hip_stream stream);
+
 // Channel-major device copy of a filterbank's channels [chan_begin,
 // chan_begin + nchans): row c holds channel chan_begin + c over time, zero
 // padded from nsamples to stride (the layout subband dedispersion reads).
@@ -216,6 +242,7 @@ struct GpuChannelMajorView {
   std::size_t nchans = 0;
   std::size_t stride = 0;
   int device_id = 0;
+  std::size_t length = 0;  // readable bytes per row from data (0 = stride)
 };
 
 // Row length a channel-major copy needs for subband dedispersion of `plan`.

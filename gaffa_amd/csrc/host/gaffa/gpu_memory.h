@@ -120,6 +120,50 @@ class GpuStream {
   gaffa_hip_stream s_ = nullptr;
 };
 
+// Page-locked host buffer (asynchronous copies to and from the device).
+class GpuPinnedBuffer {
+ public:
+  GpuPinnedBuffer() = default;
+  explicit GpuPinnedBuffer(std::size_t bytes);
+  GpuPinnedBuffer(const GpuPinnedBuffer&) = delete;
+  GpuPinnedBuffer& operator=(const GpuPinnedBuffer&) = delete;
+  GpuPinnedBuffer(GpuPinnedBuffer&& o) noexcept
+      : p_(std::exchange(o.p_, nullptr)), bytes_(std::exchange(o.bytes_, 0)) {}
+  GpuPinnedBuffer& operator=(GpuPinnedBuffer&& o) noexcept {
+    if (this != &o) {
+      release();
+      p_ = std::exchange(o.p_, nullptr);
+      bytes_ = std::exchange(o.bytes_, 0);
+    }
+    return *this;
+  }
+  ~GpuPinnedBuffer() { release(); }
+  void* data() noexcept { return p_; }
+  std::size_t bytes() const noexcept { return bytes_; }
+
+ private:
+  void release() noexcept;
+  void* p_ = nullptr;
+  std::size_t bytes_ = 0;
+};
+
+// An event created on construction (default flags), for ordering work
+// across streams.
+class GpuEvent {
+ public:
+  GpuEvent();
+  GpuEvent(const GpuEvent&) = delete;
+  GpuEvent& operator=(const GpuEvent&) = delete;
+  ~GpuEvent();
+  void record(gaffa_hip_stream stream);
+  void wait(gaffa_hip_stream stream);  // `stream` waits for the last record
+  void synchronize();                  // the host waits for the last record
+  gaffa_hip_event get() const noexcept { return e_; }
+
+ private:
+  gaffa_hip_event e_ = nullptr;
+};
+
 struct GpuLaunchOptions {
   int device_id = 0;
   std::size_t threads_per_block = 256;

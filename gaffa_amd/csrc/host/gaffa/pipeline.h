@@ -8,6 +8,7 @@
 
 #include "gaffa/dedispersion.h"
 #include "gaffa/ffa_gpu.h"
+#include "gaffa/filterbank.h"
 #include "gaffa/preprocessing.h"
 #include "gaffa/synth.h"
 
@@ -71,6 +72,11 @@ class GpuSearchPipeline {
   void preprocess();
   void search();
   StageTimes run();  // all three, timed with HIP events
+  // Stream an 8-bit SIGPROC file (matching the configuration) to the device
+  // and dedisperse each output range as soon as the samples it needs have
+  // arrived, then preprocess and search.  stream_stats() describes the load.
+  StageTimes run_file(const std::string& path, std::size_t chunk_samples = std::size_t{1} << 18);
+  const FilterbankStreamStats& stream_stats() const noexcept { return stream_stats_; }
   // Record events between the FFA launch stages of every tile (off by
   // default: the marks split one enqueue into four).
   void set_kernel_timing(bool on) noexcept { kernel_timing_ = on; }
@@ -97,6 +103,7 @@ class GpuSearchPipeline {
   std::size_t ndm_ = 0, n_out_ = 0, stride_ = 0;
   std::vector<double> dms_;
   std::vector<MultiDmDedispersionPlan> segment_plans_;
+  std::vector<GpuSegmentTables> segment_tables_;
   GpuDeviceBuffer<std::uint8_t> filterbank_;
   GpuDeviceBuffer<std::uint8_t> channels_;  // channel-major copy (subband path)
   std::size_t channel_stride_ = 0;
@@ -107,6 +114,9 @@ class GpuSearchPipeline {
   std::vector<DmPeak> peaks_;
   std::vector<gaffa_hip_event> events_;
   bool kernel_timing_ = false;
+  FilterbankStreamStats stream_stats_{};
+  void dedisperse_range(std::size_t begin, std::size_t end);
+  std::size_t lookahead_ = 0;  // input samples past an output range its kernels read
   std::vector<FfaStageMarks> marks_;  // one per tile, events created on demand
   std::size_t marks_used_ = 0;
 };

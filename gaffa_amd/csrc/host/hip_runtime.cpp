@@ -98,6 +98,32 @@ void GpuStream::destroy() noexcept {
   s_ = nullptr;
 }
 
+GpuPinnedBuffer::GpuPinnedBuffer(std::size_t bytes) : bytes_(bytes) {
+  hip_ok(gaffa_hip_host_malloc(&p_, bytes), "hipHostMalloc");
+}
+
+void GpuPinnedBuffer::release() noexcept {
+  if (p_ != nullptr) (void)gaffa_hip_host_free(p_);
+  p_ = nullptr;
+  bytes_ = 0;
+}
+
+GpuEvent::GpuEvent() { hip_ok(gaffa_hip_event_create(&e_), "hipEventCreate"); }
+
+GpuEvent::~GpuEvent() {
+  if (e_ != nullptr) (void)gaffa_hip_event_destroy(e_);
+}
+
+void GpuEvent::record(gaffa_hip_stream stream) {
+  hip_ok(gaffa_hip_event_record(e_, stream), "hipEventRecord");
+}
+
+void GpuEvent::wait(gaffa_hip_stream stream) {
+  hip_ok(gaffa_hip_stream_wait_event(stream, e_), "hipStreamWaitEvent");
+}
+
+void GpuEvent::synchronize() { hip_ok(gaffa_hip_event_synchronize(e_), "hipEventSynchronize"); }
+
 void copy_to_device(void* dst, const void* src, std::size_t bytes, gaffa_hip_stream stream) {
   hip_ok(gaffa_hip_memcpy(dst, src, bytes, stream), "hipMemcpy host-to-device");
 }

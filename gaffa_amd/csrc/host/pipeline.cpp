@@ -34,6 +34,11 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
     ndm_ += s.ndm;
   }
   stride_ = (n_out_ + 63) / 64 * 64;
+  for (const MultiDmDedispersionPlan& p : segment_plans_) {
+    segment_tables_.push_back(
+        make_gpu_segment_tables(cfg_.frequency_mhz, p, cfg_.subband, cfg_.subband_options));
+    lookahead_ = std::max(lookahead_, segment_tables_.back().lookahead);
+  }
   filterbank_ = GpuDeviceBuffer<std::uint8_t>(cfg_.nsamples * cfg_.nchans);
   if (cfg_.subband) {
     for (const MultiDmDedispersionPlan& p : segment_plans_)
@@ -90,30 +95,84 @@ void GpuSearchPipeline::download_filterbank(std::uint8_t* out) const {
   copy_to_host(out, filterbank_.data(), filterbank_.size(), stream_.get());
 }
 
-void GpuSearchPipeline::dedisperse() {
-  GpuDeviceScope scope(cfg_.device_id);
-  const GpuSampleView v{filterbank_.data(), SampleShape{cfg_.nsamples, 1, cfg_.nchans},
-                        cfg_.device_id};
-  GpuDedispersionOptions o;
-  o.device_id = cfg_.device_id;
-  o.stream = stream_.get();
-  if (cfg_.subband) {
-    // one channel-major copy serves every DM segment
-    transpose_filterbank_gpu_into(v, 0, cfg_.nchans, channels_.data(), channel_stride_, o);
-  }
+void GpuSearchPipeline::dedisperse_range(std::size_t begin, std::size_t end) {
   const GpuChannelMajorView cm{channels_.data(), cfg_.nsamples, 0, cfg_.nchans, channel_stride_,
                                cfg_.device_id};
   std::size_t row = 0;
-  for (const MultiDmDedispersionPlan& p : segment_plans_) {
-    std::uint32_t* out = plane_.data() + row * stride_;
-    if (cfg_.subband) {
-      dedisperse_subband_gpu_cm_into(cm, cfg_.frequency_mhz, p, cfg_.subband_options, out,
-                                     stride_, o, n_out_);
-    } else {
-      dedisperse_multi_dm_gpu_into(v, cfg_.frequency_mhz, p, out, stride_, o, n_out_);
-    }
-    row += p.ndm;
+  for (std::size_t k = 0; k < segment_tables_.size(); ++k) {
+    enqueue_segment_range(segment_tables_[k], filterbank_.data(), cm, cfg_.nsamples, cfg_.nchans,
+                          plane_.data() + row * stride_, stride_, begin, end, stream_.get());
+    row += segment_plans_[k].ndm;
   }
+}
+
+void GpuSearchPipeline::dedisperse() {
+  GpuDeviceScope scope(cfg_.device_id);
+  if (cfg_.subband) {
+    // one channel-major copy serves every DM segment
+    const GpuSampleView v{filterbank_.data(), SampleShape{cfg_.nsamples, 1, cfg_.nchans},
+                          cfg_.device_id};
+    GpuDedispersionOptions o;
+    o.device_id = cfg_.device_id;
+    o.stream = stream_.get();
+    transpose_filterbank_gpu_into(v, 0, cfg_.nchans, channels_.data(), channel_stride_, o);
+  }
+  dedisperse_range(0, n_out_);
+  // one check of the kernels' delay-bound guard for the whole plane
+  hip_ok(gaffa_hip_dedisperse_check(stream_.get()), "dedisperse");
+}
+
+StageTimes GpuSearchPipeline::run_file(const std::string& path, std::size_t chunk_samples) {
+  GpuDeviceScope scope(cfg_.device_id);
+  const FilterbankHeader h = read_filterbank_header(path);
+  check_arg(h.nbits == 8 && h.nifs == 1, "pipeline files must be 8-bit with nifs == 1");
+  check_arg(static_cast<std::size_t>(h.nchans) == cfg_.nchans &&
+                h.nsamples == cfg_.nsamples && h.tsamp == cfg_.tsamp,
+            "filterbank file shape does not match the pipeline");
+  for (std::size_t c = 0; c < cfg_.nchans; ++c)
+    check_arg(std::abs(h.frequency_table[c] - cfg_.frequency_mhz[c]) <= 1e-9,
+              "filterbank file frequencies do not match the pipeline");
+  chunk_samples = std::max<std::size_t>(256, (chunk_samples + 255) / 256 * 256);
+  gaffa_hip_stream st = stream_.get();
+  hip_ok(gaffa_hip_event_record(events_[0], st), "hipEventRecord");
+  std::size_t transposed = 0, done = 0;
+  stream_stats_ = stream_filterbank_u8(path, h, filterbank_.data(), st, chunk_samples,
+                                       [&](std::size_t ready) {
+    if (cfg_.subband) {
+      gaffa_hip_transpose_args a{};
+      a.samples = filterbank_.data();
+      a.nsamples = ready;
+      a.nchans = cfg_.nchans;
+      a.chan_begin = 0;
+      a.chan_end = cfg_.nchans;
+      a.out = channels_.data();
+      a.out_stride = channel_stride_;
+      a.t_begin = transposed;
+      // the last chunk also writes the zero padding to the end of the rows
+      a.t_count = ready == cfg_.nsamples ? channel_stride_ - transposed : ready - transposed;
+      hip_ok(gaffa_hip_filterbank_transpose_u8(&a, st), "filterbank_transpose_u8");
+      transposed = ready;
+    }
+    const std::size_t target =
+        ready == cfg_.nsamples ? n_out_ : (ready > lookahead_ ? std::min(n_out_, ready - lookahead_) : 0);
+    if (target > done) {
+      dedisperse_range(done, target);
+      done = target;
+    }
+  });
+  hip_ok(gaffa_hip_dedisperse_check(st), "dedisperse");
+  hip_ok(gaffa_hip_event_record(events_[1], st), "hipEventRecord");
+  preprocess();
+  hip_ok(gaffa_hip_event_record(events_[2], st), "hipEventRecord");
+  if (prep_) prep_->synchronize();
+  search();
+  stream_.synchronize();
+  StageTimes t;
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[0], events_[1], &t.dedisperse_ms), "elapsed");
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[1], events_[2], &t.preprocess_ms), "elapsed");
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[3], events_[4], &t.ffa_ms), "elapsed");
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[0], events_[4], &t.total_ms), "elapsed");
+  return t;
 }
 
 void GpuSearchPipeline::preprocess() {

@@ -154,6 +154,36 @@ int gaffa_hip_event_record(gaffa_hip_event event, gaffa_hip_stream stream) {
   });
 }
 
+int gaffa_hip_event_synchronize(gaffa_hip_event event) {
+  return guarded([&] {
+    check(event != nullptr, "event must not be null");
+    hip_check(hipEventSynchronize(reinterpret_cast<hipEvent_t>(event)), "hipEventSynchronize");
+  });
+}
+
+int gaffa_hip_stream_wait_event(gaffa_hip_stream stream, gaffa_hip_event event) {
+  return guarded([&] {
+    check(event != nullptr, "event must not be null");
+    hip_check(hipStreamWaitEvent(as_stream(stream), reinterpret_cast<hipEvent_t>(event), 0),
+              "hipStreamWaitEvent");
+  });
+}
+
+int gaffa_hip_host_malloc(void** ptr, uint64_t bytes) {
+  return guarded([&] {
+    check(ptr != nullptr, "host allocation output must not be null");
+    *ptr = nullptr;
+    if (bytes == 0) return;
+    hip_check(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "hipHostMalloc");
+  });
+}
+
+int gaffa_hip_host_free(void* ptr) {
+  return guarded([&] {
+    if (ptr != nullptr) hip_check(hipHostFree(ptr), "hipHostFree");
+  });
+}
+
 int gaffa_hip_event_elapsed_ms(gaffa_hip_event start, gaffa_hip_event stop, float* ms) {
   return guarded([&] {
     hip_check(hipEventSynchronize(reinterpret_cast<hipEvent_t>(stop)), "hipEventSynchronize");

@@ -156,3 +156,14 @@ class Filterbank:
 
 
 __all__ = ["Filterbank", "FilterbankHeader", "read_header"]
+
+
+def write_filterbank(path: str | os.PathLike, samples: np.ndarray, *, tsamp: float, fch1: float,
+                     foff: float, source_name: str = "", tstart: float = 0.0) -> None:
+    """Write an 8-bit SIGPROC filterbank: ``samples`` is (nsamples, nchans) in
+    the file's channel order (channel c at fch1 + c * foff)."""
+    from ._core import _write_filterbank_u8
+
+    _write_filterbank_u8(os.fspath(path), np.ascontiguousarray(samples, dtype=np.uint8),
+                         tsamp=tsamp, fch1=fch1, foff=foff, source_name=source_name,
+                         tstart=tstart)

@@ -80,6 +80,12 @@ int gaffa_hip_event_destroy(gaffa_hip_event event);
 int gaffa_hip_event_record(gaffa_hip_event event, gaffa_hip_stream stream);
 int gaffa_hip_event_elapsed_ms(gaffa_hip_event start, gaffa_hip_event stop,
                                float* ms);
+int gaffa_hip_event_synchronize(gaffa_hip_event event);
+/* Work enqueued on `stream` after this call waits for `event`. */
+int gaffa_hip_stream_wait_event(gaffa_hip_stream stream, gaffa_hip_event event);
+/* Page-locked host memory (hipHostMalloc) for asynchronous copies. */
+int gaffa_hip_host_malloc(void** ptr, uint64_t bytes);
+int gaffa_hip_host_free(void* ptr);
 /* dst[0 .. count) = value with a kernel (graph-safe reset of counters; no
  * gaffa counterpart). */
 int gaffa_hip_fill_u32(uint32_t* dst, uint32_t value, uint64_t count,
@@ -134,7 +140,16 @@ typedef struct gaffa_hip_transpose_args {
   uint64_t chan_end;
   uint8_t* out;           /* [chan_end - chan_begin][out_stride] */
   uint64_t out_stride;
+  uint64_t t_begin;       /* time range written: [t_begin, t_begin + t_count), */
+  uint64_t t_count;       /* multiples of 256 except at out_stride; 0 = all */
 } gaffa_hip_transpose_args;
+
+/* dst[r][c] = src[r][reverse ? nchans - 1 - c : c] for rows r < rows (uint8
+ * filterbank rows; reverse turns a descending-frequency file into gaffa's
+ * ascending channel order).  No gaffa counterpart (its reader reverses on
+ * the host, io/filterbank.cpp:79-118). */
+int gaffa_hip_copy_rows_u8(const uint8_t* src, uint8_t* dst, uint64_t rows,
+                           uint64_t nchans, int32_t reverse, gaffa_hip_stream stream);
 
 int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* args,
                                       gaffa_hip_stream stream);
@@ -154,7 +169,10 @@ uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,
  * every coarse delay. */
 typedef struct gaffa_hip_dedisperse_subband_args {
   const uint8_t* channels;  /* channel-major [nsel][channel_stride] */
-  uint64_t channel_stride;  /* bytes per channel row, multiple of 4 */
+  uint64_t channel_stride;  /* bytes between channel rows, multiple of 4 */
+  uint64_t channel_length;  /* readable bytes from `channels` along a row
+                               (0 = channel_stride; less when `channels`
+                               starts inside the rows) */
   uint64_t nsamples;        /* valid samples per row */
   uint64_t nsel;            /* channels (the selected range) */
   uint64_t subband_channels;

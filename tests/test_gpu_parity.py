@@ -499,3 +499,42 @@ def test_spectrum_filterbank_and_fold(gpu, ref):
                                    round(0.5 / fb.header.tsamp), output_channels=4)
     np.testing.assert_array_equal(r.cube, cube)
     np.testing.assert_array_equal(r.exposure, expo)
+
+
+@pytest.mark.parametrize("which,descending,chunk", [
+    ("small", True, 1 << 14), ("small", False, 5000), ("lband", True, 1 << 15)])
+def test_streamed_file_equals_resident(gpu, tmp_path, which, descending, chunk):
+    """run_file streams a SIGPROC file to the device in pinned chunks and
+    dedisperses each output range as soon as its lookahead has arrived; the
+    searched plane and the peaks must equal the resident run() on the same
+    samples, for files stored in either channel order."""
+    from gaffa_amd.io import write_filterbank
+    from gaffa_amd.pipeline import lband_workload, small_workload
+
+    w = small_workload() if which == "small" else lband_workload(nsamples=1 << 19, npulsars=3)
+    say(f"streaming: {which} {w.nchans}ch x {w.nsamples} descending={descending} chunk={chunk}")
+    ref = w.pipeline(ffa_workspace_bytes=4 << 30)
+    ref.generate(w.synth_spec())
+    x = ref.download_filterbank(w.nsamples, w.nchans)
+    ref.run()
+    want_plane = ref.download_plane_f32()
+    want = ref.peaks
+    path = tmp_path / "obs.fil"
+    if descending:  # the workloads' own (fch1, foff < 0) header
+        write_filterbank(path, x[:, ::-1], tsamp=w.tsamp, fch1=w.fch1, foff=w.foff,
+                         source_name="synthetic")
+    else:
+        write_filterbank(path, x, tsamp=w.tsamp, fch1=float(w.frequencies[0]), foff=-w.foff,
+                         source_name="synthetic")
+    del ref
+    pipe = w.pipeline(ffa_workspace_bytes=4 << 30)
+    st = pipe.run_file(str(path), chunk_samples=chunk)
+    assert st["stream_bytes"] == w.nsamples * w.nchans
+    assert st["stream_chunks"] == -(-w.nsamples // max(256, -(-chunk // 256) * 256))
+    np.testing.assert_array_equal(pipe.download_filterbank(w.nsamples, w.nchans), x)
+    np.testing.assert_array_equal(pipe.download_plane_f32(), want_plane)
+    got = pipe.peaks
+    assert len(got) == len(want) > 0
+    for p, q in zip(got, want):
+        assert (p.dm_index, p.peak.shift, p.peak.width, p.peak.bins, p.peak.snr) == \
+            (q.dm_index, q.peak.shift, q.peak.width, q.peak.bins, q.peak.snr)
