@@ -117,5 +117,5 @@ def ffa_transform_rows(batch, plan: FfaPlan, task: int, *, device_id: int = 0) -
     return _ffa_transform_rows_gpu(batch, plan, task, device_id=device_id)
 
 
-__all__ = ["FfaPeak", "FfaPlan", "ffa_search", "ffa_search_batch", "ffa_transform_rows",
-           "make_riptide_plan"]
+__all__ = ["FfaBatchedSearch", "FfaPeak", "FfaPlan", "ffa_search", "ffa_search_batch",
+           "ffa_transform_rows", "make_octave_plan", "make_riptide_plan"]
