@@ -229,7 +229,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["lband", "small", "batched"], default="lband")
+    ap.add_argument("--workload", choices=["lband", "small", "large", "batched"],
+                    default="lband")
     ap.add_argument("--batched-series", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -244,14 +245,15 @@ def main() -> int:
     import numpy as np
 
     import gaffa_amd
-    from gaffa_amd.pipeline import lband_workload, small_workload
+    from gaffa_amd.pipeline import large_workload, lband_workload, small_workload
 
     if gaffa_amd.gpu_device_count() == 0:
         print("no HIP device visible", file=sys.stderr)
         return 1
     if args.workload == "batched":
         return bench_batched(args)
-    w = lband_workload() if args.workload == "lband" else small_workload()
+    w = {"lband": lband_workload, "small": small_workload,
+         "large": large_workload}[args.workload]()
     print(f"[bench] workload {w.name}: {w.nchans} ch x {w.nsamples} samples, {w.ndm} DMs",
           file=sys.stderr, flush=True)
     pipe = w.pipeline()
@@ -309,7 +311,8 @@ def main() -> int:
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u8->u32 sums (dedispersion), f32 (detrend, FFA, S/N)",
-        "data": "synthetic (seeded GPU generator, 8 injected pulsars)",
+        "data": (f"synthetic (seeded GPU generator, {len(w.pulsars)} injected pulsars"
+                 + (f", {len(w.rfi_channels)} RFI channels" if w.rfi_channels else "") + ")"),
         "config": {
             "workload": (f"{w.name}: {w.nchans} ch {w.fch1}/{w.foff} MHz, tsamp {w.tsamp}, "
                          f"{w.nsamples} samples; subband {w.subband_channels} ch/subband x "

@@ -177,6 +177,31 @@ def lband_workload(nsamples: int = 1 << 22, seed: int = 20260202, ndm: int = 102
         duty_cycle_max=0.18, width_trial_spacing=2.0)
 
 
+def large_workload(nsamples: int = 1 << 22, seed: int = 20260505, ndm: int = 4096,
+                   npulsars: int = 32) -> Workload:
+    """4096 channels (1700 MHz, -0.195 MHz: an 800 MHz band), 32 pulsars,
+    5 narrowband RFI channels at a constant +40, subband dedispersion with 64
+    channels per subband over 4096 trials 0-1000."""
+    rng = np.random.default_rng(seed)
+    nchans, fch1, foff, tsamp = 4096, 1700.0, -0.195, 64e-6
+    pulsars = []
+    for _ in range(npulsars):
+        period = float(np.exp(rng.uniform(np.log(0.05), np.log(10.0))))
+        dm = float(rng.uniform(0.0, 1000.0))
+        duty = float(rng.uniform(0.005, 0.10))
+        snr = float(rng.uniform(6.0, 25.0))
+        pulsars.append((period, dm, duty, pulse_amplitude(snr, nchans, nsamples, duty),
+                        float(rng.uniform(0.0, 1.0))))
+    rfi = sorted(int(c) for c in rng.choice(nchans, 5, replace=False))
+    freq = channel_frequencies(nchans, fch1, foff)
+    segs = smearing_dm_segments(dm_max=1000.0, ndm=ndm, tsamp=tsamp, freq=freq)
+    return Workload(
+        name="large", nsamples=nsamples, nchans=nchans, fch1=fch1, foff=foff, tsamp=tsamp,
+        seed=seed, pulsars=pulsars, rfi_channels=rfi, rfi_offset=40, dm_segments=segs,
+        subband=True, subband_channels=64, ndm_per_nominal=32, median_seconds=2.0,
+        period_min=0.1, period_max=10.0, duty_cycle_max=0.18, width_trial_spacing=2.0)
+
+
 @dataclass
 class BatchedWorkload:
     """Many float series of different lengths, each searched with its own

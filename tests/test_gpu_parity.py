@@ -117,6 +117,33 @@ def test_subband_lband_slice_vs_oracle(gpu, port):
         np.testing.assert_array_equal(r.data, want)
 
 
+def test_subband_large_slice_vs_oracle(gpu, port):
+    """The 4096-channel configuration: 64 channels per subband (stage-1 sums
+    of 64 x 255 still fit 16 bits; stage 2 flushes every 4 subbands), RFI
+    channels in the input, and the first and last DM segments (DM 0 and
+    ~990, delays up to 57 k samples)."""
+    from gaffa_amd import dedispersion
+    from gaffa_amd.io import Filterbank
+    from gaffa_amd.pipeline import large_workload
+
+    say("dedispersion: 4096-ch x 2^17 synthetic with RFI, nsub 64, DM 0 and ~990")
+    w = large_workload(nsamples=1 << 17, npulsars=4)
+    x = gpu._synth_filterbank_gpu(w.synth_spec())
+    want_x = port.synth_filterbank(w.nsamples, w.nchans, w.fch1, w.foff, w.tsamp, w.seed,
+                                   pulsars=w.pulsars, rfi_channels=w.rfi_channels,
+                                   rfi_offset=w.rfi_offset, t_count=4096)
+    np.testing.assert_array_equal(x[:4096], want_x)
+    assert x[:, w.rfi_channels].mean() > x.mean() + 5  # the +40 channels are there
+    fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
+    for seg in (w.dm_segments[0], w.dm_segments[-1]):
+        r = dedispersion.dedisperse_subband(fb, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
+                                            subband_channels=64, ndm_per_nominal=32)
+        want = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
+                               tsamp=w.tsamp, subband=True, subband_channels=64,
+                               ndm_per_nominal=32)
+        np.testing.assert_array_equal(r.data, want)
+
+
 def test_c_abi_direct_call(gpu, port):
     """Drive gaffa_hip_dedisperse_multi_u8 through ctypes with raw device memory."""
     import gaffa_amd

@@ -187,3 +187,17 @@ def test_native_filterbank_reader_and_writer(core, tmp_path):
         assert hw["frequency_table"] == list(back.header.frequency_table)
         want = x[:, ::-1] if foff < 0 else x
         np.testing.assert_array_equal(back.data[:, 0, :], want)
+
+
+def test_large_workload_shape(core):
+    """The 4096-channel configuration: 4096 trials 0-1000 in 32 smearing
+    segments, 64-channel subbands, 5 RFI channels, 32 pulsars."""
+    from gaffa_amd.pipeline import large_workload
+
+    w = large_workload()
+    assert (w.nchans, w.ndm, w.subband_channels, len(w.pulsars), len(w.rfi_channels)) == \
+        (4096, 4096, 64, 32, 5)
+    assert w.dms[0] == 0.0 and 990.0 < w.dms[-1] <= 1000.0 and np.all(np.diff(w.dms) > 0)
+    f = w.frequencies
+    dmax = core.dedispersion_delay_bins(float(w.dms[-1]), float(f[0]), float(f.max()), w.tsamp)
+    assert 57000 < dmax < 57400  # n_out ~ 4.137 M of 2^22
