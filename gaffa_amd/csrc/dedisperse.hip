@@ -380,6 +380,33 @@ int32_t* error_flag_for_device() {
 
 }  // namespace
 
+// Largest time tile whose LDS (the u16 subband intermediate of nsub rows of
+// tile + max_residual samples, plus the residual table) fits two workgroups
+// per CU, else one; tile 0 when none does.
+static void subband_geometry(uint64_t nsub, uint64_t ndm_per_nominal, int32_t max_residual,
+                             uint32_t& tile, uint32_t& inter_len, uint64_t& lds) {
+  const uint64_t res_bytes = ndm_per_nominal * nsub * 4;
+  tile = 0;
+  inter_len = 0;
+  lds = 0;
+  for (const uint64_t budget : {kMaxLds / 2 - 2048, kMaxLds - 4096}) {
+    for (uint32_t cand = 2048; cand >= 64 && tile == 0; cand /= 2) {
+      const uint64_t jlen4 = (cand + static_cast<uint64_t>(max_residual) + 4 + 3) & ~3ull;
+      const uint64_t il = jlen4 + 8;
+      // stage 1 writes (computes or zeroes) inter[0, il) in kSbQ dword
+      // pairs per thread
+      if (il > static_cast<uint64_t>(kThreads) * kSbQ * 4) continue;
+      const uint64_t bytes = nsub * il * 2 + res_bytes;
+      if (bytes <= budget) {
+        tile = cand;
+        inter_len = static_cast<uint32_t>(il);
+        lds = bytes;
+      }
+    }
+    if (tile != 0) return;
+  }
+}
+
 // Reads and clears the LDS-bound guard flag (a host/kernel table mismatch).
 static void check_dedispersion_guard(hipStream_t stream) {
   int32_t* flag = error_flag_for_device();
@@ -503,26 +530,9 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
     check(a->ndm_per_nominal <= 1024, "ndm_per_nominal must be <= 1024 on the GPU");
     const uint32_t flush = static_cast<uint32_t>(
         std::max<uint64_t>(1, 65535 / (255 * a->subband_channels)));
-    // Largest time tile whose LDS fits two workgroups per CU, else one.
-    const uint64_t res_bytes = a->ndm_per_nominal * nsub * 4;
     uint32_t tile = 0, inter_len = 0;
     uint64_t lds = 0;
-    for (const uint64_t budget : {kMaxLds / 2 - 2048, kMaxLds - 4096}) {
-      for (uint32_t cand = 2048; cand >= 64 && tile == 0; cand /= 2) {
-        const uint64_t jlen4 = (cand + static_cast<uint64_t>(a->max_residual) + 4 + 3) & ~3ull;
-        const uint64_t il = jlen4 + 8;
-        // stage 1 writes (computes or zeroes) inter[0, il) in kSbQ dword
-        // pairs per thread
-        if (il > static_cast<uint64_t>(kThreads) * kSbQ * 4) continue;
-        const uint64_t bytes = nsub * il * 2 + res_bytes;
-        if (bytes <= budget) {
-          tile = cand;
-          inter_len = static_cast<uint32_t>(il);
-          lds = bytes;
-        }
-      }
-      if (tile != 0) break;
-    }
+    subband_geometry(nsub, a->ndm_per_nominal, a->max_residual, tile, inter_len, lds);
     check_overflow(tile != 0, "subband dedispersion delays too large for LDS");
     const uint64_t tiles = ceil_div(a->out_nsamples, tile);
     check_overflow(tiles * nnom < 0x7FFFFFF0ull, "dedispersion grid too large");
@@ -548,6 +558,16 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
                        error_flag_for_device());
     launch_check("dedisperse_subband_cm_kernel");
   });
+}
+
+uint32_t gaffa_hip_subband_time_tile(uint64_t nsel, uint64_t subband_channels,
+                                     uint64_t ndm_per_nominal, int32_t max_residual) {
+  if (nsel == 0 || subband_channels == 0 || ndm_per_nominal == 0 || max_residual < 0) return 0;
+  uint32_t tile = 0, inter_len = 0;
+  uint64_t lds = 0;
+  subband_geometry(ceil_div(nsel, subband_channels), ndm_per_nominal, max_residual, tile,
+                   inter_len, lds);
+  return tile;
 }
 
 uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,

@@ -1256,6 +1256,8 @@ PYBIND11_MODULE(_core, m) {
                                               static_cast<py::ssize_t>(nchans)});
       })
       .def_property_readonly("ndm", &GpuSearchPipeline::ndm)
+      .def_property_readonly("segment_ndm_per_nominal",
+                             &GpuSearchPipeline::segment_ndm_per_nominal)
       .def_property_readonly("out_nsamples", &GpuSearchPipeline::out_nsamples)
       .def_property_readonly("dms", &GpuSearchPipeline::dms)
       .def_property_readonly("ffa_plan", &GpuSearchPipeline::ffa_plan)

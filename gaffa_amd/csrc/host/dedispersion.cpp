@@ -206,7 +206,20 @@ GpuSegmentTables make_gpu_segment_tables(std::span<const double> f,
   t.subband_channels = so.subband_channels;
   t.ndm_per_nominal = so.ndm_per_nominal;
   if (subband) {
-    const SubbandTables st = make_subband_tables(f, p, so);
+    // Narrow the nominal DM group (halving) until the residual span of a
+    // group fits the kernel's LDS intermediate: wide DM steps at high DM
+    // (4096 channels, DM ~1000) need it.  The sums are then gaffa's subband
+    // dedispersion at that ndm_per_nominal.
+    SubbandDedispersionOptions o = so;
+    SubbandTables st = make_subband_tables(f, p, o);
+    const std::size_t nsel = p.chan_end - p.chan_begin;
+    while (o.ndm_per_nominal > 1 &&
+           gaffa_hip_subband_time_tile(nsel, o.subband_channels, o.ndm_per_nominal,
+                                       st.max_residual) == 0) {
+      o.ndm_per_nominal = (o.ndm_per_nominal + 1) / 2;
+      st = make_subband_tables(f, p, o);
+    }
+    t.ndm_per_nominal = o.ndm_per_nominal;
     for (std::int32_t d : st.coarse) check_arg(d >= 0, "dedispersion requires non-negative delays");
     t.first = upload(st.coarse.data(), st.coarse.size());
     t.second = upload(st.residual.data(), st.residual.size());

@@ -82,6 +82,15 @@ class GpuSearchPipeline {
   void set_kernel_timing(bool on) noexcept { kernel_timing_ = on; }
 
   std::size_t ndm() const noexcept { return ndm_; }
+  // DM trials per nominal group each segment's subband tables use (the
+  // configured value, halved where the group's residual span would not fit
+  // the kernel's LDS); empty for brute force.
+  std::vector<std::size_t> segment_ndm_per_nominal() const {
+    std::vector<std::size_t> v;
+    for (const GpuSegmentTables& t : segment_tables_)
+      if (t.subband) v.push_back(t.ndm_per_nominal);
+    return v;
+  }
   std::size_t out_nsamples() const noexcept { return n_out_; }
   std::size_t plane_stride() const noexcept { return stride_; }
   double tsamp() const noexcept { return cfg_.tsamp; }

@@ -154,6 +154,13 @@ int gaffa_hip_copy_rows_u8(const uint8_t* src, uint8_t* dst, uint64_t rows,
 int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* args,
                                       gaffa_hip_stream stream);
 
+/* Time tile gaffa_hip_dedisperse_subband_u8 uses for this geometry (the
+ * u16 subband intermediate of every subband over tile + max_residual
+ * samples must fit LDS), or 0 when none fits: the call would fail with
+ * GAFFA_HIP_OVERFLOW and a smaller ndm_per_nominal is needed. */
+uint32_t gaffa_hip_subband_time_tile(uint64_t nsel, uint64_t subband_channels,
+                                     uint64_t ndm_per_nominal, int32_t max_residual);
+
 /* Row length (bytes) a channel-major copy needs so that every stage-1 read
  * of gaffa_hip_dedisperse_subband_u8 stays in the row. */
 uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,
@@ -487,6 +494,101 @@ int gaffa_hip_synth_series(const gaffa_hip_synth_series_args* args,
 
 int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* args,
                                gaffa_hip_stream stream);
+
+/* ----------------------------------------------------- candidate scorer ----
+ * No gaffa counterpart: a learned score that decides which search
+ * candidates go on to full folding.  Inputs per candidate: a 32-subint x
+ * 64-phase folded cube and a 128-trial periodogram patch, each normalised
+ * to zero mean and unit variance.  Network (fixed shape):
+ *   conv1..conv4: circular 1D convolutions over phase, kernel 3,
+ *                 32 -> 64 -> 64 -> 64 -> 64 channels (the subints are
+ *                 conv1's input channels), each + bias, ReLU;
+ *   pool:         mean and max over phase (128) ++ the patch (128) = 256;
+ *   fc1:          256 -> 512, + bias, ReLU;  fc2: 512 -> 1 logit.
+ * Weights and activations are bfloat16 (round to nearest even), every
+ * product sum accumulates in FP32 on MFMA (v_mfma_f32_16x16x32_bf16).
+ * Parameters live in one FP32 master vector (layout below); a bfloat16
+ * copy feeds the kernels and is refreshed by gaffa_hip_scorer_adam /
+ * gaffa_hip_scorer_sync_params. */
+
+#define GAFFA_HIP_SCORER_SUBINTS 32
+#define GAFFA_HIP_SCORER_PHASE_BINS 64
+#define GAFFA_HIP_SCORER_PATCH 128
+#define GAFFA_HIP_SCORER_CHANNELS 64
+#define GAFFA_HIP_SCORER_HIDDEN 512
+#define GAFFA_HIP_SCORER_LAYERS 6
+
+/* Float offsets of each layer's weight and bias in the parameter vector.
+ * Conv weights are [out][tap][in] (tap 0 reads phase p - 1), fc weights
+ * [out][in]. */
+typedef struct gaffa_hip_scorer_layout {
+  uint64_t weight[GAFFA_HIP_SCORER_LAYERS];
+  uint64_t bias[GAFFA_HIP_SCORER_LAYERS];
+  uint64_t fan_in[GAFFA_HIP_SCORER_LAYERS];
+  uint64_t nparams;        /* padded to a multiple of 8 */
+} gaffa_hip_scorer_layout;
+
+int gaffa_hip_scorer_get_layout(gaffa_hip_scorer_layout* layout);
+
+/* Device workspace for up to `batch` candidates per call. */
+uint64_t gaffa_hip_scorer_workspace_bytes(uint32_t batch);
+
+typedef struct gaffa_hip_scorer_net {
+  float* params;      /* [nparams] FP32 master */
+  uint16_t* params16; /* [nparams] bfloat16 copy used by the kernels */
+  float* grads;       /* [nparams], written by backward */
+  float* adam_m;      /* [nparams] */
+  float* adam_v;      /* [nparams] */
+  void* workspace;    /* gaffa_hip_scorer_workspace_bytes(capacity) */
+  uint32_t capacity;  /* batch capacity of the workspace */
+} gaffa_hip_scorer_net;
+
+/* Forward pass of `batch` candidates.  Candidate b reads cube
+ * rows index[b] (or b when index is null) of cubes [N][32][64] and
+ * patches [N][128] (FP32, device).  Writes logits[b] (device, FP32) and
+ * keeps the activations in the workspace for gaffa_hip_scorer_backward. */
+int gaffa_hip_scorer_forward(const gaffa_hip_scorer_net* net, const float* cubes,
+                             const float* patches, const uint32_t* index, uint32_t batch,
+                             float* logits, gaffa_hip_stream stream);
+
+/* Backward pass of the binary cross-entropy with logits, mean over the
+ * batch, for the last forward: labels[b] (device, 0 or 1, read at the same
+ * index as the inputs).  Writes every parameter gradient to net->grads and
+ * the loss to *loss (device). */
+int gaffa_hip_scorer_backward(const gaffa_hip_scorer_net* net, const float* labels,
+                              const uint32_t* index, uint32_t batch, float* loss,
+                              gaffa_hip_stream stream);
+
+/* One Adam step (bias-corrected, step >= 1) on the FP32 master, then the
+ * bfloat16 copy is refreshed. */
+int gaffa_hip_scorer_adam(const gaffa_hip_scorer_net* net, float lr, float beta1, float beta2,
+                          float eps, uint32_t step, gaffa_hip_stream stream);
+
+/* params16 = bfloat16(params), after the master was written directly. */
+int gaffa_hip_scorer_sync_params(const gaffa_hip_scorer_net* net, gaffa_hip_stream stream);
+
+/* The MFMA GEMM the scorer is built from, exposed for testing:
+ * C(i, j) = sum_k A(i, k) B(k, j) over bfloat16 A(i, k) = a[i*a_row + k*a_col]
+ * and B(k, j) = b[k*b_row + j*b_col], FP32 accumulation; then + bias[j]
+ * (if bias), max(., 0) (if relu), stored to c32 and/or c16 (bfloat16) at
+ * [i*ldc + j].  Split over k_splits workgroups along k (partial sums in
+ * `partial`, [k_splits][m][n] floats, added in split order). */
+typedef struct gaffa_hip_gemm_args {
+  const uint16_t* a;
+  int64_t a_row, a_col;
+  const uint16_t* b;
+  int64_t b_row, b_col;
+  uint32_t m, n, k;
+  const float* bias;
+  uint32_t relu;
+  float* c32;
+  uint16_t* c16;
+  uint64_t ldc;
+  uint32_t k_splits;
+  float* partial;
+} gaffa_hip_gemm_args;
+
+int gaffa_hip_gemm_bf16(const gaffa_hip_gemm_args* args, gaffa_hip_stream stream);
 
 #ifdef __cplusplus
 }

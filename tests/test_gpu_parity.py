@@ -135,13 +135,46 @@ def test_subband_large_slice_vs_oracle(gpu, port):
     np.testing.assert_array_equal(x[:4096], want_x)
     assert x[:, w.rfi_channels].mean() > x.mean() + 5  # the +40 channels are there
     fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
-    for seg in (w.dm_segments[0], w.dm_segments[-1]):
+    # at DM ~990 a 32-trial nominal group spans ~1700 residual samples: the
+    # 64-subband intermediate does not fit LDS, and 16 trials per group do
+    top = w.dm_segments[-1]
+    with pytest.raises(OverflowError, match="too large for LDS"):
+        dedispersion.dedisperse_subband(fb, dm_low=top.dm_low, dm_step=top.dm_step, ndm=64,
+                                        subband_channels=64, ndm_per_nominal=32)
+    for seg, npn in ((w.dm_segments[0], 32), (top, 16)):
         r = dedispersion.dedisperse_subband(fb, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
-                                            subband_channels=64, ndm_per_nominal=32)
+                                            subband_channels=64, ndm_per_nominal=npn)
         want = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
                                tsamp=w.tsamp, subband=True, subband_channels=64,
-                               ndm_per_nominal=32)
+                               ndm_per_nominal=npn)
         np.testing.assert_array_equal(r.data, want)
+
+
+def test_large_pipeline_segments_vs_oracle(gpu, port):
+    """The pipeline at 4096 channels narrows the nominal group of the
+    high-DM segments to fit LDS and reports it; the plane rows of the first
+    and last segments equal the oracle at the reported group size."""
+    import dataclasses
+
+    from gaffa_amd.pipeline import large_workload
+
+    say("pipeline: 4096-ch x 2^17 dedispersion, segment nominal groups vs oracle")
+    w = dataclasses.replace(large_workload(nsamples=1 << 17, npulsars=4), period_max=1.0)
+    pipe = w.pipeline(ffa_workspace_bytes=2 << 30)
+    npn = pipe.segment_ndm_per_nominal
+    assert len(npn) == len(w.dm_segments) and npn[0] == 32 and npn[-1] < 32
+    assert all(a >= b for a, b in zip(npn, npn[1:]))
+    pipe.generate(w.synth_spec())
+    pipe.dedisperse()
+    plane = pipe.download_plane_u32()
+    x = pipe.download_filterbank(w.nsamples, w.nchans)
+    n_out = plane.shape[1]
+    for k in (0, len(w.dm_segments) - 1):
+        seg = w.dm_segments[k]
+        want = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step,
+                               ndm=seg.ndm, tsamp=w.tsamp, subband=True, subband_channels=64,
+                               ndm_per_nominal=npn[k])
+        np.testing.assert_array_equal(plane[k * seg.ndm:(k + 1) * seg.ndm], want[:, :n_out])
 
 
 def test_c_abi_direct_call(gpu, port):

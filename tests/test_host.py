@@ -16,7 +16,7 @@ from conftest import GOLDEN, ROOT
 def declared_functions():
     text = open(os.path.join(ROOT, "include", "gaffa_hip.h")).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"^\s*(?:int|uint64_t|const char\*)\s+(gaffa_hip_\w+)\s*\(", text, re.M)
+    names = re.findall(r"^\s*(?:int|uint32_t|uint64_t|const char\*)\s+(gaffa_hip_\w+)\s*\(", text, re.M)
     assert len(names) > 20
     return names
 
