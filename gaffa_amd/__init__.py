@@ -23,11 +23,13 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
     ) from exc
 
 from ._core import cuda_device_count, gpu_device_count, gpu_runtime_version  # noqa: E402
-from . import dedispersion, ffa, io, pipeline  # noqa: E402
+from . import dedispersion, ffa, io, pipeline, scorer  # noqa: E402
+from .scorer import CandidateScorer  # noqa: E402
 
 HIP_LIBRARY = os.path.join(_HERE, "libgaffa_hip.so")
 
 __all__ = [
+    "CandidateScorer",
     "HIP_LIBRARY",
     "cuda_device_count",
     "dedispersion",
@@ -36,4 +38,5 @@ __all__ = [
     "gpu_runtime_version",
     "io",
     "pipeline",
+    "scorer",
 ]

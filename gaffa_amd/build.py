@@ -31,7 +31,7 @@ HIP_SOURCES = ["runtime.hip", "dedisperse.hip", "preprocess.hip", "ffa.hip", "sy
 HOST_SOURCES = ["host/dedispersion.cpp", "host/preprocessing.cpp", "host/ffa_plan.cpp",
                 "host/ffa_gpu.cpp", "host/hip_runtime.cpp", "host/synth.cpp", "host/pipeline.cpp",
                 "host/ffa_batch.cpp", "host/candidate.cpp", "host/folding.cpp",
-                "host/filterbank.cpp",
+                "host/filterbank.cpp", "host/scorer.cpp",
                 "host/bindings.cpp"]
 
 # gfx950 only.  -ffp-contract=off keeps every FP32/FP64 operation separately

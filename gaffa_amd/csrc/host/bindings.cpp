@@ -10,6 +10,7 @@
 #include "gaffa/folding.h"
 #include "gaffa/pipeline.h"
 #include "gaffa/preprocessing.h"
+#include "gaffa/scorer.h"
 #include "gaffa/synth.h"
 
 #include <pybind11/numpy.h>
@@ -549,6 +550,29 @@ PyFoldResult py_fold_spectrum(const py::array& data, double tsamp, double period
   p.tsubint = r.tsubint;
   return p;
 }
+
+namespace {
+
+// Scorer inputs: float32 C-contiguous cubes (n, 32, 64), patches (n, 128)
+// and, when given, labels (n,).
+using FloatArray = py::array_t<float, py::array::c_style | py::array::forcecast>;
+
+std::size_t scorer_rows(const FloatArray& cubes, const FloatArray& patches,
+                        const FloatArray* labels) {
+  if (cubes.ndim() != 3 || cubes.shape(1) != GAFFA_HIP_SCORER_SUBINTS ||
+      cubes.shape(2) != GAFFA_HIP_SCORER_PHASE_BINS)
+    throw py::value_error("cubes must have shape (n, 32, 64)");
+  if (patches.ndim() != 2 || patches.shape(1) != GAFFA_HIP_SCORER_PATCH)
+    throw py::value_error("patches must have shape (n, 128)");
+  const auto n = static_cast<std::size_t>(cubes.shape(0));
+  if (n == 0 || static_cast<std::size_t>(patches.shape(0)) != n)
+    throw py::value_error("cubes and patches must hold the same non-zero number of candidates");
+  if (labels != nullptr && (labels->ndim() != 1 || static_cast<std::size_t>(labels->shape(0)) != n))
+    throw py::value_error("labels must have shape (n,)");
+  return n;
+}
+
+}  // namespace
 
 PYBIND11_MODULE(_core, m) {
   m.doc() = "gaffa search hot path on AMD Instinct MI355X (HIP, gfx950)";
@@ -1288,4 +1312,89 @@ PYBIND11_MODULE(_core, m) {
             return d;
           },
           "Device layout of the FFA plan: LDS depth per task, launches and items");
+
+  // learned candidate scorer
+  py::class_<GpuCandidateScorer>(m, "CandidateScorer")
+      .def(py::init<std::size_t, int>(), py::arg("capacity") = 1024, py::arg("device_id") = 0)
+      .def_property_readonly("parameter_count", &GpuCandidateScorer::parameter_count)
+      .def_property_readonly("capacity", &GpuCandidateScorer::capacity)
+      .def_static("layout",
+                  [] {
+                    const gaffa_hip_scorer_layout l = GpuCandidateScorer::layout();
+                    py::dict d;
+                    d["weight"] = std::vector<std::uint64_t>(l.weight, l.weight + 6);
+                    d["bias"] = std::vector<std::uint64_t>(l.bias, l.bias + 6);
+                    d["fan_in"] = std::vector<std::uint64_t>(l.fan_in, l.fan_in + 6);
+                    d["nparams"] = l.nparams;
+                    return d;
+                  })
+      .def("initialize", &GpuCandidateScorer::initialize, py::arg("seed"))
+      .def_property(
+          "parameters",
+          [](const GpuCandidateScorer& s) {
+            const std::vector<float> p = s.parameters();
+            py::array_t<float> out(static_cast<py::ssize_t>(p.size()));
+            std::memcpy(out.mutable_data(), p.data(), p.size() * sizeof(float));
+            return out;
+          },
+          [](GpuCandidateScorer& s, const FloatArray& p) {
+            if (p.ndim() != 1) throw py::value_error("parameters must be 1D");
+            s.set_parameters(std::span<const float>(p.data(), static_cast<std::size_t>(p.size())));
+          })
+      .def(
+          "forward",
+          [](GpuCandidateScorer& s, const FloatArray& cubes, const FloatArray& patches) {
+            const std::size_t n = scorer_rows(cubes, patches, nullptr);
+            const float* c = cubes.data();
+            const float* q = patches.data();
+            std::vector<float> z;
+            {
+              py::gil_scoped_release release;
+              z = s.forward(c, q, n);
+            }
+            py::array_t<float> out(static_cast<py::ssize_t>(n));
+            std::memcpy(out.mutable_data(), z.data(), n * sizeof(float));
+            return out;
+          },
+          py::arg("cubes"), py::arg("patches"), "Logits (n,) of the candidates")
+      .def(
+          "gradients",
+          [](GpuCandidateScorer& s, const FloatArray& cubes, const FloatArray& patches,
+             const FloatArray& labels) {
+            const std::size_t n = scorer_rows(cubes, patches, &labels);
+            const float* c = cubes.data();
+            const float* q = patches.data();
+            const float* y = labels.data();
+            std::vector<float> g;
+            float loss = 0.0f;
+            {
+              py::gil_scoped_release release;
+              loss = s.gradients(c, q, y, n, g);
+            }
+            py::array_t<float> out(static_cast<py::ssize_t>(g.size()));
+            std::memcpy(out.mutable_data(), g.data(), g.size() * sizeof(float));
+            return py::make_tuple(loss, out);
+          },
+          py::arg("cubes"), py::arg("patches"), py::arg("labels"),
+          "(mean BCE loss, gradient of every parameter) for one batch")
+      .def(
+          "fit",
+          [](GpuCandidateScorer& s, const FloatArray& cubes, const FloatArray& patches,
+             const FloatArray& labels, std::size_t epochs, std::size_t batch, float lr,
+             std::uint64_t seed) {
+            const std::size_t n = scorer_rows(cubes, patches, &labels);
+            ScorerTrainOptions o;
+            o.epochs = epochs;
+            o.batch = batch;
+            o.learning_rate = lr;
+            o.seed = seed;
+            const float* c = cubes.data();
+            const float* q = patches.data();
+            const float* y = labels.data();
+            py::gil_scoped_release release;
+            return s.fit(c, q, y, n, o);
+          },
+          py::arg("cubes"), py::arg("patches"), py::arg("labels"), py::kw_only(),
+          py::arg("epochs") = 4, py::arg("batch") = 256, py::arg("learning_rate") = 1e-3f,
+          py::arg("seed") = 1, "Adam training on the GPU; returns the mean loss per epoch");
 }

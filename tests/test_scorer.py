@@ -1,0 +1,241 @@
+"""The learned candidate scorer: the FP64 oracle pinned by finite differences
+and by torch autograd (CPU), the parameter layout of the C ABI (CPU), and the
+GPU's MFMA GEMM, forward and backward passes against the oracle, plus a
+training run (GPU)."""
+
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+
+sys.path.insert(0, ROOT)
+from oracle import scorer_oracle as so  # noqa: E402
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def random_case(seed, batch, scale=1.0):
+    rng = np.random.default_rng(seed)
+    lay = so.layout()
+    p = np.zeros(lay["nparams"])
+    for i in range(6):
+        n = lay["bias"][i] - lay["weight"][i]
+        p[lay["weight"][i]:lay["bias"][i]] = rng.uniform(-1, 1, n) * np.sqrt(6 / lay["fan_in"][i])
+        nb = so.CHANNELS if i < 4 else (so.HIDDEN if i == 4 else 1)
+        p[lay["bias"][i]:lay["bias"][i] + nb] = rng.normal(0, 0.1, nb)
+    cubes = rng.normal(0, scale, (batch, so.SUBINTS, so.PHASE))
+    patches = rng.normal(0, scale, (batch, so.PATCH))
+    labels = rng.integers(0, 2, batch).astype(np.float64)
+    return p, cubes, patches, labels
+
+
+# ----------------------------------------------------------- CPU: oracle ----
+def test_layout_matches_c_abi():
+    import gaffa_amd
+
+    class Layout(ctypes.Structure):
+        _fields_ = [("weight", ctypes.c_uint64 * 6), ("bias", ctypes.c_uint64 * 6),
+                    ("fan_in", ctypes.c_uint64 * 6), ("nparams", ctypes.c_uint64)]
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    lay = Layout()
+    assert lib.gaffa_hip_scorer_get_layout(ctypes.byref(lay)) == 0
+    want = so.layout()
+    assert list(lay.weight) == want["weight"] and list(lay.bias) == want["bias"]
+    assert list(lay.fan_in) == want["fan_in"] and lay.nparams == want["nparams"]
+    assert 150_000 < want["nparams"] < 250_000  # ~200k parameters
+
+
+def test_oracle_matches_torch_autograd():
+    """The FP64 oracle's loss and every gradient equal torch's autograd of
+    the same network (circular conv1d, mean/max pooling, BCE with logits)."""
+    torch = pytest.importorskip("torch")
+    import torch.nn.functional as F
+
+    p, cubes, patches, labels = random_case(3, 5)
+    lay = so.layout()
+    loss, g = so.loss_and_gradients(p, cubes, patches, labels, emulate=False)
+
+    P = torch.tensor(p, dtype=torch.float64, requires_grad=True)
+    x = torch.tensor(cubes, dtype=torch.float64)  # [B][32 ch][64 phase]
+    for l in range(4):
+        w = P[lay["weight"][l]:lay["bias"][l]].reshape(so.CHANNELS, 3, so.CIN[l])
+        b = P[lay["bias"][l]:lay["bias"][l] + so.CHANNELS]
+        x = F.relu(F.conv1d(F.pad(x, (1, 1), mode="circular"), w.permute(0, 2, 1), b))
+    mean = x.mean(dim=2)
+    mx, _ = x.max(dim=2)
+    f = torch.cat([mean, mx, torch.tensor(patches, dtype=torch.float64)], dim=1)
+    w5 = P[lay["weight"][4]:lay["bias"][4]].reshape(so.HIDDEN, so.FEATURES)
+    h = F.relu(f @ w5.T + P[lay["bias"][4]:lay["bias"][4] + so.HIDDEN])
+    z = h @ P[lay["weight"][5]:lay["bias"][5]] + P[lay["bias"][5]]
+    t_loss = F.binary_cross_entropy_with_logits(z, torch.tensor(labels, dtype=torch.float64))
+    t_loss.backward()
+    assert abs(loss - t_loss.item()) <= 1e-12 * max(1.0, abs(loss))
+    np.testing.assert_allclose(g, P.grad.numpy(), rtol=1e-9, atol=1e-12)
+
+
+def test_oracle_finite_differences():
+    """Central differences of the FP64 loss at parameters of every layer."""
+    p, cubes, patches, labels = random_case(4, 3)
+    lay = so.layout()
+    _, g = so.loss_and_gradients(p, cubes, patches, labels, emulate=False)
+    rng = np.random.default_rng(0)
+    idx = []
+    for i in range(6):
+        idx += list(rng.integers(lay["weight"][i], lay["bias"][i], 3)) + [lay["bias"][i]]
+    h = 1e-6
+    for i in idx:
+        e = np.zeros_like(p)
+        e[i] = h
+        lp, _ = so.loss_and_gradients(p + e, cubes, patches, labels, emulate=False)
+        lm, _ = so.loss_and_gradients(p - e, cubes, patches, labels, emulate=False)
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1e-3, abs(g[i])), (i, fd, g[i])
+
+
+def test_oracle_emulation_is_close_to_fp64():
+    """bfloat16 rounding moves the loss and gradients by a few percent at
+    most (it is the GPU's arithmetic, so the GPU is held to the emulation)."""
+    p, cubes, patches, labels = random_case(5, 16)
+    l64, g64 = so.loss_and_gradients(p, cubes, patches, labels, emulate=False)
+    l16, g16 = so.loss_and_gradients(p, cubes, patches, labels, emulate=True)
+    assert abs(l16 - l64) <= 0.02 * abs(l64)
+    assert np.linalg.norm(g16 - g64) <= 0.05 * np.linalg.norm(g64)
+
+
+# ------------------------------------------------------------ GPU: MFMA ----
+def _bf16_bits(x):
+    u = np.asarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    return (((u + 0x7FFF + ((u >> 16) & 1)) >> 16) & 0xFFFF).astype(np.uint16)
+
+
+def _bits_to_f64(b):
+    return (b.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["fast", "strided_split", "edge"])
+def test_mfma_gemm_vs_fp64(case):
+    """gaffa_hip_gemm_bf16 through the C ABI: the MFMA fragment layout, the
+    K-contiguous and strided operand paths, split-K, bias, ReLU and both
+    outputs, against an FP64 product of the same bfloat16 operands (FP32
+    accumulation error only: <= 1e-5 of the largest |C|)."""
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    lib.gaffa_hip_last_error.restype = ctypes.c_char_p
+
+    class G(ctypes.Structure):
+        _fields_ = [("a", ctypes.c_void_p), ("a_row", ctypes.c_int64), ("a_col", ctypes.c_int64),
+                    ("b", ctypes.c_void_p), ("b_row", ctypes.c_int64), ("b_col", ctypes.c_int64),
+                    ("m", ctypes.c_uint32), ("n", ctypes.c_uint32), ("k", ctypes.c_uint32),
+                    ("bias", ctypes.c_void_p), ("relu", ctypes.c_uint32),
+                    ("c32", ctypes.c_void_p), ("c16", ctypes.c_void_p), ("ldc", ctypes.c_uint64),
+                    ("k_splits", ctypes.c_uint32), ("partial", ctypes.c_void_p)]
+
+    rng = np.random.default_rng({"fast": 1, "strided_split": 2, "edge": 3}[case])
+    m, n, k, splits, relu = {"fast": (70, 50, 96, 1, 0), "strided_split": (100, 130, 1000, 4, 1),
+                             "edge": (1, 1, 5, 1, 0)}[case]
+    A = _bf16_bits(rng.normal(size=(m, k)))
+    B = _bf16_bits(rng.normal(size=(k, n)))
+    bias = rng.normal(size=n).astype(np.float32)
+    if case == "fast":  # A row-major, B stored as [n][k] (K-contiguous both)
+        a_host, a_row, a_col = A, k, 1
+        b_host, b_row, b_col = np.ascontiguousarray(B.T), 1, k
+    else:  # A stored transposed ([k][m]), B row-major ([k][n]): strided paths
+        a_host, a_row, a_col = np.ascontiguousarray(A.T), 1, m
+        b_host, b_row, b_col = B, n, 1
+    want = _bits_to_f64(A) @ _bits_to_f64(B) + (bias if case != "edge" else 0)
+    if relu:
+        want = np.maximum(want, 0)
+
+    def dmalloc(nbytes):
+        ptr = ctypes.c_void_p()
+        assert lib.gaffa_hip_malloc(ctypes.byref(ptr), ctypes.c_uint64(max(nbytes, 256))) == 0
+        return ptr
+
+    def put(arr):
+        arr = np.ascontiguousarray(arr)
+        ptr = dmalloc(arr.nbytes)
+        assert lib.gaffa_hip_memcpy(ptr, arr.ctypes.data_as(ctypes.c_void_p),
+                                    ctypes.c_uint64(arr.nbytes), None) == 0
+        return ptr
+
+    say(f"gemm {case}: m={m} n={n} k={k} splits={splits}")
+    ptrs = [put(a_host), put(b_host), put(bias), dmalloc(m * n * 4), dmalloc(m * n * 2),
+            dmalloc(splits * m * n * 4)]
+    try:
+        g = G(ptrs[0], a_row, a_col, ptrs[1], b_row, b_col, m, n, k,
+              ptrs[2] if case != "edge" else None, relu, ptrs[3], ptrs[4], n, splits, ptrs[5])
+        assert lib.gaffa_hip_gemm_bf16(ctypes.byref(g), None) == 0, lib.gaffa_hip_last_error()
+        c32 = np.empty((m, n), np.float32)
+        c16 = np.empty((m, n), np.uint16)
+        for host, dev in ((c32, ptrs[3]), (c16, ptrs[4])):
+            assert lib.gaffa_hip_memcpy(host.ctypes.data_as(ctypes.c_void_p), dev,
+                                        ctypes.c_uint64(host.nbytes), None) == 0
+    finally:
+        for ptr in ptrs:
+            lib.gaffa_hip_free(ptr)
+    tol = 1e-5 * max(1.0, np.abs(want).max())
+    np.testing.assert_allclose(c32, want, rtol=0, atol=tol)
+    np.testing.assert_array_equal(c16, _bf16_bits(c32))
+
+
+# --------------------------------------------------------- GPU: network ----
+@pytest.mark.gpu
+def test_forward_and_gradients_vs_oracle():
+    """Logits and every layer's gradient from the MFMA kernels against the
+    oracle with bfloat16 emulation: they differ only in FP32 accumulation
+    order (and the rare bfloat16 rounding flip it causes), so the logits
+    agree to 2e-3 and each layer's gradient to 2 % in L2 norm."""
+    import gaffa_amd
+
+    p, cubes, patches, labels = random_case(7, 96)
+    s = gaffa_amd.CandidateScorer(capacity=128)
+    s.parameters = p.astype(np.float32)
+    p32 = p.astype(np.float32).astype(np.float64)
+    z_want, _ = so.forward(p32, cubes.astype(np.float32), patches.astype(np.float32))
+    z = s.forward(cubes.astype(np.float32), patches.astype(np.float32))
+    np.testing.assert_allclose(z, z_want, rtol=0, atol=2e-3 * max(1.0, np.abs(z_want).max()))
+    loss, g = s.gradients(cubes.astype(np.float32), patches.astype(np.float32),
+                          labels.astype(np.float32))
+    l_want, g_want = so.loss_and_gradients(p32, cubes.astype(np.float32),
+                                           patches.astype(np.float32), labels)
+    assert abs(loss - l_want) <= 1e-3 * max(1.0, abs(l_want))
+    lay = so.layout()
+    for i in range(6):
+        for lo, hi in ((lay["weight"][i], lay["bias"][i]),
+                       (lay["bias"][i], lay["bias"][i] + (so.CHANNELS if i < 4 else
+                                                           (so.HIDDEN if i == 4 else 1)))):
+            err = np.linalg.norm(g[lo:hi] - g_want[lo:hi])
+            assert err <= 0.02 * np.linalg.norm(g_want[lo:hi]) + 1e-7, (i, lo, err)
+
+
+@pytest.mark.gpu
+def test_training_learns_a_separable_set():
+    """Adam on the GPU drives the loss down on a set whose label is a pulse
+    in the cube (a bright phase column) plus a peak in the patch."""
+    import gaffa_amd
+
+    rng = np.random.default_rng(11)
+    n = 4096
+    cubes = rng.normal(size=(n, 32, 64)).astype(np.float32)
+    patches = rng.normal(size=(n, 128)).astype(np.float32)
+    labels = rng.integers(0, 2, n).astype(np.float32)
+    phase = rng.integers(0, 64, n)
+    for i in np.nonzero(labels)[0]:
+        cubes[i, :, phase[i]] += 1.5
+        patches[i, 60:68] += 2.0
+    s = gaffa_amd.CandidateScorer(capacity=256)
+    s.initialize(5)
+    losses = s.fit(cubes, patches, labels, epochs=3, batch=256, learning_rate=1e-3, seed=2)
+    say(f"training losses {losses}")
+    assert losses[-1] < 0.2 < losses[0]
+    z = s.forward(cubes[:512], patches[:512])
+    acc = np.mean((z > 0) == (labels[:512] > 0))
+    assert acc > 0.95
