@@ -65,6 +65,9 @@ struct KArgs {
   gaffa_hip_ffa_peak* peaks;
   uint64_t peak_capacity;
   uint32_t* peak_count;
+  float* periodogram;
+  uint64_t periodogram_stride;
+  uint32_t periodogram_widths;
 };
 
 KArgs kargs(const gaffa_hip_ffa_search_args* a) {
@@ -82,6 +85,9 @@ KArgs kargs(const gaffa_hip_ffa_search_args* a) {
   k.peaks = a->peaks;
   k.peak_capacity = a->peak_capacity;
   k.peak_count = a->peak_count;
+  k.periodogram = a->periodogram;
+  k.periodogram_stride = a->periodogram_stride;
+  k.periodogram_widths = a->periodogram_widths;
   return k;
 }
 
@@ -536,6 +542,10 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
       const float best = boxcar_max(x, b, w, total);
       const float hb = task.height[widx] + task.baseline[widx];
       const float snr = (hb * best - task.baseline[widx] * total) * task.inv_stdnoise;
+      if (a.periodogram != nullptr)
+        a.periodogram[(sidx + task.series) * a.periodogram_stride +
+                      static_cast<uint64_t>(task.periodogram_row + s0 + i) * a.periodogram_widths +
+                      widx] = snr;
       if (snr >= a.snr_threshold) {
         // first phase attaining the maximum (ffa_detection.cpp:158-167)
         float A2 = 0.0f, B2 = 0.0f;
@@ -575,6 +585,9 @@ void validate(const gaffa_hip_ffa_search_args* a) {
   check(a->nsamples > 0, "FFA executor time series must not be empty");
   check(a->series_stride >= a->nsamples, "FFA series stride must be >= nsamples");
   check(a->work != nullptr, "FFA work buffer must not be null");
+  check(a->periodogram == nullptr ||
+            (a->periodogram_widths > 0 && a->periodogram_widths <= GAFFA_HIP_FFA_MAX_WIDTHS),
+        "FFA periodogram width count must be in 1..16");
   check(p.max_bins > 1 && p.max_bins < 65536, "FFA bins must be in (1, 65536)");
   check(p.terminal_lds_bytes <= kMaxLdsBytes, "FFA detection tile exceeds LDS");
   check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");

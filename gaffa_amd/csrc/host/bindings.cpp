@@ -1230,6 +1230,69 @@ PYBIND11_MODULE(_core, m) {
           py::arg("dm_index"), py::arg("period"), py::kw_only(), py::arg("nbin") = 64,
           py::arg("nsubint") = 32,
           "Subint x phase cubes of the preprocessed plane rows (after run())")
+      .def("enable_periodogram", &GpuSearchPipeline::enable_periodogram, py::arg("on") = true,
+           "Keep the dense S/N periodogram of the following searches (scorer patches)")
+      .def_property_readonly("trial_periods",
+                             [](const GpuSearchPipeline& p) {
+                               const std::vector<double>& v = p.trial_periods();
+                               py::array_t<double> out(static_cast<py::ssize_t>(v.size()));
+                               std::memcpy(out.mutable_data(), v.data(), v.size() * sizeof(double));
+                               return out;
+                             })
+      .def(
+          "periodogram",
+          [](const GpuSearchPipeline& p, std::size_t dm_index) {
+            const auto rows = static_cast<py::ssize_t>(p.trial_periods().size());
+            const auto w = static_cast<py::ssize_t>(p.periodogram_widths());
+            py::array_t<float> out({rows, w});
+            p.download_periodogram(dm_index, out.mutable_data());
+            return out;
+          },
+          py::arg("dm_index"), "S/N [trial period][width trial] of one DM trial")
+      .def(
+          "candidate_features",
+          [](GpuSearchPipeline& p, const std::vector<Candidate>& c) {
+            const std::size_t n = c.size();
+            py::array_t<float> cubes({static_cast<py::ssize_t>(n),
+                                      static_cast<py::ssize_t>(GAFFA_HIP_SCORER_SUBINTS),
+                                      static_cast<py::ssize_t>(GAFFA_HIP_SCORER_PHASE_BINS)});
+            py::array_t<float> patches({static_cast<py::ssize_t>(n),
+                                        static_cast<py::ssize_t>(GAFFA_HIP_SCORER_PATCH)});
+            float* hc = cubes.mutable_data();
+            float* hp = patches.mutable_data();
+            {
+              py::gil_scoped_release release;
+              GpuDeviceScope scope(p.device_id());
+              GpuDeviceBuffer<float> dc(n * kScorerCubeFloats), dp(n * kScorerPatchFloats);
+              p.candidate_features(c, dc.data(), dp.data());
+              copy_to_host(hc, dc.data(), dc.bytes(), p.stream());
+              copy_to_host(hp, dp.data(), dp.bytes(), p.stream());
+              hip_ok(gaffa_hip_stream_synchronize(p.stream()), "hipStreamSynchronize");
+            }
+            return py::make_tuple(cubes, patches);
+          },
+          py::arg("candidates"),
+          "Scorer inputs (cubes (n, 32, 64), patches (n, 128)) of candidates of the last search")
+      .def(
+          "score_candidates",
+          [](GpuSearchPipeline& p, GpuCandidateScorer& scorer, const std::vector<Candidate>& c) {
+            const std::size_t n = c.size();
+            py::array_t<float> out(static_cast<py::ssize_t>(n));
+            float* ho = out.mutable_data();
+            {
+              py::gil_scoped_release release;
+              GpuDeviceScope scope(p.device_id());
+              GpuDeviceBuffer<float> dc(n * kScorerCubeFloats), dp(n * kScorerPatchFloats),
+                  dz(n);
+              p.candidate_features(c, dc.data(), dp.data());
+              scorer.forward_device(dc.data(), dp.data(), n, dz.data(), p.stream());
+              copy_to_host(ho, dz.data(), dz.bytes(), p.stream());
+              hip_ok(gaffa_hip_stream_synchronize(p.stream()), "hipStreamSynchronize");
+            }
+            return out;
+          },
+          py::arg("scorer"), py::arg("candidates"),
+          "Scorer logits of candidates of the last search, features and network on the GPU")
       .def(
           "run_file",
           [](GpuSearchPipeline& p, const std::string& path, std::size_t chunk_samples) {

@@ -129,6 +129,7 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       groups_.push_back(g);
     }
   }
+  std::uint64_t periodogram_row = 0;
   for (const FfaSearchTask& t : plan.tasks) {
     const FfaDetectionPlan det = make_ffa_detection_plan(plan.width_trials, t.bins);
     const float stdnoise = ffa_task_stdnoise(t);
@@ -142,6 +143,9 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
     d.nwidths = static_cast<std::uint32_t>(det.boxcar_trials.size());
     d.inv_stdnoise = 1.0F / stdnoise;
     d.series = series;
+    d.periodogram_row = static_cast<std::uint32_t>(periodogram_row);
+    periodogram_row += t.rows_eval;
+    check_arg(periodogram_row < (1ULL << 32), "FFA plan has too many evaluated rows");
     for (std::size_t w = 0; w < det.boxcar_trials.size(); ++w) {
       d.width[w] = static_cast<std::uint32_t>(det.boxcar_trials[w].width);
       d.height[w] = det.boxcar_trials[w].height;
@@ -274,7 +278,30 @@ gaffa_hip_ffa_search_args GpuFfaProgram::args(const float* series, std::size_t n
   a.peaks = peaks_.data();
   a.peak_capacity = peaks_.size();
   a.peak_count = peak_count_.data();
+  if (periodogram_ != nullptr) {
+    a.periodogram = periodogram_ + static_cast<std::size_t>(base) * periodogram_stride_;
+    a.periodogram_stride = periodogram_stride_;
+    a.periodogram_widths = periodogram_widths_;
+  }
   return a;
+}
+
+void GpuFfaProgram::set_periodogram(float* base, std::size_t stride, std::uint32_t widths) {
+  if (base != nullptr) {
+    check_arg(widths >= exec_plan_.plan().width_trials.size() &&
+                  widths <= GAFFA_HIP_FFA_MAX_WIDTHS,
+              "FFA periodogram needs a slot for every width trial");
+    check_arg(stride >= periodogram_rows() * widths, "FFA periodogram stride too small");
+  }
+  periodogram_ = base;
+  periodogram_stride_ = stride;
+  periodogram_widths_ = widths;
+}
+
+std::size_t GpuFfaProgram::periodogram_rows() const noexcept {
+  std::size_t n = 0;
+  for (const FfaSearchTask& t : exec_plan_.plan().tasks) n += t.rows_eval;
+  return n;
 }
 
 void GpuFfaProgram::enqueue_tile(const float* series, std::size_t nseries, std::size_t stride,

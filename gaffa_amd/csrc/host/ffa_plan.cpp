@@ -171,16 +171,23 @@ float ffa_task_stdnoise(const FfaSearchTask& task) {
   return static_cast<float>(std::sqrt(static_cast<double>(task.rows) * variance));
 }
 
+double ffa_trial_period(const FfaSearchTask& task, std::size_t shift) {
+  if (task.rows <= 1) return task.effective_tsamp * static_cast<double>(task.bins);
+  const double b = static_cast<double>(task.bins);
+  return task.effective_tsamp * b * b /
+         (b - static_cast<double>(shift) / static_cast<double>(task.rows - 1));
+}
+
+std::vector<double> ffa_plan_trial_periods(const FfaSearchPlan& plan) {
+  std::vector<double> v;
+  for (const FfaSearchTask& t : plan.tasks)
+    for (std::size_t s = 0; s < t.rows_eval; ++s) v.push_back(ffa_trial_period(t, s));
+  return v;
+}
+
 FfaPeak make_ffa_peak(const FfaSearchTask& task, const FfaBoxcarTrial& trial, std::size_t shift,
                       std::size_t phase, float snr) {
-  double period;
-  if (task.rows <= 1) {
-    period = task.effective_tsamp * static_cast<double>(task.bins);
-  } else {
-    const double b = static_cast<double>(task.bins);
-    period = task.effective_tsamp * b * b /
-             (b - static_cast<double>(shift) / static_cast<double>(task.rows - 1));
-  }
+  const double period = ffa_trial_period(task, shift);
   FfaPeak p;
   p.period = period;
   p.frequency = 1.0 / period;

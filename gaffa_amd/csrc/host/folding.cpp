@@ -39,6 +39,60 @@ std::size_t fold_subint_samples(double tsubint, double tsamp) {
   return std::max<std::size_t>(1, static_cast<std::size_t>(rounded));
 }
 
+namespace {
+
+// Validates, uploads the jobs and launches the fold kernel into out /
+// exposure (device).  Returns nsubint.
+std::size_t launch_fold(const GpuFoldInput& in, std::span<const FoldJob> jobs, double tsamp,
+                        std::size_t nbin, std::size_t subint_samples, std::size_t nout,
+                        float* out, double* exposure, gaffa_hip_stream stream) {
+  const std::size_t nsubint = (in.nsamples + subint_samples - 1) / subint_samples;
+  std::vector<gaffa_hip_fold_job> hj;
+  for (const FoldJob& j : jobs) hj.push_back(gaffa_hip_fold_job{j.input_offset, j.period, j.start_sample});
+  GpuDeviceBuffer<gaffa_hip_fold_job> d_jobs = upload(hj.data(), hj.size());
+  gaffa_hip_fold_args a{};
+  a.data = in.data;
+  a.dtype = static_cast<std::int32_t>(in.dtype);
+  a.nsamples = in.nsamples;
+  a.nchans = static_cast<std::uint32_t>(in.nchans);
+  a.row_stride = in.row_stride == 0 ? in.nchans : in.row_stride;
+  a.tsamp = tsamp;
+  a.nbin = static_cast<std::uint32_t>(nbin);
+  a.subint_samples = subint_samples;
+  a.nsubint = nsubint;
+  a.output_channels = static_cast<std::uint32_t>(nout);
+  a.jobs = d_jobs.data();
+  a.njobs = static_cast<std::uint32_t>(jobs.size());
+  a.out = out;
+  a.exposure = exposure;
+  hip_ok(gaffa_hip_fold_batch(&a, stream), "fold_batch");
+  // the job table must outlive the kernel
+  hip_ok(gaffa_hip_stream_synchronize(stream), "hipStreamSynchronize");
+  return nsubint;
+}
+
+}  // namespace
+
+void fold_batch_gpu_device(const GpuFoldInput& in, std::span<const FoldJob> jobs, double tsamp,
+                           std::size_t nbin, std::size_t subint_samples,
+                           std::size_t output_channels, float* out, gaffa_hip_stream stream) {
+  check_arg(in.data != nullptr && out != nullptr, "fold data and output must not be null");
+  check_arg(in.nsamples != 0 && in.nchans != 0, "fold spectrum shape must be non-empty");
+  check_arg(!jobs.empty(), "fold batch must contain at least one job");
+  check_arg(subint_samples != 0, "fold subint_samples must be positive");
+  const std::size_t nout = output_channels == 0 ? in.nchans : output_channels;
+  check_arg(nout <= in.nchans && in.nchans % nout == 0,
+            "fold output_channels must divide the input channel count");
+  for (const FoldJob& j : jobs) {
+    validate_shape(j.period, tsamp, nbin);
+    validate_phase_range(j.start_sample, in.nsamples, tsamp / j.period * static_cast<double>(nbin));
+  }
+  GpuDeviceScope scope(in.device_id);
+  const std::size_t nsubint = (in.nsamples + subint_samples - 1) / subint_samples;
+  GpuDeviceBuffer<double> d_exp(jobs.size() * nsubint * nbin);
+  launch_fold(in, jobs, tsamp, nbin, subint_samples, nout, out, d_exp.data(), stream);
+}
+
 FoldedCube fold_batch_gpu(const GpuFoldInput& in, std::span<const FoldJob> jobs, double tsamp,
                           std::size_t nbin, std::size_t subint_samples,
                           std::size_t output_channels, std::vector<double>* exposure,

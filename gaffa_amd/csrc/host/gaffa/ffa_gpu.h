@@ -137,6 +137,12 @@ class GpuFfaProgram {
   std::size_t peak_capacity() const noexcept { return peaks_.size(); }
   void grow_peaks(std::size_t count);
   gaffa_hip_stream stream() const noexcept { return exec_.stream; }
+  // Dense periodogram output for the following tiles: series i of the
+  // whole batch (series_index_base + tile index) writes the S/N of every
+  // evaluated (row, width) at base + i * stride + row * widths + width, rows
+  // in plan order (periodogram_rows() of them).  nullptr switches it off.
+  void set_periodogram(float* base, std::size_t stride, std::uint32_t widths);
+  std::size_t periodogram_rows() const noexcept;
   // Transform-only access for testing: rows_eval final rows of one task.
   void transform_rows(const float* series, std::size_t nseries, std::size_t stride,
                       std::size_t task, float* out);
@@ -155,6 +161,9 @@ class GpuFfaProgram {
   GpuDeviceBuffer<gaffa_hip_ffa_peak> peaks_;
   GpuDeviceBuffer<std::uint32_t> peak_count_;
   gaffa_hip_ffa_plan_desc desc_{};
+  float* periodogram_ = nullptr;
+  std::size_t periodogram_stride_ = 0;
+  std::uint32_t periodogram_widths_ = 0;
 };
 
 // Expand compact device peaks into FfaBatchPeak records (make_ffa_peak).

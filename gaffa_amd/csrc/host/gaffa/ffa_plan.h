@@ -81,6 +81,11 @@ FfaDetectionPlan make_ffa_detection_plan(std::span<const std::size_t> width_tria
 float ffa_task_stdnoise(const FfaSearchTask& task);
 FfaPeak make_ffa_peak(const FfaSearchTask& task, const FfaBoxcarTrial& trial, std::size_t shift,
                       std::size_t phase, float snr);
+// The trial period make_ffa_peak reports for (task, shift).
+double ffa_trial_period(const FfaSearchTask& task, std::size_t shift);
+// Every evaluated trial period of the plan, task by task (rows_eval each):
+// the row order of the dense periodogram.  Ascending for riptide plans.
+std::vector<double> ffa_plan_trial_periods(const FfaSearchPlan& plan);
 
 struct FfaSearchOptions {
   float snr_threshold = 6.0F;

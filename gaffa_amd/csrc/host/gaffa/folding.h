@@ -102,6 +102,12 @@ FoldDedispersedSpectrumResult fold_dedispersed_spectrum_gpu(
 // Many folds of one input layout in one launch (e.g. candidates of a DM x
 // time plane, each its own row and period): cubes [njobs][nsubint][nchans_out]
 // [nbin] and exposure [njobs][nsubint][nbin].
+// As fold_batch_gpu, leaving the folded values on the device in `out`
+// ([jobs][nsubint][output_channels][nbin] floats, enqueued on `stream`).
+void fold_batch_gpu_device(const GpuFoldInput& input, std::span<const FoldJob> jobs, double tsamp,
+                           std::size_t nbin, std::size_t subint_samples,
+                           std::size_t output_channels, float* out, gaffa_hip_stream stream);
+
 FoldedCube fold_batch_gpu(const GpuFoldInput& input, std::span<const FoldJob> jobs, double tsamp,
                           std::size_t nbin, std::size_t subint_samples,
                           std::size_t output_channels, std::vector<double>* exposure,

@@ -6,6 +6,7 @@
 // one stream with no host round trip between stages.
 #pragma once
 
+#include "gaffa/candidate.h"
 #include "gaffa/dedispersion.h"
 #include "gaffa/ffa_gpu.h"
 #include "gaffa/filterbank.h"
@@ -15,6 +16,7 @@
 #include <cstddef>
 #include <cstdint>
 #include <memory>
+#include <span>
 #include <vector>
 
 namespace gaffa {
@@ -94,6 +96,7 @@ class GpuSearchPipeline {
   std::size_t out_nsamples() const noexcept { return n_out_; }
   std::size_t plane_stride() const noexcept { return stride_; }
   double tsamp() const noexcept { return cfg_.tsamp; }
+  int device_id() const noexcept { return cfg_.device_id; }
   const std::vector<double>& dms() const noexcept { return dms_; }
   const FfaSearchPlan& ffa_plan() const noexcept { return ffa_plan_; }
   const GpuFfaProgram& ffa_program() const noexcept { return *ffa_; }
@@ -105,6 +108,23 @@ class GpuSearchPipeline {
   // Copies rows [0, ndm) x [0, out_nsamples) of the plane (uint32 or float
   // bits, depending on the stage reached) into out.
   void download_plane(void* out) const;
+
+  // Dense S/N periodogram of every DM trial, written by the following
+  // searches: [ndm][trial rows][width trials] floats, rows in the order of
+  // trial_periods() (0 where a task has fewer widths).  For the candidate
+  // scorer's periodogram patches.
+  void enable_periodogram(bool on);
+  bool periodogram_enabled() const noexcept { return periodogram_.size() != 0; }
+  const std::vector<double>& trial_periods() const noexcept { return trial_periods_; }
+  std::size_t periodogram_widths() const noexcept { return ffa_plan_.width_trials.size(); }
+  void download_periodogram(std::size_t dm_index, float* out) const;
+  // Scorer inputs of candidates of the last search (preprocessed plane and
+  // periodogram): cubes [n][32][64], the candidate's DM-trial series folded
+  // at its period into 32 subints of 64 phase bins, and patches [n][128],
+  // the 128 periodogram trials around its period at its width (edges
+  // repeated); each normalised to zero mean and unit variance.  Device
+  // outputs, enqueued on the pipeline stream.
+  void candidate_features(std::span<const Candidate> candidates, float* cubes, float* patches);
 
  private:
   SearchPipelineConfig cfg_;
@@ -128,6 +148,8 @@ class GpuSearchPipeline {
   std::size_t lookahead_ = 0;  // input samples past an output range its kernels read
   std::vector<FfaStageMarks> marks_;  // one per tile, events created on demand
   std::size_t marks_used_ = 0;
+  GpuDeviceBuffer<float> periodogram_;
+  std::vector<double> trial_periods_;
 };
 
 }  // namespace gaffa

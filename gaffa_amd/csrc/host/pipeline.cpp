@@ -1,6 +1,8 @@
 // Device-resident DM search pipeline (see gaffa/pipeline.h).
 #include "gaffa/pipeline.h"
 
+#include "gaffa/folding.h"
+
 #include <algorithm>
 #include <cmath>
 #include <parallel/algorithm>
@@ -265,6 +267,85 @@ void GpuSearchPipeline::download_plane(void* out) const {
   auto* o = static_cast<std::uint32_t*>(out);
   for (std::size_t i = 0; i < ndm_; ++i)
     std::copy_n(tmp.data() + i * stride_, n_out_, o + i * n_out_);
+}
+
+}  // namespace gaffa
+
+namespace gaffa {
+
+void GpuSearchPipeline::enable_periodogram(bool on) {
+  GpuDeviceScope scope(cfg_.device_id);
+  if (!on) {
+    ffa_->set_periodogram(nullptr, 0, 0);
+    periodogram_ = GpuDeviceBuffer<float>();
+    return;
+  }
+  if (trial_periods_.empty()) trial_periods_ = ffa_plan_trial_periods(ffa_plan_);
+  const std::size_t widths = periodogram_widths();
+  const std::size_t stride = trial_periods_.size() * widths;
+  periodogram_ = GpuDeviceBuffer<float>(ndm_ * stride);
+  // entries of widths a task does not evaluate stay 0
+  hip_ok(gaffa_hip_fill_u32(reinterpret_cast<std::uint32_t*>(periodogram_.data()), 0,
+                            periodogram_.size(), stream_.get()),
+         "fill_u32");
+  ffa_->set_periodogram(periodogram_.data(), stride, static_cast<std::uint32_t>(widths));
+  stream_.synchronize();
+}
+
+void GpuSearchPipeline::download_periodogram(std::size_t dm_index, float* out) const {
+  check_arg(periodogram_enabled(), "the periodogram is not enabled");
+  check_arg(dm_index < ndm_, "dm_index out of range");
+  GpuDeviceScope scope(cfg_.device_id);
+  const std::size_t stride = trial_periods_.size() * periodogram_widths();
+  copy_to_host(out, periodogram_.data() + dm_index * stride, stride * sizeof(float),
+               stream_.get());
+  stream_.synchronize();
+}
+
+void GpuSearchPipeline::candidate_features(std::span<const Candidate> candidates, float* cubes,
+                                           float* patches) {
+  check_arg(!candidates.empty(), "candidate list is empty");
+  check_arg(periodogram_enabled(), "candidate features need the periodogram (enable it "
+                                   "before the search)");
+  check_arg(candidates.size() < 65536, "at most 65535 candidates per call");
+  GpuDeviceScope scope(cfg_.device_id);
+  const std::size_t widths = periodogram_widths();
+  const std::size_t nsub = GAFFA_HIP_SCORER_SUBINTS, nbin = GAFFA_HIP_SCORER_PHASE_BINS;
+  std::vector<FoldJob> jobs;
+  std::vector<std::uint32_t> series, row, width;
+  for (const Candidate& c : candidates) {
+    check_arg(c.dm_index < ndm_, "candidate dm_index out of range");
+    const FfaPeak& p = c.best_peak.peak;
+    const auto it = std::lower_bound(trial_periods_.begin(), trial_periods_.end(), p.period);
+    check_arg(it != trial_periods_.end() && *it == p.period,
+              "candidate period is not a trial period of this pipeline's plan");
+    check_arg(p.width_index < widths, "candidate width index out of range");
+    jobs.push_back(FoldJob{c.dm_index * stride_, p.period, 0});
+    series.push_back(static_cast<std::uint32_t>(c.dm_index));
+    row.push_back(static_cast<std::uint32_t>(it - trial_periods_.begin()));
+    width.push_back(static_cast<std::uint32_t>(p.width_index));
+  }
+  const GpuFoldInput in{plane_.data(), FoldDtype::F32, n_out_, 1, 1, cfg_.device_id};
+  fold_batch_gpu_device(in, jobs, cfg_.tsamp, nbin, (n_out_ + nsub - 1) / nsub, 1, cubes,
+                        stream_.get());
+  hip_ok(gaffa_hip_scorer_normalise_rows(cubes, static_cast<std::uint32_t>(candidates.size()),
+                                         static_cast<std::uint32_t>(nsub * nbin), stream_.get()),
+         "scorer_normalise_rows");
+  GpuDeviceBuffer<std::uint32_t> d_series = gaffa::upload(series.data(), series.size());
+  GpuDeviceBuffer<std::uint32_t> d_row = gaffa::upload(row.data(), row.size());
+  GpuDeviceBuffer<std::uint32_t> d_width = gaffa::upload(width.data(), width.size());
+  gaffa_hip_scorer_patch_args a{};
+  a.periodogram = periodogram_.data();
+  a.series_stride = trial_periods_.size() * widths;
+  a.rows = trial_periods_.size();
+  a.widths = static_cast<std::uint32_t>(widths);
+  a.series = d_series.data();
+  a.row = d_row.data();
+  a.width = d_width.data();
+  a.n = static_cast<std::uint32_t>(candidates.size());
+  a.out = patches;
+  hip_ok(gaffa_hip_scorer_gather_patches(&a, stream_.get()), "scorer_gather_patches");
+  stream_.synchronize();
 }
 
 }  // namespace gaffa

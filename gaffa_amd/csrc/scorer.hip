@@ -469,6 +469,47 @@ sync_kernel(const float* __restrict__ p, uint16_t* __restrict__ p16, uint64_t n)
   if (e < n) p16[e] = to_bf16(p[e]);
 }
 
+// ------------------------------------------------------ input features ----
+__global__ void __launch_bounds__(kPatch)
+gather_patch_kernel(gaffa_hip_scorer_patch_args a) {
+  const uint32_t i = blockIdx.x, q = threadIdx.x;
+  const int64_t r = static_cast<int64_t>(a.row[i]) - static_cast<int64_t>(kPatch / 2) + q;
+  const uint64_t rc = static_cast<uint64_t>(
+      min<int64_t>(max<int64_t>(r, 0), static_cast<int64_t>(a.rows) - 1));
+  a.out[static_cast<uint64_t>(i) * kPatch + q] =
+      a.periodogram[a.series[i] * a.series_stride + rc * a.widths + a.width[i]];
+}
+
+// One block per row: FP32 mean, then FP32 variance about it (fixed-order
+// reductions), then (x - mean) / sqrt(variance).
+__global__ void __launch_bounds__(256)
+normalise_rows_kernel(float* __restrict__ x, uint32_t length) {
+  __shared__ float red[256];
+  float* row = x + static_cast<uint64_t>(blockIdx.x) * length;
+  auto reduce = [&](float v) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (uint32_t s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    const float r = red[0];
+    __syncthreads();
+    return r;
+  };
+  float s = 0.0f;
+  for (uint32_t k = threadIdx.x; k < length; k += 256) s += row[k];
+  const float mean = reduce(s) / static_cast<float>(length);
+  float v = 0.0f;
+  for (uint32_t k = threadIdx.x; k < length; k += 256) {
+    const float d = row[k] - mean;
+    v += d * d;
+  }
+  const float var = reduce(v) / static_cast<float>(length);
+  const float inv = var > 0.0f ? 1.0f / sqrtf(var) : 0.0f;
+  for (uint32_t k = threadIdx.x; k < length; k += 256) row[k] = (row[k] - mean) * inv;
+}
+
 dim3 blocks(uint64_t n) { return dim3(static_cast<uint32_t>(ceil_div(n, 256))); }
 
 void colsum(const float* x, uint64_t rows, uint32_t cols, float* part, float* out,
@@ -649,6 +690,32 @@ int gaffa_hip_scorer_adam(const gaffa_hip_scorer_net* net, float lr, float beta1
                        net->params16, net->grads, net->adam_m, net->adam_v, n, lr, beta1, beta2,
                        eps, c1, c2);
     launch_check("adam_kernel");
+  });
+}
+
+int gaffa_hip_scorer_gather_patches(const gaffa_hip_scorer_patch_args* a,
+                                    gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(a != nullptr && a->periodogram != nullptr && a->series != nullptr &&
+              a->row != nullptr && a->width != nullptr && a->out != nullptr,
+          "scorer patch arguments must not be null");
+    check(a->n > 0 && a->rows > 0 && a->widths > 0, "scorer patch sizes must be positive");
+    check(a->series_stride >= a->rows * a->widths, "periodogram series stride too small");
+    hipLaunchKernelGGL(gather_patch_kernel, dim3(a->n), dim3(kPatch), 0, as_stream(stream), *a);
+    launch_check("gather_patch_kernel");
+    hipLaunchKernelGGL(normalise_rows_kernel, dim3(a->n), dim3(256), 0, as_stream(stream),
+                       a->out, kPatch);
+    launch_check("normalise_rows_kernel");
+  });
+}
+
+int gaffa_hip_scorer_normalise_rows(float* x, uint32_t rows, uint32_t length,
+                                    gaffa_hip_stream stream) {
+  return guarded([&] {
+    check(x != nullptr && rows > 0 && length > 0, "normalise_rows needs a non-empty array");
+    hipLaunchKernelGGL(normalise_rows_kernel, dim3(rows), dim3(256), 0, as_stream(stream), x,
+                       length);
+    launch_check("normalise_rows_kernel");
   });
 }
 

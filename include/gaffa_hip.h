@@ -291,6 +291,8 @@ typedef struct gaffa_hip_ffa_task {
   float inv_stdnoise;       /* 1.0f / ffa_task_stdnoise(task) */
   uint32_t series;          /* added to the peak's series index (batches of
                                series with their own plans in one row) */
+  uint32_t periodogram_row; /* first row of this task in the series'
+                               periodogram (plan order, rows_eval each) */
   uint32_t width[GAFFA_HIP_FFA_MAX_WIDTHS];
   float height[GAFFA_HIP_FFA_MAX_WIDTHS];
   float baseline[GAFFA_HIP_FFA_MAX_WIDTHS];
@@ -366,6 +368,14 @@ typedef struct gaffa_hip_ffa_search_args {
   gaffa_hip_ffa_peak* peaks;
   uint64_t peak_capacity;
   uint32_t* peak_count; /* device; incremented, never reset here */
+  /* Optional dense periodogram (null = none): the S/N of every evaluated
+   * (row, width) of every task, periodogram[s * periodogram_stride +
+   * (task.periodogram_row + row) * periodogram_widths + width_index] for
+   * series s of this call.  No gaffa counterpart (its search keeps only the
+   * peaks above threshold); read by the candidate scorer. */
+  float* periodogram;
+  uint64_t periodogram_stride;
+  uint32_t periodogram_widths;
 } gaffa_hip_ffa_search_args;
 
 /* Downsample every prepare group of every series (weighted-sum, serial FP32
@@ -566,6 +576,28 @@ int gaffa_hip_scorer_adam(const gaffa_hip_scorer_net* net, float lr, float beta1
 
 /* params16 = bfloat16(params), after the master was written directly. */
 int gaffa_hip_scorer_sync_params(const gaffa_hip_scorer_net* net, gaffa_hip_stream stream);
+
+/* Scorer input preparation.  Patches: out[i][q] = periodogram[series[i] *
+ * series_stride + clamp(row[i] - 64 + q, 0, rows - 1) * widths + width[i]]
+ * for q < 128 (the trials around the candidate's, edges repeated), then
+ * normalised.  normalise_rows: each row of x [rows][length] to zero mean and
+ * unit variance (FP32 mean and variance; a constant row becomes 0). */
+typedef struct gaffa_hip_scorer_patch_args {
+  const float* periodogram;
+  uint64_t series_stride;
+  uint64_t rows;
+  uint32_t widths;
+  const uint32_t* series; /* device [n] */
+  const uint32_t* row;    /* device [n] */
+  const uint32_t* width;  /* device [n] */
+  uint32_t n;
+  float* out;             /* device [n][128] */
+} gaffa_hip_scorer_patch_args;
+
+int gaffa_hip_scorer_gather_patches(const gaffa_hip_scorer_patch_args* args,
+                                    gaffa_hip_stream stream);
+int gaffa_hip_scorer_normalise_rows(float* x, uint32_t rows, uint32_t length,
+                                    gaffa_hip_stream stream);
 
 /* The MFMA GEMM the scorer is built from, exposed for testing:
  * C(i, j) = sum_k A(i, k) B(k, j) over bfloat16 A(i, k) = a[i*a_row + k*a_col]

@@ -239,3 +239,54 @@ def test_training_learns_a_separable_set():
     z = s.forward(cubes[:512], patches[:512])
     acc = np.mean((z > 0) == (labels[:512] > 0))
     assert acc > 0.95
+
+
+# ------------------------------------------------- GPU: candidate inputs ----
+def _normalised(x):
+    x = np.asarray(x, np.float64)
+    m = x.mean(axis=-1, keepdims=True)
+    s = x.std(axis=-1, keepdims=True)
+    return np.where(s > 0, (x - m) / np.where(s > 0, s, 1), 0)
+
+
+@pytest.mark.gpu
+def test_periodogram_and_candidate_features():
+    """The dense periodogram holds the S/N of every evaluated (trial, width):
+    exactly the peaks reach the threshold, each at its peak's S/N (so it is
+    pinned to the oracle through the peak tests).  Candidate cubes are the
+    candidate's DM series folded at its period (equal to fold_candidates),
+    patches the periodogram around it; both normalised."""
+    from gaffa_amd.pipeline import small_workload
+
+    import gaffa_amd
+
+    say("periodogram + candidate features: small workload, 2^16 samples")
+    w = small_workload(nsamples=1 << 16)
+    pipe = w.pipeline(ffa_workspace_bytes=2 << 30)
+    pipe.enable_periodogram(True)
+    pipe.generate(w.synth_spec())
+    pipe.run()
+    peaks = pipe.peaks
+    periods = pipe.trial_periods
+    assert np.all(np.diff(periods) > 0)
+    pgs = [pipe.periodogram(d) for d in range(pipe.ndm)]
+    assert sum(int(np.count_nonzero(pg >= w.snr_threshold)) for pg in pgs) == len(peaks)
+    for p in peaks[:500]:
+        row = int(np.searchsorted(periods, p.peak.period))
+        assert periods[row] == p.peak.period
+        assert pgs[p.dm_index][row, p.peak.width_index] == np.float32(p.peak.snr)
+    cands = pipe.select_candidates()[:48]
+    cubes, patches = pipe.candidate_features(cands)
+    assert cubes.shape == (len(cands), 32, 64) and patches.shape == (len(cands), 128)
+    folded, _ = pipe.fold_candidates([c.dm_index for c in cands], [c.period for c in cands],
+                                     nbin=64, nsubint=32)
+    np.testing.assert_allclose(cubes.reshape(len(cands), -1),
+                               _normalised(folded.reshape(len(cands), -1)), atol=2e-5)
+    for i, c in enumerate(cands):
+        row = int(np.searchsorted(periods, c.best_peak.peak.period))
+        idx = np.clip(np.arange(row - 64, row + 64), 0, len(periods) - 1)
+        want = _normalised(pgs[c.dm_index][idx, c.best_peak.peak.width_index])
+        np.testing.assert_allclose(patches[i], want, atol=2e-5)
+    s = gaffa_amd.CandidateScorer(capacity=64)
+    z = pipe.score_candidates(s, cands)
+    np.testing.assert_array_equal(z, s.forward(cubes, patches))
