@@ -245,6 +245,7 @@ void enqueue_segment_range(const GpuSegmentTables& t, const std::uint8_t* sample
                            std::size_t begin, std::size_t end, gaffa_hip_stream stream) {
   check_arg(begin < end && end <= nsamples, "dedispersion output range is empty");
   if (t.subband) {
+    check_arg(begin % 4 == 0, "subband output ranges must start on a multiple of 4 samples");
     gaffa_hip_dedisperse_subband_args a{};
     a.channels = cm.data + (t.chan_begin - cm.chan_begin) * cm.stride + begin;
     a.channel_stride = cm.stride;

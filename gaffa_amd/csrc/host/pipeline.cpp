@@ -155,8 +155,12 @@ StageTimes GpuSearchPipeline::run_file(const std::string& path, std::size_t chun
       hip_ok(gaffa_hip_filterbank_transpose_u8(&a, st), "filterbank_transpose_u8");
       transposed = ready;
     }
+    // ranges start on 256-sample boundaries: the subband kernel reads its
+    // channel rows in aligned dwords from the range start
     const std::size_t target =
-        ready == cfg_.nsamples ? n_out_ : (ready > lookahead_ ? std::min(n_out_, ready - lookahead_) : 0);
+        ready == cfg_.nsamples
+            ? n_out_
+            : (ready > lookahead_ ? std::min(n_out_, (ready - lookahead_) / 256 * 256) : 0);
     if (target > done) {
       dedisperse_range(done, target);
       done = target;
