@@ -43,10 +43,9 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
     subtree     per series and task with rows >= 2: read + write its rows x bins
                 floats once (8 rows bins)
     levels      per series and task: 8 rows bins for each level done through
-                global memory (depth_t - min(depth_t, 4) of them: the detection
-                items merge the top four levels in LDS), averaged over the launches
-    detect      per series and task: the rows at the cone's depth once (4 rows
-                bins; the input series row for 1-row tasks) + 20 B per peak
+                global memory (depth_t - 1 of them), averaged over the launches
+    detect      per series and task: the depth-1 rows once (4 rows bins; the input
+                series row for 1-row tasks) + 20 B per peak
     """
     import numpy as np
 
@@ -61,7 +60,7 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
         if f != 1.0:
             prep += 4.0 * float(t["prepared_nsamples"][t["downsample_factor"] == f][0])
     subtree = 8.0 * float(rb[t["rows"] >= 2].sum())
-    levels = 8.0 * float((rb * np.maximum(depth - np.minimum(depth, 4.0), 0.0)).sum())
+    levels = 8.0 * float((rb * np.maximum(depth - 1.0, 0.0)).sum())
     detect = 4.0 * float(rb.sum())
     nlev = max(1, lay["level_launches"])
     nseg = len(workload.dm_segments)

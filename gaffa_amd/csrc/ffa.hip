@@ -278,7 +278,7 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
-  const uint32_t wave = tid >> 6;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
   constexpr uint32_t kWaves = kSubtreeThreads / 64;
   const uint32_t half = plan.subtree_half_floats;
   uint2* desc = reinterpret_cast<uint2*>(lds + 2 * static_cast<size_t>(half));
@@ -330,27 +330,29 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     // One level: rows of `src` (LDS) merged into `dst`, written once per
     // destination kind so every access keeps its address space (LDS rows
     // are ds_* operations, the top level's stores global_*).  Lanes cover
-    // 4 phases each per 256-phase pass; each wave merges two rows at a time,
-    // decoding each row's descriptor once into 32-bit LDS indices.
+    // 4 phases each per 256-phase pass; each wave merges two rows at a time.
+    // Descriptors are wave-uniform (scalar row math) and read one pair
+    // ahead.  No store is predicated: a lane past the row end recomputes
+    // phase b - 1 and a wave without a second row recomputes row nrows - 1,
+    // storing the very value another lane stores there.
     auto level = [&](const uint2* dd, const float* src, auto&& store) {
+      if (wave >= nrows) return;
       for (uint32_t p0 = 0; p0 < b; p0 += 256) {
         uint32_t pc[4];
-        bool ok[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          const uint32_t p = p0 + lane + 64u * j;
-          ok[j] = p < b;
-          pc[j] = min(p, b - 1);  // clamped reads, predicated stores
-        }
+        for (uint32_t j = 0; j < 4; ++j) pc[j] = min(p0 + lane + 64u * j, b - 1);
+        uint2 e0 = dd[wave], e1 = dd[min(wave + kWaves, nrows - 1)];
         for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
-          const uint32_t r1 = min(r0 + kWaves, nrows - 1);  // duplicate of a valid row
-          const bool has1 = r0 + kWaves < nrows;
-          // descriptors are wave-uniform: keep the row math on the scalar unit
-          const uint2 e0 = dd[r0], e1 = dd[r1];
+          const uint32_t r1 = min(r0 + kWaves, nrows - 1);
           const uint2 d0 = make_uint2(__builtin_amdgcn_readfirstlane(e0.x),
                                       __builtin_amdgcn_readfirstlane(e0.y));
           const uint2 d1 = make_uint2(__builtin_amdgcn_readfirstlane(e1.x),
                                       __builtin_amdgcn_readfirstlane(e1.y));
+          const uint32_t n0 = r0 + 2 * kWaves;
+          if (n0 < nrows) {
+            e0 = dd[n0];
+            e1 = dd[min(n0 + kWaves, nrows - 1)];
+          }
           const uint32_t hb0 = (d0.x & 0xFFFFu) * b, hb1 = (d1.x & 0xFFFFu) * b;
           const bool cp0 = (d0.x >> 16) == kCopyTail, cp1 = (d1.x >> 16) == kCopyTail;
           const uint32_t tb0 = cp0 ? hb0 : (d0.x >> 16) * b, tb1 = cp1 ? hb1 : (d1.x >> 16) * b;
@@ -370,10 +372,8 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
           const uint32_t ob0 = r0 * b, ob1 = r1 * b;
 #pragma unroll
           for (uint32_t j = 0; j < 4; ++j) {
-            if (ok[j]) {
-              store(ob0 + pc[j], v0[j]);
-              if (has1) store(ob1 + pc[j], v1[j]);
-            }
+            store(ob0 + pc[j], v0[j]);
+            store(ob1 + pc[j], v1[j]);
           }
         }
       }
@@ -465,74 +465,11 @@ __device__ __forceinline__ float boxcar_max(const float* x, uint32_t b, uint32_t
   return best;
 }
 
-// Cone of the top levels (fused with detection).  A terminal item is up to
-// kTerminalRows consecutive final rows [s0, s0 + n) of one task.  Every final
-// row reads one row of each child of the root, which reads one row of each
-// of its children, and so on: the rows an item needs at depth d form one
-// contiguous range per node at that depth (the row picks are monotone in
-// the row index).  The item loads those ranges of the depth-kc nodes (kc =
-// min(task depth, kConeLevels); below kc the subtree and level kernels ran)
-// and merges upward in LDS, so levels kc - 1 .. 0 never touch global memory.
-constexpr uint32_t kConeLevels = 4;
-constexpr uint32_t kConeRanges = (2u << kConeLevels) - 1;  // nodes of depths 0..kConeLevels
-constexpr uint32_t kConeMaxRows = 128;                     // rows one depth may need
-constexpr uint32_t kConeStaticBytes = 8192;                // static LDS of the kernel, bound
-
-struct ConeRange {
-  uint32_t start;  // node rows [start, start + size)
-  uint32_t size;
-  uint32_t a;      // rows [a, b] of the node are needed
-  uint32_t b;
-  uint32_t slot;   // LDS slot of row a
-};
-
-__device__ __host__ __forceinline__ uint32_t cone_levels(uint32_t task_depth) {
-  return task_depth < kConeLevels ? task_depth : kConeLevels;
-}
-
-__device__ __host__ __forceinline__ uint32_t cone_pick(uint32_t child_rows, uint32_t rows,
-                                                       uint32_t s) {
-  const float scale = static_cast<float>(child_rows - 1) / static_cast<float>(rows - 1);
-  return static_cast<uint32_t>(scale * static_cast<float>(s) + 0.5f);
-}
-
-// Ranges of depths 0..kc (depth d at r[2^d - 1 .. 2^(d+1) - 2], left to
-// right); rows[d] = rows needed at depth d.  Returns the largest rows[d].
-__device__ __host__ inline uint32_t cone_ranges(uint32_t task_rows, uint32_t s0, uint32_t n,
-                                                uint32_t kc, ConeRange* r, uint32_t* rows) {
-  r[0] = ConeRange{0, task_rows, s0, s0 + n - 1, 0};
-  rows[0] = n;
-  uint32_t most = n;
-  for (uint32_t d = 0; d < kc; ++d) {
-    const uint32_t base = (1u << d) - 1, cbase = (2u << d) - 1;
-    uint32_t slot = 0;
-    for (uint32_t j = 0; j < (1u << d); ++j) {
-      const ConeRange p = r[base + j];
-      const uint32_t hn = p.size / 2, tn = p.size - hn;
-      const uint32_t sa = p.a - p.start, sz = p.b - p.start;
-      ConeRange h{p.start, hn, p.start + cone_pick(hn, p.size, sa),
-                  p.start + cone_pick(hn, p.size, sz), slot};
-      slot += h.b - h.a + 1;
-      ConeRange t{p.start + hn, tn, p.start + hn + cone_pick(tn, p.size, sa),
-                  p.start + hn + cone_pick(tn, p.size, sz), slot};
-      slot += t.b - t.a + 1;
-      r[cbase + 2 * j] = h;
-      r[cbase + 2 * j + 1] = t;
-    }
-    rows[d + 1] = slot;
-    most = slot > most ? slot : most;
-  }
-  return most;
-}
-
 template <bool kDetect>
 __global__ void __launch_bounds__(kTerminalThreads)
 ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   extern __shared__ float lds[];
-  __shared__ ConeRange cr[kConeRanges];
-  __shared__ uint32_t crows[kConeLevels + 1];
-  __shared__ Desc cdesc[kConeLevels][kConeMaxRows];
-  __shared__ uint32_t csrc[kConeMaxRows];
+  __shared__ Desc sdesc[kTerminalRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.terminal_items[blockIdx.x];
   if (!kDetect && item.task != only_task) return;
@@ -550,90 +487,36 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   uint32_t maxw = 0;
   for (uint32_t w = 0; w < task.nwidths; ++w) maxw = max(maxw, task.width[w]);
   const uint32_t sb = terminal_stride(b, maxw);
-  float* buf[2] = {lds, lds + static_cast<size_t>(plan.cone_rows) * sb};
 
+  const float* src;
   if (task.rows == 1) {
-    const float* src = task_input(a, group, sidx);
+    src = task_input(a, group, sidx);
+  } else {
+    src = a.work + 1ull * a.nseries * plan.work_stride + sidx * plan.work_stride +
+          task.work_offset;
+    if (threadIdx.x < nrows) {
+      const uint32_t s = s0 + threadIdx.x;
+      sdesc[threadIdx.x] = merge_desc(Node{0, task.rows}, s, b);
+    }
+    __syncthreads();
+  }
+  if (task.rows == 1) {
     for (uint32_t p = threadIdx.x; p < b; p += kTerminalThreads) lds[p] = src[p];
   } else {
-    const uint32_t kc = cone_levels(task.depth);
-    __shared__ uint32_t cone_ok;
-    if (threadIdx.x == 0) {
-      // the host sized LDS with the same function; a mismatch skips the item
-      const uint32_t most = cone_ranges(task.rows, s0, nrows, kc, cr, crows);
-      cone_ok = most <= plan.cone_rows && most <= kConeMaxRows;
-    }
-    __syncthreads();
-    if (!cone_ok) return;
-    // per-row merge descriptors of depths 0..kc-1 (slots of the two input
-    // rows at depth d + 1 and the roll), and the source rows of depth kc
-    for (uint32_t d = 0; d < kc; ++d) {
-      const uint32_t base = (1u << d) - 1, cbase = (2u << d) - 1;
-      for (uint32_t q = threadIdx.x; q < crows[d]; q += kTerminalThreads) {
-        uint32_t j = 0;
-        while (j + 1 < (1u << d) && cr[base + j + 1].slot <= q) ++j;
-        const ConeRange p = cr[base + j];
-        const uint32_t s = p.a + (q - p.slot) - p.start;
-        const uint32_t hn = p.size / 2, tn = p.size - hn;
-        const uint32_t h = cone_pick(hn, p.size, s), t = cone_pick(tn, p.size, s);
-        const ConeRange ch = cr[cbase + 2 * j], ct = cr[cbase + 2 * j + 1];
-        cdesc[d][q] = Desc{ch.slot + (p.start + h - ch.a), ct.slot + (p.start + hn + t - ct.a),
-                           (s - t) % b};
-      }
-    }
-    {
-      const uint32_t base = (1u << kc) - 1;
-      for (uint32_t q = threadIdx.x; q < crows[kc]; q += kTerminalThreads) {
-        uint32_t j = 0;
-        while (j + 1 < (1u << kc) && cr[base + j + 1].slot <= q) ++j;
-        csrc[q] = cr[base + j].a + (q - cr[base + j].slot);
-      }
-    }
-    __syncthreads();
-    // depth-kc rows from the work region the level below wrote
-    const float* src = a.work + static_cast<uint64_t>(kc & 1u) * a.nseries * plan.work_stride +
-                       sidx * plan.work_stride + task.work_offset;
-    float* bottom = buf[kc & 1u];
-    for (uint32_t q0 = wave; q0 < crows[kc]; q0 += 2 * kWaves) {
-      const uint32_t q1 = q0 + kWaves;
-      const float* r0 = src + static_cast<uint64_t>(csrc[q0]) * b;
-      const float* r1 = src + static_cast<uint64_t>(csrc[q1 < crows[kc] ? q1 : q0]) * b;
-      float v0[4], v1[4];
-#pragma unroll
-      for (uint32_t k = 0; k < 4; ++k) {
-        const uint32_t p = min(lane + 64u * k, b - 1);
-        v0[k] = r0[p];
-        v1[k] = r1[p];
-      }
-#pragma unroll
-      for (uint32_t k = 0; k < 4; ++k) {
-        const uint32_t p = lane + 64u * k;
-        if (p < b) {
-          bottom[static_cast<size_t>(q0) * sb + p] = v0[k];
-          if (q1 < crows[kc]) bottom[static_cast<size_t>(q1) * sb + p] = v1[k];
-        }
-      }
-    }
-    __syncthreads();
-    // merge upward: depth d rows from depth d + 1 rows, all in LDS
-    for (uint32_t dd = kc; dd-- > 0;) {
-      const float* in = buf[(dd + 1) & 1u];
-      float* out = buf[dd & 1u];
-      for (uint32_t q = wave; q < crows[dd]; q += kWaves) {
-        const Desc ds = cdesc[dd][q];
-        const float* hr = in + static_cast<size_t>(ds.h) * sb;
-        const float* tr = in + static_cast<size_t>(ds.t) * sb;
-        for (uint32_t p = lane; p < b; p += 64) {
-          uint32_t x = p + ds.roll;
-          x = min(x, x - b);  // (p + roll) mod b for p, roll < b
-          out[static_cast<size_t>(q) * sb + p] = hr[p] + tr[x];
-        }
-      }
-      __syncthreads();
+    constexpr uint32_t kBatch = 4;
+    for (uint32_t i0 = wave; i0 < nrows; i0 += kBatch * kWaves) {
+      merge_batch<kBatch>(
+          i0, kWaves, nrows, b, lane,
+          [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
+            const Desc ds = sdesc[i];
+            h = src + static_cast<uint64_t>(ds.h) * b;
+            t = src + static_cast<uint64_t>(ds.t) * b;
+            roll = ds.roll;
+          },
+          [&](uint32_t i, uint32_t p, float v) { lds[static_cast<size_t>(i) * sb + p] = v; });
     }
   }
   __syncthreads();
-  // final rows s0 .. s0 + nrows - 1 are rows 0 .. nrows - 1 of buf[0]
   if constexpr (!kDetect) {
     for (uint32_t i = wave; i < nrows; i += kWaves)
       for (uint32_t p = lane; p < b; p += 64)
@@ -706,9 +589,7 @@ void validate(const gaffa_hip_ffa_search_args* a) {
             (a->periodogram_widths > 0 && a->periodogram_widths <= GAFFA_HIP_FFA_MAX_WIDTHS),
         "FFA periodogram width count must be in 1..16");
   check(p.max_bins > 1 && p.max_bins < 65536, "FFA bins must be in (1, 65536)");
-  check(p.terminal_lds_bytes <= kMaxLdsBytes - kConeStaticBytes,
-        "FFA detection tile exceeds LDS");
-  check(p.cone_rows <= kConeMaxRows, "FFA detection cone needs too many rows");
+  check(p.terminal_lds_bytes <= kMaxLdsBytes, "FFA detection tile exceeds LDS");
   check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");
   check(p.nsubtree_items == 0 ||
             (p.subtree_half_floats > 0 &&
@@ -896,25 +777,6 @@ uint32_t gaffa_hip_ffa_level_rows(void) { return kLevelRows; }
 
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {
   return kTerminalRows * terminal_stride(bins, max_width) * 4u;
-}
-
-uint32_t gaffa_hip_ffa_cone_levels(void) { return kConeLevels; }
-
-uint32_t gaffa_hip_ffa_cone_rows(uint32_t task_rows, uint32_t task_depth, uint32_t first_row,
-                                 uint32_t nrows) {
-  if (task_rows < 2 || nrows == 0 || first_row + nrows > task_rows) return 0;
-  ConeRange r[kConeRanges];
-  uint32_t rows[kConeLevels + 1];
-  return cone_ranges(task_rows, first_row, nrows, cone_levels(task_depth), r, rows);
-}
-
-uint32_t gaffa_hip_ffa_cone_lds_bytes(uint32_t cone_rows, uint32_t bins, uint32_t max_width) {
-  const uint64_t rows = cone_rows < kTerminalRows ? kTerminalRows : cone_rows;
-  const uint64_t bytes = 2ull * rows * terminal_stride(bins, max_width) * 4u;
-  // the kernel's static tables (ranges, descriptors, source rows) share LDS
-  return cone_rows > kConeMaxRows || bytes > kMaxLdsBytes - kConeStaticBytes
-             ? 0xFFFFFFFFu
-             : static_cast<uint32_t>(bytes);
 }
 
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins) {

@@ -1274,6 +1274,28 @@ PYBIND11_MODULE(_core, m) {
           py::arg("candidates"),
           "Scorer inputs (cubes (n, 32, 64), patches (n, 128)) of candidates of the last search")
       .def(
+          "fold_candidates_full",
+          [](GpuSearchPipeline& p, const std::vector<Candidate>& c, std::size_t nbin,
+             std::size_t nsubint, std::size_t output_channels, bool keep) {
+            std::vector<float> cubes;
+            double ms = 0.0;
+            {
+              py::gil_scoped_release release;
+              ms = p.fold_candidates_full(c, nbin, nsubint, output_channels,
+                                          keep ? &cubes : nullptr);
+            }
+            if (!keep) return py::make_tuple(ms, py::none());
+            const std::size_t nout = output_channels == 0 ? p.nchans() : output_channels;
+            return py::make_tuple(
+                ms, owning_array(std::move(cubes),
+                                 {static_cast<py::ssize_t>(c.size()),
+                                  static_cast<py::ssize_t>(nsubint),
+                                  static_cast<py::ssize_t>(nout), static_cast<py::ssize_t>(nbin)}));
+          },
+          py::arg("candidates"), py::kw_only(), py::arg("nbin") = 64, py::arg("nsubint") = 32,
+          py::arg("output_channels") = 64, py::arg("keep_cubes") = false,
+          "gaffa's full fold of each candidate from the filterbank: (device ms, cubes or None)")
+      .def(
           "score_candidates",
           [](GpuSearchPipeline& p, GpuCandidateScorer& scorer, const std::vector<Candidate>& c) {
             const std::size_t n = c.size();

@@ -133,7 +133,6 @@ void GpuFfaBatchedSearch::build_tiles() {
     d.subtree_half_floats = e.subtree_half_floats();
     d.max_subtree_desc = e.max_subtree_desc();
     d.terminal_lds_bytes = e.terminal_lds_bytes();
-    d.cone_rows = e.cone_rows();
     d.prep_blocks = e.prep_blocks();
     d.max_prepared_nsamples = e.max_prepared();
     d.prepared_stride = e.prepared_stride();

@@ -153,6 +153,9 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
     }
     check_arg(det.max_width <= t.bins, "FFA detection width exceeds bins");
     max_bins_ = std::max(max_bins_, d.bins);
+    terminal_lds_bytes_ = std::max(
+        terminal_lds_bytes_,
+        gaffa_hip_ffa_terminal_lds_bytes(d.bins, static_cast<std::uint32_t>(det.max_width)));
     std::size_t lds_rows = std::min(kSubtreeMaxRows, kSubtreeNodeFloats / t.bins);
     while (lds_rows > 2 && tree_levels(lds_rows) * lds_rows > kSubtreeMaxDesc) --lds_rows;
     check_arg(lds_rows >= 2, "GPU FFA bins too large for an LDS subtree");
@@ -174,37 +177,20 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       }
       subtree_lds_bytes_ = 8u * subtree_half_ + 8u * subtree_desc_;
       if (by_depth_.size() < depth) by_depth_.resize(depth);
-      // depths below the detection items' cone go through the level kernel
-      const std::uint32_t cone = std::min(depth, gaffa_hip_ffa_cone_levels());
-      for (std::uint32_t l = cone; l < depth; ++l)
+      for (std::uint32_t l = 1; l < depth; ++l)
         for (std::uint32_t r = 0; r < d.rows; r += level_rows)
           by_depth_[l].push_back(
               gaffa_hip_ffa_item{task_index, l, r, std::min(level_rows, d.rows - r)});
       max_depth_ = std::max<std::size_t>(max_depth_, depth);
     }
-    std::uint32_t task_cone_rows = 0;
-    for (std::uint32_t r = 0; r < d.rows_eval; r += terminal_rows) {
-      const std::uint32_t n = std::min(terminal_rows, d.rows_eval - r);
-      terminal_.push_back(gaffa_hip_ffa_item{task_index, r, n, 0});
-      if (t.rows >= 2)
-        task_cone_rows =
-            std::max(task_cone_rows, gaffa_hip_ffa_cone_rows(d.rows, d.depth, r, n));
-    }
-    cone_rows_ = std::max(cone_rows_, task_cone_rows);
+    for (std::uint32_t r = 0; r < d.rows_eval; r += terminal_rows)
+      terminal_.push_back(
+          gaffa_hip_ffa_item{task_index, r, std::min(terminal_rows, d.rows_eval - r), 0});
     tasks_.push_back(d);
   }
 }
 
 void GpuFfaExecutionPlan::finish() {
-  // the detection kernel's two row buffers hold the plan's widest cone at
-  // each task's padded row stride
-  for (const gaffa_hip_ffa_task& d : tasks_) {
-    std::uint32_t maxw = 0;
-    for (std::uint32_t w = 0; w < d.nwidths; ++w) maxw = std::max(maxw, d.width[w]);
-    const std::uint32_t lds = gaffa_hip_ffa_cone_lds_bytes(cone_rows_, d.bins, maxw);
-    check_arg(lds != 0xFFFFFFFFu, "GPU FFA detection cone exceeds the kernel limits");
-    terminal_lds_bytes_ = std::max(terminal_lds_bytes_, lds);
-  }
   level_offsets_.assign(max_depth_ + 1, 0);
   for (std::size_t l = 0; l <= max_depth_; ++l) {
     level_offsets_[l] = static_cast<std::uint32_t>(level_.size());
@@ -266,7 +252,6 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.subtree_half_floats = exec_plan_.subtree_half_floats();
   desc_.max_subtree_desc = exec_plan_.max_subtree_desc();
   desc_.terminal_lds_bytes = exec_plan_.terminal_lds_bytes();
-  desc_.cone_rows = exec_plan_.cone_rows();
   desc_.prep_blocks = exec_plan_.prep_blocks();
   desc_.max_prepared_nsamples = exec_plan_.max_prepared();
   desc_.prepared_stride = exec_plan_.prepared_stride();

@@ -80,7 +80,6 @@ class GpuFfaExecutionPlan {
   std::uint32_t subtree_half_floats() const noexcept { return subtree_half_; }
   std::uint32_t max_subtree_desc() const noexcept { return subtree_desc_; }
   std::uint32_t terminal_lds_bytes() const noexcept { return terminal_lds_bytes_; }
-  std::uint32_t cone_rows() const noexcept { return cone_rows_; }
   std::uint32_t prep_blocks() const noexcept { return prep_blocks_; }
   std::uint32_t max_bins() const noexcept { return max_bins_; }
   std::size_t max_prepared() const noexcept { return max_prepared_; }
@@ -101,7 +100,6 @@ class GpuFfaExecutionPlan {
   std::uint32_t subtree_half_ = 0;
   std::uint32_t subtree_desc_ = 0;
   std::uint32_t terminal_lds_bytes_ = 0;
-  std::uint32_t cone_rows_ = 0;
   std::uint32_t prep_blocks_ = 0;
   std::uint32_t max_bins_ = 0;
   std::vector<gaffa_hip_ffa_group> groups_;

@@ -97,6 +97,7 @@ class GpuSearchPipeline {
   std::size_t plane_stride() const noexcept { return stride_; }
   double tsamp() const noexcept { return cfg_.tsamp; }
   int device_id() const noexcept { return cfg_.device_id; }
+  std::size_t nchans() const noexcept { return cfg_.nchans; }
   const std::vector<double>& dms() const noexcept { return dms_; }
   const FfaSearchPlan& ffa_plan() const noexcept { return ffa_plan_; }
   const GpuFfaProgram& ffa_program() const noexcept { return *ffa_; }
@@ -125,6 +126,14 @@ class GpuSearchPipeline {
   // repeated); each normalised to zero mean and unit variance.  Device
   // outputs, enqueued on the pipeline stream.
   void candidate_features(std::span<const Candidate> candidates, float* cubes, float* patches);
+  // gaffa's full fold (pfold) of candidates from the resident filterbank:
+  // per candidate the aligned spectrum at its DM (dedisperse_spectrum),
+  // folded at its period into nsubint subints x output_channels x nbin.
+  // Cubes go to `cubes` on the host when given ([n][nsubint][output
+  // channels][nbin]); returns the device time in ms.
+  double fold_candidates_full(std::span<const Candidate> candidates, std::size_t nbin,
+                              std::size_t nsubint, std::size_t output_channels,
+                              std::vector<float>* cubes);
 
  private:
   SearchPipelineConfig cfg_;

@@ -67,7 +67,7 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   // so the first run rarely repeats a tile
   fe.initial_peak_buffer_bytes = std::max<std::size_t>(fe.initial_peak_buffer_bytes, 256u << 20);
   ffa_ = std::make_unique<GpuFfaProgram>(ffa_plan_, GpuFfaProgramOptions{cfg_.device_id}, fe);
-  events_.resize(5, nullptr);
+  events_.resize(7, nullptr);
   for (auto& e : events_) hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
 }
 
@@ -350,6 +350,46 @@ void GpuSearchPipeline::candidate_features(std::span<const Candidate> candidates
   a.out = patches;
   hip_ok(gaffa_hip_scorer_gather_patches(&a, stream_.get()), "scorer_gather_patches");
   stream_.synchronize();
+}
+
+double GpuSearchPipeline::fold_candidates_full(std::span<const Candidate> candidates,
+                                               std::size_t nbin, std::size_t nsubint,
+                                               std::size_t output_channels,
+                                               std::vector<float>* cubes) {
+  check_arg(nbin > 0 && nsubint > 0, "fold nbin and nsubint must be positive");
+  const std::size_t nout = output_channels == 0 ? cfg_.nchans : output_channels;
+  check_arg(nout <= cfg_.nchans && cfg_.nchans % nout == 0,
+            "fold output_channels must divide the channel count");
+  GpuDeviceScope scope(cfg_.device_id);
+  const gaffa_hip_stream st = stream_.get();
+  GpuDeviceBuffer<std::uint8_t> spectrum(cfg_.nsamples * cfg_.nchans);
+  GpuDeviceBuffer<float> cube(nsubint * nout * nbin);
+  if (cubes != nullptr) cubes->assign(candidates.size() * cube.size(), 0.0f);
+  const double fmax = *std::max_element(cfg_.frequency_mhz.begin(), cfg_.frequency_mhz.end());
+  GpuDedispersionOptions o;
+  o.device_id = cfg_.device_id;
+  o.stream = st;
+  hip_ok(gaffa_hip_event_record(events_[5], st), "hipEventRecord");
+  for (std::size_t i = 0; i < candidates.size(); ++i) {
+    const Candidate& c = candidates[i];
+    const SingleDmDedispersionPlan plan{c.dm, fmax, cfg_.tsamp, 0, cfg_.nchans};
+    const std::size_t n = spectrum_output_nsamples(cfg_.nsamples, cfg_.frequency_mhz, plan);
+    check_arg(n > 0, "candidate DM leaves no aligned samples");
+    dedisperse_spectrum_gpu_raw(filterbank_.data(), 1, SampleShape{cfg_.nsamples, 1, cfg_.nchans},
+                                cfg_.frequency_mhz, plan, spectrum.data(), n, o);
+    const FoldJob job{0, c.period, 0};
+    fold_batch_gpu_device(GpuFoldInput{spectrum.data(), FoldDtype::U8, n, cfg_.nchans, 0,
+                                       cfg_.device_id},
+                          std::span<const FoldJob>(&job, 1), cfg_.tsamp, nbin,
+                          (n + nsubint - 1) / nsubint, nout, cube.data(), st);
+    if (cubes != nullptr)
+      copy_to_host(cubes->data() + i * cube.size(), cube.data(), cube.bytes(), st);
+  }
+  hip_ok(gaffa_hip_event_record(events_[6], st), "hipEventRecord");
+  stream_.synchronize();
+  float ms = 0.0f;
+  hip_ok(gaffa_hip_event_elapsed_ms(events_[5], events_[6], &ms), "elapsed");
+  return ms;
 }
 
 }  // namespace gaffa

@@ -331,8 +331,7 @@ typedef struct gaffa_hip_ffa_plan_desc {
   uint32_t subtree_lds_bytes;  /* 8 * subtree_half_floats + 8 * max_subtree_desc */
   uint32_t subtree_half_floats; /* max over subtree items of rows * bins */
   uint32_t max_subtree_desc;    /* max over subtree items of levels * rows */
-  uint32_t terminal_lds_bytes; /* max over tasks of ..._cone_lds_bytes() */
-  uint32_t cone_rows;          /* max over terminal items of ..._cone_rows() */
+  uint32_t terminal_lds_bytes; /* max over tasks of ..._terminal_lds_bytes() */
   uint32_t prep_blocks;        /* max over downsampled groups of
                                   ceil(prepared_nsamples / outputs per block) */
   uint64_t max_prepared_nsamples;
@@ -355,16 +354,6 @@ uint32_t gaffa_hip_ffa_level_rows(void);
 uint32_t gaffa_hip_ffa_prepare_outputs_per_block(double factor);
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width);
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins);
-/* The detection items also merge the top levels of each task in LDS:
- * levels below depth min(task depth, cone_levels()) are the subtree and
- * level kernels' work (level items exist only for depths >= that).
- * cone_rows: rows a terminal item (first_row, nrows) needs at its widest
- * depth; cone_lds_bytes: dynamic LDS of the detection kernel for a plan
- * whose largest cone_rows is `cone_rows` (0xFFFFFFFF = does not fit). */
-uint32_t gaffa_hip_ffa_cone_levels(void);
-uint32_t gaffa_hip_ffa_cone_rows(uint32_t task_rows, uint32_t task_depth, uint32_t first_row,
-                                 uint32_t nrows);
-uint32_t gaffa_hip_ffa_cone_lds_bytes(uint32_t cone_rows, uint32_t bins, uint32_t max_width);
 
 typedef struct gaffa_hip_ffa_search_args {
   const gaffa_hip_ffa_plan_desc* plan;

@@ -290,10 +290,9 @@ def test_ffa_rows_and_search_vs_golden(gpu):
                                        (2679, 180), (2657, 180), (1203, 200), (1090, 256),
                                        (9000, 3), (5003, 2)])
 def test_ffa_transform_shapes_bit_exact(gpu, port, rows_bins):
-    """Single-task plans over odd shapes: LDS-only trees, trees whose top
-    levels all fit the detection cone, deep trees (9000 x 3, 5003 x 2) with
-    several global levels below it, and level work items of 1, 2 and 3 rows
-    (rows % 16 in {1, 2, 3})."""
+    """Single-task plans over odd shapes: LDS-only trees, forced global
+    levels, deep trees (9000 x 3, 5003 x 2: five and six global levels), and
+    level work items of 1, 2 and 3 rows (rows % 16 in {1, 2, 3})."""
     from oracle.oracle import TASK_DTYPE
 
     rows, bins = rows_bins
