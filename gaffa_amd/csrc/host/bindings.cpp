@@ -1276,7 +1276,7 @@ PYBIND11_MODULE(_core, m) {
       .def(
           "fold_candidates_full",
           [](GpuSearchPipeline& p, const std::vector<Candidate>& c, std::size_t nbin,
-             std::size_t nsubint, std::size_t output_channels, bool keep) {
+             std::size_t nsubint, std::size_t output_channels, bool keep) -> py::tuple {
             std::vector<float> cubes;
             double ms = 0.0;
             {
