@@ -10,7 +10,7 @@ Training data is made by the search itself: synthetic uint8 N(128, 16)
 filterbanks, half of them noise only, the rest with injected pulsars, go
 through the GPU pipeline (dedispersion, detrend, FFA, candidate selection);
 each candidate is labelled by whether it matches an injection (its
-frequency within a Fourier bin of n or 1/n times the pulsar's, n <= 4, its
+frequency within a Fourier bin of n / d times the pulsar's, n, d <= 8, its
 DM error smearing the pulse by no more than its width or two DM trials).  The plain pipeline folds every
 candidate above the S/N threshold; the scored pipeline folds only those the
 network keeps.  ``evaluate`` measures both on a held-out seed: recall of the
@@ -100,16 +100,17 @@ def _draw_pulsars(rng, cfg: ScorerDataConfig) -> list:
     return out
 
 
-def _ratios() -> np.ndarray:
-    """Harmonic ratios a candidate may sit at: n and 1/n for n <= 4."""
-    return np.array(sorted({float(n) for n in range(1, 5)} | {1.0 / n for n in range(2, 5)}))
+def _ratios(max_n: int = 8) -> np.ndarray:
+    """Harmonic ratios a candidate may sit at: reduced n / d, n, d <= 8."""
+    return np.array(sorted({n / d for n in range(1, max_n + 1) for d in range(1, max_n + 1)
+                            if math.gcd(n, d) == 1}))
 
 
 def _match(period, dm, pulsars, obs_seconds, band_delay_per_dm, dm_step, ratios) -> int:
     """Index of the injected pulsar this candidate belongs to, or -1: its
-    frequency within one Fourier bin (1 / observation) of n or 1/n times the
-    pulsar's (n <= 4), its DM error smearing the pulse across the band by no
-    more than the pulse width or two DM trials, whichever is larger."""
+    frequency within one Fourier bin (1 / observation) of n / d times the
+    pulsar's (n, d <= 8), its DM error smearing the pulse across the band by
+    no more than the pulse width or two DM trials, whichever is larger."""
     f = 1.0 / period
     for k, (p_period, p_dm, duty, *_rest) in enumerate(pulsars):
         smear = abs(dm - p_dm) * band_delay_per_dm
