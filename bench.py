@@ -171,6 +171,59 @@ def cpu_baseline(workload, filterbank, threads: int) -> dict:
     }
 
 
+def scored_comparison(workload, pipe, search_ms: float) -> dict:
+    """Plain pipeline: full fold (gaffa's pfold of the filterbank at the
+    candidate's DM: aligned spectrum + 32 subints x 64 channels x 64 bins)
+    of every candidate of the last search.  Scored pipeline: the learned
+    scorer on every candidate (features and network on the GPU), full fold
+    of those it keeps.  Detections: injected pulsars matched (frequency within
+    a Fourier bin of n/d times theirs, n, d <= 8; DM within the pulse width
+    or two trials) by a folded candidate with S/N >= 8."""
+    import numpy as np
+
+    import gaffa_amd
+    from gaffa_amd import scorer as sc
+
+    cands = pipe.select_candidates()
+    net, threshold = sc.load_trained()
+    t0 = time.perf_counter()
+    z = pipe.score_candidates(net, cands)
+    score_ms = 1000.0 * (time.perf_counter() - t0)
+    kept = [c for c, v in zip(cands, z) if v >= threshold]
+    plain_fold_ms, _ = pipe.fold_candidates_full(cands)
+    scored_fold_ms, _ = pipe.fold_candidates_full(kept) if kept else (0.0, None)
+    obs = pipe.out_nsamples * workload.tsamp
+    dms = np.asarray(pipe.dms)
+    f = workload.frequencies
+    kdm = sc.DISPERSION_S * (f.min() ** -2 - f.max() ** -2)
+    ratios = sc._ratios()
+
+    def detected(cs):
+        found = set()
+        for c in cs:
+            if c.snr < 8.0:
+                continue
+            step = (dms[min(c.dm_index + 1, len(dms) - 1)] - dms[max(c.dm_index - 1, 0)]) / 2
+            k = sc._match(c.period, c.dm, workload.pulsars, obs, kdm, step, ratios)
+            if k >= 0:
+                found.add(k)
+        return found
+
+    plain = detected(cands)
+    scored = detected(kept)
+    plain_ms = search_ms + plain_fold_ms
+    scored_ms = search_ms + score_ms + scored_fold_ms
+    return {
+        "candidates": len(cands), "kept": len(kept), "threshold": threshold,
+        "fold_ms_per_candidate": plain_fold_ms / max(1, len(cands)),
+        "plain_ms": plain_ms, "scored_ms": scored_ms, "score_ms": score_ms,
+        "speedup": plain_ms / scored_ms,
+        "injected": len(workload.pulsars), "plain_detections": len(plain),
+        "scored_detections": len(plain & scored),
+        "recall": len(plain & scored) / max(1, len(plain)),
+    }
+
+
 def bench_batched(args) -> int:
     """Batched FFA: float series of different lengths, each with its own
     octave plan, searched as tiles of one launch sequence each, replayed from
@@ -234,6 +287,9 @@ def main() -> int:
     ap.add_argument("--batched-series", type=int, default=2048)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--scored", action="store_true",
+                    help="after the timed steps, compare the plain pipeline (full fold of every "
+                         "candidate) with the scorer-pruned one (committed weights)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="after the timed steps, copy MIB MiB device-to-device (pmc scaling)")
     args = ap.parse_args()
@@ -258,6 +314,8 @@ def main() -> int:
           file=sys.stderr, flush=True)
     pipe = w.pipeline()
     pipe.set_kernel_timing(True)
+    if args.scored:
+        pipe.enable_periodogram(True)
     pipe.generate(w.synth_spec())
     ndm, n_out = pipe.ndm, pipe.out_nsamples
     print(f"[bench] n_out {n_out}, FFA tasks {pipe.ffa_plan.task_count}, "
@@ -339,6 +397,8 @@ def main() -> int:
         "stages_ms": {k: v for k, v in stages.items() if k.endswith("_ms")},
         "kernels": kernels,
     }
+    if args.scored:
+        result["scored_pipeline"] = scored_comparison(w, pipe, ms_per_step)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or len(os.sched_getaffinity(0))
         print(f"[bench] cpu baseline on {threads} threads", file=sys.stderr, flush=True)

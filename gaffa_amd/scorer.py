@@ -220,9 +220,20 @@ def recall_threshold(scores: np.ndarray, s: CandidateSet, recall: float = 0.995,
 
 
 def load_weights(scorer: CandidateScorer, path: str = WEIGHTS_PATH) -> CandidateScorer:
-    """Load parameters saved by ``train`` (float32 .npy, layout order)."""
+    """Load parameters saved by ``main`` (float32 .npy, layout order)."""
     scorer.parameters = np.load(path, allow_pickle=False).astype(np.float32)
     return scorer
+
+
+def load_trained(path: str = WEIGHTS_PATH, *, capacity: int = 1024, device_id: int = 0):
+    """The committed scorer and its keep threshold (the logit chosen on the
+    validation set, stored beside the weights)."""
+    import json
+
+    with open(os.path.splitext(path)[0] + ".json") as fh:
+        meta = json.load(fh)
+    net = load_weights(CandidateScorer(capacity=capacity, device_id=device_id), path)
+    return net, float(meta["threshold"])
 
 
 def snr_cut_baseline(s: CandidateSet, recall: float, detection_snr: float = 8.0) -> dict:
@@ -295,6 +306,10 @@ def main(argv=None) -> int:
     print(json.dumps(out), flush=True)
     if args.out:
         np.save(args.out, scorer.parameters)
+        with open(os.path.splitext(args.out)[0] + ".json", "w") as fh:
+            json.dump({"threshold": thr, "recall_target": args.recall, "seed": args.seed,
+                       "train_filterbanks": args.train_filterbanks, "epochs": args.epochs,
+                       "data": "ScorerDataConfig defaults", "heldout": res}, fh, indent=1)
     return 0
 
 
