@@ -247,6 +247,80 @@ ffa_prepare_kernel(const float* __restrict__ series, uint64_t series_stride,
   prepared[sidx * prepared_stride + g.prep_offset + k] = sum;
 }
 
+// All downsample groups of a series in one pass over it: block (span x,
+// series z) stages input samples [s0, s0 + kPrepSpan + halo) once and
+// computes every output k of every group whose window starts in
+// [s0, s0 + kPrepSpan) (floor(k f) in the span; the window reaches at most
+// ceil(f) + 1 samples further, inside the halo), with the same serial
+// weighted sum as ffa_prepare_kernel.  For plans whose downsample groups all
+// read one series (input_offset 0, one input length) and whose largest
+// factor fits the halo; the series is then read once instead of once per
+// group.
+constexpr uint32_t kPrepSpan = 8192;
+constexpr uint32_t kPrepMaxHalo = 4096;
+constexpr uint32_t kPrepMaxGroups = 64;
+
+__global__ void __launch_bounds__(kPrepThreads)
+ffa_prepare_fused_kernel(const float* __restrict__ series, uint64_t series_stride,
+                         const gaffa_hip_ffa_group* __restrict__ groups, uint32_t ngroups,
+                         uint32_t halo, float* __restrict__ prepared, uint64_t prepared_stride) {
+  extern __shared__ float stage[];
+  __shared__ uint64_t k_lo[kPrepMaxGroups];
+  __shared__ uint32_t first_out[kPrepMaxGroups + 1];
+  const uint64_t n = groups[0].input_nsamples;
+  const uint64_t s0 = static_cast<uint64_t>(blockIdx.x) * kPrepSpan;
+  if (s0 >= n) return;
+  const uint64_t s1 = min(s0 + kPrepSpan, n);  // owned window starts
+  const uint64_t sidx = blockIdx.z;
+  const float* x = series + sidx * series_stride;
+  const uint64_t staged = min(n, s1 + halo) - s0;
+  for (uint32_t e = threadIdx.x; e < staged; e += kPrepThreads) stage[e] = x[s0 + e];
+  // per group: the outputs whose window starts in [s0, s1)
+  if (threadIdx.x < ngroups) {
+    const gaffa_hip_ffa_group g = groups[threadIdx.x];
+    uint64_t lo = 0, hi = 0;
+    if (g.factor != 1.0) {
+      const double f = g.factor;
+      auto start = [&](uint64_t k) { return static_cast<uint64_t>(floor(static_cast<double>(k) * f)); };
+      lo = static_cast<uint64_t>(ceil(static_cast<double>(s0) / f));
+      while (lo > 0 && start(lo - 1) >= s0) --lo;
+      while (start(lo) < s0) ++lo;
+      hi = static_cast<uint64_t>(ceil(static_cast<double>(s1) / f));
+      while (hi > 0 && start(hi - 1) >= s1) --hi;
+      while (start(hi) < s1) ++hi;
+      lo = min(lo, g.prepared_nsamples);
+      hi = min(hi, g.prepared_nsamples);
+    }
+    k_lo[threadIdx.x] = lo;
+    first_out[threadIdx.x + 1] = static_cast<uint32_t>(hi - lo);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    first_out[0] = 0;
+    for (uint32_t gi = 0; gi < ngroups; ++gi) first_out[gi + 1] += first_out[gi];
+  }
+  __syncthreads();
+  const double last_index = static_cast<double>(n - 1);
+  const uint32_t total = first_out[ngroups];
+  uint32_t gi = 0;
+  for (uint32_t e = threadIdx.x; e < total; e += kPrepThreads) {
+    while (first_out[gi + 1] <= e) ++gi;
+    const gaffa_hip_ffa_group g = groups[gi];
+    const uint64_t k = k_lo[gi] + (e - first_out[gi]);
+    const double start = static_cast<double>(k) * g.factor;
+    const double end = start + g.factor;
+    const uint64_t first = static_cast<uint64_t>(floor(start));
+    const uint64_t last = static_cast<uint64_t>(fmin(floor(end), last_index));
+    const float w0 = static_cast<float>(static_cast<double>(first + 1) - start);
+    const float w1 = static_cast<float>(end - static_cast<double>(last));
+    const float* src = stage - s0;
+    float sum = w0 * src[first];
+    for (uint64_t i = first + 1; i < last; ++i) sum += src[i];
+    sum += w1 * src[last];
+    prepared[sidx * prepared_stride + g.prep_offset + k] = sum;
+  }
+}
+
 // ------------------------------------------------------------- subtree ----
 // Persistent: one workgroup per CU walks the (node, series) units of the
 // launch, unit u = (subtree item u % nitems, series u / nitems), so that the
@@ -640,6 +714,19 @@ void launch_prepare(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
   if (p.ngroups == 0 || p.max_prepared_nsamples == 0) return;
   check(a->prepared != nullptr, "FFA prepared buffer must not be null");
   check(p.prep_blocks > 0 && p.prep_blocks < (1u << 31), "FFA prepare geometry missing");
+  if (p.prep_fused_halo > 0) {
+    check(p.prep_fused_halo <= kPrepMaxHalo && p.ngroups <= kPrepMaxGroups &&
+              p.prep_input_nsamples > 0,
+          "FFA fused prepare geometry out of range");
+    const dim3 grid(static_cast<uint32_t>(ceil_div(p.prep_input_nsamples, kPrepSpan)), 1,
+                    static_cast<uint32_t>(a->nseries));
+    const uint32_t lds = (kPrepSpan + p.prep_fused_halo) * 4u;
+    hipLaunchKernelGGL(ffa_prepare_fused_kernel, grid, dim3(kPrepThreads), lds, st, a->series,
+                       a->series_stride, p.groups, p.ngroups, p.prep_fused_halo, a->prepared,
+                       p.prepared_stride);
+    launch_check("ffa_prepare_fused_kernel");
+    return;
+  }
   // groups needing fewer blocks than the widest return at once
   const dim3 grid(p.prep_blocks, p.ngroups, static_cast<uint32_t>(a->nseries));
   hipLaunchKernelGGL(ffa_prepare_kernel, grid, dim3(kPrepThreads), 0, st, a->series,

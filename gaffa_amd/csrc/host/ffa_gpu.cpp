@@ -191,6 +191,18 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
 }
 
 void GpuFfaExecutionPlan::finish() {
+  // one prepare pass per series when every downsample group reads the same
+  // series and the widest window fits the fused kernel's halo
+  double max_factor = 0.0;
+  bool one_series = groups_.size() <= 64;
+  for (const gaffa_hip_ffa_group& g : groups_) {
+    one_series = one_series && g.input_offset == 0 && g.input_nsamples == input_nsamples_;
+    if (g.factor != 1.0) max_factor = std::max(max_factor, g.factor);
+  }
+  const double halo = std::ceil(max_factor) + 2.0;
+  prep_fused_halo_ = one_series && max_factor > 0.0 && halo <= 4096.0
+                         ? static_cast<std::uint32_t>(halo)
+                         : 0u;
   level_offsets_.assign(max_depth_ + 1, 0);
   for (std::size_t l = 0; l <= max_depth_; ++l) {
     level_offsets_[l] = static_cast<std::uint32_t>(level_.size());
@@ -253,6 +265,8 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.max_subtree_desc = exec_plan_.max_subtree_desc();
   desc_.terminal_lds_bytes = exec_plan_.terminal_lds_bytes();
   desc_.prep_blocks = exec_plan_.prep_blocks();
+  desc_.prep_fused_halo = exec_plan_.prep_fused_halo();
+  desc_.prep_input_nsamples = exec_plan_.input_nsamples();
   desc_.max_prepared_nsamples = exec_plan_.max_prepared();
   desc_.prepared_stride = exec_plan_.prepared_stride();
   desc_.work_stride = exec_plan_.work_stride();

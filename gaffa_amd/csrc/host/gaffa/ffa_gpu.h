@@ -81,6 +81,9 @@ class GpuFfaExecutionPlan {
   std::uint32_t max_subtree_desc() const noexcept { return subtree_desc_; }
   std::uint32_t terminal_lds_bytes() const noexcept { return terminal_lds_bytes_; }
   std::uint32_t prep_blocks() const noexcept { return prep_blocks_; }
+  // > 0 when one pass per series serves every downsample group (all read
+  // the one series): samples a window reaches past its start.
+  std::uint32_t prep_fused_halo() const noexcept { return prep_fused_halo_; }
   std::uint32_t max_bins() const noexcept { return max_bins_; }
   std::size_t max_prepared() const noexcept { return max_prepared_; }
   // Bytes of device workspace for a tile of `series` series.
@@ -101,6 +104,7 @@ class GpuFfaExecutionPlan {
   std::uint32_t subtree_desc_ = 0;
   std::uint32_t terminal_lds_bytes_ = 0;
   std::uint32_t prep_blocks_ = 0;
+  std::uint32_t prep_fused_halo_ = 0;
   std::uint32_t max_bins_ = 0;
   std::vector<gaffa_hip_ffa_group> groups_;
   std::vector<gaffa_hip_ffa_task> tasks_;

@@ -159,6 +159,7 @@ class GpuSearchPipeline {
   std::size_t marks_used_ = 0;
   GpuDeviceBuffer<float> periodogram_;
   std::vector<double> trial_periods_;
+  std::vector<std::uint32_t> trial_order_;
 };
 
 }  // namespace gaffa

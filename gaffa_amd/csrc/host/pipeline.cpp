@@ -284,7 +284,18 @@ void GpuSearchPipeline::enable_periodogram(bool on) {
     periodogram_ = GpuDeviceBuffer<float>();
     return;
   }
-  if (trial_periods_.empty()) trial_periods_ = ffa_plan_trial_periods(ffa_plan_);
+  if (trial_periods_.empty()) {
+    trial_periods_ = ffa_plan_trial_periods(ffa_plan_);
+    // rows by period (plan order is ascending for riptide plans, but the
+    // lookup must not depend on it)
+    trial_order_.resize(trial_periods_.size());
+    for (std::size_t i = 0; i < trial_order_.size(); ++i)
+      trial_order_[i] = static_cast<std::uint32_t>(i);
+    std::stable_sort(trial_order_.begin(), trial_order_.end(),
+                     [&](std::uint32_t a, std::uint32_t b) {
+                       return trial_periods_[a] < trial_periods_[b];
+                     });
+  }
   const std::size_t widths = periodogram_widths();
   const std::size_t stride = trial_periods_.size() * widths;
   periodogram_ = GpuDeviceBuffer<float>(ndm_ * stride);
@@ -320,13 +331,15 @@ void GpuSearchPipeline::candidate_features(std::span<const Candidate> candidates
   for (const Candidate& c : candidates) {
     check_arg(c.dm_index < ndm_, "candidate dm_index out of range");
     const FfaPeak& p = c.best_peak.peak;
-    const auto it = std::lower_bound(trial_periods_.begin(), trial_periods_.end(), p.period);
-    check_arg(it != trial_periods_.end() && *it == p.period,
+    const auto it = std::lower_bound(
+        trial_order_.begin(), trial_order_.end(), p.period,
+        [&](std::uint32_t r, double v) { return trial_periods_[r] < v; });
+    check_arg(it != trial_order_.end() && trial_periods_[*it] == p.period,
               "candidate period is not a trial period of this pipeline's plan");
     check_arg(p.width_index < widths, "candidate width index out of range");
     jobs.push_back(FoldJob{c.dm_index * stride_, p.period, 0});
     series.push_back(static_cast<std::uint32_t>(c.dm_index));
-    row.push_back(static_cast<std::uint32_t>(it - trial_periods_.begin()));
+    row.push_back(*it);
     width.push_back(static_cast<std::uint32_t>(p.width_index));
   }
   const GpuFoldInput in{plane_.data(), FoldDtype::F32, n_out_, 1, 1, cfg_.device_id};

@@ -334,6 +334,12 @@ typedef struct gaffa_hip_ffa_plan_desc {
   uint32_t terminal_lds_bytes; /* max over tasks of ..._terminal_lds_bytes() */
   uint32_t prep_blocks;        /* max over downsampled groups of
                                   ceil(prepared_nsamples / outputs per block) */
+  uint32_t prep_fused_halo;    /* > 0: every downsampled group reads the one
+                                  series (input_offset 0, prep_input_nsamples
+                                  samples) and a window reaches at most this
+                                  many samples (ceil(max factor) + 2, <= 4096)
+                                  past its start: one pass per series */
+  uint64_t prep_input_nsamples;
   uint64_t max_prepared_nsamples;
   uint64_t prepared_stride; /* floats per series in `prepared` */
   uint64_t work_stride;     /* floats per series per work region */
