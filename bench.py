@@ -75,7 +75,7 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
 
 KERNEL_NAMES = {
     "dedisperse": "dedisperse_subband_cm_kernel",
-    "ffa_prepare": "ffa_prepare_kernel",
+    "ffa_prepare": "ffa_prepare_fused_kernel",
     "ffa_subtree": "ffa_subtree_kernel",
     "ffa_levels": "ffa_level_kernel",
     "ffa_detect": "ffa_terminal_kernel<true>",
