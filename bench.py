@@ -42,8 +42,9 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
                 (4 P_g per downsample group)
     subtree     per series and task with rows >= 2: read + write its rows x bins
                 floats once (8 rows bins)
-    levels      per series and task: 8 rows bins for each level done through
-                global memory (depth_t - 1 of them), averaged over the launches
+    levels      per series and task: 8 rows bins for each pass through global
+                memory (floor(depth_t / 2) of them: a pass merges two tree levels
+                where two remain), averaged over the launches
     detect      per series and task: the depth-1 rows once (4 rows bins; the input
                 series row for 1-row tasks) + 20 B per peak
     """
@@ -60,7 +61,7 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
         if f != 1.0:
             prep += 4.0 * float(t["prepared_nsamples"][t["downsample_factor"] == f][0])
     subtree = 8.0 * float(rb[t["rows"] >= 2].sum())
-    levels = 8.0 * float((rb * np.maximum(depth - 1.0, 0.0)).sum())
+    levels = 8.0 * float((rb * np.floor(depth / 2.0)).sum())
     detect = 4.0 * float(rb.sum())
     nlev = max(1, lay["level_launches"])
     nseg = len(workload.dm_segments)
@@ -177,7 +178,7 @@ def scored_comparison(workload, pipe, search_ms: float) -> dict:
     of every candidate of the last search.  Scored pipeline: the learned
     scorer on every candidate (features and network on the GPU), full fold
     of those it keeps.  Detections: injected pulsars matched (frequency within
-    a Fourier bin of n/d times theirs, n, d <= 8; DM within the pulse width
+    a Fourier bin of 1/2, 1 or 2 times theirs; DM within the pulse width
     or two trials) by a folded candidate with S/N >= 8."""
     import numpy as np
 
@@ -200,7 +201,6 @@ def scored_comparison(workload, pipe, search_ms: float) -> dict:
     dms = np.asarray(pipe.dms)
     f = workload.frequencies
     kdm = sc.DISPERSION_S * (f.min() ** -2 - f.max() ** -2)
-    ratios = sc._ratios()
 
     def detected(cs):
         found = set()
@@ -208,7 +208,7 @@ def scored_comparison(workload, pipe, search_ms: float) -> dict:
             if c.snr < 8.0:
                 continue
             step = (dms[min(c.dm_index + 1, len(dms) - 1)] - dms[max(c.dm_index - 1, 0)]) / 2
-            k = sc._match(c.period, c.dm, workload.pulsars, obs, kdm, step, ratios)
+            k = sc._match(c.period, c.dm, workload.pulsars, obs, kdm, step, sc.MAIN_RATIOS)
             if k >= 0:
                 found.add(k)
         return found

@@ -396,7 +396,9 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     // the next unit's rows travel while this one computes
     if (u + gridDim.x < nunits) fetch(u + gridDim.x);
 
-    float* out = a.work + static_cast<uint64_t>(task.depth & 1u) * a.nseries * plan.work_stride +
+    float* out = a.work +
+                 static_cast<uint64_t>((task.regions >> task.depth) & 1u) * a.nseries *
+                     plan.work_stride +
                  sidx * plan.work_stride + task.work_offset + static_cast<uint64_t>(item.a) * b;
     if (levels == 0) {
       for (uint32_t e = tid; e < total; e += kSubtreeThreads) out[e] = cur[e];
@@ -466,40 +468,101 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
 }
 
 // --------------------------------------------------------------- level ----
-// Rows [first, first + count) of the depth-`depth` nodes of a task, merged
-// from the depth+1 outputs in work[(depth+1) % 2] into work[depth % 2].
+
+// Rows [first, first + count) of the depth-d nodes of a task (item.a = d |
+// levels << 16).  One level: merged from the depth d + 1 rows.  Two levels:
+// merged straight from the depth d + 2 rows, each output phase p summed as
+// (A[p] + B[p + rh]) + (C[q] + D[q + rt]) with q = p + ro (indices mod b),
+// which is exactly the two one-level merges (the depth d + 1 values are
+// FP32 sums in between), without writing the depth d + 1 rows.  The work
+// region of each depth comes from the task's region bits.
+struct Desc2 {
+  uint32_t a, b, rh;  // head child row: its two inputs and roll
+  uint32_t c, d, rt;  // tail child row: its two inputs and roll
+  uint32_t ro;        // roll of the tail child row in the output merge
+};
+
 __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
   __shared__ Desc sdesc[kLevelRows];
+  __shared__ Desc2 sdesc2[kLevelRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
-  const gaffa_hip_ffa_task task = plan.tasks[item.task];
+  const gaffa_hip_ffa_task& task = plan.tasks[item.task];
   const uint64_t sidx = blockIdx.y;
   const uint32_t b = task.bins;
-  const uint32_t depth = item.a;
+  const uint32_t depth = item.a & 0xFFFFu;
+  const uint32_t levels = item.a >> 16;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t wave = threadIdx.x >> 6;
+  const uint64_t region = a.nseries * plan.work_stride;
+  const float* src = a.work + static_cast<uint64_t>((task.regions >> (depth + levels)) & 1u) * region +
+                     sidx * plan.work_stride + task.work_offset;
+  float* dst = a.work + static_cast<uint64_t>((task.regions >> depth) & 1u) * region +
+               sidx * plan.work_stride + task.work_offset;
+  constexpr uint32_t kWaves = kLevelThreads / 64;
+  if (levels == 1) {
+    if (threadIdx.x < item.c) {
+      const uint32_t r = item.b + threadIdx.x;
+      sdesc[threadIdx.x] = merge_desc(find_node(0, task.rows, depth, r), r, b);
+    }
+    __syncthreads();
+    constexpr uint32_t kBatch = kLevelRows / kWaves;
+    merge_batch<kBatch>(
+        wave, kWaves, item.c, b, lane,
+        [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
+          const Desc ds = sdesc[i];
+          h = src + static_cast<uint64_t>(ds.h) * b;
+          t = src + static_cast<uint64_t>(ds.t) * b;
+          roll = ds.roll;
+        },
+        [&](uint32_t i, uint32_t p, float v) { dst[static_cast<uint64_t>(item.b + i) * b + p] = v; });
+    return;
+  }
   if (threadIdx.x < item.c) {
     const uint32_t r = item.b + threadIdx.x;
-    sdesc[threadIdx.x] = merge_desc(find_node(0, task.rows, depth, r), r, b);
+    const Node nd = find_node(0, task.rows, depth, r);
+    const Desc top = merge_desc(nd, r, b);
+    const uint32_t hn = nd.size / 2;
+    const Desc dh = merge_desc(Node{nd.start, hn}, top.h, b);
+    const Desc dt = merge_desc(Node{nd.start + hn, nd.size - hn}, top.t, b);
+    sdesc2[threadIdx.x] = Desc2{dh.h, dh.t, dh.roll, dt.h, dt.t, dt.roll, top.roll};
   }
   __syncthreads();
-  const uint64_t region = a.nseries * plan.work_stride;
-  const float* src = a.work + static_cast<uint64_t>((depth + 1) & 1u) * region +
-                     sidx * plan.work_stride + task.work_offset;
-  float* dst = a.work + static_cast<uint64_t>(depth & 1u) * region + sidx * plan.work_stride +
-               task.work_offset;
-  constexpr uint32_t kWaves = kLevelThreads / 64;
-  constexpr uint32_t kBatch = kLevelRows / kWaves;
-  merge_batch<kBatch>(
-      wave, kWaves, item.c, b, lane,
-      [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
-        const Desc ds = sdesc[i];
-        h = src + static_cast<uint64_t>(ds.h) * b;
-        t = src + static_cast<uint64_t>(ds.t) * b;
-        roll = ds.roll;
-      },
-      [&](uint32_t i, uint32_t p, float v) { dst[static_cast<uint64_t>(item.b + i) * b + p] = v; });
+  // two rows per wave at a time, four phases per lane: 32 loads in flight
+  for (uint32_t i0 = wave; i0 < item.c; i0 += 2 * kWaves) {
+    const uint32_t i1 = min(i0 + kWaves, item.c - 1);  // duplicate of a valid row
+    const Desc2 e[2] = {sdesc2[i0], sdesc2[i1]};
+    for (uint32_t p0 = lane; p0 < b; p0 += 256) {
+      float v[2][4];
+#pragma unroll
+      for (uint32_t k = 0; k < 2; ++k) {
+        const float* A = src + static_cast<uint64_t>(e[k].a) * b;
+        const float* B = src + static_cast<uint64_t>(e[k].b) * b;
+        const float* C = src + static_cast<uint64_t>(e[k].c) * b;
+        const float* D = src + static_cast<uint64_t>(e[k].d) * b;
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t p = min(p0 + 64u * j, b - 1);
+          uint32_t pb = p + e[k].rh;
+          pb = min(pb, pb - b);
+          uint32_t q = p + e[k].ro;
+          q = min(q, q - b);
+          uint32_t qd = q + e[k].rt;
+          qd = min(qd, qd - b);
+          v[k][j] = (A[p] + B[pb]) + (C[q] + D[qd]);
+        }
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t p = p0 + 64u * j;
+        if (p < b) {
+          dst[static_cast<uint64_t>(item.b + i0) * b + p] = v[0][j];
+          if (i0 + kWaves < item.c) dst[static_cast<uint64_t>(item.b + i1) * b + p] = v[1][j];
+        }
+      }
+    }
+  }
 }
 
 // ------------------------------------------------------------ terminal ----
@@ -566,7 +629,8 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   if (task.rows == 1) {
     src = task_input(a, group, sidx);
   } else {
-    src = a.work + 1ull * a.nseries * plan.work_stride + sidx * plan.work_stride +
+    src = a.work + static_cast<uint64_t>((task.regions >> 1) & 1u) * a.nseries * plan.work_stride +
+          sidx * plan.work_stride +
           task.work_offset;
     if (threadIdx.x < nrows) {
       const uint32_t s = s0 + threadIdx.x;

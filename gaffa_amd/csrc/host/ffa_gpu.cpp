@@ -177,10 +177,20 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       }
       subtree_lds_bytes_ = 8u * subtree_half_ + 8u * subtree_desc_;
       if (by_depth_.size() < depth) by_depth_.resize(depth);
-      for (std::uint32_t l = 1; l < depth; ++l)
+      // global passes from depth `depth` up to depth 1, two tree levels per
+      // pass where two remain; the region bits make every pass read one work
+      // region and write the other
+      std::uint32_t regions = 0;  // the subtrees write region 0
+      for (std::uint32_t din = depth; din > 1;) {
+        const std::uint32_t levels = din >= 3 ? 2u : 1u;
+        const std::uint32_t dout = din - levels;
+        regions |= (1u - ((regions >> din) & 1u)) << dout;
         for (std::uint32_t r = 0; r < d.rows; r += level_rows)
-          by_depth_[l].push_back(
-              gaffa_hip_ffa_item{task_index, l, r, std::min(level_rows, d.rows - r)});
+          by_depth_[dout].push_back(gaffa_hip_ffa_item{task_index, dout | (levels << 16), r,
+                                                       std::min(level_rows, d.rows - r)});
+        din = dout;
+      }
+      d.regions = regions;
       max_depth_ = std::max<std::size_t>(max_depth_, depth);
     }
     for (std::uint32_t r = 0; r < d.rows_eval; r += terminal_rows)

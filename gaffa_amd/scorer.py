@@ -10,8 +10,10 @@ Training data is made by the search itself: synthetic uint8 N(128, 16)
 filterbanks, half of them noise only, the rest with injected pulsars, go
 through the GPU pipeline (dedispersion, detrend, FFA, candidate selection);
 each candidate is labelled by whether it matches an injection (its
-frequency within a Fourier bin of n / d times the pulsar's, n, d <= 8, its
-DM error smearing the pulse by no more than its width or two DM trials).  The plain pipeline folds every
+frequency within a Fourier bin of 1/2, 1 or 2 times the pulsar's, its DM
+error smearing the pulse by no more than its width or two DM trials);
+candidates at other small harmonic ratios (n / d, n, d <= 8) are related
+to a pulsar and left out of training and of the noise counts.  The plain pipeline folds every
 candidate above the S/N threshold; the scored pipeline folds only those the
 network keeps.  ``evaluate`` measures both on a held-out seed: recall of the
 plain pipeline's detections (pulsars with a matching candidate at S/N >= 8),
@@ -78,6 +80,7 @@ class CandidateSet:
     labels: np.ndarray       # (n,) float32, 1 = matches an injected pulsar
     snr: np.ndarray          # (n,) candidate S/N
     pulsar: np.ndarray       # (n,) index into pulsar arrays, -1 = none
+    related: np.ndarray      # (n,) bool: matches a pulsar at a ratio n/d <= 8
     pulsar_snr: np.ndarray   # (npulsars,) injected S/N
     pulsar_period: np.ndarray
     pulsar_dm: np.ndarray
@@ -101,9 +104,15 @@ def _draw_pulsars(rng, cfg: ScorerDataConfig) -> list:
 
 
 def _ratios(max_n: int = 8) -> np.ndarray:
-    """Harmonic ratios a candidate may sit at: reduced n / d, n, d <= 8."""
+    """Harmonic ratios a candidate may sit at: reduced n / d, n, d <= max_n."""
     return np.array(sorted({n / d for n in range(1, max_n + 1) for d in range(1, max_n + 1)
                             if math.gcd(n, d) == 1}))
+
+
+# A candidate is a detection of a pulsar at its period, twice it or half it;
+# other small ratios mark it as related (a weaker harmonic), neither signal
+# nor noise for training.
+MAIN_RATIOS = np.array([0.5, 1.0, 2.0])
 
 
 def _match(period, dm, pulsars, obs_seconds, band_delay_per_dm, dm_step, ratios) -> int:
@@ -135,7 +144,7 @@ def candidate_set(n_filterbanks: int, seed: int, cfg: ScorerDataConfig = ScorerD
     kdm = cfg.band_delay_per_dm()
     dms = np.asarray(pipe.dms)
     ratios = _ratios()
-    cubes, patches, labels, snrs, owner = [], [], [], [], []
+    cubes, patches, labels, snrs, owner, related = [], [], [], [], [], []
     p_snr, p_period, p_dm = [], [], []
     t0 = time.perf_counter()
     for i in range(n_filterbanks):
@@ -157,9 +166,11 @@ def candidate_set(n_filterbanks: int, seed: int, cfg: ScorerDataConfig = ScorerD
         patches.append(q)
         for cand in cands:
             step = dms[min(cand.dm_index + 1, len(dms) - 1)] - dms[max(cand.dm_index - 1, 0)]
-            k = _match(cand.period, cand.dm, pulsars, obs, kdm, step / 2, ratios)
+            k = _match(cand.period, cand.dm, pulsars, obs, kdm, step / 2, MAIN_RATIOS)
+            rel = k >= 0 or _match(cand.period, cand.dm, pulsars, obs, kdm, step / 2, ratios) >= 0
             labels.append(1.0 if k >= 0 else 0.0)
             owner.append(base + k if k >= 0 else -1)
+            related.append(rel)
             snrs.append(cand.snr)
         if log is not None and (i + 1) % 50 == 0:
             log(f"[scorer] {i + 1}/{n_filterbanks} filterbanks, {len(labels)} candidates, "
@@ -168,17 +179,19 @@ def candidate_set(n_filterbanks: int, seed: int, cfg: ScorerDataConfig = ScorerD
         cubes=np.concatenate(cubes) if cubes else np.zeros((0, 32, 64), np.float32),
         patches=np.concatenate(patches) if patches else np.zeros((0, 128), np.float32),
         labels=np.asarray(labels, np.float32), snr=np.asarray(snrs, np.float32),
-        pulsar=np.asarray(owner, np.int64), pulsar_snr=np.asarray(p_snr),
+        pulsar=np.asarray(owner, np.int64), related=np.asarray(related, bool),
+        pulsar_snr=np.asarray(p_snr),
         pulsar_period=np.asarray(p_period), pulsar_dm=np.asarray(p_dm),
         filterbanks=n_filterbanks, seconds=time.perf_counter() - t0)
 
 
-def balanced_indices(labels: np.ndarray, seed: int) -> np.ndarray:
-    """All positives and as many negatives (or all of the smaller class
-    with an equal draw of the other), shuffled."""
+def balanced_indices(labels: np.ndarray, seed: int, related=None) -> np.ndarray:
+    """An equal draw of positives and negatives (all of the smaller class),
+    shuffled; negatives related to a pulsar (weaker harmonics) are left
+    out."""
     rng = np.random.default_rng(seed)
     pos = np.nonzero(labels > 0)[0]
-    neg = np.nonzero(labels == 0)[0]
+    neg = np.nonzero((labels == 0) & (~related if related is not None else True))[0]
     k = min(len(pos), len(neg))
     idx = np.concatenate([rng.choice(pos, k, replace=False), rng.choice(neg, k, replace=False)])
     rng.shuffle(idx)
@@ -198,7 +211,7 @@ def evaluate(scores: np.ndarray, s: CandidateSet, threshold: float,
     strong = s.snr >= detection_snr
     plain = set(s.pulsar[(s.pulsar >= 0) & strong].tolist())
     scored = set(s.pulsar[(s.pulsar >= 0) & strong & keep].tolist())
-    noise = s.labels == 0
+    noise = (s.labels == 0) & ~s.related
     return {
         "plain_detections": len(plain),
         "scored_detections": len(plain & scored),
@@ -272,7 +285,7 @@ def train(train_set: CandidateSet, *, epochs: int = 6, batch: int = 256, lr: flo
           seed: int = 1, capacity: int = 1024, device_id: int = 0):
     """A scorer trained on a class-balanced draw of the set; returns
     (scorer, per-epoch loss)."""
-    idx = balanced_indices(train_set.labels, seed)
+    idx = balanced_indices(train_set.labels, seed, train_set.related)
     scorer = CandidateScorer(capacity=capacity, device_id=device_id)
     scorer.initialize(seed)
     losses = scorer.fit(train_set.cubes[idx], train_set.patches[idx], train_set.labels[idx],

@@ -293,6 +293,8 @@ typedef struct gaffa_hip_ffa_task {
                                series with their own plans in one row) */
   uint32_t periodogram_row; /* first row of this task in the series'
                                periodogram (plan order, rows_eval each) */
+  uint32_t regions;         /* bit k: work region (0 or 1) holding the
+                               depth-k rows of this task's tree */
   uint32_t width[GAFFA_HIP_FFA_MAX_WIDTHS];
   float height[GAFFA_HIP_FFA_MAX_WIDTHS];
   float baseline[GAFFA_HIP_FFA_MAX_WIDTHS];
@@ -300,7 +302,8 @@ typedef struct gaffa_hip_ffa_task {
 
 typedef struct gaffa_hip_ffa_item {
   uint32_t task;
-  uint32_t a; /* subtree: node start row; level: depth; terminal: first row */
+  uint32_t a; /* subtree: node start row; level: output depth | levels << 16
+                 (1 or 2 tree levels merged in one pass); terminal: first row */
   uint32_t b; /* subtree: node rows;      level: first row; terminal: rows */
   uint32_t c; /* level: rows in this item */
 } gaffa_hip_ffa_item;
