@@ -5,7 +5,7 @@
 // modulo nbin); exposure collects the overlaps; the folded value is
 // sum / exposure where exposure > 0.
 //
-// One thread per (job, subint, output channel, phase bin) *gathers*: it walks
+// One thread per (job, subint, phase bin, output channel) *gathers*: it walks
 // the periods of its subint (bin k = phase + m * nbin for increasing m) and,
 // per period, the few samples overlapping bin k in increasing time order.
 // A sample overlaps at most one bin of a given phase (a period spans more than
@@ -36,8 +36,11 @@ fold_kernel(const T* __restrict__ data, gaffa_hip_fold_args a,
   const uint64_t per_job = static_cast<uint64_t>(a.nsubint) * nout * a.nbin;
   const uint64_t e = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
   if (e >= per_job) return;
-  const uint32_t phase = static_cast<uint32_t>(e % a.nbin);
-  const uint32_t ch = static_cast<uint32_t>((e / a.nbin) % nout);
+  // output channel fastest: the lanes of a wave read adjacent channel
+  // groups of the same samples (whole rows per load) and walk the same
+  // sample sequence (one phase per wave where nout >= 64)
+  const uint32_t ch = static_cast<uint32_t>(e % nout);
+  const uint32_t phase = static_cast<uint32_t>((e / nout) % a.nbin);
   const uint32_t sub = static_cast<uint32_t>(e / (static_cast<uint64_t>(a.nbin) * nout));
   const uint32_t factor = a.nchans / nout;
 

@@ -123,8 +123,9 @@ class GpuSearchPipeline {
   // periodogram): cubes [n][32][64], the candidate's DM-trial series folded
   // at its period into 32 subints of 64 phase bins, and patches [n][128],
   // the 128 periodogram trials around its period at its width (edges
-  // repeated); each normalised to zero mean and unit variance.  Device
-  // outputs, enqueued on the pipeline stream.
+  // repeated).  Cubes are normalised to zero mean and unit variance,
+  // patches to zero mean in S/N units.  Device outputs, enqueued on the
+  // pipeline stream.
   void candidate_features(std::span<const Candidate> candidates, float* cubes, float* patches);
   // gaffa's full fold (pfold) of candidates from the resident filterbank:
   // per candidate the aligned spectrum at its DM (dedisperse_spectrum),

@@ -2,8 +2,9 @@
 // parameters, Adam state and workspace on one device and drives the MFMA
 // forward / backward kernels of libgaffa_hip (gaffa_hip_scorer_*).
 //
-// Inputs per candidate: a [32 subint][64 phase] folded cube and a 128-trial
-// periodogram patch, FP32, each normalised to zero mean and unit variance.
+// Inputs per candidate: a [32 subint][64 phase] folded cube (normalised to
+// zero mean and unit variance) and a 128-trial periodogram patch (mean
+// removed, S/N units), FP32.
 #pragma once
 
 #include "gaffa/gpu_memory.h"

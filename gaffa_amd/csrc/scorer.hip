@@ -481,9 +481,10 @@ gather_patch_kernel(gaffa_hip_scorer_patch_args a) {
 }
 
 // One block per row: FP32 mean, then FP32 variance about it (fixed-order
-// reductions), then (x - mean) / sqrt(variance).
+// reductions), then (x - mean) / sqrt(variance), or only x - mean when
+// `scale` is 0 (periodogram patches keep their S/N units).
 __global__ void __launch_bounds__(256)
-normalise_rows_kernel(float* __restrict__ x, uint32_t length) {
+normalise_rows_kernel(float* __restrict__ x, uint32_t length, uint32_t scale) {
   __shared__ float red[256];
   float* row = x + static_cast<uint64_t>(blockIdx.x) * length;
   auto reduce = [&](float v) {
@@ -506,7 +507,7 @@ normalise_rows_kernel(float* __restrict__ x, uint32_t length) {
     v += d * d;
   }
   const float var = reduce(v) / static_cast<float>(length);
-  const float inv = var > 0.0f ? 1.0f / sqrtf(var) : 0.0f;
+  const float inv = !scale ? 1.0f : (var > 0.0f ? 1.0f / sqrtf(var) : 0.0f);
   for (uint32_t k = threadIdx.x; k < length; k += 256) row[k] = (row[k] - mean) * inv;
 }
 
@@ -704,7 +705,7 @@ int gaffa_hip_scorer_gather_patches(const gaffa_hip_scorer_patch_args* a,
     hipLaunchKernelGGL(gather_patch_kernel, dim3(a->n), dim3(kPatch), 0, as_stream(stream), *a);
     launch_check("gather_patch_kernel");
     hipLaunchKernelGGL(normalise_rows_kernel, dim3(a->n), dim3(256), 0, as_stream(stream),
-                       a->out, kPatch);
+                       a->out, kPatch, 0u);
     launch_check("normalise_rows_kernel");
   });
 }
@@ -714,7 +715,7 @@ int gaffa_hip_scorer_normalise_rows(float* x, uint32_t rows, uint32_t length,
   return guarded([&] {
     check(x != nullptr && rows > 0 && length > 0, "normalise_rows needs a non-empty array");
     hipLaunchKernelGGL(normalise_rows_kernel, dim3(rows), dim3(256), 0, as_stream(stream), x,
-                       length);
+                       length, 1u);
     launch_check("normalise_rows_kernel");
   });
 }

@@ -2,8 +2,9 @@
 
 ``CandidateScorer`` holds a small 1D-conv + MLP network (~175k parameters,
 bfloat16 weights and activations, FP32 accumulation on MFMA) that scores a
-search candidate from its folded [32 subint][64 phase] cube and a 128-trial
-periodogram patch, both normalised to zero mean and unit variance.  Forward,
+search candidate from its folded [32 subint][64 phase] cube (normalised to
+zero mean and unit variance) and a 128-trial periodogram patch (mean
+removed, in S/N units).  Forward,
 backward and Adam run on the GPU through libgaffa_hip's scorer kernels.
 
 Training data is made by the search itself: synthetic uint8 N(128, 16)
@@ -304,7 +305,7 @@ def main(argv=None) -> int:
     ap.add_argument("--train-filterbanks", type=int, default=400)
     ap.add_argument("--val-filterbanks", type=int, default=60)
     ap.add_argument("--eval-filterbanks", type=int, default=100)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--epochs", type=int, default=5)
     ap.add_argument("--seed", type=int, default=20260707)
     ap.add_argument("--recall", type=float, default=0.99)
     ap.add_argument("--out", default="")

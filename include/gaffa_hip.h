@@ -588,9 +588,11 @@ int gaffa_hip_scorer_sync_params(const gaffa_hip_scorer_net* net, gaffa_hip_stre
 
 /* Scorer input preparation.  Patches: out[i][q] = periodogram[series[i] *
  * series_stride + clamp(row[i] - 64 + q, 0, rows - 1) * widths + width[i]]
- * for q < 128 (the trials around the candidate's, edges repeated), then
- * normalised.  normalise_rows: each row of x [rows][length] to zero mean and
- * unit variance (FP32 mean and variance; a constant row becomes 0). */
+ * for q < 128 (the trials around the candidate's, edges repeated), minus
+ * the patch mean (the values stay in S/N units, so the network sees how
+ * strong the candidate is).  normalise_rows: each row of x [rows][length]
+ * to zero mean and unit variance (FP32 mean and variance; a constant row
+ * becomes 0). */
 typedef struct gaffa_hip_scorer_patch_args {
   const float* periodogram;
   uint64_t series_stride;
