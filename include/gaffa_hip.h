@@ -517,8 +517,8 @@ int gaffa_hip_synth_filterbank(const gaffa_hip_synth_args* args,
 /* ----------------------------------------------------- candidate scorer ----
  * No gaffa counterpart: a learned score that decides which search
  * candidates go on to full folding.  Inputs per candidate: a 32-subint x
- * 64-phase folded cube and a 128-trial periodogram patch, each normalised
- * to zero mean and unit variance.  Network (fixed shape):
+ * 64-phase folded cube (zero mean, unit variance) and a 128-trial
+ * periodogram patch (mean removed, S/N units).  Network (fixed shape):
  *   conv1..conv4: circular 1D convolutions over phase, kernel 3,
  *                 32 -> 64 -> 64 -> 64 -> 64 channels (the subints are
  *                 conv1's input channels), each + bias, ReLU;
