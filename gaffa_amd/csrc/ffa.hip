@@ -331,6 +331,7 @@ ffa_prepare_fused_kernel(const float* __restrict__ series, uint64_t series_strid
 // straight to work[depth % 2].  LDS = [2][half] floats + descriptor table.
 constexpr uint32_t kSubtreePrefetch = 20;  // floats per thread: node <= 20480 floats
 constexpr uint32_t kCopyTail = 0xFFFFu;    // descriptor tail of a copied row
+constexpr uint32_t kSubtreeRowsPerPass = 4;  // rows a wave merges at once (4 phases each)
 
 // Packed merge descriptor: x = head | tail << 16 (tail kCopyTail = copy the
 // row), y = roll.
@@ -406,51 +407,56 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     // One level: rows of `src` (LDS) merged into `dst`, written once per
     // destination kind so every access keeps its address space (LDS rows
     // are ds_* operations, the top level's stores global_*).  Lanes cover
-    // 4 phases each per 256-phase pass; each wave merges two rows at a time.
-    // Descriptors are wave-uniform (scalar row math) and read one pair
-    // ahead.  No store is predicated: a lane past the row end recomputes
-    // phase b - 1 and a wave without a second row recomputes row nrows - 1,
-    // storing the very value another lane stores there.
+    // 4 phases each per 256-phase pass; each wave merges
+    // kSubtreeRowsPerPass rows at a time.  Descriptors are wave-uniform
+    // (scalar row math) and read one batch ahead.  No store is predicated: a
+    // lane past the row end recomputes phase b - 1 and a wave short of rows
+    // recomputes row nrows - 1, storing the very value another lane stores
+    // there.
     auto level = [&](const uint2* dd, const float* src, auto&& store) {
       if (wave >= nrows) return;
+      constexpr uint32_t R = kSubtreeRowsPerPass;
       for (uint32_t p0 = 0; p0 < b; p0 += 256) {
         uint32_t pc[4];
 #pragma unroll
         for (uint32_t j = 0; j < 4; ++j) pc[j] = min(p0 + lane + 64u * j, b - 1);
-        uint2 e0 = dd[wave], e1 = dd[min(wave + kWaves, nrows - 1)];
-        for (uint32_t r0 = wave; r0 < nrows; r0 += 2 * kWaves) {
-          const uint32_t r1 = min(r0 + kWaves, nrows - 1);
-          const uint2 d0 = make_uint2(__builtin_amdgcn_readfirstlane(e0.x),
-                                      __builtin_amdgcn_readfirstlane(e0.y));
-          const uint2 d1 = make_uint2(__builtin_amdgcn_readfirstlane(e1.x),
-                                      __builtin_amdgcn_readfirstlane(e1.y));
-          const uint32_t n0 = r0 + 2 * kWaves;
+        uint2 e[R];
+#pragma unroll
+        for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(wave + k * kWaves, nrows - 1)];
+        for (uint32_t r0 = wave; r0 < nrows; r0 += R * kWaves) {
+          uint32_t hb[R], tb[R], ro[R], ob[R];
+          bool cp[R];
+#pragma unroll
+          for (uint32_t k = 0; k < R; ++k) {
+            const uint32_t x = __builtin_amdgcn_readfirstlane(e[k].x);
+            const uint32_t y = __builtin_amdgcn_readfirstlane(e[k].y);
+            hb[k] = (x & 0xFFFFu) * b;
+            cp[k] = (x >> 16) == kCopyTail;
+            tb[k] = cp[k] ? hb[k] : (x >> 16) * b;
+            ro[k] = cp[k] ? 0u : y;
+            ob[k] = min(r0 + k * kWaves, nrows - 1) * b;
+          }
+          const uint32_t n0 = r0 + R * kWaves;
           if (n0 < nrows) {
-            e0 = dd[n0];
-            e1 = dd[min(n0 + kWaves, nrows - 1)];
-          }
-          const uint32_t hb0 = (d0.x & 0xFFFFu) * b, hb1 = (d1.x & 0xFFFFu) * b;
-          const bool cp0 = (d0.x >> 16) == kCopyTail, cp1 = (d1.x >> 16) == kCopyTail;
-          const uint32_t tb0 = cp0 ? hb0 : (d0.x >> 16) * b, tb1 = cp1 ? hb1 : (d1.x >> 16) * b;
-          const uint32_t ro0 = cp0 ? 0u : d0.y, ro1 = cp1 ? 0u : d1.y;
-          float v0[4], v1[4];
 #pragma unroll
-          for (uint32_t j = 0; j < 4; ++j) {
-            uint32_t q0 = pc[j] + ro0, q1 = pc[j] + ro1;
-            q0 = min(q0, q0 - b);  // (p + roll) mod b for p, roll < b
-            q1 = min(q1, q1 - b);
-            const float h0 = src[hb0 + pc[j]], t0 = src[tb0 + q0];
-            const float h1 = src[hb1 + pc[j]], t1 = src[tb1 + q1];
-            // a copied row (a leaf above this depth) keeps its own value
-            v0[j] = cp0 ? h0 : h0 + t0;
-            v1[j] = cp1 ? h1 : h1 + t1;
+            for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(n0 + k * kWaves, nrows - 1)];
           }
-          const uint32_t ob0 = r0 * b, ob1 = r1 * b;
+          float v[R][4];
 #pragma unroll
-          for (uint32_t j = 0; j < 4; ++j) {
-            store(ob0 + pc[j], v0[j]);
-            store(ob1 + pc[j], v1[j]);
+          for (uint32_t k = 0; k < R; ++k) {
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) {
+              uint32_t q = pc[j] + ro[k];
+              q = min(q, q - b);  // (p + roll) mod b for p, roll < b
+              const float h = src[hb[k] + pc[j]], t = src[tb[k] + q];
+              // a copied row (a leaf above this depth) keeps its own value
+              v[k][j] = cp[k] ? h : h + t;
+            }
           }
+#pragma unroll
+          for (uint32_t k = 0; k < R; ++k)
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) store(ob[k] + pc[j], v[k][j]);
         }
       }
     };
