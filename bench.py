@@ -215,6 +215,9 @@ def scored_comparison(workload, pipe, search_ms: float) -> dict:
 
     plain = detected(cands)
     scored = detected(kept)
+    # the method without the network: fold only candidates with S/N >= 8
+    strong = [c for c in cands if c.snr >= 8.0]
+    by_snr = detected(strong)
     plain_ms = search_ms + plain_fold_ms
     scored_ms = search_ms + score_ms + scored_fold_ms
     return {
@@ -226,6 +229,8 @@ def scored_comparison(workload, pipe, search_ms: float) -> dict:
         "injected": len(workload.pulsars), "plain_detections": len(plain),
         "scored_detections": len(plain & scored),
         "recall": len(plain & scored) / max(1, len(plain)),
+        "snr8_kept": len(strong), "snr8_detections": len(plain & by_snr),
+        "scored_and_snr8_kept": sum(1 for c in kept if c.snr >= 8.0),
     }
 
 

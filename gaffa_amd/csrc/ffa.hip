@@ -331,7 +331,7 @@ ffa_prepare_fused_kernel(const float* __restrict__ series, uint64_t series_strid
 // straight to work[depth % 2].  LDS = [2][half] floats + descriptor table.
 constexpr uint32_t kSubtreePrefetch = 20;  // floats per thread: node <= 20480 floats
 constexpr uint32_t kCopyTail = 0xFFFFu;    // descriptor tail of a copied row
-constexpr uint32_t kSubtreeRowsPerPass = 4;  // rows a wave merges at once (4 phases each)
+constexpr uint32_t kSubtreeRowsPerPass = 2;  // rows a wave merges at once (4 phases each)
 
 // Packed merge descriptor: x = head | tail << 16 (tail kCopyTail = copy the
 // row), y = roll.

@@ -332,11 +332,14 @@ def main(argv=None) -> int:
     budget = base.get("folded_fraction") or 0.0
     ranking = {"budget": budget, "pulsars_by_snr": budget_recall(ev.snr, ev, budget),
                "pulsars_by_score": budget_recall(z, ev, budget)}
+    # network and S/N cut together: fold what both keep
+    both = np.where(ev.snr >= 8.0, z, -np.inf)
+    combined = evaluate(both, ev, thr)
     out = {
         "train_candidates": len(tr), "train_positive": int(tr.labels.sum()),
         "train_filterbanks": tr.filterbanks, "data_seconds": t1 - t0, "train_seconds": t2 - t1,
         "epoch_loss": losses, "threshold": thr, "heldout": res, "snr_cut_at_equal_recall": base,
-        "equal_budget": ranking,
+        "equal_budget": ranking, "scored_and_snr8": combined,
         "heldout_seed": args.seed + 2,
     }
     print(json.dumps(out), flush=True)
