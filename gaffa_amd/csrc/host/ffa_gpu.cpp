@@ -365,6 +365,13 @@ std::vector<gaffa_hip_ffa_peak> GpuFfaProgram::read_peaks(std::uint32_t count) {
   return v;
 }
 
+std::size_t GpuFfaProgram::read_peaks_into(std::uint32_t count, gaffa_hip_ffa_peak* dst) {
+  const std::size_t n = std::min<std::size_t>(count, peaks_.size());
+  if (n != 0) copy_to_host(dst, peaks_.data(), n * sizeof(gaffa_hip_ffa_peak), exec_.stream);
+  hip_ok(gaffa_hip_stream_synchronize(exec_.stream), "hipStreamSynchronize");
+  return n;
+}
+
 void GpuFfaProgram::grow_peaks(std::size_t count) {
   const std::size_t max_count = exec_.max_peak_buffer_bytes / sizeof(gaffa_hip_ffa_peak);
   if (count > max_count) {
@@ -416,6 +423,14 @@ std::vector<FfaBatchPeak> expand_ffa_peaks(const GpuFfaExecutionPlan& plan,
 std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
                                            std::span<const gaffa_hip_ffa_peak> peaks,
                                            std::span<const double> dms) {
+  std::vector<DmPeak> out;
+  expand_sorted_dm_peaks_into(plan, peaks, dms, out);
+  return out;
+}
+
+void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
+                                 std::span<const gaffa_hip_ffa_peak> peaks,
+                                 std::span<const double> dms, std::vector<DmPeak>& out) {
   const auto& tasks = plan.plan().tasks;
   std::map<std::size_t, FfaDetectionPlan> by_bins;
   std::vector<const FfaDetectionPlan*> det(tasks.size());
@@ -482,11 +497,10 @@ std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
     }
     i = j;
   }
-  std::vector<DmPeak> out(peaks.size());
+  out.resize(peaks.size());
 #pragma omp parallel for schedule(static)
   for (std::int64_t i = 0; i < n; ++i)
     out[static_cast<std::size_t>(i)] = expand(peaks[keys[static_cast<std::size_t>(i)].index]);
-  return out;
 }
 
 FfaBatchSearchResult run_ffa_batch_gpu(GpuFfaProgram& program, GpuTimeSeriesBatchView batch,

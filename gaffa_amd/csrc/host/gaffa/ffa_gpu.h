@@ -138,6 +138,9 @@ class GpuFfaProgram {
                     const FfaStageMarks* marks = nullptr);
   std::uint32_t read_peak_count();
   std::vector<gaffa_hip_ffa_peak> read_peaks(std::uint32_t count);
+  // Copies the first min(count, capacity) compact peaks to `dst` (host,
+  // ideally pinned) and waits; returns how many.
+  std::size_t read_peaks_into(std::uint32_t count, gaffa_hip_ffa_peak* dst);
   std::size_t peak_capacity() const noexcept { return peaks_.size(); }
   void grow_peaks(std::size_t count);
   gaffa_hip_stream stream() const noexcept { return exec_.stream; }
@@ -211,6 +214,11 @@ struct DmSearchResult {
 std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
                                            std::span<const gaffa_hip_ffa_peak> peaks,
                                            std::span<const double> dms);
+// The same into `out`, reusing its storage (no allocation or first-touch
+// cost when it already holds as many peaks, as in repeated searches).
+void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
+                                 std::span<const gaffa_hip_ffa_peak> peaks,
+                                 std::span<const double> dms, std::vector<DmPeak>& out);
 
 bool is_better_dm_peak(const DmPeak& lhs, const DmPeak& rhs);
 

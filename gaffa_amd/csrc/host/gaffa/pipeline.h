@@ -159,6 +159,7 @@ class GpuSearchPipeline {
   std::vector<FfaStageMarks> marks_;  // one per tile, events created on demand
   std::size_t marks_used_ = 0;
   GpuDeviceBuffer<float> periodogram_;
+  GpuPinnedBuffer compact_host_;  // compact peaks read back after a search
   std::vector<double> trial_periods_;
   std::vector<std::uint32_t> trial_order_;
 };

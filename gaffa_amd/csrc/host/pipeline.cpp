@@ -233,8 +233,13 @@ void GpuSearchPipeline::search() {
     if (count <= prog.peak_capacity()) break;
     prog.grow_peaks(count);
   }
-  const std::vector<gaffa_hip_ffa_peak> compact = prog.read_peaks(count);
-  peaks_ = expand_sorted_dm_peaks(prog.execution_plan(), compact, dms_);
+  // compact peaks come back through a pinned buffer kept across searches
+  const std::size_t bytes = std::max<std::size_t>(count, 1) * sizeof(gaffa_hip_ffa_peak);
+  if (compact_host_.bytes() < bytes) compact_host_ = GpuPinnedBuffer(bytes + bytes / 4);
+  auto* compact = static_cast<gaffa_hip_ffa_peak*>(compact_host_.data());
+  const std::size_t n = prog.read_peaks_into(count, compact);
+  expand_sorted_dm_peaks_into(prog.execution_plan(),
+                              std::span<const gaffa_hip_ffa_peak>(compact, n), dms_, peaks_);
 }
 
 StageTimes GpuSearchPipeline::run() {
