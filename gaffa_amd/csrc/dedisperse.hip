@@ -641,6 +641,59 @@ spectrum_kernel(const T* __restrict__ x, uint64_t nsamples, uint64_t nchans, uin
   }
 }
 
+// Aligned spectrum without staging, for channel counts that fill whole
+// 32-bit words: one thread per output word (4 u8, 2 u16 or 1 32-bit
+// channel) over a run of output rows; each element is read from its
+// channel's delayed row (0 past the end, as gaffa's reader) and the word is
+// stored whole.  Lanes of a wave take adjacent words, so a wave reads
+// adjacent channels of nearly the same rows.
+constexpr uint32_t kSpecRows = 64;  // output rows per block
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+spectrum_direct_kernel(const T* __restrict__ x, uint64_t nsamples, uint64_t nchans,
+                       uint64_t chan_begin, uint32_t nsel, const int32_t* __restrict__ delays,
+                       uint32_t* __restrict__ out, uint64_t out_nsamples) {
+  constexpr uint32_t E = 4 / sizeof(T);
+  const uint32_t words = nsel / E;
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kSpecRows;
+  const uint64_t t1 = min(out_nsamples, t0 + kSpecRows);
+  for (uint32_t w = threadIdx.x; w < words; w += kThreads) {
+    uint64_t d[E];
+#pragma unroll
+    for (uint32_t j = 0; j < E; ++j) d[j] = static_cast<uint64_t>(delays[w * E + j]);
+    const T* col = x + chan_begin + static_cast<uint64_t>(w) * E;
+    uint64_t t = t0;
+    for (; t + 4 <= t1; t += 4) {
+      T v[4][E];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) {
+          const uint64_t r = t + u + d[j];
+          v[u][j] = r < nsamples ? col[min(r, nsamples - 1) * nchans + j] : T(0);
+        }
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        uint32_t packed;
+        __builtin_memcpy(&packed, v[u], 4);
+        out[(t + u) * words + w] = packed;
+      }
+    }
+    for (; t < t1; ++t) {
+      T v[E];
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j) {
+        const uint64_t r = t + d[j];
+        v[j] = r < nsamples ? col[min(r, nsamples - 1) * nchans + j] : T(0);
+      }
+      uint32_t packed;
+      __builtin_memcpy(&packed, v, 4);
+      out[t * words + w] = packed;
+    }
+  }
+}
+
 }  // namespace
 }  // namespace gaffa_hip
 
@@ -659,6 +712,34 @@ extern "C" int gaffa_hip_dedisperse_spectrum(const gaffa_hip_spectrum_args* a,
     check(a->max_tile_spread >= 0, "dedispersion requires non-negative delays");
     const uint64_t nsel = a->chan_end - a->chan_begin;
     check_overflow(nsel <= 0xFFFFFFFFull, "dedispersion channel range too large");
+    const hipStream_t st = as_stream(stream);
+    if ((nsel * a->element_size) % 4 == 0 && (reinterpret_cast<uintptr_t>(a->out) & 3u) == 0) {
+      const dim3 grid(static_cast<uint32_t>(ceil_div(a->out_nsamples, kSpecRows)));
+      check_overflow(ceil_div(a->out_nsamples, kSpecRows) < 0x7FFFFFFFull,
+                     "spectrum grid too large");
+      auto* o = static_cast<uint32_t*>(a->out);
+      switch (a->element_size) {
+        case 1:
+          hipLaunchKernelGGL(spectrum_direct_kernel<uint8_t>, grid, dim3(kThreads), 0, st,
+                             static_cast<const uint8_t*>(a->samples), a->nsamples, a->nchans,
+                             a->chan_begin, static_cast<uint32_t>(nsel), a->delays, o,
+                             a->out_nsamples);
+          break;
+        case 2:
+          hipLaunchKernelGGL(spectrum_direct_kernel<uint16_t>, grid, dim3(kThreads), 0, st,
+                             static_cast<const uint16_t*>(a->samples), a->nsamples, a->nchans,
+                             a->chan_begin, static_cast<uint32_t>(nsel), a->delays, o,
+                             a->out_nsamples);
+          break;
+        default:
+          hipLaunchKernelGGL(spectrum_direct_kernel<uint32_t>, grid, dim3(kThreads), 0, st,
+                             static_cast<const uint32_t*>(a->samples), a->nsamples, a->nchans,
+                             a->chan_begin, static_cast<uint32_t>(nsel), a->delays, o,
+                             a->out_nsamples);
+      }
+      launch_check("spectrum_direct_kernel");
+      return;
+    }
     // widest tile (time x channels) whose slab fits 96 KiB
     const uint32_t ctile = a->channel_tile;
     check(ctile == 8 || ctile == 16 || ctile == 32 || ctile == 64 || ctile == 128 || ctile == 256,
@@ -677,7 +758,6 @@ extern "C" int gaffa_hip_dedisperse_spectrum(const gaffa_hip_spectrum_args* a,
     check_overflow(tiles < 0x7FFFFFFFull && ctiles < 65536, "spectrum grid too large");
     const uint32_t lds = slab_rows * ctile * a->element_size;
     const dim3 grid(static_cast<uint32_t>(tiles), static_cast<uint32_t>(ctiles));
-    const hipStream_t st = as_stream(stream);
     int32_t* flag = error_flag_for_device();
     switch (a->element_size) {
       case 1:

@@ -543,6 +543,12 @@ def test_spectrum_bit_exact(gpu, port, dtype):
     np.testing.assert_array_equal(
         sub.data, port.dedisperse_spectrum(x, freq, 250.0, 64e-6, ref=freq.max(), cb=100,
                                            ce=612))
+    # 513 channels fill no whole 32-bit words for 8/16-bit data: the staged path
+    odd = gpu._dedisperse_spectrum_array(x, frequency_mhz=list(freq), tsamp=64e-6, dm=250.0,
+                                         chan_begin=100, chan_end=613)
+    np.testing.assert_array_equal(
+        odd.data, port.dedisperse_spectrum(x, freq, 250.0, 64e-6, ref=freq.max(), cb=100,
+                                           ce=613))
 
 
 def test_spectrum_filterbank_and_fold(gpu, ref):
