@@ -641,6 +641,70 @@ spectrum_kernel(const T* __restrict__ x, uint64_t nsamples, uint64_t nchans, uin
   }
 }
 
+// Aligned spectrum staged through LDS in 32-byte channel tiles (32 u8, 16
+// u16 or 8 32-bit channels): block (channel tile x, time tile y) loads the
+// rows [t0 + dmin, t0 + tile + dmax) of its tile with 16-byte loads, then
+// writes each output row's 32 bytes as eight words gathered from the rows at
+// each channel's delay.  Channel tiles are the fastest grid index, so the
+// blocks of one time tile run together and share the input rows in L2.
+constexpr uint32_t kSpecTileRows = 1024;  // output rows per block
+
+template <typename T>
+__global__ void __launch_bounds__(kThreads)
+spectrum_tiled_kernel(const T* __restrict__ x, uint64_t nsamples, uint64_t nchans,
+                      uint64_t chan_begin, const int32_t* __restrict__ delays,
+                      uint32_t* __restrict__ out, uint64_t out_nsamples, uint32_t out_words,
+                      uint32_t slab_rows, int32_t* __restrict__ error_flag) {
+  constexpr uint32_t CT = 32 / sizeof(T);  // channels per tile
+  constexpr uint32_t E = 4 / sizeof(T);    // channels per word
+  extern __shared__ uint4 slab4[];         // [slab_rows][2] x 16 bytes
+  __shared__ int32_t sdel[CT];
+  __shared__ int32_t dmin_s, dmax_s;
+  const uint32_t c0 = blockIdx.x * CT;
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.y) * kSpecTileRows;
+  if (threadIdx.x < CT) sdel[threadIdx.x] = delays[c0 + threadIdx.x];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int32_t lo = sdel[0], hi = sdel[0];
+    for (uint32_t c = 1; c < CT; ++c) {
+      lo = min(lo, sdel[c]);
+      hi = max(hi, sdel[c]);
+    }
+    dmin_s = lo;
+    dmax_s = hi;
+  }
+  __syncthreads();
+  const int32_t dmin = dmin_s;
+  const uint32_t rows = kSpecTileRows + static_cast<uint32_t>(dmax_s - dmin);
+  if (rows > slab_rows) {  // host/kernel table mismatch: refuse the block
+    if (threadIdx.x == 0) atomicExch(error_flag, 1);
+    return;
+  }
+  const T* src = x + chan_begin + c0;
+  for (uint32_t e = threadIdx.x; e < 2 * rows; e += kThreads) {
+    const uint64_t t = t0 + static_cast<uint64_t>(dmin) + (e >> 1);
+    uint4 v = make_uint4(0, 0, 0, 0);
+    if (t < nsamples)
+      v = *reinterpret_cast<const uint4*>(src + t * nchans + (e & 1u) * (CT / 2));
+    slab4[e] = v;
+  }
+  __syncthreads();
+  const T* slab = reinterpret_cast<const T*>(slab4);
+  const uint32_t tcount = static_cast<uint32_t>(min<uint64_t>(kSpecTileRows, out_nsamples - t0));
+  for (uint32_t e = threadIdx.x; e < tcount * 8u; e += kThreads) {
+    const uint32_t tl = e >> 3, w = e & 7u;
+    T v[E];
+#pragma unroll
+    for (uint32_t j = 0; j < E; ++j) {
+      const uint32_t c = w * E + j;
+      v[j] = slab[(tl + static_cast<uint32_t>(sdel[c] - dmin)) * CT + c];
+    }
+    uint32_t packed;
+    __builtin_memcpy(&packed, v, 4);
+    out[(t0 + tl) * out_words + blockIdx.x * 8u + w] = packed;
+  }
+}
+
 // Aligned spectrum without staging, for channel counts that fill whole
 // 32-bit words: one thread per output word (4 u8, 2 u16 or 1 32-bit
 // channel) over a run of output rows; each element is read from its
@@ -713,6 +777,45 @@ extern "C" int gaffa_hip_dedisperse_spectrum(const gaffa_hip_spectrum_args* a,
     const uint64_t nsel = a->chan_end - a->chan_begin;
     check_overflow(nsel <= 0xFFFFFFFFull, "dedispersion channel range too large");
     const hipStream_t st = as_stream(stream);
+    const uint32_t ct32 = 32u / a->element_size;  // channels per 32-byte tile
+    if ((nsel * a->element_size) % 32 == 0 && (a->nchans * a->element_size) % 16 == 0 &&
+        (a->chan_begin * a->element_size) % 16 == 0 &&
+        (reinterpret_cast<uintptr_t>(a->samples) & 15u) == 0 &&
+        (reinterpret_cast<uintptr_t>(a->out) & 3u) == 0 && a->channel_tile >= ct32 &&
+        a->channel_tile % ct32 == 0) {
+      // every 32-byte tile lies inside one channel_tile tile: its spread is bounded
+      const uint64_t slab_rows = kSpecTileRows + static_cast<uint64_t>(a->max_tile_spread);
+      const uint64_t lds = slab_rows * 32;
+      if (lds <= 64 * 1024) {
+        const uint64_t ttiles = ceil_div(a->out_nsamples, kSpecTileRows);
+        check_overflow(ttiles < 65536, "spectrum grid too large");
+        const dim3 grid(static_cast<uint32_t>(nsel / ct32), static_cast<uint32_t>(ttiles));
+        auto* o = static_cast<uint32_t*>(a->out);
+        const uint32_t words = static_cast<uint32_t>(nsel * a->element_size / 4);
+        int32_t* flag = error_flag_for_device();
+        switch (a->element_size) {
+          case 1:
+            hipLaunchKernelGGL(spectrum_tiled_kernel<uint8_t>, grid, dim3(kThreads), lds, st,
+                               static_cast<const uint8_t*>(a->samples), a->nsamples, a->nchans,
+                               a->chan_begin, a->delays, o, a->out_nsamples, words,
+                               static_cast<uint32_t>(slab_rows), flag);
+            break;
+          case 2:
+            hipLaunchKernelGGL(spectrum_tiled_kernel<uint16_t>, grid, dim3(kThreads), lds, st,
+                               static_cast<const uint16_t*>(a->samples), a->nsamples, a->nchans,
+                               a->chan_begin, a->delays, o, a->out_nsamples, words,
+                               static_cast<uint32_t>(slab_rows), flag);
+            break;
+          default:
+            hipLaunchKernelGGL(spectrum_tiled_kernel<uint32_t>, grid, dim3(kThreads), lds, st,
+                               static_cast<const uint32_t*>(a->samples), a->nsamples, a->nchans,
+                               a->chan_begin, a->delays, o, a->out_nsamples, words,
+                               static_cast<uint32_t>(slab_rows), flag);
+        }
+        launch_check("spectrum_tiled_kernel");
+        return;
+      }
+    }
     if ((nsel * a->element_size) % 4 == 0 && (reinterpret_cast<uintptr_t>(a->out) & 3u) == 0) {
       const dim3 grid(static_cast<uint32_t>(ceil_div(a->out_nsamples, kSpecRows)));
       check_overflow(ceil_div(a->out_nsamples, kSpecRows) < 0x7FFFFFFFull,
