@@ -347,7 +347,7 @@ def main() -> int:
     units = ndm * n_out
     value = units / (wall / args.steps)
     stages = {k: v / args.steps for k, v in stage_sum.items()}
-    npeaks = len(pipe.peaks)
+    npeaks = pipe.peak_count
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
 

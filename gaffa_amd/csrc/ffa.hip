@@ -332,6 +332,7 @@ ffa_prepare_fused_kernel(const float* __restrict__ series, uint64_t series_strid
 constexpr uint32_t kSubtreePrefetch = 20;  // floats per thread: node <= 20480 floats
 constexpr uint32_t kCopyTail = 0xFFFFu;    // descriptor tail of a copied row
 constexpr uint32_t kSubtreeRowsPerPass = 2;  // rows a wave merges at once (4 phases each)
+constexpr uint32_t kSubtreeLocalDepth = 4;   // levels below it run wave by wave
 
 // Packed merge descriptor: x = head | tail << 16 (tail kCopyTail = copy the
 // row), y = roll.
@@ -413,8 +414,10 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     // lane past the row end recomputes phase b - 1 and a wave short of rows
     // recomputes row nrows - 1, storing the very value another lane stores
     // there.
-    auto level = [&](const uint2* dd, const float* src, auto&& store) {
-      if (wave >= nrows) return;
+    // Rows r_first, r_first + r_step, ... below r_end (wave-uniform).
+    auto level = [&](const uint2* dd, const float* src, auto&& store, uint32_t r_first,
+                     uint32_t r_end, uint32_t r_step) {
+      if (r_first >= r_end) return;
       constexpr uint32_t R = kSubtreeRowsPerPass;
       for (uint32_t p0 = 0; p0 < b; p0 += 256) {
         uint32_t pc[4];
@@ -422,8 +425,8 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
         for (uint32_t j = 0; j < 4; ++j) pc[j] = min(p0 + lane + 64u * j, b - 1);
         uint2 e[R];
 #pragma unroll
-        for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(wave + k * kWaves, nrows - 1)];
-        for (uint32_t r0 = wave; r0 < nrows; r0 += R * kWaves) {
+        for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(r_first + k * r_step, r_end - 1)];
+        for (uint32_t r0 = r_first; r0 < r_end; r0 += R * r_step) {
           uint32_t hb[R], tb[R], ro[R], ob[R];
           bool cp[R];
 #pragma unroll
@@ -434,12 +437,12 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
             cp[k] = (x >> 16) == kCopyTail;
             tb[k] = cp[k] ? hb[k] : (x >> 16) * b;
             ro[k] = cp[k] ? 0u : y;
-            ob[k] = min(r0 + k * kWaves, nrows - 1) * b;
+            ob[k] = min(r0 + k * r_step, r_end - 1) * b;
           }
-          const uint32_t n0 = r0 + R * kWaves;
-          if (n0 < nrows) {
+          const uint32_t n0 = r0 + R * r_step;
+          if (n0 < r_end) {
 #pragma unroll
-            for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(n0 + k * kWaves, nrows - 1)];
+            for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(n0 + k * r_step, r_end - 1)];
           }
           float v[R][4];
 #pragma unroll
@@ -460,15 +463,49 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
         }
       }
     };
-    for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 1; --d) {
+    // The deepest levels need no workgroup barrier: each wave takes whole
+    // nodes of depth `dl` (16 of them for kWaves = 16) and merges every
+    // level inside them on its own, ordering its own LDS accesses only.
+    const uint32_t dl = levels > kSubtreeLocalDepth ? kSubtreeLocalDepth : levels;
+    if (dl < levels) {
+      for (uint32_t j = wave; j < (1u << dl); j += kWaves) {
+        uint32_t ns = 0, nz = nrows;  // node j of depth dl: rows [ns, ns + nz)
+        for (uint32_t bit = dl; bit-- > 0;) {
+          const uint32_t h = nz / 2;
+          if ((j >> bit) & 1u) {
+            ns += h;
+            nz -= h;
+          } else {
+            nz = h;
+          }
+        }
+        float* c = cur;
+        float* n = nxt;
+        for (uint32_t d = levels - 1; d >= dl; --d) {
+          level(desc + d * nrows, c, [&](uint32_t i, float v) { n[i] = v; }, ns, ns + nz, 1u);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          float* t = c;
+          c = n;
+          n = t;
+        }
+      }
+      if ((levels - dl) & 1u) {
+        float* t = cur;
+        cur = nxt;
+        nxt = t;
+      }
+      __syncthreads();
+    }
+    for (int32_t d = static_cast<int32_t>(dl) - 1; d >= 1; --d) {
       float* dst = nxt;
       level(desc + static_cast<uint32_t>(d) * nrows, cur,
-            [&](uint32_t i, float v) { dst[i] = v; });
+            [&](uint32_t i, float v) { dst[i] = v; }, wave, nrows, kWaves);
       __syncthreads();
       nxt = cur;
       cur = dst;
     }
-    if (levels > 0) level(desc, cur, [&](uint32_t i, float v) { out[i] = v; });
+    if (levels > 0)
+      level(desc, cur, [&](uint32_t i, float v) { out[i] = v; }, wave, nrows, kWaves);
     __syncthreads();  // LDS rows and descriptors are rewritten by the next unit
   }
 }

@@ -1371,6 +1371,8 @@ PYBIND11_MODULE(_core, m) {
       .def_property_readonly("dms", &GpuSearchPipeline::dms)
       .def_property_readonly("ffa_plan", &GpuSearchPipeline::ffa_plan)
       .def_property_readonly("peaks", &GpuSearchPipeline::peaks)
+      .def_property_readonly("peak_count",
+                             [](const GpuSearchPipeline& p) { return p.peaks().size(); })
       .def_property_readonly("ffa_tile", [](const GpuSearchPipeline& p) {
         return p.ffa_program().tile_capacity();
       })
