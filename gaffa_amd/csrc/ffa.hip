@@ -332,7 +332,6 @@ ffa_prepare_fused_kernel(const float* __restrict__ series, uint64_t series_strid
 constexpr uint32_t kSubtreePrefetch = 20;  // floats per thread: node <= 20480 floats
 constexpr uint32_t kCopyTail = 0xFFFFu;    // descriptor tail of a copied row
 constexpr uint32_t kSubtreeRowsPerPass = 2;  // rows a wave merges at once (4 phases each)
-constexpr uint32_t kSubtreeLocalDepth = 4;   // levels below it run wave by wave
 
 // Packed merge descriptor: x = head | tail << 16 (tail kCopyTail = copy the
 // row), y = roll.
@@ -463,40 +462,7 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
         }
       }
     };
-    // The deepest levels need no workgroup barrier: each wave takes whole
-    // nodes of depth `dl` (16 of them for kWaves = 16) and merges every
-    // level inside them on its own, ordering its own LDS accesses only.
-    const uint32_t dl = levels > kSubtreeLocalDepth ? kSubtreeLocalDepth : levels;
-    if (dl < levels) {
-      for (uint32_t j = wave; j < (1u << dl); j += kWaves) {
-        uint32_t ns = 0, nz = nrows;  // node j of depth dl: rows [ns, ns + nz)
-        for (uint32_t bit = dl; bit-- > 0;) {
-          const uint32_t h = nz / 2;
-          if ((j >> bit) & 1u) {
-            ns += h;
-            nz -= h;
-          } else {
-            nz = h;
-          }
-        }
-        float* c = cur;
-        float* n = nxt;
-        for (uint32_t d = levels - 1; d >= dl; --d) {
-          level(desc + d * nrows, c, [&](uint32_t i, float v) { n[i] = v; }, ns, ns + nz, 1u);
-          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-          float* t = c;
-          c = n;
-          n = t;
-        }
-      }
-      if ((levels - dl) & 1u) {
-        float* t = cur;
-        cur = nxt;
-        nxt = t;
-      }
-      __syncthreads();
-    }
-    for (int32_t d = static_cast<int32_t>(dl) - 1; d >= 1; --d) {
+    for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 1; --d) {
       float* dst = nxt;
       level(desc + static_cast<uint32_t>(d) * nrows, cur,
             [&](uint32_t i, float v) { dst[i] = v; }, wave, nrows, kWaves);
