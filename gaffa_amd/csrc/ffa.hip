@@ -28,6 +28,7 @@
 #include "common.hpp"
 
 #include <cmath>
+#include <type_traits>
 
 namespace gaffa_hip {
 namespace {
@@ -35,7 +36,7 @@ namespace {
 constexpr uint32_t kLevelThreads = 256;
 constexpr uint32_t kLevelRows = 16;  // rows per level work item (host side too)
 constexpr uint32_t kSubtreeThreads = 1024;
-constexpr uint32_t kSubtreeMaxRows = 512;  // 16-bit row fields in the descriptors
+constexpr uint32_t kSubtreeMaxRows = 512;  // rows of a subtree node
 constexpr uint32_t kTerminalThreads = 256;
 constexpr uint32_t kTerminalRows = 32;  // rows per terminal work item (host side too)
 constexpr uint32_t kPrepThreads = 256;
@@ -44,7 +45,7 @@ constexpr uint32_t kMaxLdsBytes = 160 * 1024;
 
 // ffa.cpp:133-143 in FP32, no contraction (the library is built with
 // -ffp-contract=off).
-__device__ __forceinline__ uint32_t pick_row(uint32_t child_rows, uint32_t rows,
+__host__ __device__ __forceinline__ uint32_t pick_row(uint32_t child_rows, uint32_t rows,
                                              uint32_t s) {
   const float scale = static_cast<float>(child_rows - 1) / static_cast<float>(rows - 1);
   return static_cast<uint32_t>(scale * static_cast<float>(s) + 0.5f);
@@ -98,8 +99,8 @@ struct Node {
 
 // The node at `depth` below a node (start, size) that contains row r, or the
 // leaf above it if the recursion stops first (size 1).
-__device__ __forceinline__ Node find_node(uint32_t start, uint32_t size, uint32_t depth,
-                                          uint32_t r) {
+__host__ __device__ __forceinline__ Node find_node(uint32_t start, uint32_t size,
+                                                   uint32_t depth, uint32_t r) {
   for (uint32_t d = 0; d < depth && size > 1; ++d) {
     const uint32_t h = size / 2;
     if (r < start + h) {
@@ -120,7 +121,7 @@ struct Desc {
   uint32_t roll;
 };
 
-__device__ __forceinline__ Desc merge_desc(Node nd, uint32_t r, uint32_t bins) {
+__host__ __device__ __forceinline__ Desc merge_desc(Node nd, uint32_t r, uint32_t bins) {
   const uint32_t s = r - nd.start;
   const uint32_t hn = nd.size / 2;
   const uint32_t tn = nd.size - hn;
@@ -326,25 +327,50 @@ ffa_prepare_fused_kernel(const float* __restrict__ series, uint64_t series_strid
 // launch, unit u = (subtree item u % nitems, series u / nitems), so that the
 // units in flight together share a series' prepared data in L2.  For each
 // unit: the node's rows, prefetched into registers while the previous unit
-// computed, are written to LDS; the merge descriptors of every level are
-// built in one pass; the levels run in LDS (ping-pong), the top level writing
-// straight to work[depth % 2].  LDS = [2][half] floats + descriptor table.
+// computed, are written to LDS; the levels run in LDS (ping-pong), the top
+// level writing straight to the task's work region.  The merges come from
+// the item's host-built table (gaffa_hip_ffa_merge), read with scalar loads
+// one pass ahead.  LDS = zero rows (subtree_zero_floats(max_bins) floats of
+// -0.0), [2][half] floats of rows, kSubtreeGuard floats of slack.  The zero
+// rows are the tail of a copied row (x + -0.0f == x for every x) and keep
+// the merge's tail base addresses positive; the slack holds the last-chunk
+// overreads.
 constexpr uint32_t kSubtreePrefetch = 20;  // floats per thread: node <= 20480 floats
-constexpr uint32_t kCopyTail = 0xFFFFu;    // descriptor tail of a copied row
-constexpr uint32_t kSubtreeRowsPerPass = 2;  // rows a wave merges at once (4 phases each)
+constexpr uint32_t kSubtreeGuard = 256;    // floats of slack after the row buffers
+constexpr uint32_t kSubtreeRowsPerPass = 2;  // rows a wave merges at once (table layout too)
+constexpr uint32_t kSubtreeWaves = kSubtreeThreads / 64;
 
-// Packed merge descriptor: x = head | tail << 16 (tail kCopyTail = copy the
-// row), y = roll.
-__device__ __forceinline__ uint2 pack_desc(Node nd, uint32_t r, uint32_t b) {
-  if (nd.size < 2) return make_uint2(r | (kCopyTail << 16), 0u);
-  const Desc d = merge_desc(nd, r, b);
-  return make_uint2(d.h | (d.t << 16), d.roll);
-}
-
-__device__ __forceinline__ uint32_t tree_levels(uint32_t nrows) {
+__host__ __device__ __forceinline__ uint32_t tree_levels(uint32_t nrows) {
   uint32_t levels = 0;
   for (uint32_t sz = nrows; sz > 1; sz = sz - sz / 2) ++levels;  // ceil(log2(nrows))
   return levels;
+}
+
+// Floats of -0.0 in front of the row buffers: a copied row's tail reads
+// phases up to bins + 62, and a tail base may sit 192 floats before its row.
+__host__ __device__ __forceinline__ uint32_t subtree_zero_floats(uint32_t max_bins) {
+  const uint32_t need = max_bins + 64u > 256u ? max_bins + 64u : 256u;
+  return (need + 63u) & ~63u;
+}
+
+// Passes per wave per level of a node of `rows` rows: wave w merges rows w,
+// w + 16, ..., kSubtreeRowsPerPass of them per pass.
+__host__ __device__ __forceinline__ uint32_t subtree_passes(uint32_t rows) {
+  const uint32_t per_wave = (rows + kSubtreeWaves - 1) / kSubtreeWaves;
+  return (per_wave + kSubtreeRowsPerPass - 1) / kSubtreeRowsPerPass;
+}
+
+using ConstMerge = __attribute__((address_space(4))) const gaffa_hip_ffa_merge;
+
+// A merge read through the constant address space: a uniform address, so
+// scalar loads.
+__device__ __forceinline__ gaffa_hip_ffa_merge load_merge(ConstMerge* p) {
+  gaffa_hip_ffa_merge g;
+  g.head = p->head;
+  g.tail = p->tail;
+  g.wrap = p->wrap;
+  g.out = p->out;
+  return g;
 }
 
 __global__ void __launch_bounds__(kSubtreeThreads)
@@ -354,18 +380,35 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform
-  constexpr uint32_t kWaves = kSubtreeThreads / 64;
+  constexpr uint32_t kWaves = kSubtreeWaves;
   const uint32_t half = plan.subtree_half_floats;
-  uint2* desc = reinterpret_cast<uint2*>(lds + 2 * static_cast<size_t>(half));
+  const uint32_t zero_floats = subtree_zero_floats(plan.max_bins);
+  float* const rows = lds + zero_floats;
   const uint32_t nitems = plan.nsubtree_items;
+  ConstMerge* const merges = (ConstMerge*)plan.subtree_merges;
+  // The plan tables are read through the constant address space (scalar
+  // loads), so that the only vector loads in flight are the prefetched rows
+  // and no wait for plan fields drains them early.
+  using ConstItem = __attribute__((address_space(4))) const gaffa_hip_ffa_item;
+  using ConstTask = __attribute__((address_space(4))) const gaffa_hip_ffa_task;
+  using ConstGroup = __attribute__((address_space(4))) const gaffa_hip_ffa_group;
+  ConstItem* const items = (ConstItem*)plan.subtree_items;
+  ConstTask* const tasks = (ConstTask*)plan.tasks;
+  ConstGroup* const groups = (ConstGroup*)plan.groups;
+
+  for (uint32_t e = tid; e < zero_floats; e += kSubtreeThreads) lds[e] = -0.0f;
 
   float pf[kSubtreePrefetch];
   auto fetch = [&](uint32_t u) {
-    const gaffa_hip_ffa_item it = plan.subtree_items[u % nitems];
-    const gaffa_hip_ffa_task& t = plan.tasks[it.task];
-    const float* in = task_input(a, plan.groups[t.group], u / nitems) +
-                      static_cast<uint64_t>(it.a) * t.bins;
-    const uint32_t total = it.b * t.bins;  // >= 1
+    ConstItem* it = items + u % nitems;
+    ConstTask* t = tasks + it->task;
+    ConstGroup* g = groups + t->group;
+    const uint64_t series = u / nitems;
+    const float* in = (g->factor == 1.0
+                           ? a.series + series * a.series_stride + g->input_offset
+                           : a.prepared + series * plan.prepared_stride + g->prep_offset) +
+                      static_cast<uint64_t>(it->a) * t->bins;
+    const uint32_t total = it->b * t->bins;  // >= 1
 #pragma unroll
     for (uint32_t k = 0; k < kSubtreePrefetch; ++k)
       pf[k] = in[min(tid + k * kSubtreeThreads, total - 1)];
@@ -375,104 +418,118 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
   if (u >= nunits) return;
   fetch(u);
   for (; u < nunits; u += gridDim.x) {
-    const gaffa_hip_ffa_item item = plan.subtree_items[u % nitems];
-    const gaffa_hip_ffa_task& task = plan.tasks[item.task];
+    ConstItem* const itp = items + u % nitems;
+    ConstTask* const tp = tasks + itp->task;
+    const uint32_t node_start = itp->a, nrows = itp->b, first_merge = itp->c;
+    const uint32_t b = tp->bins;
+    const uint32_t region = (tp->regions >> tp->depth) & 1u;
+    const uint64_t work_offset = tp->work_offset;
     const uint64_t sidx = u / nitems;
-    const uint32_t b = task.bins;
-    const uint32_t nrows = item.b;
     const uint32_t total = nrows * b;
-    float* cur = lds;
-    float* nxt = lds + half;
+    float* cur = rows;
+    float* nxt = rows + half;
 #pragma unroll
     for (uint32_t k = 0; k < kSubtreePrefetch; ++k) {
       const uint32_t e = tid + k * kSubtreeThreads;
       if (e < total) cur[e] = pf[k];
     }
     const uint32_t levels = tree_levels(nrows);
-    for (uint32_t e = tid; e < levels * nrows; e += kSubtreeThreads) {
-      const uint32_t d = e / nrows, r = e - d * nrows;
-      desc[e] = pack_desc(find_node(0, nrows, d, r), r, b);
-    }
     __syncthreads();
     // the next unit's rows travel while this one computes
     if (u + gridDim.x < nunits) fetch(u + gridDim.x);
 
-    float* out = a.work +
-                 static_cast<uint64_t>((task.regions >> task.depth) & 1u) * a.nseries *
-                     plan.work_stride +
-                 sidx * plan.work_stride + task.work_offset + static_cast<uint64_t>(item.a) * b;
+    float* out = a.work + static_cast<uint64_t>(region) * a.nseries * plan.work_stride +
+                 sidx * plan.work_stride + work_offset + static_cast<uint64_t>(node_start) * b;
     if (levels == 0) {
       for (uint32_t e = tid; e < total; e += kSubtreeThreads) out[e] = cur[e];
     }
-    // One level: rows of `src` (LDS) merged into `dst`, written once per
-    // destination kind so every access keeps its address space (LDS rows
-    // are ds_* operations, the top level's stores global_*).  Lanes cover
-    // 4 phases each per 256-phase pass; each wave merges
-    // kSubtreeRowsPerPass rows at a time.  Descriptors are wave-uniform
-    // (scalar row math) and read one batch ahead.  No store is predicated: a
-    // lane past the row end recomputes phase b - 1 and a wave short of rows
-    // recomputes row nrows - 1, storing the very value another lane stores
-    // there.
-    // Rows r_first, r_first + r_step, ... below r_end (wave-uniform).
-    auto level = [&](const uint2* dd, const float* src, auto&& store, uint32_t r_first,
-                     uint32_t r_end, uint32_t r_step) {
-      if (r_first >= r_end) return;
+    // One level: the wave's merges m[0 .. R * passes) of `src` (LDS) into
+    // `store`, which is written once per destination kind so every access
+    // keeps its address space (LDS rows are ds_* operations, the top level's
+    // stores global_*).  The table gives each wave its rows w, w + 16, ...,
+    // padded with repeats of its last row to whole passes.  A row is swept in
+    // blocks of 256 phases, NC <= 4 chunks of 64 lanes.  The roll splits it at
+    // w = bins - roll: phases p < w read tail[p + roll], phases p >= w read
+    // tail[p + roll - b].  Per chunk a lane picks one of two tail addresses
+    // (one compare, one select); every other address is one add per row plus
+    // the chunk's immediate offset.  The last chunk of a row loads up to 63
+    // floats past it (inside the next row or the LDS slack) and masks only
+    // its stores.  A copied row's tail is the zero rows.
+    const uint32_t passes = subtree_passes(nrows);
+    auto level = [&](ConstMerge* m, const float* src, auto&& store) {
+      if (wave >= nrows) return;
       constexpr uint32_t R = kSubtreeRowsPerPass;
       for (uint32_t p0 = 0; p0 < b; p0 += 256) {
-        uint32_t pc[4];
+        const uint32_t rem = b - p0;
+        const uint32_t nc = rem >= 256 ? 4u : (rem + 63) >> 6;
+        const uint32_t live = min(rem - 64u * (nc - 1), 64u);  // lanes of the last chunk
+        auto pass = [&](auto ncc) {
+          constexpr uint32_t NC = decltype(ncc)::value;
+          using LdsFloat = __attribute__((address_space(3))) const float;
+          auto load_row = [&](const gaffa_hip_ffa_merge& g, float (&hv)[NC], float (&tv)[NC]) {
+            const uint32_t before = g.wrap - min(g.wrap, p0);  // phases of the block before w
+            const float* hp = src + g.head + p0 + lane;
+            const float* tbase = g.tail == GAFFA_HIP_FFA_COPY_ROW ? lds : src + g.tail;
+            LdsFloat* ta = (LdsFloat*)(tbase + p0 + lane);
+            // tb[64 j] lies inside the tail row for every lane that reads it
+            // (p >= w); tb itself may sit up to 192 floats before the row,
+            // inside the zero rows.  The empty asm keeps ta and tb two
+            // registers, so that each chunk's choice is one select rather
+            // than a rebuilt address.
+            LdsFloat* tb = ta - b;
+            asm("" : "+v"(tb));
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) pc[j] = min(p0 + lane + 64u * j, b - 1);
-        uint2 e[R];
-#pragma unroll
-        for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(r_first + k * r_step, r_end - 1)];
-        for (uint32_t r0 = r_first; r0 < r_end; r0 += R * r_step) {
-          uint32_t hb[R], tb[R], ro[R], ob[R];
-          bool cp[R];
-#pragma unroll
-          for (uint32_t k = 0; k < R; ++k) {
-            const uint32_t x = __builtin_amdgcn_readfirstlane(e[k].x);
-            const uint32_t y = __builtin_amdgcn_readfirstlane(e[k].y);
-            hb[k] = (x & 0xFFFFu) * b;
-            cp[k] = (x >> 16) == kCopyTail;
-            tb[k] = cp[k] ? hb[k] : (x >> 16) * b;
-            ro[k] = cp[k] ? 0u : y;
-            ob[k] = min(r0 + k * r_step, r_end - 1) * b;
-          }
-          const uint32_t n0 = r0 + R * r_step;
-          if (n0 < r_end) {
-#pragma unroll
-            for (uint32_t k = 0; k < R; ++k) e[k] = dd[min(n0 + k * r_step, r_end - 1)];
-          }
-          float v[R][4];
-#pragma unroll
-          for (uint32_t k = 0; k < R; ++k) {
-#pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) {
-              uint32_t q = pc[j] + ro[k];
-              q = min(q, q - b);  // (p + roll) mod b for p, roll < b
-              const float h = src[hb[k] + pc[j]], t = src[tb[k] + q];
-              // a copied row (a leaf above this depth) keeps its own value
-              v[k][j] = cp[k] ? h : h + t;
+            for (uint32_t j = 0; j < NC; ++j) {
+              LdsFloat* tp = lane + 64u * j < before ? ta : tb;
+              hv[j] = hp[64u * j];
+              tv[j] = tp[64u * j];
             }
+          };
+          auto store_row = [&](uint32_t orow, const float (&hv)[NC], const float (&tv)[NC]) {
+#pragma unroll
+            for (uint32_t j = 0; j + 1 < NC; ++j)
+              store(orow + p0 + lane + 64u * j, hv[j] + tv[j]);
+            if (lane < live) store(orow + p0 + lane + 64u * (NC - 1), hv[NC - 1] + tv[NC - 1]);
+          };
+          gaffa_hip_ffa_merge g[R];
+#pragma unroll
+          for (uint32_t k = 0; k < R; ++k) g[k] = load_merge(m + k);
+          for (uint32_t ps = 0; ps < passes; ++ps) {
+            gaffa_hip_ffa_merge cg[R];
+#pragma unroll
+            for (uint32_t k = 0; k < R; ++k) cg[k] = g[k];
+            if (ps + 1 < passes) {
+#pragma unroll
+              for (uint32_t k = 0; k < R; ++k) g[k] = load_merge(m + (ps + 1) * R + k);
+            }
+            // every load of the pass is issued before the first add waits
+            float hv[R][NC], tv[R][NC];
+#pragma unroll
+            for (uint32_t k = 0; k < R; ++k) load_row(cg[k], hv[k], tv[k]);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (uint32_t k = 0; k < R; ++k) store_row(cg[k].out, hv[k], tv[k]);
           }
-#pragma unroll
-          for (uint32_t k = 0; k < R; ++k)
-#pragma unroll
-            for (uint32_t j = 0; j < 4; ++j) store(ob[k] + pc[j], v[k][j]);
-        }
+        };
+        if (nc == 4) pass(std::integral_constant<uint32_t, 4>{});
+        else if (nc == 3) pass(std::integral_constant<uint32_t, 3>{});
+        else if (nc == 2) pass(std::integral_constant<uint32_t, 2>{});
+        else pass(std::integral_constant<uint32_t, 1>{});
       }
     };
+    // level d of the table: kWaves x (R * passes) merges, wave-major
+    ConstMerge* const table = merges + first_merge + wave * kSubtreeRowsPerPass * passes;
+    const uint32_t level_size = kWaves * kSubtreeRowsPerPass * passes;
     for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 1; --d) {
       float* dst = nxt;
-      level(desc + static_cast<uint32_t>(d) * nrows, cur,
-            [&](uint32_t i, float v) { dst[i] = v; }, wave, nrows, kWaves);
+      level(table + static_cast<uint32_t>(d) * level_size, cur,
+            [&](uint32_t i, float v) { dst[i] = v; });
       __syncthreads();
       nxt = cur;
       cur = dst;
     }
-    if (levels > 0)
-      level(desc, cur, [&](uint32_t i, float v) { out[i] = v; }, wave, nrows, kWaves);
-    __syncthreads();  // LDS rows and descriptors are rewritten by the next unit
+    if (levels > 0) level(table, cur, [&](uint32_t i, float v) { out[i] = v; });
+    __syncthreads();  // the LDS rows are rewritten by the next unit
   }
 }
 
@@ -741,7 +798,9 @@ void validate(const gaffa_hip_ffa_search_args* a) {
   check(p.nsubtree_items == 0 ||
             (p.subtree_half_floats > 0 &&
              p.subtree_half_floats <= kSubtreePrefetch * kSubtreeThreads &&
-             8ull * p.subtree_half_floats + 8ull * p.max_subtree_desc <= p.subtree_lds_bytes),
+             (p.subtree_merges != nullptr || p.max_subtree_desc == 0) &&
+             8ull * p.subtree_half_floats + 4ull * kSubtreeGuard +
+                     4ull * subtree_zero_floats(p.max_bins) <= p.subtree_lds_bytes),
         "FFA subtree geometry inconsistent");
 }
 
@@ -940,13 +999,43 @@ uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {
 }
 
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins) {
-  // two row buffers + the descriptor table of every level
+  // two row buffers, the zero rows in front of them and the slack after
   if (rows == 0 || rows > kSubtreeMaxRows ||
       static_cast<uint64_t>(rows) * bins > kSubtreePrefetch * kSubtreeThreads)
     return 0xFFFFFFFFu;
-  uint32_t levels = 0;
-  for (uint32_t sz = rows; sz > 1; sz = sz - sz / 2) ++levels;
-  return 8u * rows * bins + 8u * levels * rows;
+  return 8u * rows * bins + 4u * kSubtreeGuard + 4u * subtree_zero_floats(bins);
+}
+
+uint32_t gaffa_hip_ffa_subtree_merges(uint32_t rows, uint32_t bins, gaffa_hip_ffa_merge* out) {
+  if (rows == 0 || rows > kSubtreeMaxRows || bins < 2 || bins > 65535) return 0;
+  const uint32_t levels = tree_levels(rows);
+  const uint32_t per_wave = kSubtreeRowsPerPass * subtree_passes(rows);
+  const uint32_t level_size = kSubtreeWaves * per_wave;
+  if (out == nullptr) return levels * level_size;
+  for (uint32_t d = 0; d < levels; ++d) {
+    for (uint32_t w = 0; w < kSubtreeWaves; ++w) {
+      for (uint32_t k = 0; k < per_wave; ++k) {
+        // row w + 16 k, or the wave's last row (row rows - 1 for a wave with
+        // none: its entries are never read)
+        uint32_t r = w + kSubtreeWaves * k;
+        if (r >= rows) r = w < rows ? w + (rows - 1 - w) / kSubtreeWaves * kSubtreeWaves : rows - 1;
+        const Node nd = find_node(0, rows, d, r);
+        gaffa_hip_ffa_merge& m = out[d * level_size + w * per_wave + k];
+        m.out = r * bins;
+        if (nd.size < 2) {  // a leaf above depth d: copied down unchanged
+          m.head = r * bins;
+          m.tail = GAFFA_HIP_FFA_COPY_ROW;
+          m.wrap = bins;
+        } else {
+          const Desc ds = merge_desc(nd, r, bins);
+          m.head = ds.h * bins;
+          m.tail = ds.t * bins + ds.roll;
+          m.wrap = bins - ds.roll;
+        }
+      }
+    }
+  }
+  return levels * level_size;
 }
 
 }  // extern "C"

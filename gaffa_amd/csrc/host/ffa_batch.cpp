@@ -39,6 +39,7 @@ struct GpuFfaBatchedSearch::Tile {
   GpuDeviceBuffer<gaffa_hip_ffa_group> groups;
   GpuDeviceBuffer<gaffa_hip_ffa_task> tasks;
   GpuDeviceBuffer<gaffa_hip_ffa_item> subtree, level, terminal;
+  GpuDeviceBuffer<gaffa_hip_ffa_merge> merges;
   gaffa_hip_ffa_plan_desc desc{};
 };
 
@@ -114,6 +115,7 @@ void GpuFfaBatchedSearch::build_tiles() {
     t->groups = gaffa::upload(e.groups().data(), e.groups().size());
     t->tasks = gaffa::upload(e.tasks().data(), e.tasks().size());
     t->subtree = gaffa::upload(e.subtree_items().data(), e.subtree_items().size());
+    t->merges = gaffa::upload(e.subtree_merges().data(), e.subtree_merges().size());
     t->level = gaffa::upload(e.level_items().data(), e.level_items().size());
     t->terminal = gaffa::upload(e.terminal_items().data(), e.terminal_items().size());
     gaffa_hip_ffa_plan_desc& d = t->desc;
@@ -123,6 +125,7 @@ void GpuFfaBatchedSearch::build_tiles() {
     d.ntasks = static_cast<std::uint32_t>(e.tasks().size());
     d.subtree_items = t->subtree.data();
     d.nsubtree_items = static_cast<std::uint32_t>(e.subtree_items().size());
+    d.subtree_merges = t->merges.data();
     d.level_items = t->level.data();
     d.level_item_offsets = e.level_offsets().data();
     d.max_depth = static_cast<std::uint32_t>(e.max_depth());

@@ -16,9 +16,21 @@ namespace gaffa {
 namespace {
 
 constexpr std::size_t kAlign = 64;               // floats
-constexpr std::size_t kSubtreeNodeFloats = 19456;  // per ping-pong buffer (2 x 76 KiB)
-constexpr std::size_t kSubtreeMaxDesc = 1024;      // levels x rows descriptors (8 KiB)
 constexpr std::size_t kSubtreeMaxRows = 512;
+constexpr std::size_t kSubtreeLdsBytes = 160 * 1024;
+
+// LDS of a subtree launch besides its two row buffers (include/gaffa_hip.h,
+// subtree_lds_bytes): the -0.0 zero rows and 1 KiB of slack.
+std::size_t subtree_fixed_lds(std::size_t max_bins) {
+  const std::size_t zero = (std::max<std::size_t>(256, max_bins + 64) + 63) / 64 * 64;
+  return 4 * zero + 1024;
+}
+
+std::size_t plan_max_bins(const FfaSearchPlan& plan) {
+  std::size_t b = 0;
+  for (const FfaSearchTask& t : plan.tasks) b = std::max(b, t.bins);
+  return b;
+}
 
 std::size_t tree_levels(std::size_t rows) {
   std::size_t levels = 0;
@@ -77,6 +89,7 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
   check_arg(!plan.tasks.empty(), "FFA executor plan must contain at least one task");
   input_nsamples_ = plan.tasks.front().input_nsamples;
   check_arg(input_nsamples_ > 0, "FFA executor time series must not be empty");
+  lds_max_bins_ = plan_max_bins(plan);
   append(plan, 0, 0);
   finish();
 }
@@ -84,6 +97,8 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
 GpuFfaExecutionPlan::GpuFfaExecutionPlan(std::span<const FfaBatchMember> members) {
   check_arg(!members.empty(), "FFA batch must contain at least one series");
   plan_.width_trials = members.front().plan->width_trials;
+  for (const FfaBatchMember& m : members)
+    if (m.plan != nullptr) lds_max_bins_ = std::max(lds_max_bins_, plan_max_bins(*m.plan));
   for (const FfaBatchMember& m : members) {
     check_arg(m.plan != nullptr && !m.plan->tasks.empty(),
               "FFA executor plan must contain at least one task");
@@ -156,8 +171,11 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
     terminal_lds_bytes_ = std::max(
         terminal_lds_bytes_,
         gaffa_hip_ffa_terminal_lds_bytes(d.bins, static_cast<std::uint32_t>(det.max_width)));
-    std::size_t lds_rows = std::min(kSubtreeMaxRows, kSubtreeNodeFloats / t.bins);
-    while (lds_rows > 2 && tree_levels(lds_rows) * lds_rows > kSubtreeMaxDesc) --lds_rows;
+    // rows of a node: two ping-pong buffers in what the fixed LDS leaves
+    check_arg(subtree_fixed_lds(lds_max_bins_) < kSubtreeLdsBytes,
+              "GPU FFA bins too large for an LDS subtree");
+    const std::size_t node_floats = (kSubtreeLdsBytes - subtree_fixed_lds(lds_max_bins_)) / 8;
+    const std::size_t lds_rows = std::min(kSubtreeMaxRows, node_floats / t.bins);
     check_arg(lds_rows >= 2, "GPU FFA bins too large for an LDS subtree");
     const std::uint32_t task_index = static_cast<std::uint32_t>(tasks_.size());
     if (t.rows >= 2) {
@@ -169,13 +187,28 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       const std::size_t before = subtree_.size();
       collect_nodes(0, d.rows, 0, depth, task_index, subtree_);
       for (std::size_t i = before; i < subtree_.size(); ++i) {
-        check_arg(gaffa_hip_ffa_subtree_lds_bytes(subtree_[i].b, d.bins) != 0xFFFFFFFFu,
+        gaffa_hip_ffa_item& it = subtree_[i];
+        check_arg(gaffa_hip_ffa_subtree_lds_bytes(it.b, d.bins) != 0xFFFFFFFFu,
                   "GPU FFA subtree node exceeds the kernel limits");
-        subtree_half_ = std::max<std::uint32_t>(subtree_half_, subtree_[i].b * d.bins);
-        subtree_desc_ = std::max<std::uint32_t>(
-            subtree_desc_, static_cast<std::uint32_t>(tree_levels(subtree_[i].b) * subtree_[i].b));
+        subtree_half_ = std::max<std::uint32_t>(subtree_half_, it.b * d.bins);
+        const std::uint32_t entries = gaffa_hip_ffa_subtree_merges(it.b, d.bins, nullptr);
+        check_arg(entries > 0 || it.b < 2, "GPU FFA subtree merge table unavailable");
+        subtree_desc_ = std::max(subtree_desc_, entries);
+        // one merge table per (rows, bins), shared by every node of that shape
+        auto [pos, fresh] = merge_table_of_.try_emplace(
+            (std::uint64_t{it.b} << 32) | d.bins, static_cast<std::uint32_t>(merges_.size()));
+        if (fresh) {
+          check_arg(merges_.size() + entries < (std::size_t{1} << 32),
+                    "GPU FFA subtree merge tables too large");
+          merges_.resize(merges_.size() + entries);
+          check_arg(gaffa_hip_ffa_subtree_merges(it.b, d.bins, merges_.data() + pos->second) ==
+                        entries,
+                    "GPU FFA subtree merge table size mismatch");
+        }
+        it.c = pos->second;
       }
-      subtree_lds_bytes_ = 8u * subtree_half_ + 8u * subtree_desc_;
+      subtree_lds_bytes_ =
+          static_cast<std::uint32_t>(8u * subtree_half_ + subtree_fixed_lds(lds_max_bins_));
       if (by_depth_.size() < depth) by_depth_.resize(depth);
       // global passes from depth `depth` up to depth 1, two tree levels per
       // pass where two remain; the region bits make every pass read one work
@@ -250,6 +283,7 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   d_groups_ = upload(g.data(), g.size());
   d_tasks_ = upload(t.data(), t.size());
   d_subtree_ = upload(exec_plan_.subtree_items().data(), exec_plan_.subtree_items().size());
+  d_merges_ = upload(exec_plan_.subtree_merges().data(), exec_plan_.subtree_merges().size());
   d_level_ = upload(exec_plan_.level_items().data(), exec_plan_.level_items().size());
   d_terminal_ = upload(exec_plan_.terminal_items().data(), exec_plan_.terminal_items().size());
   prepared_ = GpuDeviceBuffer<float>(std::max<std::size_t>(1, tile_ * exec_plan_.prepared_stride()));
@@ -264,6 +298,7 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.ntasks = static_cast<std::uint32_t>(t.size());
   desc_.subtree_items = d_subtree_.data();
   desc_.nsubtree_items = static_cast<std::uint32_t>(exec_plan_.subtree_items().size());
+  desc_.subtree_merges = d_merges_.data();
   desc_.level_items = d_level_.data();
   desc_.level_item_offsets = exec_plan_.level_offsets().data();
   desc_.max_depth = static_cast<std::uint32_t>(exec_plan_.max_depth());

@@ -11,6 +11,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <span>
 #include <vector>
 
@@ -73,6 +74,8 @@ class GpuFfaExecutionPlan {
   const std::vector<gaffa_hip_ffa_group>& groups() const noexcept { return groups_; }
   const std::vector<gaffa_hip_ffa_task>& tasks() const noexcept { return tasks_; }
   const std::vector<gaffa_hip_ffa_item>& subtree_items() const noexcept { return subtree_; }
+  // the subtree items' merge tables (item.c = first entry)
+  const std::vector<gaffa_hip_ffa_merge>& subtree_merges() const noexcept { return merges_; }
   const std::vector<gaffa_hip_ffa_item>& level_items() const noexcept { return level_; }
   const std::vector<std::uint32_t>& level_offsets() const noexcept { return level_offsets_; }
   const std::vector<gaffa_hip_ffa_item>& terminal_items() const noexcept { return terminal_; }
@@ -99,6 +102,7 @@ class GpuFfaExecutionPlan {
   std::size_t work_stride_ = 0;
   std::size_t max_depth_ = 0;
   std::size_t max_prepared_ = 0;
+  std::size_t lds_max_bins_ = 0;  // bins of the widest task of every plan appended
   std::uint32_t subtree_lds_bytes_ = 0;
   std::uint32_t subtree_half_ = 0;
   std::uint32_t subtree_desc_ = 0;
@@ -109,6 +113,8 @@ class GpuFfaExecutionPlan {
   std::vector<gaffa_hip_ffa_group> groups_;
   std::vector<gaffa_hip_ffa_task> tasks_;
   std::vector<gaffa_hip_ffa_item> subtree_, level_, terminal_;
+  std::vector<gaffa_hip_ffa_merge> merges_;
+  std::map<std::uint64_t, std::uint32_t> merge_table_of_;  // rows << 32 | bins -> first entry
   std::vector<std::uint32_t> level_offsets_;
 };
 
@@ -164,6 +170,7 @@ class GpuFfaProgram {
   GpuDeviceBuffer<gaffa_hip_ffa_group> d_groups_;
   GpuDeviceBuffer<gaffa_hip_ffa_task> d_tasks_;
   GpuDeviceBuffer<gaffa_hip_ffa_item> d_subtree_, d_level_, d_terminal_;
+  GpuDeviceBuffer<gaffa_hip_ffa_merge> d_merges_;
   GpuDeviceBuffer<float> prepared_, work_;
   GpuDeviceBuffer<gaffa_hip_ffa_peak> peaks_;
   GpuDeviceBuffer<std::uint32_t> peak_count_;

@@ -318,22 +318,41 @@ typedef struct gaffa_hip_ffa_peak {
   float snr;
 } gaffa_hip_ffa_peak;
 
+/* One row of one subtree level: the LDS merge out[i] = head[i] +
+ * tail[(i + roll) mod bins] in float offsets from the level's row buffer
+ * (16 bytes).  A subtree item's table holds, per level (level 0, the node's
+ * own merge, first), 16 wave lists of 2 * ceil(ceil(rows / 16) / 2) merges:
+ * wave w's rows w, w + 16, ..., padded with repeats of its last row.  The
+ * item's `c` field is the index of its first entry; items with the same
+ * rows and bins share a table.  No gaffa counterpart: gaffa recurses per row
+ * (ffa.cpp:110-251). */
+#define GAFFA_HIP_FFA_COPY_ROW 0xFFFFFFFFu
+typedef struct gaffa_hip_ffa_merge {
+  uint32_t head; /* head row * bins */
+  uint32_t tail; /* tail row * bins + roll, or GAFFA_HIP_FFA_COPY_ROW: the
+                    row is a leaf above this level and is copied */
+  uint32_t wrap; /* bins - roll: phases whose tail index does not wrap */
+  uint32_t out;  /* row * bins */
+} gaffa_hip_ffa_merge;
+
 typedef struct gaffa_hip_ffa_plan_desc {
   const gaffa_hip_ffa_group* groups; /* device */
   uint32_t ngroups;
   const gaffa_hip_ffa_task* tasks;   /* device */
   uint32_t ntasks;
-  const gaffa_hip_ffa_item* subtree_items; /* device */
+  const gaffa_hip_ffa_item* subtree_items; /* device; c = first merge */
   uint32_t nsubtree_items;
+  const gaffa_hip_ffa_merge* subtree_merges; /* device */
   const gaffa_hip_ffa_item* level_items;   /* device, sorted by depth desc */
   const uint32_t* level_item_offsets;      /* host, [max_depth + 1] */
   uint32_t max_depth;
   const gaffa_hip_ffa_item* terminal_items; /* device */
   uint32_t nterminal_items;
   uint32_t max_bins;
-  uint32_t subtree_lds_bytes;  /* 8 * subtree_half_floats + 8 * max_subtree_desc */
+  uint32_t subtree_lds_bytes;  /* 8 * subtree_half_floats + 1024 + 4 * Z, Z =
+                                  max(256, max_bins + 64) rounded up to 64 */
   uint32_t subtree_half_floats; /* max over subtree items of rows * bins */
-  uint32_t max_subtree_desc;    /* max over subtree items of levels * rows */
+  uint32_t max_subtree_desc;    /* max over subtree items of merges per item */
   uint32_t terminal_lds_bytes; /* max over tasks of ..._terminal_lds_bytes() */
   uint32_t prep_blocks;        /* max over downsampled groups of
                                   ceil(prepared_nsamples / outputs per block) */
@@ -363,6 +382,11 @@ uint32_t gaffa_hip_ffa_level_rows(void);
 uint32_t gaffa_hip_ffa_prepare_outputs_per_block(double factor);
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width);
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins);
+/* Fills the merge table of a subtree node of `rows` rows (host memory; a
+ * null `out` only counts) with the kernels' own row-selection arithmetic
+ * (gaffa's ffa.cpp:133-143 in FP32).  Returns the entry count, 0 for rows
+ * outside 1..512 or bins outside 2..65535. */
+uint32_t gaffa_hip_ffa_subtree_merges(uint32_t rows, uint32_t bins, gaffa_hip_ffa_merge* out);
 
 typedef struct gaffa_hip_ffa_search_args {
   const gaffa_hip_ffa_plan_desc* plan;
