@@ -1024,7 +1024,8 @@ PYBIND11_MODULE(_core, m) {
          const py::array_t<std::uint32_t, py::array::c_style>& shift,
          const py::array_t<std::uint32_t, py::array::c_style>& phase,
          const py::array_t<std::uint32_t, py::array::c_style>& width_index,
-         const py::array_t<float, py::array::c_style>& snr, const std::vector<double>& dms) {
+         const py::array_t<float, py::array::c_style>& snr, const std::vector<double>& dms,
+         bool ranked) {
         const std::size_t n = static_cast<std::size_t>(series.size());
         if (task.size() != series.size() || shift.size() != series.size() ||
             phase.size() != series.size() || width_index.size() != series.size() ||
@@ -1039,10 +1040,14 @@ PYBIND11_MODULE(_core, m) {
                                     snr.data()[i]};
         }
         const GpuFfaExecutionPlan e(plan);
-        return expand_sorted_dm_peaks(e, c, dms);
+        std::vector<DmPeak> out;
+        const std::vector<std::uint32_t> ranks =
+            ranked ? ffa_period_ranks(plan) : std::vector<std::uint32_t>{};
+        expand_sorted_dm_peaks_into(e, c, dms, out, ranks);
+        return out;
       },
       py::arg("plan"), py::arg("series"), py::arg("task"), py::arg("shift"), py::arg("phase"),
-      py::arg("width_index"), py::arg("snr"), py::arg("dms"),
+      py::arg("width_index"), py::arg("snr"), py::arg("dms"), py::arg("ranked") = true,
       "Host-side expansion + is_better_dm_peak ordering of compact GPU peaks (no device use)");
   m.def(
       "_find_dm_peaks_gpu",

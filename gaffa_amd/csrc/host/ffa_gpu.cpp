@@ -463,10 +463,34 @@ std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
   return out;
 }
 
+std::vector<std::uint32_t> ffa_period_ranks(const FfaSearchPlan& plan) {
+  const std::vector<double> periods = ffa_plan_trial_periods(plan);
+  if (periods.size() >= (std::size_t{1} << 31)) return {};
+  for (double v : periods)
+    if (!std::isfinite(v)) return {};
+  std::vector<std::uint32_t> order(periods.size());
+  for (std::size_t i = 0; i < order.size(); ++i) order[i] = static_cast<std::uint32_t>(i);
+  __gnu_parallel::sort(order.begin(), order.end(), [&](std::uint32_t a, std::uint32_t b) {
+    return periods[a] < periods[b] || (periods[a] == periods[b] && a < b);
+  });
+  std::vector<std::uint32_t> rank(periods.size());
+  std::uint32_t r = 0;
+  for (std::size_t i = 0; i < order.size(); ++i) {
+    if (i > 0 && periods[order[i]] != periods[order[i - 1]]) ++r;
+    rank[order[i]] = r;
+  }
+  return rank;
+}
+
 void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
                                  std::span<const gaffa_hip_ffa_peak> peaks,
-                                 std::span<const double> dms, std::vector<DmPeak>& out) {
+                                 std::span<const double> dms, std::vector<DmPeak>& out,
+                                 std::span<const std::uint32_t> period_ranks) {
   const auto& tasks = plan.plan().tasks;
+  // first periodogram row of each task (plan order, rows_eval each)
+  std::vector<std::size_t> first_row(tasks.size() + 1, 0);
+  for (std::size_t k = 0; k < tasks.size(); ++k) first_row[k + 1] = first_row[k] + tasks[k].rows_eval;
+  const bool ranked = !period_ranks.empty() && period_ranks.size() == first_row.back();
   std::map<std::size_t, FfaDetectionPlan> by_bins;
   std::vector<const FfaDetectionPlan*> det(tasks.size());
   for (std::size_t k = 0; k < tasks.size(); ++k) {
@@ -478,8 +502,8 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
   }
   for (const gaffa_hip_ffa_peak& p : peaks) {
     if (p.task >= tasks.size() || (p.phase_width >> 16) >= det[p.task]->boxcar_trials.size() ||
-        p.series >= dms.size()) {
-      throw std::logic_error("FFA compact peak refers to an unknown task, width or series");
+        p.series >= dms.size() || p.shift >= tasks[p.task].rows_eval) {
+      throw std::logic_error("FFA compact peak refers to an unknown task, row, width or series");
     }
   }
   auto expand = [&](const gaffa_hip_ffa_peak& p) {
@@ -488,10 +512,12 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
                                 p.shift, p.phase_width & 0xFFFFu, p.snr)};
   };
   // Sort (key, index) pairs instead of 96-byte records.  The key orders by
-  // is_better_dm_peak's first two criteria (finite peaks first, then S/N
-  // descending; -0 and +0 share a key because they compare equal); runs of
-  // equal keys are then ordered with the full comparator, so the result is
-  // exactly std::sort(..., is_better_dm_peak) of the expanded peaks.
+  // is_better_dm_peak's first criteria: finite peaks first, then S/N
+  // descending (-0 and +0 share a key because they compare equal), then, for
+  // finite peaks of a plan with period ranks, period ascending (frequency
+  // follows the period).  Runs of equal keys are then ordered with the full
+  // comparator, so the result is exactly std::sort(..., is_better_dm_peak)
+  // of the expanded peaks.
   struct Key {
     std::uint64_t key;
     std::uint32_t index;
@@ -504,33 +530,48 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
     const FfaPeak e = expand(c).peak;
     const bool finite =
         std::isfinite(e.snr) && std::isfinite(e.period) && std::isfinite(e.frequency);
-    std::uint64_t k = finite ? 0 : (1ULL << 33);
+    std::uint64_t k = finite ? 0 : (1ULL << 63);
     if (std::isnan(c.snr)) {
-      k |= 0xFFFFFFFFULL;  // one run, ordered by the comparator below
+      k |= 0xFFFFFFFFULL << 31;  // one run, ordered by the comparator below
     } else {
       const float snr = c.snr == 0.0F ? 0.0F : c.snr;  // canonical zero
       std::uint32_t bits = 0;
       std::memcpy(&bits, &snr, 4);
       const std::uint32_t ascending = (bits & 0x80000000U) ? ~bits : (bits | 0x80000000U);
-      k |= static_cast<std::uint64_t>(~ascending);  // descending S/N
+      k |= static_cast<std::uint64_t>(~ascending) << 31;  // descending S/N
+      if (finite && ranked) k |= period_ranks[first_row[c.task] + c.shift];
     }
     keys[static_cast<std::size_t>(i)] = Key{k, static_cast<std::uint32_t>(i)};
   }
   __gnu_parallel::sort(keys.begin(), keys.end(),
                        [](const Key& a, const Key& b) { return a.key < b.key; });
-  for (std::size_t i = 0; i < keys.size();) {
-    std::size_t j = i + 1;
-    while (j < keys.size() && keys[j].key == keys[i].key) ++j;
-    if (j - i > 1) {
-      std::vector<std::pair<DmPeak, std::uint32_t>> run;
-      run.reserve(j - i);
-      for (std::size_t k = i; k < j; ++k) run.emplace_back(expand(peaks[keys[k].index]), keys[k].index);
-      std::sort(run.begin(), run.end(), [](const auto& a, const auto& b) {
-        return is_better_dm_peak(a.first, b.first);
-      });
-      for (std::size_t k = i; k < j; ++k) keys[k].index = run[k - i].second;
+  // equal-key runs, found and ordered in parallel (each run is independent)
+  std::vector<std::pair<std::size_t, std::size_t>> runs;
+#pragma omp parallel
+  {
+    std::vector<std::pair<std::size_t, std::size_t>> mine;
+#pragma omp for schedule(static) nowait
+    for (std::int64_t s = 0; s < n; ++s) {
+      const std::size_t i = static_cast<std::size_t>(s);
+      if (i > 0 && keys[i].key == keys[i - 1].key) continue;  // not a run start
+      std::size_t j = i + 1;
+      while (j < keys.size() && keys[j].key == keys[i].key) ++j;
+      if (j - i > 1) mine.emplace_back(i, j);
     }
-    i = j;
+#pragma omp critical
+    runs.insert(runs.end(), mine.begin(), mine.end());
+  }
+  const std::int64_t nruns = static_cast<std::int64_t>(runs.size());
+#pragma omp parallel for schedule(dynamic, 64)
+  for (std::int64_t r = 0; r < nruns; ++r) {
+    const auto [i, j] = runs[static_cast<std::size_t>(r)];
+    std::vector<std::pair<DmPeak, std::uint32_t>> run;
+    run.reserve(j - i);
+    for (std::size_t k = i; k < j; ++k) run.emplace_back(expand(peaks[keys[k].index]), keys[k].index);
+    std::sort(run.begin(), run.end(), [](const auto& a, const auto& b) {
+      return is_better_dm_peak(a.first, b.first);
+    });
+    for (std::size_t k = i; k < j; ++k) keys[k].index = run[k - i].second;
   }
   out.resize(peaks.size());
 #pragma omp parallel for schedule(static)

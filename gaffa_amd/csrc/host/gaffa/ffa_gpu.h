@@ -223,9 +223,17 @@ std::vector<DmPeak> expand_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
                                            std::span<const double> dms);
 // The same into `out`, reusing its storage (no allocation or first-touch
 // cost when it already holds as many peaks, as in repeated searches).
+// `period_ranks` (ffa_period_ranks of the plan, optional) lets the sort key
+// carry the period order too, so that far fewer equal-key runs need the full
+// comparator.
 void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
                                  std::span<const gaffa_hip_ffa_peak> peaks,
-                                 std::span<const double> dms, std::vector<DmPeak>& out);
+                                 std::span<const double> dms, std::vector<DmPeak>& out,
+                                 std::span<const std::uint32_t> period_ranks = {});
+// Dense rank of every evaluated (task, shift) by trial period (equal periods
+// share a rank), indexed by the task's first periodogram row + shift; empty
+// when a period is not finite or there are 2^31 rows or more.
+std::vector<std::uint32_t> ffa_period_ranks(const FfaSearchPlan& plan);
 
 bool is_better_dm_peak(const DmPeak& lhs, const DmPeak& rhs);
 

@@ -160,6 +160,7 @@ class GpuSearchPipeline {
   std::size_t marks_used_ = 0;
   GpuDeviceBuffer<float> periodogram_;
   GpuPinnedBuffer compact_host_;  // compact peaks read back after a search
+  std::vector<std::uint32_t> period_ranks_;  // ffa_period_ranks(ffa_plan_)
   std::vector<double> trial_periods_;
   std::vector<std::uint32_t> trial_order_;
 };

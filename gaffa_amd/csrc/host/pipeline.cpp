@@ -67,6 +67,7 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   // so the first run rarely repeats a tile
   fe.initial_peak_buffer_bytes = std::max<std::size_t>(fe.initial_peak_buffer_bytes, 256u << 20);
   ffa_ = std::make_unique<GpuFfaProgram>(ffa_plan_, GpuFfaProgramOptions{cfg_.device_id}, fe);
+  period_ranks_ = ffa_period_ranks(ffa_plan_);
   events_.resize(7, nullptr);
   for (auto& e : events_) hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
 }
@@ -239,7 +240,8 @@ void GpuSearchPipeline::search() {
   auto* compact = static_cast<gaffa_hip_ffa_peak*>(compact_host_.data());
   const std::size_t n = prog.read_peaks_into(count, compact);
   expand_sorted_dm_peaks_into(prog.execution_plan(),
-                              std::span<const gaffa_hip_ffa_peak>(compact, n), dms_, peaks_);
+                              std::span<const gaffa_hip_ffa_peak>(compact, n), dms_, peaks_,
+                              period_ranks_);
 }
 
 StageTimes GpuSearchPipeline::run() {

@@ -131,11 +131,13 @@ def test_workload_shapes():
     assert steps == sorted(steps) and all(s.ndm % 32 == 0 for s in w.dm_segments)
 
 
-def test_dm_peak_key_sort_matches_comparator(core):
+@pytest.mark.parametrize("ranked", [True, False])
+def test_dm_peak_key_sort_matches_comparator(core, ranked):
     """expand_sorted_dm_peaks sorts (key, index) pairs and fixes equal-key runs
     with the full comparator: the order must equal a plain sort by
     is_better_dm_peak (finite first, S/N desc, period, frequency, width, phase,
-    shift, bins asc, then dm_index, dm), ties and +-0 included."""
+    shift, bins asc, then dm_index, dm), ties and +-0 included, with and
+    without the period ranks in the key."""
     plan = core._make_riptide_ffa_plan(nsamples=130870, tsamp=256e-6, period_min=0.2,
                                        period_max=2.0, duty_cycle_max=0.09,
                                        width_trial_spacing=2.0)
@@ -144,14 +146,15 @@ def test_dm_peak_key_sort_matches_comparator(core):
     rng = np.random.default_rng(11)
     n = 20000
     task = rng.integers(0, len(tasks), n).astype(np.uint32)
-    shift = (rng.random(n) * tasks["rows"][task]).astype(np.uint32)
+    shift = (rng.random(n) * tasks["rows_eval"][task]).astype(np.uint32)
     phase = (rng.random(n) * tasks["bins"][task]).astype(np.uint32)
     widx = rng.integers(0, nw, n).astype(np.uint32)
     series = rng.integers(0, 64, n).astype(np.uint32)
     # few distinct S/N values -> long equal-S/N runs; both signed zeros
     snr = rng.choice(np.array([7.5, 7.25, 6.0, 0.0, -0.0, 12.125], np.float32), n)
     dms = list(np.arange(64) * 0.5)
-    got = core._expand_sorted_dm_peaks(plan, series, task, shift, phase, widx, snr, dms)
+    got = core._expand_sorted_dm_peaks(plan, series, task, shift, phase, widx, snr, dms,
+                                       ranked=ranked)
     assert len(got) == n
 
     def key(p):
