@@ -32,6 +32,16 @@ UNIT = "DM-trial·samples/s"
 HBM_PEAK_GBS = 8000.0
 
 
+def global_passes(depth: int) -> int:
+    """Global-memory merge passes of a task whose LDS subtrees sit at `depth`:
+    depth - 1 tree levels, three per pass, two where two or four remain."""
+    rem, passes = max(depth - 1, 0), 0
+    while rem > 0:
+        rem -= 2 if rem in (2, 4) else min(rem, 3)
+        passes += 1
+    return passes
+
+
 def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
     """Algorithmic HBM bytes per launch of each hot kernel (the minimum any
     implementation must move), keyed like the stage times:
@@ -43,8 +53,8 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
     subtree     per series and task with rows >= 2: read + write its rows x bins
                 floats once (8 rows bins)
     levels      per series and task: 8 rows bins for each pass through global
-                memory (floor(depth_t / 2) of them: a pass merges two tree levels
-                where two remain), averaged over the launches
+                memory (global_passes(depth_t) of them: a pass merges three tree
+                levels, two where two or four remain), averaged over the launches
     detect      per series and task: the depth-1 rows once (4 rows bins; the input
                 series row for 1-row tasks) + 20 B per peak
     """
@@ -61,7 +71,8 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
         if f != 1.0:
             prep += 4.0 * float(t["prepared_nsamples"][t["downsample_factor"] == f][0])
     subtree = 8.0 * float(rb[t["rows"] >= 2].sum())
-    levels = 8.0 * float((rb * np.floor(depth / 2.0)).sum())
+    passes = np.array([global_passes(int(d)) for d in depth], np.float64)
+    levels = 8.0 * float((rb * passes).sum())
     detect = 4.0 * float(rb.sum())
     nlev = max(1, lay["level_launches"])
     nseg = len(workload.dm_segments)

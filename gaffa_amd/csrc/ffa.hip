@@ -536,7 +536,8 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
 // --------------------------------------------------------------- level ----
 
 // Rows [first, first + count) of the depth-d nodes of a task (item.a = d |
-// levels << 16).  One level: merged from the depth d + 1 rows.  Two levels:
+// levels << 16).  One level: merged from the depth d + 1 rows.  Two or three
+// levels (Desc3 below):
 // merged straight from the depth d + 2 rows, each output phase p summed as
 // (A[p] + B[p + rh]) + (C[q] + D[q + rt]) with q = p + ro (indices mod b),
 // which is exactly the two one-level merges (the depth d + 1 values are
@@ -548,10 +549,21 @@ struct Desc2 {
   uint32_t ro;        // roll of the tail child row in the output merge
 };
 
+// Three levels: the eight depth-(d + 3) rows of an output row and the
+// combined roll of each (mod b, the first is 0):
+//   out[p] = ((r0[p] + r1[p+R1]) + (r2[p+R2] + r3[p+R3]))
+//          + ((r4[p+R4] + r5[p+R5]) + (r6[p+R6] + r7[p+R7])),
+// the FP32 sums of three one-level merges in their order.
+struct Desc3 {
+  uint32_t row[8];
+  uint32_t roll[7];
+};
+
 __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
   __shared__ Desc sdesc[kLevelRows];
   __shared__ Desc2 sdesc2[kLevelRows];
+  __shared__ Desc3 sdesc3[kLevelRows];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
   const gaffa_hip_ffa_task& task = plan.tasks[item.task];
@@ -583,6 +595,63 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
           roll = ds.roll;
         },
         [&](uint32_t i, uint32_t p, float v) { dst[static_cast<uint64_t>(item.b + i) * b + p] = v; });
+    return;
+  }
+  if (levels == 3) {
+    // the host schedules three levels only where every node at depths d + 1
+    // and d + 2 has two rows or more
+    if (threadIdx.x < item.c) {
+      const uint32_t r = item.b + threadIdx.x;
+      const Node nd = find_node(0, task.rows, depth, r);
+      const Desc top = merge_desc(nd, r, b);
+      const Node nh{nd.start, nd.size / 2}, nt{nd.start + nd.size / 2, nd.size - nd.size / 2};
+      const Desc dh = merge_desc(nh, top.h, b);
+      const Desc dt = merge_desc(nt, top.t, b);
+      const Desc da = merge_desc(Node{nh.start, nh.size / 2}, dh.h, b);
+      const Desc db = merge_desc(Node{nh.start + nh.size / 2, nh.size - nh.size / 2}, dh.t, b);
+      const Desc dc = merge_desc(Node{nt.start, nt.size / 2}, dt.h, b);
+      const Desc dd = merge_desc(Node{nt.start + nt.size / 2, nt.size - nt.size / 2}, dt.t, b);
+      auto mod = [&](uint32_t x) { return x % b; };
+      Desc3 e;
+      e.row[0] = da.h; e.row[1] = da.t; e.row[2] = db.h; e.row[3] = db.t;
+      e.row[4] = dc.h; e.row[5] = dc.t; e.row[6] = dd.h; e.row[7] = dd.t;
+      e.roll[0] = da.roll;
+      e.roll[1] = dh.roll;
+      e.roll[2] = mod(dh.roll + db.roll);
+      e.roll[3] = top.roll;
+      e.roll[4] = mod(top.roll + dc.roll);
+      e.roll[5] = mod(top.roll + dt.roll);
+      e.roll[6] = mod(top.roll + dt.roll + dd.roll);
+      sdesc3[threadIdx.x] = e;
+    }
+    __syncthreads();
+    // one row per wave at a time, four phases per lane: 32 loads in flight
+    for (uint32_t i = wave; i < item.c; i += kWaves) {
+      const Desc3 e = sdesc3[i];
+      const float* r[8];
+#pragma unroll
+      for (uint32_t k = 0; k < 8; ++k) r[k] = src + static_cast<uint64_t>(e.row[k]) * b;
+      for (uint32_t p0 = lane; p0 < b; p0 += 256) {
+        float v[4];
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t p = min(p0 + 64u * j, b - 1);
+          uint32_t x[7];
+#pragma unroll
+          for (uint32_t k = 0; k < 7; ++k) {
+            x[k] = p + e.roll[k];
+            x[k] = min(x[k], x[k] - b);  // (p + roll) mod b for p, roll < b
+          }
+          v[j] = ((r[0][p] + r[1][x[0]]) + (r[2][x[1]] + r[3][x[2]])) +
+                 ((r[4][x[3]] + r[5][x[4]]) + (r[6][x[5]] + r[7][x[6]]));
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < 4; ++j) {
+          const uint32_t p = p0 + 64u * j;
+          if (p < b) dst[static_cast<uint64_t>(item.b + i) * b + p] = v[j];
+        }
+      }
+    }
     return;
   }
   if (threadIdx.x < item.c) {

@@ -210,12 +210,15 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       subtree_lds_bytes_ =
           static_cast<std::uint32_t>(8u * subtree_half_ + subtree_fixed_lds(lds_max_bins_));
       if (by_depth_.size() < depth) by_depth_.resize(depth);
-      // global passes from depth `depth` up to depth 1, two tree levels per
-      // pass where two remain; the region bits make every pass read one work
-      // region and write the other
+      // global passes from depth `depth` up to depth 1: three tree levels per
+      // pass, two where two or four remain (2 + 2 reads less than 3 + 1); the
+      // region bits make every pass read one work region and write the
+      // other.  Every node above the LDS depth has two rows or more (the
+      // depth is the first whose nodes fit), as a multi-level pass needs.
       std::uint32_t regions = 0;  // the subtrees write region 0
       for (std::uint32_t din = depth; din > 1;) {
-        const std::uint32_t levels = din >= 3 ? 2u : 1u;
+        const std::uint32_t rem = din - 1;
+        const std::uint32_t levels = rem == 4 ? 2u : std::min<std::uint32_t>(rem, 3u);
         const std::uint32_t dout = din - levels;
         regions |= (1u - ((regions >> din) & 1u)) << dout;
         for (std::uint32_t r = 0; r < d.rows; r += level_rows)

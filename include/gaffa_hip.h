@@ -303,7 +303,7 @@ typedef struct gaffa_hip_ffa_task {
 typedef struct gaffa_hip_ffa_item {
   uint32_t task;
   uint32_t a; /* subtree: node start row; level: output depth | levels << 16
-                 (1 or 2 tree levels merged in one pass); terminal: first row */
+                 (1, 2 or 3 tree levels merged in one pass); terminal: first row */
   uint32_t b; /* subtree: node rows;      level: first row; terminal: rows */
   uint32_t c; /* level: rows in this item */
 } gaffa_hip_ffa_item;
