@@ -410,6 +410,19 @@ std::size_t GpuFfaProgram::read_peaks_into(std::uint32_t count, gaffa_hip_ffa_pe
   return n;
 }
 
+void GpuFfaProgram::enqueue_peak_count_copy(std::uint32_t* host_slot) {
+  hip_ok(gaffa_hip_memcpy(host_slot, peak_count_.data(), sizeof(std::uint32_t), exec_.stream),
+         "hipMemcpyAsync peak count");
+}
+
+void GpuFfaProgram::enqueue_peak_copy(std::size_t first, std::size_t count, gaffa_hip_ffa_peak* dst,
+                                      gaffa_hip_stream stream) const {
+  check_arg(first + count <= peaks_.size(), "peak range exceeds the peak buffer");
+  if (count != 0)
+    hip_ok(gaffa_hip_memcpy(dst, peaks_.data() + first, count * sizeof(gaffa_hip_ffa_peak), stream),
+           "hipMemcpyAsync peaks");
+}
+
 void GpuFfaProgram::grow_peaks(std::size_t count) {
   const std::size_t max_count = exec_.max_peak_buffer_bytes / sizeof(gaffa_hip_ffa_peak);
   if (count > max_count) {
@@ -485,16 +498,12 @@ std::vector<std::uint32_t> ffa_period_ranks(const FfaSearchPlan& plan) {
   return rank;
 }
 
-void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
-                                 std::span<const gaffa_hip_ffa_peak> peaks,
-                                 std::span<const double> dms, std::vector<DmPeak>& out,
-                                 std::span<const std::uint32_t> period_ranks) {
+namespace {
+
+// Detection plans by bins, one per task (make_ffa_peak's width tables).
+std::vector<const FfaDetectionPlan*> detection_plans(const GpuFfaExecutionPlan& plan,
+                                                     std::map<std::size_t, FfaDetectionPlan>& by_bins) {
   const auto& tasks = plan.plan().tasks;
-  // first periodogram row of each task (plan order, rows_eval each)
-  std::vector<std::size_t> first_row(tasks.size() + 1, 0);
-  for (std::size_t k = 0; k < tasks.size(); ++k) first_row[k + 1] = first_row[k] + tasks[k].rows_eval;
-  const bool ranked = !period_ranks.empty() && period_ranks.size() == first_row.back();
-  std::map<std::size_t, FfaDetectionPlan> by_bins;
   std::vector<const FfaDetectionPlan*> det(tasks.size());
   for (std::size_t k = 0; k < tasks.size(); ++k) {
     auto it = by_bins.find(tasks[k].bins);
@@ -503,39 +512,61 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
                            make_ffa_detection_plan(plan.plan().width_trials, tasks[k].bins)).first;
     det[k] = &it->second;
   }
+  return det;
+}
+
+DmPeak expand_dm_peak(const std::vector<FfaSearchTask>& tasks,
+                      const std::vector<const FfaDetectionPlan*>& det, std::span<const double> dms,
+                      const gaffa_hip_ffa_peak& p) {
+  return DmPeak{dms[p.series], p.series,
+                make_ffa_peak(tasks[p.task], det[p.task]->boxcar_trials[p.phase_width >> 16],
+                              p.shift, p.phase_width & 0xFFFFu, p.snr)};
+}
+
+}  // namespace
+
+void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
+                                 std::span<const gaffa_hip_ffa_peak> peaks,
+                                 std::span<const double> dms, std::vector<DmPeak>& out,
+                                 std::span<const std::uint32_t> period_ranks) {
+  std::vector<DmPeakKey> keys(peaks.size());
+  dm_peak_keys(plan, peaks, 0, dms, period_ranks, keys.data());
+  finish_sorted_dm_peaks(plan, peaks, dms, keys, out);
+}
+
+// Sort (key, index) pairs instead of 96-byte records.  The key orders by
+// is_better_dm_peak's first criteria: finite peaks first, then S/N
+// descending (-0 and +0 share a key because they compare equal), then, for
+// finite peaks of a plan with period ranks, period ascending (frequency
+// follows the period).  Runs of equal keys are then ordered with the full
+// comparator, so the result is exactly std::sort(..., is_better_dm_peak) of
+// the expanded peaks.
+void dm_peak_keys(const GpuFfaExecutionPlan& plan, std::span<const gaffa_hip_ffa_peak> peaks,
+                  std::size_t first, std::span<const double> dms,
+                  std::span<const std::uint32_t> period_ranks, DmPeakKey* keys) {
+  const auto& tasks = plan.plan().tasks;
+  std::map<std::size_t, FfaDetectionPlan> by_bins;
+  const std::vector<const FfaDetectionPlan*> det = detection_plans(plan, by_bins);
+  // first periodogram row of each task (plan order, rows_eval each)
+  std::vector<std::size_t> first_row(tasks.size() + 1, 0);
+  for (std::size_t k = 0; k < tasks.size(); ++k) first_row[k + 1] = first_row[k] + tasks[k].rows_eval;
+  const bool ranked = !period_ranks.empty() && period_ranks.size() == first_row.back();
   for (const gaffa_hip_ffa_peak& p : peaks) {
     if (p.task >= tasks.size() || (p.phase_width >> 16) >= det[p.task]->boxcar_trials.size() ||
         p.series >= dms.size() || p.shift >= tasks[p.task].rows_eval) {
       throw std::logic_error("FFA compact peak refers to an unknown task, row, width or series");
     }
   }
-  auto expand = [&](const gaffa_hip_ffa_peak& p) {
-    return DmPeak{dms[p.series], p.series,
-                  make_ffa_peak(tasks[p.task], det[p.task]->boxcar_trials[p.phase_width >> 16],
-                                p.shift, p.phase_width & 0xFFFFu, p.snr)};
-  };
-  // Sort (key, index) pairs instead of 96-byte records.  The key orders by
-  // is_better_dm_peak's first criteria: finite peaks first, then S/N
-  // descending (-0 and +0 share a key because they compare equal), then, for
-  // finite peaks of a plan with period ranks, period ascending (frequency
-  // follows the period).  Runs of equal keys are then ordered with the full
-  // comparator, so the result is exactly std::sort(..., is_better_dm_peak)
-  // of the expanded peaks.
-  struct Key {
-    std::uint64_t key;
-    std::uint32_t index;
-  };
   const std::int64_t n = static_cast<std::int64_t>(peaks.size());
-  std::vector<Key> keys(peaks.size());
 #pragma omp parallel for schedule(static)
   for (std::int64_t i = 0; i < n; ++i) {
     const gaffa_hip_ffa_peak& c = peaks[static_cast<std::size_t>(i)];
-    const FfaPeak e = expand(c).peak;
+    const FfaPeak e = expand_dm_peak(tasks, det, dms, c).peak;
     const bool finite =
         std::isfinite(e.snr) && std::isfinite(e.period) && std::isfinite(e.frequency);
     std::uint64_t k = finite ? 0 : (1ULL << 63);
     if (std::isnan(c.snr)) {
-      k |= 0xFFFFFFFFULL << 31;  // one run, ordered by the comparator below
+      k |= 0xFFFFFFFFULL << 31;  // one run, ordered by the comparator
     } else {
       const float snr = c.snr == 0.0F ? 0.0F : c.snr;  // canonical zero
       std::uint32_t bits = 0;
@@ -544,10 +575,21 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
       k |= static_cast<std::uint64_t>(~ascending) << 31;  // descending S/N
       if (finite && ranked) k |= period_ranks[first_row[c.task] + c.shift];
     }
-    keys[static_cast<std::size_t>(i)] = Key{k, static_cast<std::uint32_t>(i)};
+    keys[i] = DmPeakKey{k, static_cast<std::uint32_t>(first + static_cast<std::size_t>(i))};
   }
+}
+
+void finish_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
+                            std::span<const gaffa_hip_ffa_peak> peaks,
+                            std::span<const double> dms, std::vector<DmPeakKey>& keys,
+                            std::vector<DmPeak>& out) {
+  check_arg(keys.size() == peaks.size(), "one sort key per peak");
+  const auto& tasks = plan.plan().tasks;
+  std::map<std::size_t, FfaDetectionPlan> by_bins;
+  const std::vector<const FfaDetectionPlan*> det = detection_plans(plan, by_bins);
+  const std::int64_t n = static_cast<std::int64_t>(peaks.size());
   __gnu_parallel::sort(keys.begin(), keys.end(),
-                       [](const Key& a, const Key& b) { return a.key < b.key; });
+                       [](const DmPeakKey& a, const DmPeakKey& b) { return a.key < b.key; });
   // equal-key runs, found and ordered in parallel (each run is independent)
   std::vector<std::pair<std::size_t, std::size_t>> runs;
 #pragma omp parallel
@@ -570,7 +612,8 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
     const auto [i, j] = runs[static_cast<std::size_t>(r)];
     std::vector<std::pair<DmPeak, std::uint32_t>> run;
     run.reserve(j - i);
-    for (std::size_t k = i; k < j; ++k) run.emplace_back(expand(peaks[keys[k].index]), keys[k].index);
+    for (std::size_t k = i; k < j; ++k)
+      run.emplace_back(expand_dm_peak(tasks, det, dms, peaks[keys[k].index]), keys[k].index);
     std::sort(run.begin(), run.end(), [](const auto& a, const auto& b) {
       return is_better_dm_peak(a.first, b.first);
     });
@@ -579,7 +622,8 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
   out.resize(peaks.size());
 #pragma omp parallel for schedule(static)
   for (std::int64_t i = 0; i < n; ++i)
-    out[static_cast<std::size_t>(i)] = expand(peaks[keys[static_cast<std::size_t>(i)].index]);
+    out[static_cast<std::size_t>(i)] =
+        expand_dm_peak(tasks, det, dms, peaks[keys[static_cast<std::size_t>(i)].index]);
 }
 
 FfaBatchSearchResult run_ffa_batch_gpu(GpuFfaProgram& program, GpuTimeSeriesBatchView batch,

@@ -148,6 +148,13 @@ class GpuFfaProgram {
   // ideally pinned) and waits; returns how many.
   std::size_t read_peaks_into(std::uint32_t count, gaffa_hip_ffa_peak* dst);
   std::size_t peak_capacity() const noexcept { return peaks_.size(); }
+  // Enqueues on the program stream a copy of the running peak count into
+  // `host_slot` (pinned host memory); valid once the stream reaches it.
+  void enqueue_peak_count_copy(std::uint32_t* host_slot);
+  // Enqueues on `stream` the copy of compact peaks [first, first + count)
+  // (within the capacity) into `dst` (pinned host memory).
+  void enqueue_peak_copy(std::size_t first, std::size_t count, gaffa_hip_ffa_peak* dst,
+                         gaffa_hip_stream stream) const;
   void grow_peaks(std::size_t count);
   gaffa_hip_stream stream() const noexcept { return exec_.stream; }
   // Dense periodogram output for the following tiles: series i of the
@@ -230,6 +237,21 @@ void expand_sorted_dm_peaks_into(const GpuFfaExecutionPlan& plan,
                                  std::span<const gaffa_hip_ffa_peak> peaks,
                                  std::span<const double> dms, std::vector<DmPeak>& out,
                                  std::span<const std::uint32_t> period_ranks = {});
+// The two halves of expand_sorted_dm_peaks_into, so that the keys of one
+// part of the peaks can be computed while the device still searches the
+// rest: dm_peak_keys fills keys[i] for peaks[i] (index = first + i) and
+// validates them; finish_sorted_dm_peaks sorts every key and expands.
+struct DmPeakKey {
+  std::uint64_t key;
+  std::uint32_t index;
+};
+void dm_peak_keys(const GpuFfaExecutionPlan& plan, std::span<const gaffa_hip_ffa_peak> peaks,
+                  std::size_t first, std::span<const double> dms,
+                  std::span<const std::uint32_t> period_ranks, DmPeakKey* keys);
+void finish_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
+                            std::span<const gaffa_hip_ffa_peak> peaks,
+                            std::span<const double> dms, std::vector<DmPeakKey>& keys,
+                            std::vector<DmPeak>& out);
 // Dense rank of every evaluated (task, shift) by trial period (equal periods
 // share a rank), indexed by the task's first periodogram row + shift; empty
 // when a period is not finite or there are 2^31 rows or more.

@@ -159,7 +159,11 @@ class GpuSearchPipeline {
   std::vector<FfaStageMarks> marks_;  // one per tile, events created on demand
   std::size_t marks_used_ = 0;
   GpuDeviceBuffer<float> periodogram_;
-  GpuPinnedBuffer compact_host_;  // compact peaks read back after a search
+  GpuPinnedBuffer compact_host_;  // compact peaks read back during a search
+  GpuPinnedBuffer tile_counts_;   // running peak count after each FFA tile
+  std::vector<gaffa_hip_event> tile_done_;  // after each tile's count copy
+  GpuStream copy_stream_;         // peak downloads beside the search
+  std::vector<DmPeakKey> peak_keys_;
   std::vector<std::uint32_t> period_ranks_;  // ffa_period_ranks(ffa_plan_)
   std::vector<double> trial_periods_;
   std::vector<std::uint32_t> trial_order_;

@@ -4,6 +4,7 @@
 #include "gaffa/folding.h"
 
 #include <algorithm>
+#include <cstring>
 #include <cmath>
 #include <parallel/algorithm>
 #include <stdexcept>
@@ -74,6 +75,8 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
 
 GpuSearchPipeline::~GpuSearchPipeline() {
   for (auto e : events_)
+    if (e != nullptr) (void)gaffa_hip_event_destroy(e);
+  for (auto e : tile_done_)
     if (e != nullptr) (void)gaffa_hip_event_destroy(e);
   for (auto& m : marks_)
     for (auto e : m.at)
@@ -219,29 +222,69 @@ void GpuSearchPipeline::search() {
       marks_.push_back(m);
     }
   }
-  std::uint32_t count = 0;
+  // per tile: the running peak count, copied back on the search stream, and
+  // an event after it
+  while (tile_done_.size() < ntiles) {
+    gaffa_hip_event e = nullptr;
+    hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
+    tile_done_.push_back(e);
+  }
+  if (tile_counts_.bytes() < ntiles * sizeof(std::uint32_t))
+    tile_counts_ = GpuPinnedBuffer(ntiles * sizeof(std::uint32_t));
+  auto* counts = static_cast<std::uint32_t*>(tile_counts_.data());
+  if (copy_stream_.get() == nullptr) copy_stream_ = GpuStream::create();
   for (;;) {
+    const std::size_t capacity = prog.peak_capacity();
     hip_ok(gaffa_hip_event_record(events_[3], stream_.get()), "hipEventRecord");
     for (std::size_t k = 0; k < ntiles; ++k) {
       const std::size_t s0 = k * tile;
       const std::size_t ns = std::min(tile, ndm_ - s0);
       prog.enqueue_tile(series + s0 * stride_, ns, stride_, static_cast<std::uint32_t>(s0),
                         cfg_.snr_threshold, s0 == 0, kernel_timing_ ? &marks_[k] : nullptr);
+      prog.enqueue_peak_count_copy(counts + k);
+      hip_ok(gaffa_hip_event_record(tile_done_[k], stream_.get()), "hipEventRecord");
     }
     marks_used_ = kernel_timing_ ? ntiles : 0;
     hip_ok(gaffa_hip_event_record(events_[4], stream_.get()), "hipEventRecord");
-    count = prog.read_peak_count();
-    if (count <= prog.peak_capacity()) break;
-    prog.grow_peaks(count);
+    // While the device searches tile k + 1 and on, the host brings tile k's
+    // peaks back (they follow tile k - 1's in the append-only buffer) and
+    // computes their sort keys.
+    std::size_t done = 0;
+    bool overflow = false;
+    for (std::size_t k = 0; k < ntiles; ++k) {
+      hip_ok(gaffa_hip_event_synchronize(tile_done_[k]), "hipEventSynchronize");
+      const std::size_t c = counts[k];
+      if (c > capacity) {
+        overflow = true;
+        break;
+      }
+      if (c == done) continue;
+      if (peak_keys_.size() < c) peak_keys_.resize(std::max(c, peak_keys_.size() * 2));
+      if (compact_host_.bytes() < c * sizeof(gaffa_hip_ffa_peak)) {
+        // grow the pinned buffer, keeping the peaks already brought back
+        GpuPinnedBuffer bigger(std::min(capacity, c + c / 2) * sizeof(gaffa_hip_ffa_peak));
+        if (done != 0) std::memcpy(bigger.data(), compact_host_.data(), done * sizeof(gaffa_hip_ffa_peak));
+        compact_host_ = std::move(bigger);
+      }
+      auto* compact = static_cast<gaffa_hip_ffa_peak*>(compact_host_.data());
+      prog.enqueue_peak_copy(done, c - done, compact + done, copy_stream_.get());
+      copy_stream_.synchronize();
+      dm_peak_keys(prog.execution_plan(), std::span<const gaffa_hip_ffa_peak>(compact + done, c - done),
+                   done, dms_, period_ranks_, peak_keys_.data() + done);
+      done = c;
+    }
+    if (overflow) {
+      stream_.synchronize();
+      prog.grow_peaks(counts[ntiles - 1]);  // repeat the search with room for every peak
+      continue;
+    }
+    peak_keys_.resize(done);
+    finish_sorted_dm_peaks(
+        prog.execution_plan(),
+        std::span<const gaffa_hip_ffa_peak>(static_cast<const gaffa_hip_ffa_peak*>(compact_host_.data()), done),
+        dms_, peak_keys_, peaks_);
+    break;
   }
-  // compact peaks come back through a pinned buffer kept across searches
-  const std::size_t bytes = std::max<std::size_t>(count, 1) * sizeof(gaffa_hip_ffa_peak);
-  if (compact_host_.bytes() < bytes) compact_host_ = GpuPinnedBuffer(bytes + bytes / 4);
-  auto* compact = static_cast<gaffa_hip_ffa_peak*>(compact_host_.data());
-  const std::size_t n = prog.read_peaks_into(count, compact);
-  expand_sorted_dm_peaks_into(prog.execution_plan(),
-                              std::span<const gaffa_hip_ffa_peak>(compact, n), dms_, peaks_,
-                              period_ranks_);
 }
 
 StageTimes GpuSearchPipeline::run() {
