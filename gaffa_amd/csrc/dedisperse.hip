@@ -204,7 +204,7 @@ transpose_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples, uint64_t n
 // 16-bit pairs (<= 256 channels x 255 fits 16 bits).  Stage 2 sums the
 // subbands at the residual delays for the group's DMs.
 constexpr int kSbQ = 4;  // stage-1 dword columns per thread (jlen4 <= 4096)
-constexpr int kSbUnroll = 8;  // channels whose loads are in flight together
+constexpr int kSbUnroll = 16;  // channels whose loads are in flight together
 constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, one L2 each
 
 __global__ void __launch_bounds__(kThreads)
@@ -254,6 +254,15 @@ dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
 
   // Stage 1: the loads of kSbUnroll channels are issued together.
   const int32_t* kdel = coarse + static_cast<uint64_t>(k) * nsel;
+  // largest delay whose stage-1 reads stay inside the padded rows, as an
+  // unsigned bound (a negative delay compares above it)
+  const int64_t room = static_cast<int64_t>(cm_length) - static_cast<int64_t>(t0) -
+                       static_cast<int64_t>(jlen4) - 4;
+  if (room < 0) {  // uniform
+    if (tid == 0) atomicExch(error_flag, 1);
+    return;
+  }
+  const uint32_t delay_limit = static_cast<uint32_t>(min<int64_t>(room, 0x7FFFFFFF));
   uint32_t* inter32 = reinterpret_cast<uint32_t*>(inter);
   const uint32_t nw = jlen4 / 4;                        // dword columns
   const uint32_t nq = (nw + kThreads - 1) / kThreads;  // <= kSbQ (host)
@@ -267,15 +276,18 @@ dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
       const uint32_t nu = min(static_cast<uint32_t>(kSbUnroll), ce - c0);
       const uint32_t* rows[kSbUnroll];
       uint32_t sh[kSbUnroll];
+      // the batch's delays first, all loads in flight together
+      int32_t dl[kSbUnroll];
+#pragma unroll
+      for (int u = 0; u < kSbUnroll; ++u) dl[u] = kdel[min(c0 + u, ce - 1)];  // spare slots repeat
       bool bad = false;
 #pragma unroll
       for (int u = 0; u < kSbUnroll; ++u) {
-        const uint32_t c = min(c0 + u, ce - 1);  // spare slots repeat a valid channel
-        const int32_t d = kdel[c];
-        const uint64_t base = t0 + static_cast<uint64_t>(d);
+        const uint32_t c = min(c0 + u, ce - 1);
         // rows are zero-padded to cm_length; a delay beyond the padding
-        // would read past the row
-        bad |= d < 0 || base + jlen4 + 4 > cm_length;
+        // would read past the row (delay_limit < 0: every delay would)
+        bad |= static_cast<uint32_t>(dl[u]) > delay_limit;
+        const uint64_t base = t0 + static_cast<uint32_t>(dl[u]);
         rows[u] = reinterpret_cast<const uint32_t*>(cm + static_cast<uint64_t>(c) * cm_stride) +
                   (base >> 2);
         sh[u] = static_cast<uint32_t>(base & 3u);
