@@ -401,8 +401,25 @@ static void subband_geometry(uint64_t nsub, uint64_t ndm_per_nominal, int32_t ma
   tile = 0;
   inter_len = 0;
   lds = 0;
+  // candidates, largest first: tiles whose stage-1 span fills whole passes of
+  // the workgroup (kThreads dword columns each, no idle lanes), then powers
+  // of two
+  uint32_t cands[16];
+  uint32_t ncand = 0;
+  for (uint32_t q = kSbQ; q >= 1; --q) {
+    const int64_t t = (static_cast<int64_t>(q) * kThreads * 4 - max_residual - 12) & ~int64_t{63};
+    if (t >= 64 && t <= 2048) cands[ncand++] = static_cast<uint32_t>(t);
+  }
+  for (uint32_t t = 2048; t >= 64; t /= 2) cands[ncand++] = t;
+  for (uint32_t i = 1; i < ncand; ++i)  // descending
+    for (uint32_t j = i; j > 0 && cands[j] > cands[j - 1]; --j) {
+      const uint32_t x = cands[j];
+      cands[j] = cands[j - 1];
+      cands[j - 1] = x;
+    }
   for (const uint64_t budget : {kMaxLds / 2 - 2048, kMaxLds - 4096}) {
-    for (uint32_t cand = 2048; cand >= 64 && tile == 0; cand /= 2) {
+    for (uint32_t ci = 0; ci < ncand && tile == 0; ++ci) {
+      const uint32_t cand = cands[ci];
       const uint64_t jlen4 = (cand + static_cast<uint64_t>(max_residual) + 4 + 3) & ~3ull;
       const uint64_t il = jlen4 + 8;
       // stage 1 writes (computes or zeroes) inter[0, il) in kSbQ dword
