@@ -204,3 +204,54 @@ def test_large_workload_shape(core):
     f = w.frequencies
     dmax = core.dedispersion_delay_bins(float(w.dms[-1]), float(f[0]), float(f.max()), w.tsamp)
     assert 57000 < dmax < 57400  # n_out ~ 4.137 M of 2^22
+
+
+class _Merge(ctypes.Structure):
+    _fields_ = [("head", ctypes.c_uint32), ("tail", ctypes.c_uint32),
+                ("wrap", ctypes.c_uint32), ("out", ctypes.c_uint32)]
+
+
+@pytest.mark.parametrize("rows,bins", [(2, 7), (3, 5), (7, 9), (16, 4), (37, 11), (84, 180),
+                                       (113, 181), (512, 2)])
+def test_subtree_merge_tables_replay_the_oracle_transform(port, rows, bins):
+    """The host-built subtree merge tables (gaffa_hip_ffa_subtree_merges),
+    replayed level by level in float32 exactly as the subtree kernel applies
+    them (head + tail at the split-at-wrap index, -0.0 for a copied row, the
+    wave lists' repeats included), give the oracle's FFA transform of the node
+    bit for bit."""
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    lib.gaffa_hip_ffa_subtree_merges.restype = ctypes.c_uint32
+    lib.gaffa_hip_ffa_subtree_merges.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
+    n = lib.gaffa_hip_ffa_subtree_merges(rows, bins, None)
+    levels = int(np.ceil(np.log2(rows))) if rows > 1 else 0
+    assert n % max(levels, 1) == 0 and (n == 0) == (levels == 0)
+    table = (_Merge * max(n, 1))()
+    assert lib.gaffa_hip_ffa_subtree_merges(rows, bins, ctypes.cast(table, ctypes.c_void_p)) == n
+    head = np.array([m.head for m in table[:n]], np.int64)
+    tail = np.array([m.tail for m in table[:n]], np.int64)
+    wrap = np.array([m.wrap for m in table[:n]], np.int64)
+    out = np.array([m.out for m in table[:n]], np.int64)
+    rng = np.random.default_rng(rows * 1000 + bins)
+    x = rng.normal(size=rows * bins).astype(np.float32)
+    cur = x.copy()
+    per_level = n // levels if levels else 0
+    p = np.arange(bins)
+    for d in range(levels - 1, -1, -1):
+        nxt = np.full(rows * bins, np.nan, np.float32)
+        sl = slice(d * per_level, (d + 1) * per_level)
+        for h, t, w, o in zip(head[sl], tail[sl], wrap[sl], out[sl]):
+            hv = cur[h + p]
+            if t == 0xFFFFFFFF:
+                tv = np.full(bins, -0.0, np.float32)
+            else:
+                tv = cur[np.where(p < w, t + p, t + p - bins)]
+            v = hv + tv
+            if not np.isnan(nxt[o]):  # a repeat must store the same row again
+                np.testing.assert_array_equal(nxt[o + p], v)
+            nxt[o + p] = v
+        assert not np.isnan(nxt).any()  # every row written at every level
+        cur = nxt
+    want = port.ffa_transform(x, bins).reshape(-1)
+    assert cur.tobytes() == want.tobytes()
