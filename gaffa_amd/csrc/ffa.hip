@@ -12,19 +12,22 @@
 //             workgroup stages a contiguous input span in LDS with coalesced
 //             loads, then each output is summed serially in the CPU's FP32
 //             order
-//   subtree   one workgroup per (node at the task's LDS depth D, series): the
-//             node's rows live in LDS and every level below it runs there,
-//             ping-ponging between two LDS buffers
-//   level     one launch per depth D-1 .. 1: merges through L2 / HBM
+//   subtree   persistent, one workgroup per CU walking (node at the task's
+//             LDS depth D, series) units: the node's rows live in LDS and
+//             every level below it runs there, ping-ponging between two LDS
+//             buffers, with merges from host-built tables
+//   level     one launch per output depth: one to three levels per pass
+//             through L2 / HBM
 //   terminal  one workgroup per (32 final rows, series): the root merge into
 //             LDS (rows padded with a circular extension), then one lane per
 //             (row, width) runs the serial FP32 prefix twice, P[phi] and
 //             P[phi + w], so every boxcar sum equals the CPU's circular-prefix
 //             difference bit for bit (ffa_detection.cpp:126-239); significant
 //             peaks are appended with one atomic each.
-// In every merge kernel the (head row, tail row, roll) descriptor of each
-// output row is computed once, by one lane, into LDS; the waves then stream
-// whole rows (lanes over phase bins) reading only that descriptor.
+// In the level and terminal kernels the (head row, tail row, roll)
+// descriptor of each output row is computed once, by one lane, into LDS; the
+// subtree kernel reads its merges from host-built tables.  The waves then
+// stream whole rows (lanes over phase bins) reading only those.
 #include "common.hpp"
 
 #include <cmath>
