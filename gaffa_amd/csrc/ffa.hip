@@ -450,7 +450,7 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     // `store`, which is written once per destination kind so every access
     // keeps its address space (LDS rows are ds_* operations, the top level's
     // stores global_*).  The table gives each wave its rows w, w + 16, ...,
-    // padded with repeats of its last row to whole passes.  A row is swept in
+    // padded to whole passes with entries that are skipped.  A row is swept in
     // blocks of 256 phases, NC <= 4 chunks of 64 lanes.  The roll splits it at
     // w = bins - roll: phases p < w read tail[p + roll], phases p >= w read
     // tail[p + roll - b].  Per chunk a lane picks one of two tail addresses
@@ -505,13 +505,23 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
 #pragma unroll
               for (uint32_t k = 0; k < R; ++k) g[k] = load_merge(m + (ps + 1) * R + k);
             }
-            // every load of the pass is issued before the first add waits
-            float hv[R][NC], tv[R][NC];
+            // every load of the pass is issued before the first add waits;
+            // a wave with fewer rows than the longest list skips its padding
+            // (the last one or two entries of its list)
+            static_assert(R == 2, "the padding cases below assume two rows per pass");
+            if (cg[1].out != GAFFA_HIP_FFA_SKIP_ROW) {
+              float hv[R][NC], tv[R][NC];
 #pragma unroll
-            for (uint32_t k = 0; k < R; ++k) load_row(cg[k], hv[k], tv[k]);
-            __builtin_amdgcn_sched_barrier(0);
+              for (uint32_t k = 0; k < R; ++k) load_row(cg[k], hv[k], tv[k]);
+              __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (uint32_t k = 0; k < R; ++k) store_row(cg[k].out, hv[k], tv[k]);
+              for (uint32_t k = 0; k < R; ++k) store_row(cg[k].out, hv[k], tv[k]);
+            } else if (cg[0].out != GAFFA_HIP_FFA_SKIP_ROW) {
+              float hv[NC], tv[NC];
+              load_row(cg[0], hv, tv);
+              __builtin_amdgcn_sched_barrier(0);
+              store_row(cg[0].out, hv, tv);
+            }
           }
         };
         if (nc == 4) pass(std::integral_constant<uint32_t, 4>{});
@@ -1087,12 +1097,14 @@ uint32_t gaffa_hip_ffa_subtree_merges(uint32_t rows, uint32_t bins, gaffa_hip_ff
   for (uint32_t d = 0; d < levels; ++d) {
     for (uint32_t w = 0; w < kSubtreeWaves; ++w) {
       for (uint32_t k = 0; k < per_wave; ++k) {
-        // row w + 16 k, or the wave's last row (row rows - 1 for a wave with
-        // none: its entries are never read)
-        uint32_t r = w + kSubtreeWaves * k;
-        if (r >= rows) r = w < rows ? w + (rows - 1 - w) / kSubtreeWaves * kSubtreeWaves : rows - 1;
-        const Node nd = find_node(0, rows, d, r);
+        // row w + 16 k, or padding past the wave's last row
+        const uint32_t r = w + kSubtreeWaves * k;
         gaffa_hip_ffa_merge& m = out[d * level_size + w * per_wave + k];
+        if (r >= rows) {
+          m = gaffa_hip_ffa_merge{0, GAFFA_HIP_FFA_COPY_ROW, bins, GAFFA_HIP_FFA_SKIP_ROW};
+          continue;
+        }
+        const Node nd = find_node(0, rows, d, r);
         m.out = r * bins;
         if (nd.size < 2) {  // a leaf above depth d: copied down unchanged
           m.head = r * bins;

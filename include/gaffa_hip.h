@@ -322,17 +322,18 @@ typedef struct gaffa_hip_ffa_peak {
  * tail[(i + roll) mod bins] in float offsets from the level's row buffer
  * (16 bytes).  A subtree item's table holds, per level (level 0, the node's
  * own merge, first), 16 wave lists of 2 * ceil(ceil(rows / 16) / 2) merges:
- * wave w's rows w, w + 16, ..., padded with repeats of its last row.  The
- * item's `c` field is the index of its first entry; items with the same
- * rows and bins share a table.  No gaffa counterpart: gaffa recurses per row
- * (ffa.cpp:110-251). */
+ * wave w's rows w, w + 16, ..., then padding entries (out =
+ * GAFFA_HIP_FFA_SKIP_ROW) up to the list length.  The item's `c` field is
+ * the index of its first entry; items with the same rows and bins share a
+ * table.  No gaffa counterpart: gaffa recurses per row (ffa.cpp:110-251). */
 #define GAFFA_HIP_FFA_COPY_ROW 0xFFFFFFFFu
+#define GAFFA_HIP_FFA_SKIP_ROW 0xFFFFFFFFu
 typedef struct gaffa_hip_ffa_merge {
   uint32_t head; /* head row * bins */
   uint32_t tail; /* tail row * bins + roll, or GAFFA_HIP_FFA_COPY_ROW: the
                     row is a leaf above this level and is copied */
   uint32_t wrap; /* bins - roll: phases whose tail index does not wrap */
-  uint32_t out;  /* row * bins */
+  uint32_t out;  /* row * bins, or GAFFA_HIP_FFA_SKIP_ROW: a padding entry */
 } gaffa_hip_ffa_merge;
 
 typedef struct gaffa_hip_ffa_plan_desc {
