@@ -32,10 +32,12 @@ UNIT = "DM-trial·samples/s"
 HBM_PEAK_GBS = 8000.0
 
 
-def global_passes(depth: int) -> int:
-    """Global-memory merge passes of a task whose LDS subtrees sit at `depth`:
-    depth - 1 tree levels, three per pass, two where two or four remain."""
-    rem, passes = max(depth - 1, 0), 0
+def global_passes(depth: int, terminal_levels: int = 1) -> int:
+    """Global-memory merge passes of a task whose LDS subtrees sit at `depth`
+    and whose detection kernel merges `terminal_levels` levels itself:
+    depth - terminal_levels tree levels, three per pass, two where two or four
+    remain."""
+    rem, passes = max(depth - terminal_levels, 0), 0
     while rem > 0:
         rem -= 2 if rem in (2, 4) else min(rem, 3)
         passes += 1
@@ -53,10 +55,12 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
     subtree     per series and task with rows >= 2: read + write its rows x bins
                 floats once (8 rows bins)
     levels      per series and task: 8 rows bins for each pass through global
-                memory (global_passes(depth_t) of them: a pass merges three tree
-                levels, two where two or four remain), averaged over the launches
-    detect      per series and task: the depth-1 rows once (4 rows bins; the input
-                series row for 1-row tasks) + 20 B per peak
+                memory (global_passes(depth_t, terminal_levels_t) of them: a pass
+                merges three tree levels, two where two or four remain), averaged
+                over the launches
+    detect      per series and task: the rows it merges from (depth 1, or depth 2
+                where it merges two levels) once (4 rows bins; the input series
+                row for 1-row tasks) + 20 B per peak
     """
     import numpy as np
 
@@ -71,7 +75,8 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
         if f != 1.0:
             prep += 4.0 * float(t["prepared_nsamples"][t["downsample_factor"] == f][0])
     subtree = 8.0 * float(rb[t["rows"] >= 2].sum())
-    passes = np.array([global_passes(int(d)) for d in depth], np.float64)
+    tlev = lay["terminal_levels"]
+    passes = np.array([global_passes(int(d), int(k)) for d, k in zip(depth, tlev)], np.float64)
     levels = 8.0 * float((rb * passes).sum())
     detect = 4.0 * float(rb.sum())
     nlev = max(1, lay["level_launches"])

@@ -754,7 +754,7 @@ template <bool kDetect>
 __global__ void __launch_bounds__(kTerminalThreads)
 ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   extern __shared__ float lds[];
-  __shared__ Desc sdesc[kTerminalRows];
+  __shared__ Desc2 sdesc[kTerminalRows];  // one-level merges use a, b, rh
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.terminal_items[blockIdx.x];
   if (!kDetect && item.task != only_task) return;
@@ -773,33 +773,83 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   for (uint32_t w = 0; w < task.nwidths; ++w) maxw = max(maxw, task.width[w]);
   const uint32_t sb = terminal_stride(b, maxw);
 
+  // depth of the rows read: 1, or 2 when this kernel merges two levels
+  const uint32_t tl = task.rows >= 2 && task.terminal_levels == 2 ? 2u : 1u;
   const float* src;
   if (task.rows == 1) {
     src = task_input(a, group, sidx);
   } else {
-    src = a.work + static_cast<uint64_t>((task.regions >> 1) & 1u) * a.nseries * plan.work_stride +
+    src = a.work + static_cast<uint64_t>((task.regions >> tl) & 1u) * a.nseries * plan.work_stride +
           sidx * plan.work_stride +
           task.work_offset;
     if (threadIdx.x < nrows) {
       const uint32_t s = s0 + threadIdx.x;
-      sdesc[threadIdx.x] = merge_desc(Node{0, task.rows}, s, b);
+      const Desc top = merge_desc(Node{0, task.rows}, s, b);
+      if (tl == 1) {
+        sdesc[threadIdx.x] = Desc2{top.h, top.t, top.roll, 0, 0, 0, 0};
+      } else {
+        const uint32_t hn = task.rows / 2;
+        const Desc dh = merge_desc(Node{0, hn}, top.h, b);
+        const Desc dt = merge_desc(Node{hn, task.rows - hn}, top.t, b);
+        sdesc[threadIdx.x] = Desc2{dh.h, dh.t, dh.roll, dt.h, dt.t, dt.roll, top.roll};
+      }
     }
     __syncthreads();
   }
   if (task.rows == 1) {
     for (uint32_t p = threadIdx.x; p < b; p += kTerminalThreads) lds[p] = src[p];
-  } else {
+  } else if (tl == 1) {
     constexpr uint32_t kBatch = 4;
     for (uint32_t i0 = wave; i0 < nrows; i0 += kBatch * kWaves) {
       merge_batch<kBatch>(
           i0, kWaves, nrows, b, lane,
           [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
-            const Desc ds = sdesc[i];
-            h = src + static_cast<uint64_t>(ds.h) * b;
-            t = src + static_cast<uint64_t>(ds.t) * b;
-            roll = ds.roll;
+            const Desc2 ds = sdesc[i];
+            h = src + static_cast<uint64_t>(ds.a) * b;
+            t = src + static_cast<uint64_t>(ds.b) * b;
+            roll = ds.rh;
           },
           [&](uint32_t i, uint32_t p, float v) { lds[static_cast<size_t>(i) * sb + p] = v; });
+    }
+  } else {
+    // two levels: out[p] = (A[p] + B[p+rh]) + (C[q] + D[q+rt]), q = p + ro
+    // (indices mod b), the FP32 sums of the two one-level merges in order;
+    // two rows per wave at a time, four phases per lane: 32 loads in flight
+    constexpr uint32_t kBatch = 2;
+    for (uint32_t i0 = wave; i0 < nrows; i0 += kBatch * kWaves) {
+      Desc2 e[kBatch];
+#pragma unroll
+      for (uint32_t k = 0; k < kBatch; ++k) e[k] = sdesc[min(i0 + k * kWaves, nrows - 1)];
+      for (uint32_t p0 = lane; p0 < b; p0 += 256) {
+        float v[kBatch][4];
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; ++k) {
+          const float* A = src + static_cast<uint64_t>(e[k].a) * b;
+          const float* B = src + static_cast<uint64_t>(e[k].b) * b;
+          const float* C = src + static_cast<uint64_t>(e[k].c) * b;
+          const float* D = src + static_cast<uint64_t>(e[k].d) * b;
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t p = min(p0 + 64u * j, b - 1);
+            uint32_t pb = p + e[k].rh;
+            pb = min(pb, pb - b);
+            uint32_t q = p + e[k].ro;
+            q = min(q, q - b);
+            uint32_t qd = q + e[k].rt;
+            qd = min(qd, qd - b);
+            v[k][j] = (A[p] + B[pb]) + (C[q] + D[qd]);
+          }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; ++k) {
+          const uint32_t i = i0 + k * kWaves;
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) {
+            const uint32_t p = p0 + 64u * j;
+            if (i < nrows && p < b) lds[static_cast<size_t>(i) * sb + p] = v[k][j];
+          }
+        }
+      }
     }
   }
   __syncthreads();

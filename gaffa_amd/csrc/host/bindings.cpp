@@ -1385,8 +1385,11 @@ PYBIND11_MODULE(_core, m) {
           "ffa_layout",
           [](const GpuSearchPipeline& p) {
             const GpuFfaExecutionPlan& e = p.ffa_program().execution_plan();
-            std::vector<std::uint32_t> depth;
-            for (const auto& t : e.tasks()) depth.push_back(t.depth);
+            std::vector<std::uint32_t> depth, terminal_levels;
+            for (const auto& t : e.tasks()) {
+              depth.push_back(t.depth);
+              terminal_levels.push_back(t.terminal_levels);
+            }
             std::size_t level_launches = 0;
             const auto& off = e.level_offsets();
             for (std::size_t d = 1; d + 1 < off.size(); ++d)
@@ -1394,6 +1397,8 @@ PYBIND11_MODULE(_core, m) {
             py::dict d;
             d["task_depth"] = owning_array(std::move(depth),
                                            {static_cast<py::ssize_t>(e.tasks().size())});
+            d["terminal_levels"] = owning_array(
+                std::move(terminal_levels), {static_cast<py::ssize_t>(e.tasks().size())});
             d["max_depth"] = e.max_depth();
             d["level_launches"] = level_launches;
             d["subtree_items"] = e.subtree_items().size();

@@ -151,6 +151,7 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
     check_arg(stdnoise > 0.0F && std::isfinite(stdnoise),
               "FFA row detection stdnoise must be finite and > 0");
     gaffa_hip_ffa_task d{};
+    d.terminal_levels = 1;
     d.group = group_of[t.downsample_factor];
     d.bins = static_cast<std::uint32_t>(t.bins);
     d.rows = static_cast<std::uint32_t>(t.rows);
@@ -210,14 +211,27 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       subtree_lds_bytes_ =
           static_cast<std::uint32_t>(8u * subtree_half_ + subtree_fixed_lds(lds_max_bins_));
       if (by_depth_.size() < depth) by_depth_.resize(depth);
-      // global passes from depth `depth` up to depth 1: three tree levels per
-      // pass, two where two or four remain (2 + 2 reads less than 3 + 1); the
-      // region bits make every pass read one work region and write the
-      // other.  Every node above the LDS depth has two rows or more (the
-      // depth is the first whose nodes fit), as a multi-level pass needs.
+      // global passes from depth `depth` up to depth `tl`: three tree levels
+      // per pass, two where two or four remain (2 + 2 reads less than
+      // 3 + 1); the region bits make every pass read one work region and
+      // write the other.  Every node above the LDS depth has two rows or
+      // more (the depth is the first whose nodes fit), as a multi-level pass
+      // and a two-level detection merge need.  The detection kernel merges
+      // the top two levels itself (tl = 2) where that saves a pass: one, four
+      // or seven levels above depth 1.
+      const auto passes = [](std::uint32_t levels) {
+        std::uint32_t n = 0;
+        while (levels > 0) {
+          levels -= levels == 4 ? 2u : std::min<std::uint32_t>(levels, 3u);
+          ++n;
+        }
+        return n;
+      };
+      const std::uint32_t tl = depth >= 2 && passes(depth - 2) < passes(depth - 1) ? 2u : 1u;
+      d.terminal_levels = tl;
       std::uint32_t regions = 0;  // the subtrees write region 0
-      for (std::uint32_t din = depth; din > 1;) {
-        const std::uint32_t rem = din - 1;
+      for (std::uint32_t din = depth; din > tl;) {
+        const std::uint32_t rem = din - tl;
         const std::uint32_t levels = rem == 4 ? 2u : std::min<std::uint32_t>(rem, 3u);
         const std::uint32_t dout = din - levels;
         regions |= (1u - ((regions >> din) & 1u)) << dout;

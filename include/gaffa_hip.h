@@ -295,6 +295,10 @@ typedef struct gaffa_hip_ffa_task {
                                periodogram (plan order, rows_eval each) */
   uint32_t regions;         /* bit k: work region (0 or 1) holding the
                                depth-k rows of this task's tree */
+  uint32_t terminal_levels; /* tree levels the detection kernel merges from
+                               global memory: 1 (it reads the depth-1 rows)
+                               or 2 (depth-2 rows, where that saves a global
+                               pass); 2 needs depth >= 2 */
   uint32_t width[GAFFA_HIP_FFA_MAX_WIDTHS];
   float height[GAFFA_HIP_FFA_MAX_WIDTHS];
   float baseline[GAFFA_HIP_FFA_MAX_WIDTHS];

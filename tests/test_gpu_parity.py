@@ -286,13 +286,18 @@ def test_ffa_rows_and_search_vs_golden(gpu):
         assert p.period == q["period"] and p.frequency == q["frequency"]
 
 
-@pytest.mark.parametrize("rows_bins", [(1, 5), (2, 7), (3, 4), (5, 4), (163, 181), (700, 256),
-                                       (2679, 180), (2657, 180), (1203, 200), (1090, 256),
-                                       (9000, 3), (5003, 2)])
+@pytest.mark.parametrize("rows_bins", [(1, 5), (2, 7), (3, 4), (5, 4), (163, 181), (300, 181),
+                                       (700, 256), (2679, 180), (2657, 180), (1203, 200),
+                                       (1090, 256), (9000, 3), (5003, 2), (70000, 2)])
 def test_ffa_transform_shapes_bit_exact(gpu, port, rows_bins):
-    """Single-task plans over odd shapes: LDS-only trees, forced global
-    levels, deep trees (9000 x 3, 5003 x 2: five and six global levels), and
-    level work items of 1, 2 and 3 rows (rows % 16 in {1, 2, 3})."""
+    """Single-task plans over odd shapes: LDS-only trees (LDS depth 1),
+    LDS depth 2 with the top two levels merged by the detection kernel
+    (300 x 181), three global levels in one pass (700 x 256, 1203 x 200,
+    1090 x 256, 5003 x 2), four global levels of which the detection kernel
+    merges the top two (2679 x 180, 2657 x 180, 9000 x 3), seven levels
+    (70000 x 2: depth 8, one three-level pass, a second, then two levels in
+    the detection kernel), and level work items of 1, 2 and 3 rows
+    (rows % 16 in {1, 2, 3})."""
     from oracle.oracle import TASK_DTYPE
 
     rows, bins = rows_bins
