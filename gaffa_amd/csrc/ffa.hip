@@ -18,8 +18,9 @@
 //             buffers, with merges from host-built tables
 //   level     one launch per output depth: one to three levels per pass
 //             through L2 / HBM
-//   terminal  one workgroup per (32 final rows, series): the root merge into
-//             LDS (rows padded with a circular extension), then one lane per
+//   terminal  one workgroup per (32 final rows, series): the root merge, or
+//             the top two levels where that saves a global pass, into LDS
+//             (rows padded with a circular extension), then one lane per
 //             (row, width) runs the serial FP32 prefix twice, P[phi] and
 //             P[phi + w], so every boxcar sum equals the CPU's circular-prefix
 //             difference bit for bit (ffa_detection.cpp:126-239); significant
