@@ -356,6 +356,34 @@ def test_ffa_batch_search_bit_exact(gpu, port):
     assert abs(best.period - 0.714) < 0.003
 
 
+@pytest.mark.parametrize("n", [(1 << 20) + 3, (1 << 22) - 1000])
+def test_ffa_search_two_level_detection_bit_exact(gpu, port, n):
+    """Search plans whose trees are deep enough that the detection kernel
+    merges the top two levels itself: at 64 us, periods 0.1-0.45 s and bins
+    180-256, 2^20 samples give LDS depth 2 for the second octave (no global
+    pass at all) and 2^22 give depth 5 for the first (one three-level pass
+    instead of 2 + 2), beside tasks that keep the one-level merge.  Every
+    peak equals the oracle's."""
+    say(f"ffa: two-level detection merge, n={n}")
+    tsamp = 64e-6
+    rng = np.random.default_rng(n % 1000)
+    x = rng.normal(size=n).astype(np.float32)
+    ph = (np.arange(n) * tsamp / 0.1703) % 1.0
+    x += (0.6 * np.exp(-0.5 * ((ph - 0.4) / 0.01) ** 2)).astype(np.float32)
+    plan = gpu._make_riptide_ffa_plan(nsamples=n, tsamp=tsamp, period_min=0.1, period_max=0.45,
+                                      bins_min=180, bins_max=256, duty_cycle_max=0.18,
+                                      width_trial_spacing=2.0)
+    got = gpu._ffa_search_batch_gpu(x[None, :], plan, snr_threshold=6.0, series_tile=1)
+    want = port.search_ffa(x, Plan(plan.tasks, np.array(plan.width_trials, np.uint64)), 6.0)
+    assert len(got) == len(want) and len(want) > 0
+    for (i, p), q in zip(got, want):
+        assert i == 0
+        assert (p.shift, p.phase, p.width, p.bins, np.float32(p.snr)) == \
+            (q["shift"], q["phase"], q["width"], q["bins"], q["snr"])
+    best = max((p for _, p in got), key=lambda p: p.snr)
+    assert abs(best.period - 0.1703) < 0.001
+
+
 def test_ffa_max_peaks_guard(gpu):
     say("ffa: max_peaks guard")
     from gaffa_amd import ffa
