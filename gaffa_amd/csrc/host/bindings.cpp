@@ -1025,7 +1025,7 @@ PYBIND11_MODULE(_core, m) {
          const py::array_t<std::uint32_t, py::array::c_style>& phase,
          const py::array_t<std::uint32_t, py::array::c_style>& width_index,
          const py::array_t<float, py::array::c_style>& snr, const std::vector<double>& dms,
-         bool ranked) {
+         bool ranked, std::size_t parts) {
         const std::size_t n = static_cast<std::size_t>(series.size());
         if (task.size() != series.size() || shift.size() != series.size() ||
             phase.size() != series.size() || width_index.size() != series.size() ||
@@ -1043,12 +1043,31 @@ PYBIND11_MODULE(_core, m) {
         std::vector<DmPeak> out;
         const std::vector<std::uint32_t> ranks =
             ranked ? ffa_period_ranks(plan) : std::vector<std::uint32_t>{};
-        expand_sorted_dm_peaks_into(e, c, dms, out, ranks);
+        if (parts <= 1) {
+          expand_sorted_dm_peaks_into(e, c, dms, out, ranks);
+          return out;
+        }
+        // as the search pipeline does it: keys sorted and peaks expanded
+        // per part, then one merge
+        std::vector<DmPeakKey> keys(n), scratch;
+        std::vector<DmPeak> expanded(n);
+        std::vector<std::size_t> ends;
+        for (std::size_t k = 0; k < parts; ++k) {
+          const std::size_t b = n * k / parts, f = n * (k + 1) / parts;
+          const std::span<const gaffa_hip_ffa_peak> part(c.data() + b, f - b);
+          dm_peak_keys(e, part, b, dms, ranks, keys.data() + b);
+          sort_dm_peak_keys(std::span<DmPeakKey>(keys.data() + b, f - b));
+          expand_dm_peaks(e, part, dms, expanded.data() + b);
+          ends.push_back(f);
+        }
+        merge_sorted_dm_peaks(expanded, ends, keys, scratch, out);
         return out;
       },
       py::arg("plan"), py::arg("series"), py::arg("task"), py::arg("shift"), py::arg("phase"),
       py::arg("width_index"), py::arg("snr"), py::arg("dms"), py::arg("ranked") = true,
-      "Host-side expansion + is_better_dm_peak ordering of compact GPU peaks (no device use)");
+      py::arg("parts") = 1,
+      "Host-side expansion + is_better_dm_peak ordering of compact GPU peaks (no device use); "
+      "parts > 1 sorts and expands contiguous parts separately and merges them");
   m.def(
       "_find_dm_peaks_gpu",
       [](const py::array& plane, const std::vector<double>& dms, double tsamp, double period_min,

@@ -640,6 +640,74 @@ void finish_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
         expand_dm_peak(tasks, det, dms, peaks[keys[static_cast<std::size_t>(i)].index]);
 }
 
+void expand_dm_peaks(const GpuFfaExecutionPlan& plan, std::span<const gaffa_hip_ffa_peak> peaks,
+                     std::span<const double> dms, DmPeak* out) {
+  const auto& tasks = plan.plan().tasks;
+  std::map<std::size_t, FfaDetectionPlan> by_bins;
+  const std::vector<const FfaDetectionPlan*> det = detection_plans(plan, by_bins);
+  const std::int64_t n = static_cast<std::int64_t>(peaks.size());
+#pragma omp parallel for schedule(static)
+  for (std::int64_t i = 0; i < n; ++i)
+    out[i] = expand_dm_peak(tasks, det, dms, peaks[static_cast<std::size_t>(i)]);
+}
+
+void sort_dm_peak_keys(std::span<DmPeakKey> keys) {
+  __gnu_parallel::sort(keys.begin(), keys.end(),
+                       [](const DmPeakKey& a, const DmPeakKey& b) { return a.key < b.key; });
+}
+
+void merge_sorted_dm_peaks(std::span<const DmPeak> expanded, std::span<const std::size_t> run_ends,
+                           std::vector<DmPeakKey>& keys, std::vector<DmPeakKey>& scratch,
+                           std::vector<DmPeak>& out) {
+  check_arg(keys.size() == expanded.size(), "one sort key per peak");
+  check_arg(!run_ends.empty() && run_ends.back() == keys.size(), "sorted parts must cover the keys");
+  const std::size_t n = keys.size();
+  const auto by_key = [](const DmPeakKey& a, const DmPeakKey& b) { return a.key < b.key; };
+  if (run_ends.size() > 1) {
+    std::vector<std::pair<DmPeakKey*, DmPeakKey*>> parts;
+    std::size_t begin = 0;
+    for (const std::size_t end : run_ends) {
+      check_arg(end >= begin, "sorted parts must be in order");
+      if (end > begin) parts.emplace_back(keys.data() + begin, keys.data() + end);
+      begin = end;
+    }
+    scratch.resize(n);
+    __gnu_parallel::multiway_merge(parts.begin(), parts.end(), scratch.begin(), n, by_key);
+    keys.swap(scratch);
+  }
+  // equal-key runs, found and ordered in parallel (each run is independent)
+  const std::int64_t sn = static_cast<std::int64_t>(n);
+  std::vector<std::pair<std::size_t, std::size_t>> runs;
+#pragma omp parallel
+  {
+    std::vector<std::pair<std::size_t, std::size_t>> mine;
+#pragma omp for schedule(static) nowait
+    for (std::int64_t s = 0; s < sn; ++s) {
+      const std::size_t i = static_cast<std::size_t>(s);
+      if (i > 0 && keys[i].key == keys[i - 1].key) continue;  // not a run start
+      std::size_t j = i + 1;
+      while (j < n && keys[j].key == keys[i].key) ++j;
+      if (j - i > 1) mine.emplace_back(i, j);
+    }
+#pragma omp critical
+    runs.insert(runs.end(), mine.begin(), mine.end());
+  }
+  const std::int64_t nruns = static_cast<std::int64_t>(runs.size());
+#pragma omp parallel for schedule(dynamic, 64)
+  for (std::int64_t r = 0; r < nruns; ++r) {
+    const auto [i, j] = runs[static_cast<std::size_t>(r)];
+    std::sort(keys.begin() + static_cast<std::ptrdiff_t>(i),
+              keys.begin() + static_cast<std::ptrdiff_t>(j),
+              [&](const DmPeakKey& a, const DmPeakKey& b) {
+                return is_better_dm_peak(expanded[a.index], expanded[b.index]);
+              });
+  }
+  out.resize(n);
+#pragma omp parallel for schedule(static)
+  for (std::int64_t i = 0; i < sn; ++i)
+    out[static_cast<std::size_t>(i)] = expanded[keys[static_cast<std::size_t>(i)].index];
+}
+
 FfaBatchSearchResult run_ffa_batch_gpu(GpuFfaProgram& program, GpuTimeSeriesBatchView batch,
                                        const FfaSearchOptions& options) {
   check_arg(std::isfinite(options.snr_threshold), "FFA search S/N threshold must be finite");

@@ -252,6 +252,17 @@ void finish_sorted_dm_peaks(const GpuFfaExecutionPlan& plan,
                             std::span<const gaffa_hip_ffa_peak> peaks,
                             std::span<const double> dms, std::vector<DmPeakKey>& keys,
                             std::vector<DmPeak>& out);
+// The same order built part by part: expand_dm_peaks fills out[i] for
+// peaks[i]; merge_sorted_dm_peaks takes keys already sorted within each part
+// (part k ends at run_ends[k]) and `expanded` indexed like the keys, merges
+// the parts, orders equal-key runs with is_better_dm_peak and gathers the
+// records into `out`.  `scratch` is working memory.
+void expand_dm_peaks(const GpuFfaExecutionPlan& plan, std::span<const gaffa_hip_ffa_peak> peaks,
+                     std::span<const double> dms, DmPeak* out);
+void sort_dm_peak_keys(std::span<DmPeakKey> keys);
+void merge_sorted_dm_peaks(std::span<const DmPeak> expanded, std::span<const std::size_t> run_ends,
+                           std::vector<DmPeakKey>& keys, std::vector<DmPeakKey>& scratch,
+                           std::vector<DmPeak>& out);
 // Dense rank of every evaluated (task, shift) by trial period (equal periods
 // share a rank), indexed by the task's first periodogram row + shift; empty
 // when a period is not finite or there are 2^31 rows or more.

@@ -164,6 +164,9 @@ class GpuSearchPipeline {
   std::vector<gaffa_hip_event> tile_done_;  // after each tile's count copy
   GpuStream copy_stream_;         // peak downloads beside the search
   std::vector<DmPeakKey> peak_keys_;
+  std::vector<DmPeakKey> merge_scratch_;
+  std::vector<DmPeak> expanded_;        // peaks in arrival order, expanded per tile
+  std::vector<std::size_t> run_ends_;   // end of each tile's sorted keys
   std::vector<std::uint32_t> period_ranks_;  // ffa_period_ranks(ffa_plan_)
   std::vector<double> trial_periods_;
   std::vector<std::uint32_t> trial_order_;

@@ -247,10 +247,11 @@ void GpuSearchPipeline::search() {
     marks_used_ = kernel_timing_ ? ntiles : 0;
     hip_ok(gaffa_hip_event_record(events_[4], stream_.get()), "hipEventRecord");
     // While the device searches tile k + 1 and on, the host brings tile k's
-    // peaks back (they follow tile k - 1's in the append-only buffer) and
-    // computes their sort keys.
+    // peaks back (they follow tile k - 1's in the append-only buffer),
+    // computes and sorts their keys and expands them; the end only merges.
     std::size_t done = 0;
     bool overflow = false;
+    run_ends_.clear();
     for (std::size_t k = 0; k < ntiles; ++k) {
       hip_ok(gaffa_hip_event_synchronize(tile_done_[k]), "hipEventSynchronize");
       const std::size_t c = counts[k];
@@ -260,6 +261,7 @@ void GpuSearchPipeline::search() {
       }
       if (c == done) continue;
       if (peak_keys_.size() < c) peak_keys_.resize(std::max(c, peak_keys_.size() * 2));
+      if (expanded_.size() < c) expanded_.resize(std::max(c, expanded_.size() * 2));
       if (compact_host_.bytes() < c * sizeof(gaffa_hip_ffa_peak)) {
         // grow the pinned buffer, keeping the peaks already brought back
         GpuPinnedBuffer bigger(std::min(capacity, c + c / 2) * sizeof(gaffa_hip_ffa_peak));
@@ -269,8 +271,11 @@ void GpuSearchPipeline::search() {
       auto* compact = static_cast<gaffa_hip_ffa_peak*>(compact_host_.data());
       prog.enqueue_peak_copy(done, c - done, compact + done, copy_stream_.get());
       copy_stream_.synchronize();
-      dm_peak_keys(prog.execution_plan(), std::span<const gaffa_hip_ffa_peak>(compact + done, c - done),
-                   done, dms_, period_ranks_, peak_keys_.data() + done);
+      const std::span<const gaffa_hip_ffa_peak> part(compact + done, c - done);
+      dm_peak_keys(prog.execution_plan(), part, done, dms_, period_ranks_, peak_keys_.data() + done);
+      sort_dm_peak_keys(std::span<DmPeakKey>(peak_keys_.data() + done, c - done));
+      expand_dm_peaks(prog.execution_plan(), part, dms_, expanded_.data() + done);
+      run_ends_.push_back(c);
       done = c;
     }
     if (overflow) {
@@ -279,10 +284,9 @@ void GpuSearchPipeline::search() {
       continue;
     }
     peak_keys_.resize(done);
-    finish_sorted_dm_peaks(
-        prog.execution_plan(),
-        std::span<const gaffa_hip_ffa_peak>(static_cast<const gaffa_hip_ffa_peak*>(compact_host_.data()), done),
-        dms_, peak_keys_, peaks_);
+    if (run_ends_.empty()) run_ends_.push_back(0);
+    merge_sorted_dm_peaks(std::span<const DmPeak>(expanded_.data(), done), run_ends_, peak_keys_,
+                          merge_scratch_, peaks_);
     break;
   }
 }

@@ -131,13 +131,15 @@ def test_workload_shapes():
     assert steps == sorted(steps) and all(s.ndm % 32 == 0 for s in w.dm_segments)
 
 
-@pytest.mark.parametrize("ranked", [True, False])
-def test_dm_peak_key_sort_matches_comparator(core, ranked):
+@pytest.mark.parametrize("ranked,parts", [(True, 1), (False, 1), (True, 7), (False, 3)])
+def test_dm_peak_key_sort_matches_comparator(core, ranked, parts):
     """expand_sorted_dm_peaks sorts (key, index) pairs and fixes equal-key runs
     with the full comparator: the order must equal a plain sort by
     is_better_dm_peak (finite first, S/N desc, period, frequency, width, phase,
     shift, bins asc, then dm_index, dm), ties and +-0 included, with and
-    without the period ranks in the key."""
+    without the period ranks in the key.  With parts > 1 the keys are sorted
+    and the peaks expanded part by part and then merged, as the search
+    pipeline does per FFA tile."""
     plan = core._make_riptide_ffa_plan(nsamples=130870, tsamp=256e-6, period_min=0.2,
                                        period_max=2.0, duty_cycle_max=0.09,
                                        width_trial_spacing=2.0)
@@ -154,7 +156,7 @@ def test_dm_peak_key_sort_matches_comparator(core, ranked):
     snr = rng.choice(np.array([7.5, 7.25, 6.0, 0.0, -0.0, 12.125], np.float32), n)
     dms = list(np.arange(64) * 0.5)
     got = core._expand_sorted_dm_peaks(plan, series, task, shift, phase, widx, snr, dms,
-                                       ranked=ranked)
+                                       ranked=ranked, parts=parts)
     assert len(got) == n
 
     def key(p):
