@@ -48,8 +48,8 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
     """Algorithmic HBM bytes per launch of each hot kernel (the minimum any
     implementation must move), keyed like the stage times:
 
-    dedisperse  one launch per DM segment: the uint8 filterbank once
-                (nsamples * nchans) + the segment's uint32 rows (4 * ndm_seg * n_out)
+    dedisperse  one launch for every DM segment: the uint8 filterbank once
+                (nsamples * nchans) + the uint32 plane once (4 * ndm * n_out)
     prepare     per series: the float series once (4 n) + every downsampled copy
                 (4 P_g per downsample group)
     subtree     per series and task with rows >= 2: read + write its rows x bins
@@ -80,9 +80,10 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
     levels = 8.0 * float((rb * passes).sum())
     detect = 4.0 * float(rb.sum())
     nlev = max(1, lay["level_launches"])
-    nseg = len(workload.dm_segments)
+    dd_launches = 1 if workload.subband else len(workload.dm_segments)
     return {
-        "dedisperse": (nseg, workload.nsamples * workload.nchans + 4.0 * (ndm / nseg) * n_out),
+        "dedisperse": (dd_launches, (workload.nsamples * workload.nchans + 4.0 * ndm * n_out)
+                       / dd_launches),
         "ffa_prepare": (ntiles, prep * series_per_tile),
         "ffa_subtree": (ntiles, subtree * series_per_tile),
         "ffa_levels": (ntiles * nlev, levels * series_per_tile / nlev),
@@ -91,7 +92,7 @@ def kernel_bytes(workload, pipe, ntiles: int, npeaks: int) -> dict:
 
 
 KERNEL_NAMES = {
-    "dedisperse": "dedisperse_subband_cm_kernel",
+    "dedisperse": "dedisperse_subband_stream_kernel",
     "ffa_prepare": "ffa_prepare_fused_kernel",
     "ffa_subtree": "ffa_subtree_kernel",
     "ffa_levels": "ffa_level_kernel",

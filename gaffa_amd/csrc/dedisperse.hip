@@ -7,7 +7,8 @@
 // pair + v_alignbyte and accumulates them as packed 16-bit pairs (<= 32
 // channels per flush, so no 16-bit lane can overflow).  Subband dedispersion
 // reads a channel-major copy of the filterbank (transposed once, shared by
-// every DM segment) directly from global memory in the same 4-sample form.
+// every DM segment) directly from global memory, 8 samples per lane, and
+// keeps each nominal group's subband sums in a sliding LDS window.
 // Integer sums are order-independent, so results equal gaffa's CPU sums bit
 // for bit (dedispersion.cpp:197-215, 291-329, 351-453).
 #include "common.hpp"
@@ -195,56 +196,168 @@ transpose_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples, uint64_t n
 }
 
 // ------------------------------------------------------------- subband ----
-// One workgroup = (time tile of `tile` outputs, nominal DM group k), blocks
-// ordered nominal-fastest so the groups of one tile share the channel rows
-// in L2.  Stage 1 builds inter[s][j] for j < jlen4 (tile + max_residual +
-// pad) as u16 in LDS straight from the channel-major rows: lane w sums the
-// 4 samples [4w, 4w + 4) of every channel of the subband with one dword pair
-// + v_alignbyte (the channel's delay shift is wave-uniform) into two packed
-// 16-bit pairs (<= 256 channels x 255 fits 16 bits).  Stage 2 sums the
-// subbands at the residual delays for the group's DMs.
-constexpr int kSbQ = 4;  // stage-1 dword columns per thread (jlen4 <= 4096)
-constexpr int kSbUnroll = 16;  // channels whose loads are in flight together
-constexpr uint32_t kXcds = 8;  // MI355X: 8 XCDs, one L2 each
+// One workgroup = (nominal DM group, time span).  The group's u16 subband
+// intermediate is a sliding window in LDS: subband s keeps
+// [t, t + chunk + h_s) where h_s is the largest residual delay of the
+// group's DMs in that subband (rounded up to 8).  The workgroup walks its
+// span chunk by chunk; each step
+//   1. moves the last h_s samples of every window to its front (no longer
+//      needed samples drop out),
+//   2. stage 1: computes the `chunk` new intermediate samples of every
+//      subband from the channel-major rows: lane w sums samples
+//      [8w, 8w + 8) of each channel of the subband at the group's coarse
+//      delay with one 12-byte load and four v_perm_b32 byte picks into
+//      packed u16 pairs (<= 256 channels x 255 fits 16 bits),
+//   3. stage 2: for each DM of the group, adds the subbands at their
+//      residual delays (lane w: outputs [8w, 8w + 8), five LDS dwords per
+//      subband, packed u16 pairs flushed to u32 before any half can
+//      overflow) and stores the chunk of the DM's output row.
+// The residual halo is therefore computed once per span, not once per
+// chunk, and any residual span that fits the window works: no nominal
+// group is narrowed.  A wave owns the windows of subbands wave,
+// wave + waves, ...: its LDS operations run in order, so step 1 and step 2
+// of one window need no barrier between them.  Workgroups are ordered span-
+// major and group-minor over the XCDs, so the groups running together on one
+// XCD read the same stretch of the channel rows.
+constexpr int kSbThreads = 512;
+constexpr uint32_t kSbWaves = kSbThreads / 64;
+constexpr uint32_t kSbPiece = 512;   // intermediate samples per wave pass (64 lanes x 8)
+constexpr int kSbUnroll = 16;        // channels whose loads are in flight together
+constexpr uint32_t kXcds = 8;        // MI355X: 8 XCDs, one L2 each
+constexpr uint32_t kSbReadPad = 528; // stage-1 bytes a pass reads past its first sample
 
-__global__ void __launch_bounds__(kThreads)
-dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
-                             uint64_t cm_length, uint32_t nsel, uint32_t sbc, uint32_t nsub, uint32_t npn,
-                             uint32_t ndm, uint32_t nnom, uint32_t nblocks,
-                             const int32_t* __restrict__ coarse,
-                             const int32_t* __restrict__ resid,
-                             uint32_t* __restrict__ out, uint64_t out_nsamples,
-                             uint64_t out_stride, uint32_t tile, uint32_t inter_len,
-                             uint32_t flush_every, int32_t* __restrict__ error_flag) {
-  extern __shared__ uint32_t dyn_lds[];
-  // [nsub][inter_len] u16, then the group's residuals [npn][nsub] i32
-  uint16_t* inter = reinterpret_cast<uint16_t*>(dyn_lds);
-  int32_t* sres = reinterpret_cast<int32_t*>(inter + static_cast<size_t>(nsub) * inter_len);
+struct SubbandGroupDev {
+  uint32_t coarse_row, first_dm, ndm, reserved;
+};
+
+__device__ inline uint32_t sb_sel_even(uint32_t sh) {
+  // bytes sh and sh + 2 of {hi:lo} into the low bytes of two u16 halves
+  return 0x0C000C00u | sh | ((sh + 2u) << 16);
+}
+
+// Stage-1 sums of N consecutive channels (rows `stride` bytes apart from
+// rows0, delays kb[0 .. N)) at intermediate samples tb + 8 lane + [0, 8):
+// every row load of the batch is issued before the adds.
+template <int N>
+__device__ inline void sb_channels(const uint8_t* rows0, uint64_t stride,
+                                   const int32_t* __restrict__ kb, uint64_t tb, uint32_t lane,
+                                   uint32_t& a0, uint32_t& a1, uint32_t& a2, uint32_t& a3) {
+  uint32_t x0[N], x1[N], x2[N], sh[N];
+  const uint8_t* rp = rows0;
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const uint64_t b = tb + static_cast<uint32_t>(kb[u]);
+    const uint32_t* row = reinterpret_cast<const uint32_t*>(rp + (b & ~uint64_t{3}));
+    sh[u] = static_cast<uint32_t>(b & 3u);
+    x0[u] = row[2u * lane];
+    x1[u] = row[2u * lane + 1u];
+    x2[u] = row[2u * lane + 2u];
+    rp += stride;
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const uint32_t se = sb_sel_even(sh[u]);
+    const uint32_t so = se + 0x00010001u;  // bytes sh + 1, sh + 3
+    a0 += __builtin_amdgcn_perm(x1[u], x0[u], se);  // samples 0, 2
+    a1 += __builtin_amdgcn_perm(x1[u], x0[u], so);  // samples 1, 3
+    a2 += __builtin_amdgcn_perm(x2[u], x1[u], se);  // samples 4, 6
+    a3 += __builtin_amdgcn_perm(x2[u], x1[u], so);  // samples 5, 7
+  }
+}
+
+__global__ void __launch_bounds__(kSbThreads)
+dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
+                                 uint64_t cm_length, uint32_t nsel, uint32_t sbc, uint32_t nsub,
+                                 const SubbandGroupDev* __restrict__ groups, uint32_t ngroups,
+                                 uint32_t npn, uint32_t ndm, uint32_t n_coarse_rows,
+                                 const int32_t* __restrict__ coarse,
+                                 const int32_t* __restrict__ resid,
+                                 uint32_t* __restrict__ out, uint64_t out_nsamples,
+                                 uint64_t out_stride, uint32_t chunk, uint32_t span,
+                                 uint32_t nspans, uint32_t max_group, uint32_t window_cap,
+                                 uint32_t flush_every, int32_t* __restrict__ error_flag) {
+  extern __shared__ __align__(16) uint32_t dyn_lds[];
+  const uint32_t nsub4 = (nsub + 3u) & ~3u;
+  // [max_group][nsub4] window addresses (u16 index) of each (DM, subband),
+  // then [nsub4] window starts and [nsub4] halos, then the windows
+  int32_t* saddr = reinterpret_cast<int32_t*>(dyn_lds);
+  int32_t* soff = saddr + static_cast<size_t>(max_group) * nsub4;
+  int32_t* shalo = soff + nsub4;
+  uint16_t* win = reinterpret_cast<uint16_t*>(shalo + nsub4);
+  uint32_t* win32 = reinterpret_cast<uint32_t*>(win);
 
   const uint32_t tid = threadIdx.x;
-  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
-  // so XCD x takes the contiguous work range [x * per_xcd, (x + 1) * per_xcd)
-  // and the nominal groups of one tile, and neighbouring tiles, share its L2.
-  const uint32_t nwork = nblocks;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t nwork = ngroups * nspans;
   const uint32_t per_xcd = (nwork + kXcds - 1) / kXcds;
   const uint32_t w_id = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
   if (w_id >= nwork) return;
-  const uint32_t k = w_id % nnom;
-  const uint64_t t0 = static_cast<uint64_t>(w_id / nnom) * tile;
-  const uint32_t dm0 = k * npn;
-  const uint32_t nd = min(npn, ndm - dm0);
-  const uint32_t jlen4 = inter_len - 8;  // stage-1 samples computed, multiple of 4
+  const uint32_t g = w_id % ngroups;
+  const uint64_t t_begin = static_cast<uint64_t>(w_id / ngroups) * span;
+  if (t_begin >= out_nsamples) return;
+  const uint64_t t_end = min(t_begin + span, out_nsamples);
+  uint32_t k, dm_first, nd;
+  if (groups != nullptr) {
+    k = groups[g].coarse_row;
+    dm_first = groups[g].first_dm;
+    nd = groups[g].ndm;
+    // rows outside the tables: refuse the group (uniform)
+    if (k >= n_coarse_rows || dm_first >= ndm || nd > ndm - dm_first) nd = 0;
+  } else {
+    k = g;
+    dm_first = g * npn;
+    nd = min(npn, ndm - dm_first);
+  }
 
   __shared__ int32_t sbad;
   if (tid == 0) sbad = 0;
   __syncthreads();
-  // stage 2 reads inter[j + r .. j + r + 5] with j <= tile - 4: r must stay
-  // within the computed samples or the block is refused (uniformly)
-  const int32_t rmax = static_cast<int32_t>(jlen4 - tile - 4);
-  for (uint32_t e = tid; e < nd * nsub; e += kThreads) {
-    const int32_t r = resid[static_cast<uint64_t>(dm0) * nsub + e];
-    if (r < 0 || r > rmax) sbad = 1;
-    sres[e] = r;
+  if (nd == 0 || nd > max_group) {
+    if (tid == 0) atomicExch(error_flag, 1);
+    return;
+  }
+  for (uint32_t e = tid; e < nd * nsub; e += kSbThreads) {
+    const uint32_t i = e / nsub, s = e % nsub;
+    const int32_t r = resid[static_cast<uint64_t>(dm_first + i) * nsub + s];
+    if (r < 0 || r > (1 << 24)) sbad = 1;
+    saddr[i * nsub4 + s] = r;
+  }
+  __syncthreads();
+  for (uint32_t s = tid; s < nsub; s += kSbThreads) {
+    int32_t h = 0;
+    for (uint32_t i = 0; i < nd; ++i) h = max(h, saddr[i * nsub4 + s]);
+    shalo[s] = (h + 7) & ~7;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t acc = 0;
+    for (uint32_t s = 0; s < nsub; ++s) {
+      soff[s] = static_cast<int32_t>(acc);
+      acc += chunk + static_cast<uint32_t>(shalo[s]) + 8u;
+    }
+    if (acc > window_cap) sbad = 1;
+  }
+  __syncthreads();
+  if (sbad) {
+    if (tid == 0) atomicExch(error_flag, 1);
+    return;
+  }
+  for (uint32_t e = tid; e < nd * nsub; e += kSbThreads) {
+    const uint32_t i = e / nsub, s = e % nsub;
+    saddr[i * nsub4 + s] += soff[s];
+  }
+  // every stage-1 read of this span stays inside the padded rows
+  const uint32_t nchunks = static_cast<uint32_t>((t_end - t_begin + chunk - 1) / chunk);
+  int32_t hmax = 0;
+  for (uint32_t s = 0; s < nsub; ++s) hmax = max(hmax, shalo[s]);  // LDS broadcast reads
+  const int64_t room = static_cast<int64_t>(cm_length) -
+                       static_cast<int64_t>(t_begin + static_cast<uint64_t>(nchunks) * chunk) -
+                       hmax - static_cast<int64_t>(kSbReadPad);
+  const int32_t* kdel = coarse + static_cast<uint64_t>(k) * nsel;
+  for (uint32_t c = tid; c < nsel; c += kSbThreads) {
+    const int32_t d = kdel[c];
+    if (d < 0 || static_cast<int64_t>(d) > room) sbad = 1;
   }
   __syncthreads();
   if (sbad) {
@@ -252,124 +365,93 @@ dedisperse_subband_cm_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
     return;
   }
 
-  // Stage 1: the loads of kSbUnroll channels are issued together.
-  const int32_t* kdel = coarse + static_cast<uint64_t>(k) * nsel;
-  // largest delay whose stage-1 reads stay inside the padded rows, as an
-  // unsigned bound (a negative delay compares above it)
-  const int64_t room = static_cast<int64_t>(cm_length) - static_cast<int64_t>(t0) -
-                       static_cast<int64_t>(jlen4) - 4;
-  if (room < 0) {  // uniform
-    if (tid == 0) atomicExch(error_flag, 1);
-    return;
-  }
-  const uint32_t delay_limit = static_cast<uint32_t>(min<int64_t>(room, 0x7FFFFFFF));
-  uint32_t* inter32 = reinterpret_cast<uint32_t*>(inter);
-  const uint32_t nw = jlen4 / 4;                        // dword columns
-  const uint32_t nq = (nw + kThreads - 1) / kThreads;  // <= kSbQ (host)
-  for (uint32_t s = 0; s < nsub; ++s) {
-    const uint32_t cs = s * sbc;
-    const uint32_t ce = min(cs + sbc, nsel);
-    uint32_t lo[kSbQ], hi[kSbQ];
-#pragma unroll
-    for (int q = 0; q < kSbQ; ++q) lo[q] = hi[q] = 0;
-    for (uint32_t c0 = cs; c0 < ce; c0 += kSbUnroll) {
-      const uint32_t nu = min(static_cast<uint32_t>(kSbUnroll), ce - c0);
-      const uint32_t* rows[kSbUnroll];
-      uint32_t sh[kSbUnroll];
-      // the batch's delays first, all loads in flight together
-      int32_t dl[kSbUnroll];
-#pragma unroll
-      for (int u = 0; u < kSbUnroll; ++u) dl[u] = kdel[min(c0 + u, ce - 1)];  // spare slots repeat
-      bool bad = false;
-#pragma unroll
-      for (int u = 0; u < kSbUnroll; ++u) {
-        const uint32_t c = min(c0 + u, ce - 1);
-        // rows are zero-padded to cm_length; a delay beyond the padding
-        // would read past the row (delay_limit < 0: every delay would)
-        bad |= static_cast<uint32_t>(dl[u]) > delay_limit;
-        const uint64_t base = t0 + static_cast<uint32_t>(dl[u]);
-        rows[u] = reinterpret_cast<const uint32_t*>(cm + static_cast<uint64_t>(c) * cm_stride) +
-                  (base >> 2);
-        sh[u] = static_cast<uint32_t>(base & 3u);
-      }
-      if (bad) {  // uniform: the delays are the same for every lane
-        if (tid == 0) atomicExch(error_flag, 1);
-        return;
-      }
-#pragma unroll
-      for (int q = 0; q < kSbQ; ++q) {
-        if (static_cast<uint32_t>(q) >= nq) break;  // uniform
-        const uint32_t w = min(tid + static_cast<uint32_t>(q) * kThreads, nw - 1);
-        uint32_t x0[kSbUnroll], x1[kSbUnroll];
-#pragma unroll
-        for (int u = 0; u < kSbUnroll; ++u) {
-          x0[u] = rows[u][w];
-          x1[u] = rows[u][w + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < kSbUnroll; ++u) {
-          if (static_cast<uint32_t>(u) < nu) {
-            const uint32_t v = __builtin_amdgcn_alignbyte(x1[u], x0[u], sh[u]);
-            lo[q] += v & 0x00FF00FFu;         // samples 4w, 4w + 2
-            hi[q] += (v >> 8) & 0x00FF00FFu;  // samples 4w + 1, 4w + 3
+  for (uint32_t j = 0; j < nchunks; ++j) {
+    const uint64_t t_out = t_begin + static_cast<uint64_t>(j) * chunk;
+    // Steps 1 and 2, per owned subband window.
+    for (uint32_t s = wave; s < nsub; s += kSbWaves) {
+      const uint32_t h = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(shalo[s]));
+      const uint32_t base = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(soff[s]));
+      if (j > 0) {
+        // forward move in steps of `chunk`: a step's stores land on the
+        // previous step's sources, which this wave has already read
+        for (uint32_t x0 = 0; x0 < h; x0 += chunk) {
+          const uint32_t x = x0 + 8u * lane;
+          if (8u * lane < chunk && x < h) {
+            const uint4 v = *reinterpret_cast<const uint4*>(win + base + chunk + x);
+            *reinterpret_cast<uint4*>(win + base + x) = v;
           }
         }
       }
-    }
-    uint32_t* row = inter32 + static_cast<size_t>(s) * (inter_len / 2);
-#pragma unroll
-    for (int q = 0; q < kSbQ; ++q) {
-      const uint32_t w = tid + static_cast<uint32_t>(q) * kThreads;
-      if (4u * w < jlen4) {
-        row[2 * w] = (lo[q] & 0xFFFFu) | (hi[q] << 16);
-        row[2 * w + 1] = (lo[q] >> 16) | (hi[q] & 0xFFFF0000u);
-      } else if (4u * w < inter_len) {
-        row[2 * w] = 0;
-        row[2 * w + 1] = 0;
+      asm volatile("" ::: "memory");  // the moves' LDS reads stay before stage-1 stores
+      const uint32_t lo = j == 0 ? 0u : h;
+      const uint32_t hi = h + chunk;
+      const uint32_t cs = s * sbc;
+      const uint32_t ce = min(cs + sbc, nsel);
+      for (uint32_t p0 = lo; p0 < hi; p0 += kSbPiece) {
+        const uint64_t tb = t_out + p0;  // intermediate time of the pass's first sample
+        uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        uint32_t c0 = cs;
+        for (; c0 + kSbUnroll <= ce; c0 += kSbUnroll)
+          sb_channels<kSbUnroll>(cm + static_cast<uint64_t>(c0) * cm_stride, cm_stride,
+                                 kdel + c0, tb, lane, a0, a1, a2, a3);
+        for (; c0 < ce; ++c0)
+          sb_channels<1>(cm + static_cast<uint64_t>(c0) * cm_stride, cm_stride, kdel + c0, tb,
+                         lane, a0, a1, a2, a3);
+        if (p0 + 8u * lane < hi) {
+          uint4 v;
+          v.x = __builtin_amdgcn_perm(a1, a0, 0x05040100u);  // s0 | s1 << 16
+          v.y = __builtin_amdgcn_perm(a1, a0, 0x07060302u);  // s2 | s3 << 16
+          v.z = __builtin_amdgcn_perm(a3, a2, 0x05040100u);
+          v.w = __builtin_amdgcn_perm(a3, a2, 0x07060302u);
+          *reinterpret_cast<uint4*>(win + base + p0 + 8u * lane) = v;
+        }
       }
     }
-  }
-  __syncthreads();
+    __syncthreads();
 
-  // Stage 2: thread -> (dm, quad) pairs; packed 16-bit partial sums over
-  // flush_every subbands at a time (no lane can overflow), read 4 subbands
-  // ahead.  The three dwords around sample j are read unconditionally and
-  // v_alignbyte picks the pair (shift 0 or 2 bytes).
-  const uint32_t tquads = tile / 4;
-  const uint32_t half_len = inter_len / 2;
-  for (uint32_t e = tid; e < nd * tquads; e += kThreads) {
-    const uint32_t i = e / tquads;
-    const uint32_t q = e % tquads;
-    const int32_t* ri = sres + i * nsub;
-    uint32_t tot0 = 0, tot1 = 0, tot2 = 0, tot3 = 0;
-    for (uint32_t s0 = 0; s0 < nsub; s0 += flush_every) {
-      const uint32_t s1 = min(s0 + flush_every, nsub);
-      uint32_t p01 = 0, p23 = 0;
+    // Step 3: one wave pass per (DM, up to 512 outputs).
+    const uint32_t q = min(8u * lane, chunk - 8u);  // idle lanes of a short chunk re-read
+    for (uint32_t i = wave; i < nd; i += kSbWaves) {
+      const int32_t* arow = saddr + i * nsub4;
+      uint32_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t s0 = 0; s0 < nsub; s0 += flush_every) {
+        const uint32_t s1 = min(s0 + flush_every, nsub);
+        uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
 #pragma unroll 4
-      for (uint32_t s = s0; s < s1; ++s) {
-        const uint32_t j = 4u * q + static_cast<uint32_t>(ri[s]);
-        const uint32_t* r32 = inter32 + static_cast<size_t>(s) * half_len + (j >> 1);
-        const uint32_t d0 = r32[0], d1 = r32[1], d2 = r32[2];
-        const uint32_t shb = (j & 1u) * 2u;
-        p01 += __builtin_amdgcn_alignbyte(d1, d0, shb);
-        p23 += __builtin_amdgcn_alignbyte(d2, d1, shb);
+        for (uint32_t s = s0; s < s1; ++s) {
+          const uint32_t u = static_cast<uint32_t>(arow[s]) + q;
+          const uint32_t* r32 = win32 + (u >> 1);
+          const uint32_t shb = (u & 1u) << 1;
+          const uint32_t d0 = r32[0], d1 = r32[1], d2 = r32[2], d3 = r32[3], d4 = r32[4];
+          p0 += __builtin_amdgcn_alignbyte(d1, d0, shb);
+          p1 += __builtin_amdgcn_alignbyte(d2, d1, shb);
+          p2 += __builtin_amdgcn_alignbyte(d3, d2, shb);
+          p3 += __builtin_amdgcn_alignbyte(d4, d3, shb);
+        }
+        tot[0] += p0 & 0xFFFFu;
+        tot[1] += p0 >> 16;
+        tot[2] += p1 & 0xFFFFu;
+        tot[3] += p1 >> 16;
+        tot[4] += p2 & 0xFFFFu;
+        tot[5] += p2 >> 16;
+        tot[6] += p3 & 0xFFFFu;
+        tot[7] += p3 >> 16;
       }
-      tot0 += p01 & 0xFFFFu;
-      tot1 += p01 >> 16;
-      tot2 += p23 & 0xFFFFu;
-      tot3 += p23 >> 16;
-    }
-    uint32_t* o = out + static_cast<uint64_t>(dm0 + i) * out_stride;
-    const uint64_t t = t0 + 4u * q;
-    if (t + 3 < out_nsamples && (out_stride & 3u) == 0 &&
-        (reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
-      *reinterpret_cast<uint4*>(o + t) = make_uint4(tot0, tot1, tot2, tot3);
-    } else {
-      const uint32_t v[4] = {tot0, tot1, tot2, tot3};
+      const uint64_t t = t_out + 8u * lane;
+      if (8u * lane < chunk && t < t_end) {
+        uint32_t* o = out + static_cast<uint64_t>(dm_first + i) * out_stride + t;
+        if (t + 8 <= t_end && (out_stride & 3u) == 0 &&
+            (reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
+          reinterpret_cast<uint4*>(o)[0] = make_uint4(tot[0], tot[1], tot[2], tot[3]);
+          reinterpret_cast<uint4*>(o)[1] = make_uint4(tot[4], tot[5], tot[6], tot[7]);
+        } else {
 #pragma unroll
-      for (int jj = 0; jj < 4; ++jj)
-        if (t + jj < out_nsamples) o[t + jj] = v[jj];
+          for (int jj = 0; jj < 8; ++jj)
+            if (t + jj < t_end) o[jj] = tot[jj];
+        }
+      }
     }
+    __syncthreads();
   }
 }
 
@@ -392,48 +474,29 @@ int32_t* error_flag_for_device() {
 
 }  // namespace
 
-// Largest time tile whose LDS (the u16 subband intermediate of nsub rows of
-// tile + max_residual samples, plus the residual table) fits two workgroups
-// per CU, else one; tile 0 when none does.
-static void subband_geometry(uint64_t nsub, uint64_t ndm_per_nominal, int32_t max_residual,
-                             uint32_t& tile, uint32_t& inter_len, uint64_t& lds) {
-  const uint64_t res_bytes = ndm_per_nominal * nsub * 4;
-  tile = 0;
-  inter_len = 0;
+// LDS bytes of the subband kernel: the (DM, subband) address table, the
+// window starts and halos, and the u16 windows (chunk + halo + 8 per subband).
+static uint64_t subband_lds_bytes(uint64_t nsub, uint64_t max_group, uint64_t halo_sum,
+                                  uint32_t chunk) {
+  const uint64_t nsub4 = (nsub + 3) & ~3ull;
+  return 4 * (max_group * nsub4 + 2 * nsub4) + 2 * (nsub * (chunk + 8ull) + halo_sum);
+}
+
+// Largest chunk (512 down to 8 samples) whose LDS fits one workgroup; 0 when
+// none does.
+static uint32_t subband_chunk(uint64_t nsub, uint64_t max_group, uint64_t halo_sum,
+                              uint64_t& lds) {
+  for (uint32_t chunk = kSbPiece; chunk >= 8; chunk /= 2) {
+    lds = subband_lds_bytes(nsub, max_group, halo_sum, chunk);
+    if (lds <= kMaxLds - 256) return chunk;
+  }
   lds = 0;
-  // candidates, largest first: tiles whose stage-1 span fills whole passes of
-  // the workgroup (kThreads dword columns each, no idle lanes), then powers
-  // of two
-  uint32_t cands[16];
-  uint32_t ncand = 0;
-  for (uint32_t q = kSbQ; q >= 1; --q) {
-    const int64_t t = (static_cast<int64_t>(q) * kThreads * 4 - max_residual - 12) & ~int64_t{63};
-    if (t >= 64 && t <= 2048) cands[ncand++] = static_cast<uint32_t>(t);
-  }
-  for (uint32_t t = 2048; t >= 64; t /= 2) cands[ncand++] = t;
-  for (uint32_t i = 1; i < ncand; ++i)  // descending
-    for (uint32_t j = i; j > 0 && cands[j] > cands[j - 1]; --j) {
-      const uint32_t x = cands[j];
-      cands[j] = cands[j - 1];
-      cands[j - 1] = x;
-    }
-  for (const uint64_t budget : {kMaxLds / 2 - 2048, kMaxLds - 4096}) {
-    for (uint32_t ci = 0; ci < ncand && tile == 0; ++ci) {
-      const uint32_t cand = cands[ci];
-      const uint64_t jlen4 = (cand + static_cast<uint64_t>(max_residual) + 4 + 3) & ~3ull;
-      const uint64_t il = jlen4 + 8;
-      // stage 1 writes (computes or zeroes) inter[0, il) in kSbQ dword
-      // pairs per thread
-      if (il > static_cast<uint64_t>(kThreads) * kSbQ * 4) continue;
-      const uint64_t bytes = nsub * il * 2 + res_bytes;
-      if (bytes <= budget) {
-        tile = cand;
-        inter_len = static_cast<uint32_t>(il);
-        lds = bytes;
-      }
-    }
-    if (tile != 0) return;
-  }
+  return 0;
+}
+
+static uint64_t subband_halo_sum(uint64_t nsub, int32_t max_residual, int64_t residual_window) {
+  if (residual_window > 0) return static_cast<uint64_t>(residual_window);
+  return nsub * ((static_cast<uint64_t>(std::max(max_residual, 0)) + 7) & ~7ull);
 }
 
 // Reads and clears the LDS-bound guard flag (a host/kernel table mismatch).
@@ -544,7 +607,6 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
           "channel rows must be dword aligned and hold nsamples");
     check(a->ndm > 0, "dedispersion requires at least one DM");
     check(a->subband_channels > 0, "subband_channels must be positive");
-    check(a->ndm_per_nominal > 0, "ndm_per_nominal must be positive");
     check(a->subband_channels <= 256,
           "GPU subband dedispersion supports at most 256 channels per subband");
     check(a->out_nsamples > 0 && a->out_nsamples <= a->nsamples,
@@ -553,56 +615,85 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
     check(a->max_residual >= 0 && a->max_coarse >= 0,
           "dedispersion requires non-negative delays");
     const uint64_t nsel = a->nsel;
-    check_overflow(nsel <= 0xFFFFFFFFull, "dedispersion channel range too large");
+    check_overflow(nsel <= 0xFFFFFFFFull && a->ndm <= 0xFFFFFFFFull,
+                   "dedispersion table too large");
     const uint64_t nsub = ceil_div(nsel, a->subband_channels);
-    const uint64_t nnom = ceil_div(a->ndm, a->ndm_per_nominal);
-    check(a->ndm_per_nominal <= 1024, "ndm_per_nominal must be <= 1024 on the GPU");
+    uint64_t ngroups = 0, max_group = 0, n_coarse = 0, npn = 0;
+    if (a->groups != nullptr) {
+      check(a->ngroups > 0 && a->max_group_ndm > 0 && a->n_coarse_rows > 0,
+            "subband groups need ngroups, max_group_ndm and n_coarse_rows");
+      ngroups = a->ngroups;
+      max_group = a->max_group_ndm;
+      n_coarse = a->n_coarse_rows;
+    } else {
+      check(a->ndm_per_nominal > 0, "ndm_per_nominal must be positive");
+      npn = a->ndm_per_nominal;
+      ngroups = ceil_div(a->ndm, npn);
+      max_group = std::min<uint64_t>(npn, a->ndm);
+      n_coarse = ngroups;
+    }
+    check(max_group <= 1024, "ndm_per_nominal must be <= 1024 on the GPU");
+    check_overflow(ngroups <= 0xFFFFFFull && n_coarse <= 0xFFFFFFFFull,
+                   "too many nominal DM groups");
     const uint32_t flush = static_cast<uint32_t>(
         std::max<uint64_t>(1, 65535 / (255 * a->subband_channels)));
-    uint32_t tile = 0, inter_len = 0;
     uint64_t lds = 0;
-    subband_geometry(nsub, a->ndm_per_nominal, a->max_residual, tile, inter_len, lds);
-    check_overflow(tile != 0, "subband dedispersion delays too large for LDS");
-    const uint64_t tiles = ceil_div(a->out_nsamples, tile);
-    check_overflow(tiles * nnom < 0x7FFFFFF0ull, "dedispersion grid too large");
-    // every stage-1 read of the last tile stays inside the padded rows
-    check((tiles - 1) * tile + static_cast<uint64_t>(a->max_coarse) + (inter_len - 8) + 4 <=
+    const uint64_t halo_sum = subband_halo_sum(nsub, a->max_residual, a->residual_window);
+    const uint32_t chunk = subband_chunk(nsub, max_group, halo_sum, lds);
+    check_overflow(chunk != 0, "subband dedispersion delays too large for LDS");
+    // spans: enough workgroups to fill the chip a few times over, each at
+    // least 16 chunks long so the halo is a small part of its stage-1 work
+    const uint64_t out_n = a->out_nsamples;
+    const uint64_t max_spans = std::max<uint64_t>(1, ceil_div(out_n, 16ull * chunk));
+    uint64_t nspans = std::min(max_spans, std::max<uint64_t>(1, ceil_div(2048, ngroups)));
+    const uint64_t span = ceil_div(ceil_div(out_n, nspans), chunk) * chunk;
+    nspans = ceil_div(out_n, span);
+    check_overflow(span <= 0xFFFFFFFFull && ngroups * nspans < 0x7FFFFFF0ull,
+                   "dedispersion grid too large");
+    // every stage-1 read of the last span stays inside the padded rows
+    check(ceil_div(out_n, chunk) * chunk +
+                  ((static_cast<uint64_t>(a->max_residual) + 7) & ~7ull) + kSbReadPad +
+                  static_cast<uint64_t>(a->max_coarse) <=
               row_len,
           "channel rows are not padded far enough for the delays (see "
           "gaffa_hip_subband_channel_stride)");
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(dedisperse_subband_cm_kernel),
+    const uint64_t nsub4 = (nsub + 3) & ~3ull;
+    const uint64_t window_cap = (lds - 4 * (max_group * nsub4 + 2 * nsub4)) / 2;
+    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(dedisperse_subband_stream_kernel),
                                   hipFuncAttributeMaxDynamicSharedMemorySize,
                                   static_cast<int>(lds)),
               "hipFuncSetAttribute");
-    const uint64_t nblocks = tiles * nnom;
-    const uint64_t grid = ceil_div(nblocks, kXcds) * kXcds;  // whole rounds over the XCDs
-    hipLaunchKernelGGL(dedisperse_subband_cm_kernel, dim3(static_cast<uint32_t>(grid)),
-                       dim3(kThreads), static_cast<uint32_t>(lds), as_stream(stream),
+    const uint64_t nwork = ngroups * nspans;
+    const uint64_t grid = ceil_div(nwork, kXcds) * kXcds;  // whole rounds over the XCDs
+    hipLaunchKernelGGL(dedisperse_subband_stream_kernel, dim3(static_cast<uint32_t>(grid)),
+                       dim3(kSbThreads), static_cast<uint32_t>(lds), as_stream(stream),
                        a->channels, a->channel_stride, row_len, static_cast<uint32_t>(nsel),
                        static_cast<uint32_t>(a->subband_channels), static_cast<uint32_t>(nsub),
-                       static_cast<uint32_t>(a->ndm_per_nominal), static_cast<uint32_t>(a->ndm),
-                       static_cast<uint32_t>(nnom), static_cast<uint32_t>(nblocks),
-                       a->coarse_delays, a->residual_delays, a->out,
-                       a->out_nsamples, a->out_stride, tile, inter_len, flush,
-                       error_flag_for_device());
-    launch_check("dedisperse_subband_cm_kernel");
+                       reinterpret_cast<const SubbandGroupDev*>(a->groups),
+                       static_cast<uint32_t>(ngroups), static_cast<uint32_t>(npn),
+                       static_cast<uint32_t>(a->ndm), static_cast<uint32_t>(n_coarse),
+                       a->coarse_delays, a->residual_delays, a->out, out_n, a->out_stride,
+                       chunk, static_cast<uint32_t>(span), static_cast<uint32_t>(nspans),
+                       static_cast<uint32_t>(max_group), static_cast<uint32_t>(window_cap),
+                       flush, error_flag_for_device());
+    launch_check("dedisperse_subband_stream_kernel");
   });
 }
 
 uint32_t gaffa_hip_subband_time_tile(uint64_t nsel, uint64_t subband_channels,
-                                     uint64_t ndm_per_nominal, int32_t max_residual) {
-  if (nsel == 0 || subband_channels == 0 || ndm_per_nominal == 0 || max_residual < 0) return 0;
-  uint32_t tile = 0, inter_len = 0;
+                                     uint64_t max_group_ndm, int32_t max_residual,
+                                     int64_t residual_window) {
+  if (nsel == 0 || subband_channels == 0 || max_group_ndm == 0 || max_residual < 0) return 0;
+  const uint64_t nsub = ceil_div(nsel, subband_channels);
   uint64_t lds = 0;
-  subband_geometry(ceil_div(nsel, subband_channels), ndm_per_nominal, max_residual, tile,
-                   inter_len, lds);
-  return tile;
+  return subband_chunk(nsub, max_group_ndm, subband_halo_sum(nsub, max_residual, residual_window),
+                       lds);
 }
 
 uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,
                                           int32_t max_residual) {
-  // the last tile starts below nsamples; it reads tile (<= 2048) + residual
-  // + pad samples past its coarse delay
+  // the last span's stage-1 passes read up to one chunk (<= 512) past the
+  // output range, plus the residual halo, the pass padding and the coarse delay
   const uint64_t need = nsamples + static_cast<uint64_t>(std::max(max_coarse, 0)) +
                         static_cast<uint64_t>(std::max(max_residual, 0)) + 2048 + 64;
   return (need + 255) / 256 * 256;

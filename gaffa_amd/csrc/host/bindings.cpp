@@ -29,15 +29,33 @@ using namespace gaffa;
 
 namespace {
 
-// "gpu" is the backend; "cuda" is accepted as an alias so gaffa scripts run
-// unchanged.  gaffa_amd has no CPU backend: its CPU path is the test oracle.
+// gaffa's backend names (dedispersion_bindings.cpp:56-64): "cuda" selects the
+// device path, here HIP on the GPU, with "gpu" as a synonym; the result
+// reports the name the caller used.  "cpu" is gaffa's own CPU code, which
+// gaffa_amd does not ship (its CPU restatement is test-only), so it is refused
+// with a ValueError rather than computed.
 std::string parse_backend(const std::string& value, const char* what) {
-  if (value == "gpu" || value == "cuda") return "gpu";
+  if (value == "gpu" || value == "cuda") return value;
   if (value == "cpu") {
     throw py::value_error(std::string(what) +
-                          " backend 'cpu' is not provided by gaffa_amd; use backend='gpu'");
+                          " backend 'cpu' is gaffa's CPU path, which gaffa_amd does not "
+                          "provide; use backend='cuda' (or 'gpu')");
   }
   throw py::value_error(std::string("unknown ") + what + " backend: " + value);
+}
+
+// gaffa's reader option names and messages (filterbank_bindings.cpp:23-43).
+ChannelOrderPolicy parse_channel_order(const std::string& value) {
+  if (value == "frequency_ascending") return ChannelOrderPolicy::FrequencyAscending;
+  if (value == "preserve_file_order") return ChannelOrderPolicy::PreserveFileOrder;
+  throw py::value_error("unknown channel_order: " + value);
+}
+
+ReverseBackend parse_reverse_backend(const std::string& value) {
+  if (value == "auto") return ReverseBackend::Auto;
+  if (value == "cpu_scalar") return ReverseBackend::CpuScalar;
+  if (value == "cpu_openmp") return ReverseBackend::CpuOpenmp;
+  throw py::value_error("unknown reverse_backend: " + value);
 }
 
 template <typename T>
@@ -132,13 +150,13 @@ double reference_frequency(const std::vector<double>& f) {
 }
 
 PyDedispersedResult make_result(DedispersedResult<std::uint32_t>&& r, double dm_low,
-                                double dm_step, double tsamp) {
+                                double dm_step, double tsamp, std::string backend) {
   PyDedispersedResult p;
   p.ndm = r.shape.ndm;
   p.nsamples = r.shape.nsamples;
   p.data = owning_array(std::move(r.data), {static_cast<py::ssize_t>(p.ndm),
                                             static_cast<py::ssize_t>(p.nsamples)});
-  p.backend = "gpu";
+  p.backend = std::move(backend);
   p.dm_low = dm_low;
   p.dm_step = dm_step;
   p.tsamp = tsamp;
@@ -147,7 +165,7 @@ PyDedispersedResult make_result(DedispersedResult<std::uint32_t>&& r, double dm_
 
 PyDedispersedResult py_single(const py::object& fb, double dm, const std::string& backend,
                               int device_id, std::size_t tpb, std::size_t tile) {
-  parse_backend(backend, "dedispersion");
+  std::string name = parse_backend(backend, "dedispersion");
   FilterbankInput in = filterbank_input(fb);
   const SingleDmDedispersionPlan plan{dm, reference_frequency(in.freq), in.tsamp, 0,
                                       in.samples.shape.nchans};
@@ -157,13 +175,13 @@ PyDedispersedResult py_single(const py::object& fb, double dm, const std::string
     r = dedisperse_single_dm_gpu(in.samples, in.freq, plan,
                                  GpuDedispersionOptions{device_id, tpb, tile, nullptr});
   }
-  return make_result(std::move(r), dm, 0.0, in.tsamp);
+  return make_result(std::move(r), dm, 0.0, in.tsamp, std::move(name));
 }
 
 PyDedispersedResult py_multi(const py::object& fb, double dm_low, double dm_step, std::size_t ndm,
                              const std::string& backend, int device_id, std::size_t tpb,
                              std::size_t tile) {
-  parse_backend(backend, "dedispersion");
+  std::string name = parse_backend(backend, "dedispersion");
   FilterbankInput in = filterbank_input(fb);
   const MultiDmDedispersionPlan plan{dm_low, dm_step, ndm, reference_frequency(in.freq),
                                      in.tsamp, 0, in.samples.shape.nchans};
@@ -173,14 +191,14 @@ PyDedispersedResult py_multi(const py::object& fb, double dm_low, double dm_step
     r = dedisperse_multi_dm_gpu(in.samples, in.freq, plan,
                                 GpuDedispersionOptions{device_id, tpb, tile, nullptr});
   }
-  return make_result(std::move(r), dm_low, dm_step, in.tsamp);
+  return make_result(std::move(r), dm_low, dm_step, in.tsamp, std::move(name));
 }
 
 PyDedispersedResult py_subband(const py::object& fb, double dm_low, double dm_step,
                                std::size_t ndm, const std::string& backend,
                                std::size_t subband_channels, std::size_t ndm_per_nominal,
                                int device_id, std::size_t tpb, std::size_t tile) {
-  parse_backend(backend, "dedispersion");
+  std::string name = parse_backend(backend, "dedispersion");
   FilterbankInput in = filterbank_input(fb);
   const MultiDmDedispersionPlan plan{dm_low, dm_step, ndm, reference_frequency(in.freq),
                                      in.tsamp, 0, in.samples.shape.nchans};
@@ -191,7 +209,7 @@ PyDedispersedResult py_subband(const py::object& fb, double dm_low, double dm_st
                                SubbandDedispersionOptions{subband_channels, ndm_per_nominal},
                                GpuDedispersionOptions{device_id, tpb, tile, nullptr});
   }
-  return make_result(std::move(r), dm_low, dm_step, in.tsamp);
+  return make_result(std::move(r), dm_low, dm_step, in.tsamp, std::move(name));
 }
 
 // ----------------------------------------------------------------------- FFA --
@@ -216,6 +234,21 @@ std::size_t peak_limit(const py::object& max_peaks) {
   const auto v = max_peaks.cast<long long>();
   if (v <= 0) throw py::value_error("max_peaks must be positive or None");
   return static_cast<std::size_t>(v);
+}
+
+// The argument checks of a single-series search, in gaffa's order and words
+// (ffa_bindings.cpp:55-71, ffa_executor.cpp:31-35), without searching: the
+// Python wrapper runs them before it decides on the backend, as gaffa's
+// bindings validate whichever backend was chosen.
+void py_check_ffa_search(const py::object& ts, const FfaSearchPlan& plan,
+                         const py::object& max_peaks) {
+  py::array holder;
+  require_series(ts, holder, 1);
+  const std::size_t n = static_cast<std::size_t>(holder.request().shape[0]);
+  for (const FfaSearchTask& t : plan.tasks)
+    if (t.input_nsamples != n)
+      throw std::invalid_argument("FFA executor task input_nsamples must match time series size");
+  (void)peak_limit(max_peaks);
 }
 
 std::vector<FfaPeak> py_ffa_search(const py::object& ts, const FfaSearchPlan& plan, int device_id,
@@ -613,14 +646,16 @@ PYBIND11_MODULE(_core, m) {
         return "<DedispersedResult shape=(" + std::to_string(r.ndm) + ", " +
                std::to_string(r.nsamples) + ") backend=" + r.backend + ">";
       });
+  // gaffa's defaults (dedispersion_bindings.cpp:648-672): "cpu" for the single,
+  // multi and spectrum forms, which gaffa_amd refuses, "cuda" for subband
   m.def("dedisperse_single_dm", &py_single, py::arg("filterbank"), py::kw_only(), py::arg("dm"),
-        py::arg("backend") = "gpu", py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
+        py::arg("backend") = "cpu", py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
         py::arg("time_tile_samples") = 81920);
   m.def("dedisperse_multi_dm", &py_multi, py::arg("filterbank"), py::kw_only(), py::arg("dm_low"),
-        py::arg("dm_step"), py::arg("ndm"), py::arg("backend") = "gpu", py::arg("device_id") = 0,
+        py::arg("dm_step"), py::arg("ndm"), py::arg("backend") = "cpu", py::arg("device_id") = 0,
         py::arg("threads_per_block") = 256, py::arg("time_tile_samples") = 81920);
   m.def("dedisperse_subband", &py_subband, py::arg("filterbank"), py::kw_only(), py::arg("dm_low"),
-        py::arg("dm_step"), py::arg("ndm"), py::arg("backend") = "gpu",
+        py::arg("dm_step"), py::arg("ndm"), py::arg("backend") = "cuda",
         py::arg("subband_channels") = 32, py::arg("ndm_per_nominal") = 32,
         py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
         py::arg("time_tile_samples") = 81920);
@@ -666,6 +701,8 @@ PYBIND11_MODULE(_core, m) {
       py::arg("period_max"), py::arg("bins_min") = 180, py::arg("bins_max") = 256,
       py::arg("min_periods") = 1, py::arg("duty_cycle_max") = 0.20,
       py::arg("width_trial_spacing") = 1.5, py::arg("max_tasks") = 1'000'000);
+  m.def("_check_ffa_search", &py_check_ffa_search, py::arg("time_series"), py::arg("plan"),
+        py::arg("max_peaks") = py::none());
   m.def("_ffa_search_gpu_host", &py_ffa_search, py::arg("time_series"), py::arg("plan"),
         py::kw_only(), py::arg("device_id") = 0, py::arg("snr_threshold") = 6.0F,
         py::arg("max_peaks") = py::none());
@@ -678,28 +715,72 @@ PYBIND11_MODULE(_core, m) {
 
   // SIGPROC files (native reader / writer)
   m.def(
-      "_read_filterbank_header",
-      [](const std::string& path) {
-        const FilterbankHeader h = read_filterbank_header(path);
-        py::dict d;
-        d["header_size"] = h.header_size;
-        d["nsamples"] = h.nsamples;
-        d["nbits"] = h.nbits;
-        d["nifs"] = h.nifs;
-        d["nchans"] = h.nchans;
-        d["tsamp"] = h.tsamp;
-        d["tstart"] = h.tstart;
-        d["fch1"] = h.fch1;
-        d["foff"] = h.foff;
-        d["telescope_id"] = h.telescope_id;
-        d["machine_id"] = h.machine_id;
-        d["source_name"] = h.source_name;
-        d["frequency_table"] = h.frequency_table;
-        d["uses_frequency_table"] = h.uses_frequency_table;
-        d["reversed"] = h.reversed;
-        return d;
+      "_read_filterbank",
+      [](const std::string& path, const std::string& channel_order,
+         const std::string& reverse_backend, std::size_t io_buffer_bytes,
+         std::size_t openmp_min_rows, bool header_only) {
+        FilterbankReadOptions o;
+        o.io_buffer_bytes = io_buffer_bytes;
+        o.openmp_min_rows = openmp_min_rows;
+        o.channel_order = parse_channel_order(channel_order);
+        o.reverse_backend = parse_reverse_backend(reverse_backend);
+        FilterbankData d;
+        {
+          py::gil_scoped_release release;
+          if (header_only)
+            d.header = read_filterbank_header(path, o.channel_order);
+          else
+            d = read_filterbank(path, o);
+        }
+        const FilterbankHeader& h = d.header;
+        py::dict hd;
+        hd["header_size"] = h.header_size;
+        hd["nsamples"] = h.nsamples;
+        hd["telescope_id"] = h.telescope_id;
+        hd["machine_id"] = h.machine_id;
+        hd["data_type"] = h.data_type;
+        hd["barycentric"] = h.barycentric;
+        hd["pulsarcentric"] = h.pulsarcentric;
+        hd["ibeam"] = h.ibeam;
+        hd["nbeams"] = h.nbeams;
+        hd["npuls"] = h.npuls;
+        hd["nbins"] = h.nbins;
+        hd["nbits"] = h.nbits;
+        hd["nifs"] = h.nifs;
+        hd["nchans"] = h.nchans;
+        hd["az_start"] = h.az_start;
+        hd["za_start"] = h.za_start;
+        hd["src_raj"] = h.src_raj;
+        hd["src_dej"] = h.src_dej;
+        hd["tstart"] = h.tstart;
+        hd["tsamp"] = h.tsamp;
+        hd["fch1"] = h.fch1;
+        hd["foff"] = h.foff;
+        hd["refdm"] = h.refdm;
+        hd["period"] = h.period;
+        hd["rawdatafile"] = h.rawdatafile;
+        hd["source_name"] = h.source_name;
+        hd["frequency_table"] = h.frequency_table;
+        hd["uses_frequency_table"] = h.uses_frequency_table;
+        hd["reversed"] = h.reversed;
+        if (header_only) return py::tuple(py::make_tuple(hd, py::none()));
+        const py::dtype dt = h.nbits == 8    ? py::dtype::of<std::uint8_t>()
+                             : h.nbits == 16 ? py::dtype::of<std::uint16_t>()
+                                             : py::dtype::of<float>();
+        const std::size_t esize = static_cast<std::size_t>(h.nbits / 8);
+        auto* owner = new std::vector<std::uint8_t>(std::move(d.bytes));
+        py::capsule cap(owner, [](void* p) { delete static_cast<std::vector<std::uint8_t>*>(p); });
+        const auto ns = static_cast<py::ssize_t>(h.nsamples);
+        const auto ni = static_cast<py::ssize_t>(h.nifs);
+        const auto nc = static_cast<py::ssize_t>(h.nchans);
+        const auto es = static_cast<py::ssize_t>(esize);
+        py::array a(dt, {ns, ni, nc}, {ni * nc * es, nc * es, es}, owner->data(), cap);
+        return py::tuple(py::make_tuple(hd, a));
       },
-      py::arg("path"));
+      py::arg("path"), py::kw_only(), py::arg("channel_order") = "frequency_ascending",
+      py::arg("reverse_backend") = "auto",
+      py::arg("io_buffer_bytes") = default_filterbank_io_buffer_bytes,
+      py::arg("openmp_min_rows") = 4096, py::arg("header_only") = false);
   m.def(
       "_write_filterbank_u8",
       [](const std::string& path, const py::array_t<std::uint8_t, py::array::c_style>& samples,
@@ -762,13 +843,16 @@ PYBIND11_MODULE(_core, m) {
       "dedisperse_spectrum",
       [](const py::object& fb, double dm, const std::string& backend, int device_id, std::size_t,
          std::size_t) {
-        parse_backend(backend, "dedispersion");
+        std::string name = parse_backend(backend, "dedispersion");
         FilterbankInput in = filterbank_input(fb);
         const py::array flat = in.keep.reshape({static_cast<py::ssize_t>(in.samples.shape.nsamples),
                                                 static_cast<py::ssize_t>(in.samples.shape.nchans)});
-        return py_spectrum_array(flat, in.freq, in.tsamp, dm, 0, std::nullopt, device_id);
+        PyDedispersedSpectrum r =
+            py_spectrum_array(flat, in.freq, in.tsamp, dm, 0, std::nullopt, device_id);
+        r.backend = std::move(name);
+        return r;
       },
-      py::arg("filterbank"), py::kw_only(), py::arg("dm"), py::arg("backend") = "gpu",
+      py::arg("filterbank"), py::kw_only(), py::arg("dm"), py::arg("backend") = "cpu",
       py::arg("device_id") = 0, py::arg("threads_per_block") = 256,
       py::arg("time_tile_samples") = 81920,
       "Aligned single-DM dynamic spectrum (valid samples only), on the GPU");

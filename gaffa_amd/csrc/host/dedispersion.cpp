@@ -161,6 +161,17 @@ SubbandTables make_subband_tables(std::span<const double> f, const MultiDmDedisp
       t.max_residual = std::max(t.max_residual, r);
     }
   }
+  for (std::size_t k = 0; k < t.n_nominal; ++k) {
+    const std::size_t i1 = std::min(p.ndm, (k + 1) * o.ndm_per_nominal);
+    std::int64_t window = 0;
+    for (std::size_t s = 0; s < t.nsub; ++s) {
+      std::int32_t h = 0;
+      for (std::size_t i = k * o.ndm_per_nominal; i < i1; ++i)
+        h = std::max(h, t.residual[i * t.nsub + s]);
+      window += (static_cast<std::int64_t>(h) + 7) / 8 * 8;
+    }
+    t.residual_window = std::max(t.residual_window, window);
+  }
   return t;
 }
 
@@ -206,26 +217,23 @@ GpuSegmentTables make_gpu_segment_tables(std::span<const double> f,
   t.subband_channels = so.subband_channels;
   t.ndm_per_nominal = so.ndm_per_nominal;
   if (subband) {
-    // Narrow the nominal DM group (halving) until the residual span of a
-    // group fits the kernel's LDS intermediate: wide DM steps at high DM
-    // (4096 channels, DM ~1000) need it.  The sums are then gaffa's subband
-    // dedispersion at that ndm_per_nominal.
-    SubbandDedispersionOptions o = so;
-    SubbandTables st = make_subband_tables(f, p, o);
+    const SubbandTables st = make_subband_tables(f, p, so);
     const std::size_t nsel = p.chan_end - p.chan_begin;
-    while (o.ndm_per_nominal > 1 &&
-           gaffa_hip_subband_time_tile(nsel, o.subband_channels, o.ndm_per_nominal,
-                                       st.max_residual) == 0) {
-      o.ndm_per_nominal = (o.ndm_per_nominal + 1) / 2;
-      st = make_subband_tables(f, p, o);
-    }
-    t.ndm_per_nominal = o.ndm_per_nominal;
+    if (gaffa_hip_subband_time_tile(nsel, so.subband_channels,
+                                    std::min(so.ndm_per_nominal, p.ndm), st.max_residual,
+                                    st.residual_window) == 0)
+      throw std::overflow_error("subband dedispersion delays too large for LDS");
     for (std::int32_t d : st.coarse) check_arg(d >= 0, "dedispersion requires non-negative delays");
     t.first = upload(st.coarse.data(), st.coarse.size());
     t.second = upload(st.residual.data(), st.residual.size());
+    t.host_first = st.coarse;
+    t.host_second = st.residual;
+    t.n_nominal = st.n_nominal;
     t.bound_first = st.max_coarse;
     t.bound_second = st.max_residual;
-    // stage 1 reads up to tile (<= 2048) + residual + pad past the coarse delay
+    t.residual_window = st.residual_window;
+    // stage 1 reads up to one chunk (<= 512) + residual + pad past the
+    // coarse delay
     t.lookahead = static_cast<std::size_t>(st.max_coarse) + st.max_residual + 2048 + 64;
   } else {
     const std::vector<std::int32_t> delays = make_multi_dm_delay_table(f, p);
@@ -262,6 +270,7 @@ void enqueue_segment_range(const GpuSegmentTables& t, const std::uint8_t* sample
     a.out_stride = out_stride;
     a.max_residual = t.bound_second;
     a.max_coarse = t.bound_first;
+    a.residual_window = t.residual_window;
     hip_ok(gaffa_hip_dedisperse_subband_u8(&a, stream), "dedisperse_subband_u8");
   } else {
     gaffa_hip_dedisperse_multi_args a{};
@@ -278,6 +287,73 @@ void enqueue_segment_range(const GpuSegmentTables& t, const std::uint8_t* sample
     a.max_block_spread = t.bound_first;
     hip_ok(gaffa_hip_dedisperse_multi_u8(&a, stream), "dedisperse_multi_u8");
   }
+}
+
+GpuSubbandBatch make_gpu_subband_batch(const std::vector<GpuSegmentTables>& segments) {
+  check_arg(!segments.empty(), "subband batch requires at least one segment");
+  GpuSubbandBatch b;
+  const GpuSegmentTables& s0 = segments.front();
+  b.chan_begin = s0.chan_begin;
+  b.chan_end = s0.chan_end;
+  b.subband_channels = s0.subband_channels;
+  const std::size_t nsel = s0.chan_end - s0.chan_begin;
+  std::vector<std::int32_t> coarse, residual;
+  std::vector<gaffa_hip_subband_group> groups;
+  for (const GpuSegmentTables& t : segments) {
+    check_arg(t.subband && t.chan_begin == b.chan_begin && t.chan_end == b.chan_end &&
+                  t.subband_channels == b.subband_channels,
+              "subband batch segments must share the channel range and subband width");
+    for (std::size_t k = 0; k < t.n_nominal; ++k) {
+      const std::size_t first = k * t.ndm_per_nominal;
+      gaffa_hip_subband_group g{};
+      g.coarse_row = static_cast<std::uint32_t>(b.n_coarse_rows + k);
+      g.first_dm = static_cast<std::uint32_t>(b.ndm + first);
+      g.ndm = static_cast<std::uint32_t>(std::min(t.ndm_per_nominal, t.ndm - first));
+      b.max_group = std::max<std::size_t>(b.max_group, g.ndm);
+      groups.push_back(g);
+    }
+    coarse.insert(coarse.end(), t.host_first.begin(), t.host_first.end());
+    residual.insert(residual.end(), t.host_second.begin(), t.host_second.end());
+    b.n_coarse_rows += t.n_nominal;
+    b.ndm += t.ndm;
+    b.max_coarse = std::max(b.max_coarse, t.bound_first);
+    b.max_residual = std::max(b.max_residual, t.bound_second);
+    b.residual_window = std::max(b.residual_window, t.residual_window);
+  }
+  check_arg(coarse.size() == b.n_coarse_rows * nsel, "subband batch coarse table size mismatch");
+  b.ngroups = groups.size();
+  b.coarse = upload(coarse.data(), coarse.size());
+  b.residual = upload(residual.data(), residual.size());
+  b.groups = upload(groups.data(), groups.size());
+  return b;
+}
+
+void enqueue_subband_batch_range(const GpuSubbandBatch& b, const GpuChannelMajorView& cm,
+                                 std::size_t nsamples, std::uint32_t* out, std::size_t out_stride,
+                                 std::size_t begin, std::size_t end, gaffa_hip_stream stream) {
+  check_arg(begin < end && end <= nsamples, "dedispersion output range is empty");
+  check_arg(begin % 4 == 0, "subband output ranges must start on a multiple of 4 samples");
+  gaffa_hip_dedisperse_subband_args a{};
+  a.channels = cm.data + (b.chan_begin - cm.chan_begin) * cm.stride + begin;
+  a.channel_stride = cm.stride;
+  a.channel_length = cm.stride - begin;
+  a.nsamples = nsamples - begin;
+  a.nsel = b.chan_end - b.chan_begin;
+  a.subband_channels = b.subband_channels;
+  a.ndm = b.ndm;
+  a.coarse_delays = b.coarse.data();
+  a.residual_delays = b.residual.data();
+  a.out = out + begin;
+  a.out_nsamples = end - begin;
+  a.out_stride = out_stride;
+  a.max_residual = b.max_residual;
+  a.max_coarse = b.max_coarse;
+  a.groups = b.groups.data();
+  a.ngroups = b.ngroups;
+  a.n_coarse_rows = b.n_coarse_rows;
+  a.max_group_ndm = b.max_group;
+  a.residual_window = b.residual_window;
+  hip_ok(gaffa_hip_dedisperse_subband_u8(&a, stream), "dedisperse_subband_u8");
 }
 
 std::size_t spectrum_output_nsamples(std::size_t nsamples, std::span<const double> f,
@@ -393,6 +469,7 @@ void dedisperse_subband_gpu_cm_into(const GpuChannelMajorView& cm, std::span<con
   a.out_stride = out_stride;
   a.max_residual = t.max_residual;
   a.max_coarse = t.max_coarse;
+  a.residual_window = t.residual_window;
   hip_ok(gaffa_hip_dedisperse_subband_u8(&a, o.stream), "dedisperse_subband_u8");
   hip_ok(gaffa_hip_dedisperse_check(o.stream), "dedisperse_subband_u8");
 }

@@ -4,8 +4,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <memory>
 #include <set>
 #include <stdexcept>
@@ -26,19 +28,19 @@ struct File {
 };
 
 template <typename T>
-T read_value(std::FILE* f) {
+T read_value(std::FILE* f, const char* field) {
   T v{};
   if (std::fread(&v, sizeof(T), 1, f) != 1)
-    throw std::runtime_error("Non-standard filterbank file: truncated header");
+    throw std::runtime_error(std::string("Failed to read filterbank field: ") + field);
   return v;
 }
 
 std::string read_string(std::FILE* f) {
-  const std::int32_t n = read_value<std::int32_t>(f);
-  if (n < 0 || n > 4096) throw std::runtime_error("Non-standard filterbank file: bad header string length");
+  const std::int32_t n = read_value<std::int32_t>(f, "string length");
+  if (n <= 0 || n > 4096) throw std::runtime_error("Invalid filterbank string length");
   std::string s(static_cast<std::size_t>(n), '\0');
-  if (n > 0 && std::fread(s.data(), 1, s.size(), f) != s.size())
-    throw std::runtime_error("Non-standard filterbank file: truncated header");
+  if (std::fread(s.data(), 1, s.size(), f) != s.size())
+    throw std::runtime_error("Failed to read filterbank string");
   return s;
 }
 
@@ -62,81 +64,183 @@ double now_ms() {
 
 }  // namespace
 
-FilterbankHeader read_filterbank_header(const std::string& path) {
-  File file(path, "rb");
+namespace {
+
+// Parses the header from the current position of `f` (the file start) and
+// sizes the sample block from the file length.  Keyword handling and checks
+// follow gaffa's reader (io/filterbank.cpp:120-234): strings take the next
+// string, FREQUENCY_START ... FREQUENCY_END brackets the fchannel table (an
+// fchannel outside it is an error; a table zeroes fch1 / foff), nsamples in
+// the header is ignored, and the data must be whole rows.
+FilterbankHeader parse_header(std::FILE* f, ChannelOrderPolicy order) {
   FilterbankHeader h;
-  if (read_string(file.f) != "HEADER_START")
+  if (read_string(f) != "HEADER_START")
     throw std::runtime_error("Non-standard filterbank file: missing HEADER_START");
-  static const std::set<std::string> ints = {
-      "telescope_id", "machine_id", "data_type", "barycentric", "pulsarcentric", "ibeam",
-      "nbeams",       "npuls",      "nbins",     "nbits",       "nifs",          "nchans",
-      "nsamples"};
-  std::vector<double> table;
+  struct IntKey {
+    const char* key;
+    int FilterbankHeader::*field;
+  };
+  struct DoubleKey {
+    const char* key;
+    double FilterbankHeader::*field;
+  };
+  static const IntKey ints[] = {
+      {"nchans", &FilterbankHeader::nchans},         {"telescope_id", &FilterbankHeader::telescope_id},
+      {"machine_id", &FilterbankHeader::machine_id}, {"data_type", &FilterbankHeader::data_type},
+      {"ibeam", &FilterbankHeader::ibeam},           {"nbeams", &FilterbankHeader::nbeams},
+      {"nbits", &FilterbankHeader::nbits},           {"barycentric", &FilterbankHeader::barycentric},
+      {"pulsarcentric", &FilterbankHeader::pulsarcentric},
+      {"nbins", &FilterbankHeader::nbins},           {"nifs", &FilterbankHeader::nifs},
+      {"npuls", &FilterbankHeader::npuls}};
+  static const DoubleKey doubles[] = {
+      {"az_start", &FilterbankHeader::az_start}, {"za_start", &FilterbankHeader::za_start},
+      {"src_raj", &FilterbankHeader::src_raj},   {"src_dej", &FilterbankHeader::src_dej},
+      {"tstart", &FilterbankHeader::tstart},     {"tsamp", &FilterbankHeader::tsamp},
+      {"period", &FilterbankHeader::period},     {"fch1", &FilterbankHeader::fch1},
+      {"foff", &FilterbankHeader::foff},         {"refdm", &FilterbankHeader::refdm}};
+  bool in_table = false;
   for (;;) {
-    const std::string key = read_string(file.f);
+    const std::string key = read_string(f);
     if (key == "HEADER_END") break;
     if (key == "rawdatafile") {
-      h.rawdatafile = read_string(file.f);
+      h.rawdatafile = read_string(f);
     } else if (key == "source_name") {
-      h.source_name = read_string(file.f);
-    } else if (ints.count(key) != 0) {
-      const int v = read_value<std::int32_t>(file.f);
-      if (key == "telescope_id") h.telescope_id = v;
-      else if (key == "machine_id") h.machine_id = v;
-      else if (key == "data_type") h.data_type = v;
-      else if (key == "barycentric") h.barycentric = v;
-      else if (key == "pulsarcentric") h.pulsarcentric = v;
-      else if (key == "ibeam") h.ibeam = v;
-      else if (key == "nbeams") h.nbeams = v;
-      else if (key == "npuls") h.npuls = v;
-      else if (key == "nbins") h.nbins = v;
-      else if (key == "nbits") h.nbits = v;
-      else if (key == "nifs") h.nifs = v;
-      else if (key == "nchans") h.nchans = v;
-      // nsamples: recomputed from the file size
-    } else if (key == "fchannel") {
-      table.push_back(read_value<double>(file.f));
-    } else if (key == "FREQUENCY_START" || key == "FREQUENCY_END") {
+      h.source_name = read_string(f);
+    } else if (key == "FREQUENCY_START") {
+      in_table = true;
       h.uses_frequency_table = true;
+      h.frequency_table.clear();
+    } else if (key == "FREQUENCY_END") {
+      in_table = false;
+      h.nchans = static_cast<int>(h.frequency_table.size());
+    } else if (key == "fchannel") {
+      if (!in_table) throw std::runtime_error("fchannel outside FREQUENCY block");
+      h.frequency_table.push_back(read_value<double>(f, "fchannel"));
+      h.fch1 = 0.0;
+      h.foff = 0.0;
+    } else if (key == "nsamples") {
+      (void)read_value<std::int32_t>(f, "nsamples");  // recomputed from the file size
     } else {
-      double* dst = key == "az_start"  ? &h.az_start
-                    : key == "za_start" ? &h.za_start
-                    : key == "src_raj"  ? &h.src_raj
-                    : key == "src_dej"  ? &h.src_dej
-                    : key == "tstart"   ? &h.tstart
-                    : key == "tsamp"    ? &h.tsamp
-                    : key == "fch1"     ? &h.fch1
-                    : key == "foff"     ? &h.foff
-                    : key == "refdm"    ? &h.refdm
-                    : key == "period"   ? &h.period
-                                        : nullptr;
-      if (dst == nullptr) throw std::runtime_error("Unknown filterbank header field: " + key);
-      *dst = read_value<double>(file.f);
+      bool known = false;
+      for (const IntKey& k : ints)
+        if (key == k.key) {
+          h.*(k.field) = read_value<std::int32_t>(f, k.key);
+          known = true;
+          break;
+        }
+      for (const DoubleKey& k : doubles)
+        if (!known && key == k.key) {
+          h.*(k.field) = read_value<double>(f, k.key);
+          known = true;
+          break;
+        }
+      if (!known) throw std::runtime_error("Unknown filterbank header field: " + key);
     }
   }
-  h.header_size = static_cast<std::size_t>(std::ftell(file.f));
-  if (h.uses_frequency_table) {
-    h.nchans = static_cast<int>(table.size());
-  } else {
-    for (int c = 0; c < h.nchans; ++c) table.push_back(h.fch1 + c * h.foff);
-  }
+  const long end = std::ftell(f);
+  if (end < 0) throw std::runtime_error("Failed to determine filterbank header size");
+  h.header_size = static_cast<std::size_t>(end);
   if (h.nchans <= 0) throw std::runtime_error("Invalid filterbank nchans");
+  if (h.nifs <= 0) throw std::runtime_error("Invalid filterbank nifs");
   if (h.nbits != 8 && h.nbits != 16 && h.nbits != 32)
     throw std::runtime_error("Unsupported filterbank nbits");
-  if (h.nifs <= 0) throw std::runtime_error("Invalid filterbank nifs");
-  std::fseek(file.f, 0, SEEK_END);
-  const std::size_t size = static_cast<std::size_t>(std::ftell(file.f));
-  const std::size_t row = static_cast<std::size_t>(h.nifs) * h.nchans * (h.nbits / 8);
-  h.nsamples = (size - h.header_size) / row;
-  const bool descending = table.size() >= 2 && table.front() > table.back();
-  h.reversed = h.uses_frequency_table ? descending : h.foff < 0.0;
+  if (std::fseek(f, 0, SEEK_END) != 0) throw std::runtime_error("Failed to size filterbank file");
+  const long size = std::ftell(f);
+  if (size < 0) throw std::runtime_error("Failed to size filterbank file");
+  if (static_cast<std::size_t>(size) < h.header_size)
+    throw std::runtime_error("Filterbank header is larger than file");
+  const std::size_t data_bytes = static_cast<std::size_t>(size) - h.header_size;
+  const std::size_t row =
+      static_cast<std::size_t>(h.nifs) * static_cast<std::size_t>(h.nchans) * (h.nbits / 8);
+  if (data_bytes % row != 0)
+    throw std::runtime_error("Filterbank data size is not a whole number of rows");
+  h.nsamples = data_bytes / row;
+  if (h.uses_frequency_table) {
+    if (h.frequency_table.size() != static_cast<std::size_t>(h.nchans))
+      throw std::runtime_error("Frequency table length does not match nchans");
+  } else {
+    h.frequency_table.resize(static_cast<std::size_t>(h.nchans));
+    for (int c = 0; c < h.nchans; ++c)
+      h.frequency_table[static_cast<std::size_t>(c)] = h.fch1 + static_cast<double>(c) * h.foff;
+  }
+  const std::vector<double>& ft = h.frequency_table;
+  const bool descending = h.uses_frequency_table ? (ft.size() >= 2 && ft.front() > ft.back())
+                                                 : h.foff < 0.0;
+  h.reversed = order == ChannelOrderPolicy::FrequencyAscending && descending;
   if (h.reversed) {
-    std::reverse(table.begin(), table.end());
-    h.fch1 = table.front();
+    std::reverse(h.frequency_table.begin(), h.frequency_table.end());
+    h.fch1 = h.frequency_table.front();
     if (!h.uses_frequency_table) h.foff = std::abs(h.foff);
   }
-  h.frequency_table = std::move(table);
   return h;
+}
+
+template <typename T>
+void reverse_channel_rows(const T* src, T* dst, std::size_t rows, std::size_t nchans,
+                          bool parallel) {
+#pragma omp parallel for if (parallel) schedule(static)
+  for (std::int64_t r = 0; r < static_cast<std::int64_t>(rows); ++r) {
+    const T* a = src + static_cast<std::size_t>(r) * nchans;
+    T* b = dst + static_cast<std::size_t>(r) * nchans;
+    for (std::size_t c = 0; c < nchans; ++c) b[c] = a[nchans - 1 - c];
+  }
+}
+
+}  // namespace
+
+FilterbankHeader read_filterbank_header(const std::string& path, ChannelOrderPolicy order) {
+  File file(path, "rb");
+  return parse_header(file.f, order);
+}
+
+FilterbankData read_filterbank(const std::string& path, const FilterbankReadOptions& o) {
+  File file(path, "rb");
+  FilterbankData d;
+  d.header = parse_header(file.f, o.channel_order);
+  const FilterbankHeader& h = d.header;
+  const std::size_t esize = static_cast<std::size_t>(h.nbits / 8);
+  const std::size_t row_elems = static_cast<std::size_t>(h.nifs) * static_cast<std::size_t>(h.nchans);
+  const std::size_t nchans = static_cast<std::size_t>(h.nchans);
+  if (h.nsamples != 0 && row_elems * esize > std::numeric_limits<std::size_t>::max() / h.nsamples)
+    throw std::overflow_error("Filterbank size overflow: total elements");
+  const std::size_t total = h.nsamples * row_elems * esize;
+  d.bytes.resize(total);
+  if (std::fseek(file.f, static_cast<long>(h.header_size), SEEK_SET) != 0)
+    throw std::runtime_error("Failed to read filterbank samples");
+  if (!h.reversed) {
+    if (total != 0 && std::fread(d.bytes.data(), 1, total, file.f) != total)
+      throw std::runtime_error("Failed to read filterbank samples");
+    return d;
+  }
+  // reversed: read io_buffer_bytes of whole rows at a time and write each
+  // (row, if) block's channels back to front
+  const std::size_t row_bytes = row_elems * esize;
+  const std::size_t rows_per_chunk = std::max<std::size_t>(1, o.io_buffer_bytes / row_bytes);
+  std::vector<std::uint8_t> buf(rows_per_chunk * row_bytes);
+  const bool parallel = o.reverse_backend == ReverseBackend::CpuOpenmp ||
+                        (o.reverse_backend == ReverseBackend::Auto &&
+                         h.nsamples >= o.openmp_min_rows);
+  for (std::size_t r0 = 0; r0 < h.nsamples; r0 += rows_per_chunk) {
+    const std::size_t rows = std::min(rows_per_chunk, h.nsamples - r0);
+    if (std::fread(buf.data(), 1, rows * row_bytes, file.f) != rows * row_bytes)
+      throw std::runtime_error("Failed to read filterbank samples");
+    const std::size_t blocks = rows * static_cast<std::size_t>(h.nifs);
+    std::uint8_t* dst = d.bytes.data() + r0 * row_bytes;
+    switch (esize) {
+      case 1:
+        reverse_channel_rows(buf.data(), dst, blocks, nchans, parallel);
+        break;
+      case 2:
+        reverse_channel_rows(reinterpret_cast<const std::uint16_t*>(buf.data()),
+                             reinterpret_cast<std::uint16_t*>(dst), blocks, nchans, parallel);
+        break;
+      default:
+        reverse_channel_rows(reinterpret_cast<const std::uint32_t*>(buf.data()),
+                             reinterpret_cast<std::uint32_t*>(dst), blocks, nchans, parallel);
+        break;
+    }
+  }
+  return d;
 }
 
 void write_filterbank_u8(const std::string& path, const FilterbankHeader& h,

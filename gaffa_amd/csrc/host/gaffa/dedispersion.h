@@ -103,6 +103,9 @@ struct SubbandTables {
   std::vector<std::int32_t> residual;  // [ndm][nsub]
   std::int32_t max_residual = 0;
   std::int32_t max_coarse = 0;
+  // largest, over nominal groups, sum over subbands of the group's largest
+  // residual rounded up to 8: the GPU kernel's LDS window budget
+  std::int64_t residual_window = 0;
 };
 
 SubbandTables make_subband_tables(std::span<const double> frequency_mhz,
@@ -214,10 +217,34 @@ struct GpuSegmentTables {
   bool subband = false;
   GpuDeviceBuffer<std::int32_t> first, second;  // brute: delays; subband: coarse, residual
   std::int32_t bound_first = 0, bound_second = 0;  // brute: block spread; subband: maxima
+  std::int64_t residual_window = 0;                 // subband: SubbandTables::residual_window
+  std::vector<std::int32_t> host_first, host_second;  // subband: host copies of the tables
+  std::size_t n_nominal = 0;                          // subband: nominal groups
   std::size_t ndm = 0, chan_begin = 0, chan_end = 0;
   std::size_t subband_channels = 0, ndm_per_nominal = 0;
   std::size_t lookahead = 0;  // input samples past an output range its kernel reads
 };
+
+// Every nominal group of several subband segments (same channel range and
+// subband width) in one table set, so one launch dedisperses all of them:
+// concatenated coarse rows and residual rows, and one group entry per
+// nominal group naming its coarse row, first output row and DM count.
+struct GpuSubbandBatch {
+  GpuDeviceBuffer<std::int32_t> coarse, residual;
+  GpuDeviceBuffer<gaffa_hip_subband_group> groups;
+  std::size_t ngroups = 0, n_coarse_rows = 0, ndm = 0, max_group = 0;
+  std::size_t chan_begin = 0, chan_end = 0, subband_channels = 0;
+  std::int32_t max_coarse = 0, max_residual = 0;
+  std::int64_t residual_window = 0;
+};
+
+GpuSubbandBatch make_gpu_subband_batch(const std::vector<GpuSegmentTables>& segments);
+
+// Enqueues output samples [begin, end) of every segment of the batch (rows in
+// segment order from `out`).
+void enqueue_subband_batch_range(const GpuSubbandBatch& batch, const GpuChannelMajorView& channels,
+                                 std::size_t nsamples, std::uint32_t* out, std::size_t out_stride,
+                                 std::size_t begin, std::size_t end, gaffa_hip_stream stream);
 
 GpuSegmentTables make_gpu_segment_tables(std::span<const double> frequency_mhz,
                                          const MultiDmDedispersionPlan& plan, bool subband,

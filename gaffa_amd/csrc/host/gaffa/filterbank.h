@@ -16,6 +16,19 @@
 
 namespace gaffa {
 
+// gaffa's read options (include/gaffa/filterbank.h:12-31): the delivered
+// channel order, and how a descending file is reversed on the host.
+enum class ChannelOrderPolicy { FrequencyAscending, PreserveFileOrder };
+enum class ReverseBackend { Auto, CpuScalar, CpuOpenmp };
+inline constexpr std::size_t default_filterbank_io_buffer_bytes = 64ull << 20;
+
+struct FilterbankReadOptions {
+  std::size_t io_buffer_bytes = default_filterbank_io_buffer_bytes;
+  ChannelOrderPolicy channel_order = ChannelOrderPolicy::FrequencyAscending;
+  ReverseBackend reverse_backend = ReverseBackend::Auto;
+  std::size_t openmp_min_rows = 4096;
+};
+
 struct FilterbankHeader {
   std::size_t header_size = 0;
   std::size_t nsamples = 0;  // from the file size
@@ -24,12 +37,26 @@ struct FilterbankHeader {
   double az_start = 0.0, za_start = 0.0, src_raj = 0.0, src_dej = 0.0, tstart = 0.0;
   double tsamp = 0.0, fch1 = 0.0, foff = 0.0, refdm = 0.0, period = 0.0;
   std::string rawdatafile, source_name;
-  std::vector<double> frequency_table;  // ascending (after any reversal)
+  std::vector<double> frequency_table;  // in the delivered channel order
   bool uses_frequency_table = false;
-  bool reversed = false;  // the file stores channels in descending frequency
+  bool reversed = false;  // channels are delivered in the reverse of the file's order
 };
 
-FilterbankHeader read_filterbank_header(const std::string& path);
+// Header of a SIGPROC file with the frequency metadata of the delivered
+// channel order (ascending unless PreserveFileOrder), gaffa's checks and
+// messages (io/filterbank.cpp:120-234).
+FilterbankHeader read_filterbank_header(
+    const std::string& path, ChannelOrderPolicy order = ChannelOrderPolicy::FrequencyAscending);
+
+// Header and samples ([nsamples][nifs][nchans] elements of nbits / 8 bytes,
+// little endian as stored) in the requested channel order
+// (io/filterbank.cpp:236-373).
+struct FilterbankData {
+  FilterbankHeader header;
+  std::vector<std::uint8_t> bytes;
+};
+
+FilterbankData read_filterbank(const std::string& path, const FilterbankReadOptions& options = {});
 
 // Writes an 8-bit filterbank (channels as given, fch1/foff header).
 void write_filterbank_u8(const std::string& path, const FilterbankHeader& header,

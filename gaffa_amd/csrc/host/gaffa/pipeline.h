@@ -84,9 +84,8 @@ class GpuSearchPipeline {
   void set_kernel_timing(bool on) noexcept { kernel_timing_ = on; }
 
   std::size_t ndm() const noexcept { return ndm_; }
-  // DM trials per nominal group each segment's subband tables use (the
-  // configured value, halved where the group's residual span would not fit
-  // the kernel's LDS); empty for brute force.
+  // DM trials per nominal group each segment's subband tables use (always
+  // the configured value); empty for brute force.
   std::vector<std::size_t> segment_ndm_per_nominal() const {
     std::vector<std::size_t> v;
     for (const GpuSegmentTables& t : segment_tables_)
@@ -143,6 +142,7 @@ class GpuSearchPipeline {
   std::vector<double> dms_;
   std::vector<MultiDmDedispersionPlan> segment_plans_;
   std::vector<GpuSegmentTables> segment_tables_;
+  GpuSubbandBatch subband_batch_;  // every segment's nominal groups, one launch
   GpuDeviceBuffer<std::uint8_t> filterbank_;
   GpuDeviceBuffer<std::uint8_t> channels_;  // channel-major copy (subband path)
   std::size_t channel_stride_ = 0;

@@ -44,6 +44,7 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   }
   filterbank_ = GpuDeviceBuffer<std::uint8_t>(cfg_.nsamples * cfg_.nchans);
   if (cfg_.subband) {
+    subband_batch_ = make_gpu_subband_batch(segment_tables_);
     for (const MultiDmDedispersionPlan& p : segment_plans_)
       channel_stride_ = std::max(channel_stride_,
                                  subband_channel_stride(cfg_.nsamples, cfg_.frequency_mhz, p,
@@ -104,6 +105,11 @@ void GpuSearchPipeline::download_filterbank(std::uint8_t* out) const {
 void GpuSearchPipeline::dedisperse_range(std::size_t begin, std::size_t end) {
   const GpuChannelMajorView cm{channels_.data(), cfg_.nsamples, 0, cfg_.nchans, channel_stride_,
                                cfg_.device_id};
+  if (cfg_.subband) {
+    enqueue_subband_batch_range(subband_batch_, cm, cfg_.nsamples, plane_.data(), stride_, begin,
+                                end, stream_.get());
+    return;
+  }
   std::size_t row = 0;
   for (std::size_t k = 0; k < segment_tables_.size(); ++k) {
     enqueue_segment_range(segment_tables_[k], filterbank_.data(), cm, cfg_.nsamples, cfg_.nchans,

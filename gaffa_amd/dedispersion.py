@@ -1,8 +1,11 @@
 """Python-facing dedispersion API (gaffa's ``gaffa.dedispersion``, on the GPU).
 
 Functions take a :class:`gaffa_amd.io.Filterbank` and return host-resident
-NumPy results; ``backend`` is ``"gpu"`` (``"cuda"`` is accepted as an alias
-so gaffa scripts run unchanged).
+NumPy results.  ``backend`` takes gaffa's names with gaffa's defaults
+(``"cpu"`` for the single-DM, multi-DM and spectrum forms, ``"cuda"`` for
+subband): ``"cuda"`` (or its synonym ``"gpu"``) runs on the GPU, and
+``"cpu"``, gaffa's own CPU path, raises ValueError because gaffa_amd does
+not ship it.
 """
 
 from __future__ import annotations

@@ -1,7 +1,8 @@
 """Fast-folding-algorithm search of preprocessed time series (gaffa's ``gaffa.ffa``).
 
 ``make_riptide_plan`` and ``ffa_search`` keep gaffa's signatures, defaults
-and error messages (python/gaffa/ffa.py:12-133); the search runs on the GPU.
+and error messages (python/gaffa/ffa.py:12-133); the search runs on the GPU
+(``backend="cuda"`` or ``"gpu"``).
 ``ffa_search_batch`` searches many equal-length series with one plan.
 """
 
@@ -13,14 +14,14 @@ from ._core import (
     FfaBatchedSearch,
     FfaPeak,
     FfaPlan,
+    _check_ffa_search,
     _ffa_search_batch_gpu,
     _ffa_search_gpu_host,
     _ffa_transform_rows_gpu,
     _make_riptide_ffa_plan,
 )
 
-_GPU_BACKENDS = ("gpu", "cuda")
-
+_GPU_BACKENDS = ("cuda", "gpu")
 
 def make_riptide_plan(
     *,
@@ -71,10 +72,17 @@ def make_octave_plan(
 
 
 def _check_backend(backend: str, device_id: int) -> None:
+    """gaffa's backend rules and messages (python/gaffa/ffa.py:116-133): "cpu"
+    takes no device; "cuda" (here HIP on the GPU, "gpu" is a synonym) takes a
+    non-negative ordinal.  gaffa_amd does not ship gaffa's CPU path (its CPU
+    restatement is test-only), so backend="cpu" is refused, not computed."""
     if backend == "cpu":
-        raise ValueError("backend must be 'gpu' or 'cuda': gaffa_amd runs the FFA on the GPU")
+        if device_id != 0:
+            raise ValueError("device_id is only valid with backend='cuda'")
+        raise ValueError("backend='cpu' is gaffa's CPU path, which gaffa_amd does not provide; "
+                         "use backend='cuda' (or 'gpu')")
     if backend not in _GPU_BACKENDS:
-        raise ValueError("backend must be 'gpu' or 'cuda'")
+        raise ValueError("backend must be 'cpu' or 'cuda' ('gpu' is accepted for 'cuda')")
     if device_id < 0:
         raise ValueError("device_id must be non-negative")
 
@@ -85,10 +93,15 @@ def ffa_search(
     *,
     snr_threshold: float = 6.0,
     max_peaks: int | None = None,
-    backend: str = "gpu",
+    backend: str = "cpu",
     device_id: int = 0,
 ) -> list[FfaPeak]:
-    """Search one preprocessed float32 series; peaks in descending peak order."""
+    """Search one preprocessed float32 series; peaks in descending peak order.
+
+    The signature and default are gaffa's: ``backend="cpu"`` is gaffa's
+    default and raises ValueError here, so pass ``backend="cuda"`` (or
+    ``"gpu"``) to search on the GPU."""
+    _check_ffa_search(time_series, plan, max_peaks)
     _check_backend(backend, device_id)
     return _ffa_search_gpu_host(time_series, plan, device_id=device_id,
                                 snr_threshold=snr_threshold, max_peaks=max_peaks)
@@ -107,7 +120,7 @@ def ffa_search_batch(
 
     Returns ``(series_index, peak)`` pairs sorted by series, then peak order.
     """
-    _check_backend("gpu", device_id)
+    _check_backend("cuda", device_id)
     return _ffa_search_batch_gpu(batch, plan, device_id=device_id, snr_threshold=snr_threshold,
                                  max_peaks=max_peaks, series_tile=series_tile)
 

@@ -1,26 +1,24 @@
-"""Minimal SIGPROC filterbank container (the input type of the dedispersion API).
+"""SIGPROC filterbank files (gaffa's ``gaffa.io``).
 
-Mirrors the header fields of gaffa's ``FilterbankHeader`` (include/gaffa/
-filterbank.h:28-64) and its default channel order: channels are stored in
-ascending frequency, so files with ``foff < 0`` are reversed on read
-(src/gaffa/io/filterbank.cpp:74-118).  ``Filterbank.from_array`` wraps
-samples already in memory (the synthetic inputs of the benchmark and tests).
+``Filterbank(path, *, channel_order=..., reverse_backend=..., io_buffer_bytes=...,
+openmp_min_rows=...)`` keeps gaffa's signature, defaults and messages
+(src/gaffa/python/filterbank_bindings.cpp:23-43, 164-172).  The file is read
+by the native C++ reader (header checks of io/filterbank.cpp:120-234,
+channel reversal of :236-373): channels are delivered in ascending
+frequency unless ``channel_order="preserve_file_order"``.  The header
+mirrors gaffa's ``FilterbankHeader`` (include/gaffa/filterbank.h:33-64).
+``Filterbank.from_array`` wraps samples already in memory (the synthetic
+inputs of the benchmark and tests).
 """
 
 from __future__ import annotations
 
 import os
-import struct
 from dataclasses import dataclass, field
 
 import numpy as np
 
-_INT_KEYS = {"telescope_id", "machine_id", "data_type", "barycentric", "pulsarcentric",
-             "ibeam", "nbeams", "npuls", "nbins", "nbits", "nifs", "nchans", "nsamples"}
-_DOUBLE_KEYS = {"az_start", "za_start", "src_raj", "src_dej", "tstart", "tsamp", "fch1",
-                "foff", "refdm", "period"}
-_STRING_KEYS = {"rawdatafile", "source_name"}
-_DTYPES = {8: np.uint8, 16: np.uint16, 32: np.float32}
+DEFAULT_IO_BUFFER_BYTES = 64 * 1024 * 1024  # gaffa's default_filterbank_io_buffer_bytes
 
 
 @dataclass
@@ -54,87 +52,57 @@ class FilterbankHeader:
     frequency_table: list = field(default_factory=list)
     uses_frequency_table: bool = False
 
-
-def _read_string(buf: bytes, pos: int) -> tuple[str, int]:
-    (n,) = struct.unpack_from("<i", buf, pos)
-    if n < 0 or n > 4096:
-        raise RuntimeError("Non-standard filterbank file: bad header string length")
-    return buf[pos + 4: pos + 4 + n].decode("ascii"), pos + 4 + n
+    def __repr__(self) -> str:
+        return (f"<FilterbankHeader nsamples={self.nsamples} nifs={self.nifs} "
+                f"nchans={self.nchans} nbits={self.nbits} foff={self.foff:f}>")
 
 
-def read_header(buf: bytes) -> FilterbankHeader:
-    h = FilterbankHeader()
-    key, pos = _read_string(buf, 0)
-    if key != "HEADER_START":
-        raise RuntimeError("Non-standard filterbank file: missing HEADER_START")
-    freqs: list[float] = []
-    while True:
-        key, pos = _read_string(buf, pos)
-        if key == "HEADER_END":
-            break
-        if key in _STRING_KEYS:
-            val, pos = _read_string(buf, pos)
-            setattr(h, key, val)
-        elif key in _INT_KEYS:
-            (val,) = struct.unpack_from("<i", buf, pos)
-            pos += 4
-            if key != "nsamples":  # recomputed from the file size, as gaffa does
-                setattr(h, key, val)
-        elif key in _DOUBLE_KEYS:
-            (val,) = struct.unpack_from("<d", buf, pos)
-            pos += 8
-            setattr(h, key, val)
-        elif key in ("FREQUENCY_START", "FREQUENCY_END"):
-            h.uses_frequency_table = True
-        elif key == "fchannel":
-            (val,) = struct.unpack_from("<d", buf, pos)
-            pos += 8
-            freqs.append(val)
-        else:
-            raise RuntimeError(f"Unknown filterbank header field: {key}")
-    h.header_size = pos
-    if h.uses_frequency_table:
-        h.frequency_table = freqs
-        h.nchans = len(freqs)
-    if h.nchans <= 0:
-        raise RuntimeError("Invalid filterbank nchans")
-    if h.nbits not in _DTYPES:
-        raise RuntimeError("Unsupported filterbank nbits")
-    return h
+def _header(d: dict) -> FilterbankHeader:
+    keys = FilterbankHeader.__dataclass_fields__
+    return FilterbankHeader(**{k: v for k, v in d.items() if k in keys})
+
+
+def read_header(path: str | os.PathLike, *,
+                channel_order: str = "frequency_ascending") -> FilterbankHeader:
+    """The header of a SIGPROC file, with the frequency metadata of the
+    delivered channel order."""
+    from ._core import _read_filterbank
+
+    hd, _ = _read_filterbank(os.fspath(path), channel_order=channel_order, header_only=True)
+    return _header(hd)
 
 
 class Filterbank:
-    """Filterbank samples (nsamples, nifs, nchans), channels ascending in frequency."""
+    """Filterbank samples (nsamples, nifs, nchans) and their header."""
 
-    def __init__(self, path: str | os.PathLike | None = None, *, header: FilterbankHeader | None = None,
-                 data: np.ndarray | None = None):
+    def __init__(self, path: str | os.PathLike | None = None, *,
+                 channel_order: str = "frequency_ascending", reverse_backend: str = "auto",
+                 io_buffer_bytes: int = DEFAULT_IO_BUFFER_BYTES, openmp_min_rows: int = 4096,
+                 header: FilterbankHeader | None = None, data: np.ndarray | None = None):
         if path is not None:
-            with open(path, "rb") as f:
-                raw = f.read()
-            header = read_header(raw)
-            dtype = _DTYPES[header.nbits]
-            row = header.nifs * header.nchans * np.dtype(dtype).itemsize
-            payload = len(raw) - header.header_size
-            header.nsamples = payload // row
-            data = np.frombuffer(raw, dtype=dtype, count=header.nsamples * header.nifs * header.nchans,
-                                 offset=header.header_size)
-            data = data.reshape(header.nsamples, header.nifs, header.nchans)
-            if not header.uses_frequency_table:
-                header.frequency_table = [header.fch1 + c * header.foff for c in range(header.nchans)]
-            ft = header.frequency_table
-            descending = len(ft) >= 2 and ft[0] > ft[-1]
-            reverse = descending if header.uses_frequency_table else header.foff < 0.0
-            if reverse:
-                data = data[:, :, ::-1]
-                header.frequency_table = list(reversed(ft))
-                header.fch1 = header.frequency_table[0]
-                if not header.uses_frequency_table:
-                    header.foff = abs(header.foff)
-            data = np.ascontiguousarray(data)
+            from ._core import _read_filterbank
+
+            hd, data = _read_filterbank(os.fspath(path), channel_order=channel_order,
+                                        reverse_backend=reverse_backend,
+                                        io_buffer_bytes=io_buffer_bytes,
+                                        openmp_min_rows=openmp_min_rows)
+            header = _header(hd)
         if header is None or data is None:
             raise ValueError("Filterbank needs a path or both header and data")
         self.header = header
         self.data = data
+
+    @property
+    def shape(self):
+        return self.data.shape
+
+    @property
+    def dtype(self):
+        return self.data.dtype
+
+    @property
+    def nbytes(self):
+        return self.data.nbytes
 
     @classmethod
     def from_array(cls, samples: np.ndarray, *, tsamp: float, frequency_mhz) -> "Filterbank":
@@ -152,7 +120,7 @@ class Filterbank:
 
     def __repr__(self) -> str:
         h = self.header
-        return f"<Filterbank nsamples={h.nsamples} nchans={h.nchans} nbits={h.nbits} tsamp={h.tsamp}>"
+        return f"<Filterbank shape=({h.nsamples}, {h.nifs}, {h.nchans}) nbits={h.nbits}>"
 
 
 __all__ = ["Filterbank", "FilterbankHeader", "read_header"]

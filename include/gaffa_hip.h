@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define GAFFA_HIP_ABI_VERSION 1
+#define GAFFA_HIP_ABI_VERSION 2
 
 typedef enum gaffa_hip_status {
   GAFFA_HIP_OK = 0,
@@ -154,26 +154,45 @@ int gaffa_hip_copy_rows_u8(const uint8_t* src, uint8_t* dst, uint64_t rows,
 int gaffa_hip_filterbank_transpose_u8(const gaffa_hip_transpose_args* args,
                                       gaffa_hip_stream stream);
 
-/* Time tile gaffa_hip_dedisperse_subband_u8 uses for this geometry (the
- * u16 subband intermediate of every subband over tile + max_residual
- * samples must fit LDS), or 0 when none fits: the call would fail with
- * GAFFA_HIP_OVERFLOW and a smaller ndm_per_nominal is needed. */
+/* Time chunk gaffa_hip_dedisperse_subband_u8 uses for this geometry: the
+ * u16 subband windows (chunk + the group's residual halo per subband) must
+ * fit LDS.  residual_window as in gaffa_hip_dedisperse_subband_args (0 =
+ * nsub * max_residual).  0 when nothing fits: the call would fail with
+ * GAFFA_HIP_OVERFLOW. */
 uint32_t gaffa_hip_subband_time_tile(uint64_t nsel, uint64_t subband_channels,
-                                     uint64_t ndm_per_nominal, int32_t max_residual);
+                                     uint64_t max_group_ndm, int32_t max_residual,
+                                     int64_t residual_window);
 
 /* Row length (bytes) a channel-major copy needs so that every stage-1 read
  * of gaffa_hip_dedisperse_subband_u8 stays in the row. */
 uint64_t gaffa_hip_subband_channel_stride(uint64_t nsamples, int32_t max_coarse,
                                           int32_t max_residual);
 
+/* One nominal DM group of a multi-segment subband launch: coarse delays row
+ * `coarse_row` of coarse_delays, residual delays and output rows
+ * [first_dm, first_dm + ndm). */
+typedef struct gaffa_hip_subband_group {
+  uint32_t coarse_row;
+  uint32_t first_dm;
+  uint32_t ndm;
+  uint32_t reserved; /* 0 */
+} gaffa_hip_subband_group;
+
 /* Two-stage subband dedispersion fused in LDS: stage 1 sums each subband at
  * the coarse delays of its nominal DM, reading the channel-major rows made by
  * gaffa_hip_filterbank_transpose_u8; stage 2 adds the subbands at the residual
  * delays; the intermediate never reaches HBM.  Replaces subband_stage1_kernel
  * + subband_stage2_kernel and their tile loop (dedispersion_cuda.cu:350-417,
- * 659-707).  nsub = ceil(nsel / subband_channels); n_nominal = ceil(ndm /
- * ndm_per_nominal).  max_residual bounds every residual delay, max_coarse
- * every coarse delay. */
+ * 659-707).  nsub = ceil(nsel / subband_channels).  Groups: either `groups`
+ * (device, ngroups entries; ndm = output rows, n_coarse_rows = rows of
+ * coarse_delays, several DM segments in one launch) or, with groups NULL,
+ * gaffa's nominal groups of one plan: group k = DMs [k * ndm_per_nominal,
+ * ...) with coarse row k.  max_residual bounds every residual delay,
+ * max_coarse every coarse delay; residual_window bounds, over the groups,
+ * the sum over subbands of the group's largest residual in that subband,
+ * each rounded up to a multiple of 8 (0 = nsub * max_residual rounded up:
+ * always safe, may need a smaller chunk).  Any group size whose windows fit
+ * LDS works (no ndm_per_nominal limit besides max_group_ndm <= 1024). */
 typedef struct gaffa_hip_dedisperse_subband_args {
   const uint8_t* channels;  /* channel-major [nsel][channel_stride] */
   uint64_t channel_stride;  /* bytes between channel rows, multiple of 4 */
@@ -192,6 +211,11 @@ typedef struct gaffa_hip_dedisperse_subband_args {
   uint64_t out_stride;
   int32_t max_residual;
   int32_t max_coarse;
+  const gaffa_hip_subband_group* groups; /* NULL = gaffa's groups of one plan */
+  uint64_t ngroups;
+  uint64_t n_coarse_rows;
+  uint64_t max_group_ndm; /* largest group (groups mode; else ndm_per_nominal) */
+  int64_t residual_window;
 } gaffa_hip_dedisperse_subband_args;
 
 int gaffa_hip_dedisperse_subband_u8(

@@ -171,6 +171,17 @@ class Plan:
     widths: np.ndarray  # uint64
 
 
+class FbHeader(C.Structure):
+    _fields_ = [("header_size", C.c_int64), ("nsamples", C.c_int64)] + \
+        [(k, C.c_int32) for k in ("telescope_id", "machine_id", "data_type", "barycentric",
+                                  "pulsarcentric", "ibeam", "nbeams", "npuls", "nbins", "nbits",
+                                  "nifs", "nchans")] + \
+        [(k, C.c_double) for k in ("az_start", "za_start", "src_raj", "src_dej", "tstart",
+                                   "tsamp", "fch1", "foff", "refdm", "period")] + \
+        [("uses_frequency_table", C.c_int32), ("rawdatafile", C.c_char * 256),
+         ("source_name", C.c_char * 256)]
+
+
 class Oracle:
     def __init__(self, kind: str = "port"):
         path = LIBS[kind]
@@ -193,6 +204,26 @@ class Oracle:
     @property
     def implementation(self) -> str:
         return self.lib.orc_implementation().decode()
+
+    # -- SIGPROC reader (reference build only) ---------------------------
+    def read_filterbank(self, path, preserve_file_order=False):
+        """gaffa's read_filterbank: (header dict, frequency table, sample bytes)."""
+        h = FbHeader()
+        nbytes = C.c_uint64()
+        fn = self.lib.orc_read_filterbank
+        freq = np.empty(1 << 16, dtype=np.float64)
+        self._call(fn, str(path).encode(), C.c_int32(1 if preserve_file_order else 0),
+                   C.byref(h), _ptr(freq), C.c_uint64(freq.size), None, C.c_uint64(0),
+                   C.byref(nbytes))
+        data = np.empty(nbytes.value, dtype=np.uint8)
+        self._call(fn, str(path).encode(), C.c_int32(1 if preserve_file_order else 0),
+                   C.byref(h), _ptr(freq), C.c_uint64(freq.size), _ptr(data),
+                   C.c_uint64(data.size), C.byref(nbytes))
+        d = {k: getattr(h, k) for k, _ in FbHeader._fields_}
+        d["rawdatafile"] = h.rawdatafile.decode()
+        d["source_name"] = h.source_name.decode()
+        d["uses_frequency_table"] = bool(h.uses_frequency_table)
+        return d, freq[:h.nchans].copy(), data
 
     # -- dedispersion ---------------------------------------------------
     def delay_bins(self, dm, f, ref, tsamp) -> int:

@@ -246,6 +246,25 @@ int orc_find_dm_peaks(const uint32_t* plane, uint64_t ndm, uint64_t nsamples,
                       float snr_threshold, uint64_t max_peaks,
                       orc_dm_peak* peaks, uint64_t capacity, uint64_t* count);
 
+/* ---- reference-only: gaffa's SIGPROC reader (io/filterbank.cpp), exported
+ * by the ref shim alone to pin the product's native reader. */
+typedef struct orc_fb_header {
+  int64_t header_size, nsamples;
+  int32_t telescope_id, machine_id, data_type, barycentric, pulsarcentric, ibeam, nbeams,
+      npuls, nbins, nbits, nifs, nchans;
+  double az_start, za_start, src_raj, src_dej, tstart, tsamp, fch1, foff, refdm, period;
+  int32_t uses_frequency_table;
+  char rawdatafile[256];
+  char source_name[256];
+} orc_fb_header;
+
+/* Reads `path` with channel_order preserve_file_order (preserve != 0) or
+ * frequency_ascending; frequency table to freq (capacity freq_cap), sample
+ * bytes to data (capacity data_cap, NULL = header only). */
+int orc_read_filterbank(const char* path, int32_t preserve, orc_fb_header* header,
+                        double* freq, uint64_t freq_cap, void* data, uint64_t data_cap,
+                        uint64_t* data_bytes);
+
 /* ---- oracle-only extras (no reference counterpart; the ref shim returns 6) */
 /* FFA transform with FP64 accumulation and the same index schedule. */
 int orc_ffa_transform_f64(const double* input, uint64_t rows, uint64_t bins,

@@ -10,6 +10,7 @@
 #include "gaffa/dedispersion.h"
 #include "gaffa/harmonic.h"
 #include "gaffa/dm_search.h"
+#include "gaffa/filterbank.h"
 #include "gaffa/ffa.h"
 #include "gaffa/folding.h"
 #include "gaffa/ffa_detection.h"
@@ -21,7 +22,9 @@
 #include "gaffa/preprocessing.h"
 #include "gaffa/time_series.h"
 
+#include <cstdio>
 #include <cstring>
+#include <variant>
 #include <exception>
 #include <stdexcept>
 #include <string>
@@ -538,6 +541,55 @@ int orc_fold_spectrum(const void* data, int32_t dtype, uint64_t ns, uint64_t nc,
     if (r.data.size() > cap) throw too_small{};
     std::copy(r.data.begin(), r.data.end(), cube);
     std::copy(r.exposure.begin(), r.exposure.end(), exposure);
+  });
+}
+
+int orc_read_filterbank(const char* path, int32_t preserve, orc_fb_header* h, double* freq,
+                        uint64_t freq_cap, void* data, uint64_t data_cap, uint64_t* data_bytes) {
+  return guarded([&] {
+    gaffa::FilterbankReadOptions o;
+    o.channel_order = preserve != 0 ? gaffa::ChannelOrderPolicy::PreserveFileOrder
+                                    : gaffa::ChannelOrderPolicy::FrequencyAscending;
+    gaffa::FilterbankData d = gaffa::read_filterbank(path, o);
+    const gaffa::FilterbankHeader& g = d.header;
+    h->header_size = g.header_size;
+    h->nsamples = g.nsamples;
+    h->telescope_id = g.telescope_id;
+    h->machine_id = g.machine_id;
+    h->data_type = g.data_type;
+    h->barycentric = g.barycentric;
+    h->pulsarcentric = g.pulsarcentric;
+    h->ibeam = g.ibeam;
+    h->nbeams = g.nbeams;
+    h->npuls = g.npuls;
+    h->nbins = g.nbins;
+    h->nbits = g.nbits;
+    h->nifs = g.nifs;
+    h->nchans = g.nchans;
+    h->az_start = g.az_start;
+    h->za_start = g.za_start;
+    h->src_raj = g.src_raj;
+    h->src_dej = g.src_dej;
+    h->tstart = g.tstart;
+    h->tsamp = g.tsamp;
+    h->fch1 = g.fch1;
+    h->foff = g.foff;
+    h->refdm = g.refdm;
+    h->period = g.period;
+    h->uses_frequency_table = g.uses_frequency_table ? 1 : 0;
+    std::snprintf(h->rawdatafile, sizeof(h->rawdatafile), "%s", g.rawdatafile.c_str());
+    std::snprintf(h->source_name, sizeof(h->source_name), "%s", g.source_name.c_str());
+    if (g.frequency_table.size() > freq_cap) throw too_small{};
+    std::copy(g.frequency_table.begin(), g.frequency_table.end(), freq);
+    std::visit(
+        [&](const auto& v) {
+          const uint64_t nbytes = v.size() * sizeof(v[0]);
+          *data_bytes = nbytes;
+          if (data == nullptr) return;
+          if (nbytes > data_cap) throw too_small{};
+          std::memcpy(data, v.data(), nbytes);
+        },
+        d.samples);
   });
 }
 

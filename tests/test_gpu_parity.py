@@ -52,8 +52,8 @@ def test_basetest_dedispersion_bit_exact(gpu):
     say("dedispersion: basetest.fil 50 DMs")
     fb = Filterbank(os.path.join(GOLDEN, "basetest.fil"))
     g = np.load(os.path.join(GOLDEN, "basetest_dedispersion.npz"))
-    multi = dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=50)
-    assert multi.backend == "gpu" and multi.data.dtype == np.uint32
+    multi = dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=50, backend="cuda")
+    assert multi.backend == "cuda" and multi.data.dtype == np.uint32
     np.testing.assert_array_equal(multi.data, g["multi"])
     sub = dedispersion.dedisperse_subband(fb, dm_low=0.0, dm_step=1.0, ndm=50,
                                           subband_channels=16, ndm_per_nominal=8)
@@ -63,7 +63,7 @@ def test_basetest_dedispersion_bit_exact(gpu):
                                           subband_channels=1, ndm_per_nominal=1)
     # (4 DMs have a longer valid range than 50: compare the common prefix)
     np.testing.assert_array_equal(deg.data[:, : multi.nsamples], g["multi"][:4])
-    single = dedispersion.dedisperse_single_dm(fb, dm=3.0)
+    single = dedispersion.dedisperse_single_dm(fb, dm=3.0, backend="gpu")
     np.testing.assert_array_equal(single.data[0, : multi.nsamples], g["multi"][3])
 
 
@@ -76,7 +76,7 @@ def test_small_brute_force_bit_exact(gpu, port):
     w = small_workload()
     x = gpu._synth_filterbank_gpu(w.synth_spec())
     fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
-    got = dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=64)
+    got = dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=64, backend="cuda")
     want = port.dedisperse(x, w.frequencies, dm_low=0.0, dm_step=1.0, ndm=64, tsamp=w.tsamp)
     assert got.shape == want.shape == (64, 130870)
     np.testing.assert_array_equal(got.data, want)
@@ -135,13 +135,12 @@ def test_subband_large_slice_vs_oracle(gpu, port):
     np.testing.assert_array_equal(x[:4096], want_x)
     assert x[:, w.rfi_channels].mean() > x.mean() + 5  # the +40 channels are there
     fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
-    # at DM ~990 a 32-trial nominal group spans ~1700 residual samples: the
-    # 64-subband intermediate does not fit LDS, and 16 trials per group do
+    # at DM ~990 a 32-trial nominal group spans ~1700 residual samples in the
+    # lowest subband; 24 trials per group leave a short last group (64 = 24 +
+    # 24 + 16)
     top = w.dm_segments[-1]
-    with pytest.raises(OverflowError, match="too large for LDS"):
-        dedispersion.dedisperse_subband(fb, dm_low=top.dm_low, dm_step=top.dm_step, ndm=64,
-                                        subband_channels=64, ndm_per_nominal=32)
-    for seg, npn in ((w.dm_segments[0], 32), (top, 16)):
+    for seg, npn in ((w.dm_segments[0], 32), (top, 32), (top, 24)):
+        say(f"  DM {seg.dm_low:.1f} step {seg.dm_step:.4f}, {npn} trials per nominal group")
         r = dedispersion.dedisperse_subband(fb, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
                                             subband_channels=64, ndm_per_nominal=npn)
         want = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
@@ -151,9 +150,9 @@ def test_subband_large_slice_vs_oracle(gpu, port):
 
 
 def test_large_pipeline_segments_vs_oracle(gpu, port):
-    """The pipeline at 4096 channels narrows the nominal group of the
-    high-DM segments to fit LDS and reports it; the plane rows of the first
-    and last segments equal the oracle at the reported group size."""
+    """The pipeline at 4096 channels dedisperses every segment in one launch
+    at the configured 32 trials per nominal group; the plane rows of the
+    first, a middle and the last segment equal the oracle."""
     import dataclasses
 
     from gaffa_amd.pipeline import large_workload
@@ -161,19 +160,18 @@ def test_large_pipeline_segments_vs_oracle(gpu, port):
     say("pipeline: 4096-ch x 2^17 dedispersion, segment nominal groups vs oracle")
     w = dataclasses.replace(large_workload(nsamples=1 << 17, npulsars=4), period_max=1.0)
     pipe = w.pipeline(ffa_workspace_bytes=2 << 30)
-    npn = pipe.segment_ndm_per_nominal
-    assert len(npn) == len(w.dm_segments) and npn[0] == 32 and npn[-1] < 32
-    assert all(a >= b for a, b in zip(npn, npn[1:]))
+    assert pipe.segment_ndm_per_nominal == [32] * len(w.dm_segments)
     pipe.generate(w.synth_spec())
     pipe.dedisperse()
     plane = pipe.download_plane_u32()
     x = pipe.download_filterbank(w.nsamples, w.nchans)
     n_out = plane.shape[1]
-    for k in (0, len(w.dm_segments) - 1):
+    for k in (0, len(w.dm_segments) // 2, len(w.dm_segments) - 1):
         seg = w.dm_segments[k]
+        say(f"  segment {k}: DM {seg.dm_low:.1f} step {seg.dm_step:.4f}")
         want = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step,
                                ndm=seg.ndm, tsamp=w.tsamp, subband=True, subband_channels=64,
-                               ndm_per_nominal=npn[k])
+                               ndm_per_nominal=32)
         np.testing.assert_array_equal(plane[k * seg.ndm:(k + 1) * seg.ndm], want[:, :n_out])
 
 
@@ -277,7 +275,7 @@ def test_ffa_rows_and_search_vs_golden(gpu):
     got = np.concatenate([ffa.ffa_transform_rows(series[None, :], plan, k)[0].ravel()
                           for k in sel])
     np.testing.assert_array_equal(got, g["rows"])
-    peaks = ffa.ffa_search(series, plan, snr_threshold=6.0)
+    peaks = ffa.ffa_search(series, plan, snr_threshold=6.0, backend="cuda")
     want = g["peaks"]
     assert len(peaks) == len(want)
     for p, q in zip(peaks, want):
@@ -391,7 +389,7 @@ def test_ffa_max_peaks_guard(gpu):
     x = np.random.default_rng(1).normal(size=1 << 14).astype(np.float32)
     plan = ffa.make_riptide_plan(nsamples=x.size, tsamp=1e-3, period_min=0.5, period_max=3.0)
     with pytest.raises(RuntimeError, match="max_peaks"):
-        ffa.ffa_search(x, plan, snr_threshold=-1e6, max_peaks=3)
+        ffa.ffa_search(x, plan, snr_threshold=-1e6, max_peaks=3, backend="cuda")
 
 
 # ------------------------------------------------------------ DM search ----
@@ -590,11 +588,11 @@ def test_spectrum_filterbank_and_fold(gpu, ref):
 
     say("spectrum: basetest.fil DM 10 + fold")
     fb = Filterbank(os.path.join(GOLDEN, "basetest.fil"))
-    spec = dedispersion.dedisperse_spectrum(fb, dm=10.0)
+    spec = dedispersion.dedisperse_spectrum(fb, dm=10.0, backend="cuda")
     want = ref.dedisperse_spectrum(fb.data[:, 0, :], fb.header.frequency_table, 10.0,
                                    fb.header.tsamp)
     np.testing.assert_array_equal(spec.data, want)
-    assert spec.backend == "gpu" and spec.dm == 10.0
+    assert spec.backend == "cuda" and spec.dm == 10.0
     r = pfold.fold_spectrum(spec, period=0.25, nbin=32, tsubint=0.5, nsubband=4)
     cube, expo = ref.fold_spectrum(want, 0.25, fb.header.tsamp, 32,
                                    round(0.5 / fb.header.tsamp), output_channels=4)

@@ -28,7 +28,7 @@ def test_c_abi_exports_every_declared_symbol():
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert missing == []
     lib.gaffa_hip_abi_version.restype = ctypes.c_int
-    assert lib.gaffa_hip_abi_version() == 1
+    assert lib.gaffa_hip_abi_version() == 2
 
 
 def test_c_abi_rejects_bad_arguments_without_a_device():
@@ -69,6 +69,12 @@ def test_product_delays_equal_oracle(core, port):
 
 
 def test_python_surface_errors(core):
+    """gaffa's own Python assertions (tests/test_ffa.py:45-80), restated: the
+    same exception types and message substrings for the same calls, with
+    gaffa's default backend="cpu", which gaffa_amd refuses after the input
+    checks."""
+    import inspect
+
     from gaffa_amd import ffa
 
     plan = ffa.make_riptide_plan(nsamples=8, tsamp=1.0, period_min=2.0, period_max=4.0,
@@ -82,29 +88,46 @@ def test_python_surface_errors(core):
         ffa.ffa_search(np.zeros((1, 8), dtype=np.float32), plan)
     with pytest.raises(ValueError, match="C-contiguous"):
         ffa.ffa_search(np.zeros(16, dtype=np.float32)[::2], plan)
+    plan9 = ffa.make_riptide_plan(nsamples=9, tsamp=1.0, period_min=2.0, period_max=4.0,
+                                  bins_min=2, bins_max=2)
     with pytest.raises(ValueError, match="input_nsamples"):
-        ffa.ffa_search(np.zeros(8, np.float32),
-                       ffa.make_riptide_plan(nsamples=9, tsamp=1.0, period_min=2.0,
-                                             period_max=4.0, bins_min=2, bins_max=2))
+        ffa.ffa_search(np.zeros(8, np.float32), plan9)
     with pytest.raises(ValueError, match="positive or None"):
         ffa.ffa_search(np.zeros(8, np.float32), plan, max_peaks=0)
-    with pytest.raises(ValueError, match="'gpu' or 'cuda'"):
+    with pytest.raises(ValueError, match="'cpu' or 'cuda'"):
         ffa.ffa_search(np.zeros(8, np.float32), plan, backend="opencl")
-    with pytest.raises(ValueError, match="'gpu' or 'cuda'"):
-        ffa.ffa_search(np.zeros(8, np.float32), plan, backend="cpu")
+    with pytest.raises(ValueError, match="only valid"):
+        ffa.ffa_search(np.zeros(8, np.float32), plan, device_id=1)
+    # gaffa's default backend is its CPU path, which is not part of gaffa_amd
+    with pytest.raises(ValueError, match="backend='cuda'"):
+        ffa.ffa_search(np.zeros(8, np.float32), plan)
+    with pytest.raises(ValueError, match="non-negative"):
+        ffa.ffa_search(np.zeros(8, np.float32), plan, backend="cuda", device_id=-1)
+    assert inspect.signature(ffa.ffa_search).parameters["backend"].default == "cpu"
 
 
 def test_dedispersion_surface_errors(core):
+    """gaffa's dedispersion backend names, defaults and messages
+    (dedispersion_bindings.cpp:56-64, 648-672; tests/test_dedispersion.py:109-113)."""
     from gaffa_amd import dedispersion
     from gaffa_amd.io import Filterbank
 
     fb = Filterbank(os.path.join(GOLDEN, "basetest.fil"))
     with pytest.raises(ValueError, match="unknown dedispersion backend"):
         dedispersion.dedisperse_single_dm(fb, dm=0.0, backend="bad")
-    with pytest.raises(ValueError, match="backend 'cpu' is not provided"):
-        dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=2, backend="cpu")
-    with pytest.raises(ValueError, match="backend 'cpu' is not provided"):
-        dedispersion.dedisperse_spectrum(fb, dm=0.0, backend="cpu")
+    calls = (
+        lambda **k: dedispersion.dedisperse_multi_dm(fb, dm_low=0.0, dm_step=1.0, ndm=2, **k),
+        lambda **k: dedispersion.dedisperse_single_dm(fb, dm=0.0, **k),
+        lambda **k: dedispersion.dedisperse_spectrum(fb, dm=0.0, **k),
+    )
+    for call in calls:
+        # gaffa's default for these three is backend="cpu"
+        with pytest.raises(ValueError, match="backend 'cpu' is gaffa's CPU path"):
+            call()
+        with pytest.raises(ValueError, match="backend 'cpu'"):
+            call(backend="cpu")
+    doc = dedispersion.dedisperse_subband.__doc__
+    assert "backend: str = 'cuda'" in doc and "ndm_per_nominal: " in doc
 
 
 def test_filterbank_from_array_roundtrip():
@@ -170,12 +193,12 @@ def test_dm_peak_key_sort_matches_comparator(core, ranked, parts):
 
 
 def test_native_filterbank_reader_and_writer(core, tmp_path):
-    """The C++ header reader agrees with the Python reader on basetest.fil,
-    and a written file reads back (descending files come back ascending)."""
+    """The header-only read agrees with the full read on basetest.fil, and a
+    written file reads back (descending files come back ascending)."""
     from gaffa_amd.io import Filterbank, write_filterbank
 
     path = os.path.join(GOLDEN, "basetest.fil")
-    h = core._read_filterbank_header(path)
+    h, _ = core._read_filterbank(path, header_only=True)
     fb = Filterbank(path)
     assert (h["nsamples"], h["nchans"], h["nbits"], h["tsamp"]) == \
         (fb.header.nsamples, fb.header.nchans, fb.header.nbits, fb.header.tsamp)
@@ -186,12 +209,117 @@ def test_native_filterbank_reader_and_writer(core, tmp_path):
     for foff in (-2.0, 2.0):
         out = tmp_path / f"w{foff}.fil"
         write_filterbank(out, x, tsamp=1e-3, fch1=1400.0, foff=foff, source_name="synthetic")
-        hw = core._read_filterbank_header(str(out))
+        hw, _ = core._read_filterbank(str(out), header_only=True)
         back = Filterbank(str(out))
         assert hw["reversed"] == (foff < 0) and hw["nsamples"] == 1000
         assert hw["frequency_table"] == list(back.header.frequency_table)
         want = x[:, ::-1] if foff < 0 else x
         np.testing.assert_array_equal(back.data[:, 0, :], want)
+
+
+def _sigproc_string(s: str) -> bytes:
+    import struct
+
+    return struct.pack("<i", len(s)) + s.encode()
+
+
+def _write_raw_header(path, fields, payload: bytes):
+    """A SIGPROC file from (key, kind, value) fields, kind in i/d/s/- (a bare
+    keyword such as FREQUENCY_START)."""
+    import struct
+
+    out = _sigproc_string("HEADER_START")
+    for key, kind, value in fields:
+        out += _sigproc_string(key)
+        if kind == "i":
+            out += struct.pack("<i", value)
+        elif kind == "d":
+            out += struct.pack("<d", value)
+        elif kind == "s":
+            out += _sigproc_string(value)
+    out += _sigproc_string("HEADER_END") + payload
+    with open(path, "wb") as fh:
+        fh.write(out)
+
+
+def test_native_filterbank_reader_equals_gaffa(core, ref, tmp_path):
+    """The native reader against gaffa's own read_filterbank (io/filterbank.cpp,
+    compiled from the reference sources): every header field, the frequency
+    table and the sample bytes, in both channel orders, for basetest.fil, for
+    written ascending / descending 8-bit files, and for 16- and 32-bit files
+    with a descending FREQUENCY table; and the same errors for a partial
+    trailing row and an fchannel outside a FREQUENCY block."""
+    from gaffa_amd.io import Filterbank, write_filterbank
+
+    rng = np.random.default_rng(0)
+    x = rng.integers(0, 256, size=(1000, 16), dtype=np.uint8)
+    paths = [os.path.join(GOLDEN, "basetest.fil")]
+    for foff in (-2.0, 2.0):
+        out = tmp_path / f"w{foff}.fil"
+        write_filterbank(out, x, tsamp=1e-3, fch1=1400.0, foff=foff, source_name="synthetic")
+        paths.append(str(out))
+    table = [1500.0 - 3.0 * c for c in range(12)]
+    for nbits, dtype in ((16, np.uint16), (32, np.float32)):
+        y = rng.integers(0, 4000, size=(300, 2, 12)).astype(dtype)
+        fields = [("rawdatafile", "s", "raw.dat"), ("source_name", "s", "J0000+0000"),
+                  ("telescope_id", "i", 6), ("machine_id", "i", 10), ("data_type", "i", 1),
+                  ("nbits", "i", nbits), ("nifs", "i", 2), ("tsamp", "d", 6.4e-5),
+                  ("tstart", "d", 60000.5), ("refdm", "d", 3.5), ("FREQUENCY_START", "-", 0)]
+        fields += [("fchannel", "d", f) for f in table] + [("FREQUENCY_END", "-", 0),
+                                                          ("nsamples", "i", 999)]
+        out = tmp_path / f"table{nbits}.fil"
+        _write_raw_header(out, fields, y.tobytes())
+        paths.append(str(out))
+    for path in paths:
+        for order in ("frequency_ascending", "preserve_file_order"):
+            want_h, want_f, want_d = ref.read_filterbank(path, order == "preserve_file_order")
+            fb = Filterbank(path, channel_order=order)
+            got = fb.header
+            for key, value in want_h.items():
+                assert getattr(got, key) == value, (path, order, key)
+            assert list(got.frequency_table) == list(want_f), (path, order)
+            np.testing.assert_array_equal(np.frombuffer(fb.data.tobytes(), np.uint8), want_d)
+            assert fb.data.shape == (want_h["nsamples"], want_h["nifs"], want_h["nchans"])
+            # the cpu_scalar and cpu_openmp reversals give the same bytes
+            for backend in ("cpu_scalar", "cpu_openmp"):
+                alt = Filterbank(path, channel_order=order, reverse_backend=backend,
+                                 io_buffer_bytes=4096)
+                np.testing.assert_array_equal(alt.data, fb.data)
+
+    partial = tmp_path / "partial.fil"
+    _write_raw_header(partial, [("nbits", "i", 8), ("nchans", "i", 4), ("tsamp", "d", 1e-3),
+                                ("fch1", "d", 1400.0), ("foff", "d", -1.0)], bytes(4 * 10 + 3))
+    stray = tmp_path / "stray.fil"
+    _write_raw_header(stray, [("nbits", "i", 8), ("fchannel", "d", 1400.0)], bytes(8))
+    for bad, msg in ((partial, "not a whole number of rows"),
+                     (stray, "fchannel outside FREQUENCY block")):
+        with pytest.raises(RuntimeError, match=msg):
+            ref.read_filterbank(bad)
+        with pytest.raises(RuntimeError, match=msg):
+            Filterbank(bad)
+
+
+def test_filterbank_options_and_messages(core):
+    """gaffa's Filterbank keyword names and messages
+    (tests/test_filterbank_io.py:12-60)."""
+    from gaffa_amd.io import Filterbank
+
+    path = os.path.join(GOLDEN, "basetest.fil")
+    fb = Filterbank(path)
+    assert fb.header.nbits == 8 and fb.data.dtype == np.uint8
+    assert fb.shape == fb.data.shape == (fb.header.nsamples, fb.header.nifs, fb.header.nchans)
+    assert fb.dtype == fb.data.dtype and fb.nbytes == fb.data.nbytes
+    assert Filterbank(path, channel_order="preserve_file_order").header.foff < 0
+    with pytest.raises(ValueError, match="unknown channel_order"):
+        Filterbank(path, channel_order="bad")
+    with pytest.raises(ValueError, match="unknown reverse_backend"):
+        Filterbank(path, reverse_backend="bad")
+
+    def load_data():
+        return Filterbank(path).data
+
+    data = load_data()
+    assert data.shape[0] > 0 and data[0, 0, 0] >= 0
 
 
 def test_large_workload_shape(core):
