@@ -612,6 +612,7 @@ PYBIND11_MODULE(_core, m) {
   PYBIND11_NUMPY_DTYPE(FfaSearchTask, downsample_factor, effective_tsamp, input_nsamples,
                        prepared_nsamples, bins, rows, rows_eval, period_begin, period_end);
 
+  m.attr("DM_PEAK_RECORD_BYTES") = sizeof(DmPeak);
   m.def("gpu_device_count", &gpu_device_count);
   m.def("gpu_runtime_version", &gpu_runtime_version);
   m.def("cuda_device_count", &gpu_device_count, "Alias of gpu_device_count for gaffa scripts.");
@@ -1481,6 +1482,23 @@ PYBIND11_MODULE(_core, m) {
       .def_property_readonly("peaks", &GpuSearchPipeline::peaks)
       .def_property_readonly("peak_count",
                              [](const GpuSearchPipeline& p) { return p.peaks().size(); })
+      .def(
+          "peak_records",
+          [](const GpuSearchPipeline& p, std::size_t dm_begin, std::size_t dm_end) {
+            // raw DmPeak records, in list order, of the DM trials in
+            // [dm_begin, dm_end): a filtered sorted list stays sorted
+            if (dm_end > p.ndm()) dm_end = p.ndm();
+            std::vector<DmPeak> out;
+            {
+              py::gil_scoped_release release;
+              for (const DmPeak& q : p.peaks())
+                if (q.dm_index >= dm_begin && q.dm_index < dm_end) out.push_back(q);
+            }
+            return py::bytes(reinterpret_cast<const char*>(out.data()),
+                             out.size() * sizeof(DmPeak));
+          },
+          py::arg("dm_begin") = 0, py::arg("dm_end") = static_cast<std::size_t>(-1),
+          "Peaks of DM trials [dm_begin, dm_end) as packed DmPeak records (bytes)")
       .def_property_readonly("ffa_tile", [](const GpuSearchPipeline& p) {
         return p.ffa_program().tile_capacity();
       })

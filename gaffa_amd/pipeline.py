@@ -21,9 +21,72 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from ._core import DmSegment, GpuSearchPipeline, SynthFilterbankSpec, SynthPulsar
+from ._core import DM_PEAK_RECORD_BYTES, DmSegment, GpuSearchPipeline, SynthFilterbankSpec, \
+    SynthPulsar
 
 DISPERSION_S = 4148.808  # s MHz^2 / (pc cm^-3), dedispersion_delay.h:16
+
+# gaffa's DmPeak record (dm_search.h:16-20 around ffa_peak.h:8-19), as laid
+# out in memory by the host layer
+DM_PEAK_DTYPE = np.dtype(
+    [("dm", "f8"), ("dm_index", "u8"),
+     ("peak", np.dtype([("period", "f8"), ("frequency", "f8"), ("width", "u8"),
+                        ("duty_cycle", "f8"), ("width_index", "u8"), ("period_index", "u8"),
+                        ("phase", "u8"), ("shift", "u8"), ("bins", "u8"), ("snr", "f4")],
+                       align=True))],
+    align=True)
+assert DM_PEAK_DTYPE.itemsize == DM_PEAK_RECORD_BYTES
+
+# The candidate stage of gaffa's GPU DM-search benchmark
+# (benchmarks/cuda_dm_search_benchmark.cpp:105-136, 279-292): clustering over
+# 0.1 Fourier bins and 50 DM trials across widths, then harmonic flagging up
+# to the 32nd harmonic with denominators up to 100, harmonics removed.
+CANDIDATE_OPTIONS = dict(frequency_cluster_radius=0.1, dm_cluster_radius=50,
+                         cluster_across_widths=True)
+HARMONIC_OPTIONS = dict(max_harmonic=32, denominator_max=100, frequency_tolerance_bins=1.5,
+                        phase_distance_max=1.0, dm_distance_max=3.0)
+
+
+def peak_records(pipe: GpuSearchPipeline, dm_begin: int = 0, dm_end: int = -1) -> np.ndarray:
+    """The last search's peaks of DM trials [dm_begin, dm_end) as a structured
+    array (DM_PEAK_DTYPE), in the list's order."""
+    end = pipe.ndm if dm_end < 0 else dm_end
+    return np.frombuffer(pipe.peak_records(dm_begin, end), dtype=DM_PEAK_DTYPE)
+
+
+def search_candidates(pipe: GpuSearchPipeline, workload: "Workload") -> list:
+    """Candidates of the last search as gaffa's benchmark makes them:
+    clustered peaks with harmonics flagged and removed."""
+    from ._core import flag_harmonics, remove_harmonics
+
+    cands = pipe.select_candidates(**CANDIDATE_OPTIONS)
+    if not cands:
+        return []
+    f = workload.frequencies
+    flagged = flag_harmonics(cands, workload.nsamples * workload.tsamp, float(f.min()),
+                             float(f.max()), **HARMONIC_OPTIONS)
+    return remove_harmonics(flagged)
+
+
+def profile_snr(profile: np.ndarray, widths=(1, 2, 4, 8, 16)) -> float:
+    """Boxcar S/N of a folded pulse profile: the largest circular boxcar sum
+    of the profile minus its median, over width w bins, divided by
+    sqrt(w) times the noise sigma of one bin (1.4826 times the median
+    absolute deviation, which a pulse of up to ~20 % duty barely moves)."""
+    p = np.asarray(profile, np.float64).ravel()
+    med = float(np.median(p))
+    sigma = 1.4826 * float(np.median(np.abs(p - med)))
+    if not np.isfinite(sigma) or sigma <= 0.0:
+        return 0.0
+    x = p - med
+    c = np.concatenate([[0.0], np.cumsum(np.concatenate([x, x]))])
+    n = len(x)
+    best = -np.inf
+    for w in widths:
+        if w > n:
+            break
+        best = max(best, float(np.max(c[w:w + n] - c[:n])) / (sigma * math.sqrt(w)))
+    return best
 
 
 def channel_frequencies(nchans: int, fch1: float, foff: float) -> np.ndarray:
@@ -267,6 +330,7 @@ def batched_workload(nseries: int = 2048, seed: int = 20260303, *, tsamp: float 
                            amplitude=amp.astype(np.float32))
 
 
-__all__ = ["BatchedWorkload", "DmSegment", "GpuSearchPipeline", "SynthFilterbankSpec", "SynthPulsar", "Workload",
-           "batched_workload", "channel_frequencies", "lband_workload", "pulse_amplitude", "small_workload",
-           "smearing_dm_segments"]
+__all__ = ["BatchedWorkload", "CANDIDATE_OPTIONS", "DM_PEAK_DTYPE", "DmSegment", "GpuSearchPipeline",
+           "HARMONIC_OPTIONS", "SynthFilterbankSpec", "SynthPulsar", "Workload", "batched_workload",
+           "channel_frequencies", "large_workload", "lband_workload", "peak_records", "profile_snr", "pulse_amplitude",
+           "search_candidates", "small_workload", "smearing_dm_segments"]

@@ -408,12 +408,31 @@ def test_dm_search_vs_golden(gpu):
         assert abs(p.peak.snr - q["peak"]["snr"]) <= 1e-5 * abs(q["peak"]["snr"])
 
 
-def test_small_pipeline_end_to_end(gpu, port):
-    from gaffa_amd.pipeline import small_workload
+def assert_dm_peaks_equal(got, want):
+    """Complete DM peak lists (structured records): equal counts, identical
+    DM index, period index, shift, phase, width index, width and bins in the
+    same order, S/N within 1e-5 relative."""
+    assert len(got) == len(want), (len(got), len(want))
+    if len(want) == 0:
+        return
+    np.testing.assert_array_equal(got["dm_index"], want["dm_index"])
+    for f in ("period_index", "shift", "phase", "width_index", "width", "bins"):
+        np.testing.assert_array_equal(got["peak"][f], want["peak"][f], err_msg=f)
+    np.testing.assert_allclose(got["peak"]["snr"], want["peak"]["snr"], rtol=1e-5, atol=0)
+    np.testing.assert_array_equal(got["peak"]["period"], want["peak"]["period"])
 
-    say("pipeline: small workload end to end vs oracle")
+
+@pytest.mark.parametrize("ffa_series_tile", [0, 7])
+def test_small_pipeline_end_to_end(gpu, port, ffa_series_tile):
+    """The 64-channel workload through the device-resident pipeline: the
+    plane bit-exact and the complete peak list equal to the oracle's
+    find_dm_peaks (with 7 series per FFA tile, ten tiles' sorted lists are
+    merged)."""
+    from gaffa_amd.pipeline import peak_records, small_workload
+
+    say(f"pipeline: small workload end to end vs oracle, series tile {ffa_series_tile}")
     w = small_workload()
-    pipe = w.pipeline()
+    pipe = w.pipeline(ffa_series_tile=ffa_series_tile)
     pipe.generate(w.synth_spec())
     pipe.dedisperse()
     plane = pipe.download_plane_u32()
@@ -422,19 +441,72 @@ def test_small_pipeline_end_to_end(gpu, port):
     np.testing.assert_array_equal(plane, want_plane)
     pipe.preprocess()
     pipe.search()
-    got = pipe.peaks
     opts = port.plan_options(w.period_min, w.period_max, w.bins_min, w.bins_max, 1,
                              w.duty_cycle_max, w.width_trial_spacing)
     pp = port.preprocess_plan(window_samples=w.window_samples(), min_points=w.min_points)
     want = port.find_dm_peaks(plane, w.dms, w.tsamp, opts, pp, w.snr_threshold)
-    top = min(len(want), 50)
-    assert len(got) >= top > 0
-    for p, q in zip(got[:top], want[:top]):
-        assert (p.dm_index, p.peak.shift, p.peak.width, p.peak.bins) == \
-            (q["dm_index"], q["peak"]["shift"], q["peak"]["width"], q["peak"]["bins"])
-        assert abs(p.peak.snr - q["peak"]["snr"]) <= 1e-5 * abs(q["peak"]["snr"])
+    got = peak_records(pipe)
+    assert len(want) > 1000
+    assert_dm_peaks_equal(got, want)
     # the injected pulsar (P 0.714 s, DM 30) is the top candidate
-    assert abs(got[0].peak.period - 0.714) < 0.005 and abs(got[0].dm - 30.0) <= 3.0
+    top = pipe.peaks[0]
+    assert abs(top.peak.period - 0.714) < 0.005 and abs(top.dm - 30.0) <= 3.0
+
+
+def _full_length_group_vs_oracle(port, w, segment: int, group: int):
+    """Run the whole pipeline at the workload's full length on the GPU, then
+    compare the complete peak list of one 32-trial nominal group with the
+    oracle's subband dedispersion of that group's segment and find_dm_peaks
+    of the group's rows, on the same filterbank bytes."""
+    from gaffa_amd.pipeline import peak_records
+
+    pipe = w.pipeline()
+    pipe.generate(w.synth_spec())
+    pipe.run()
+    first = sum(s.ndm for s in w.dm_segments[:segment]) + group * w.ndm_per_nominal
+    got = peak_records(pipe, first, first + w.ndm_per_nominal).copy()
+    got["dm_index"] -= first
+    x = pipe.download_filterbank(w.nsamples, w.nchans)
+    del pipe
+    seg = w.dm_segments[segment]
+    ndm = (group + 1) * w.ndm_per_nominal  # the segment's trials up to the group
+    say(f"  oracle: subband dedispersion of {ndm} trials at DM {seg.dm_low:.2f}+"
+        f"{seg.dm_step:.4f}k, FFA of trials {group * w.ndm_per_nominal}..{ndm - 1}")
+    plane = port.dedisperse(x, w.frequencies, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=ndm,
+                            tsamp=w.tsamp, subband=True, subband_channels=w.subband_channels,
+                            ndm_per_nominal=w.ndm_per_nominal)
+    del x
+    rows = np.ascontiguousarray(plane[group * w.ndm_per_nominal:])
+    del plane
+    dms = seg.dm_low + seg.dm_step * np.arange(group * w.ndm_per_nominal, ndm)
+    opts = port.plan_options(w.period_min, w.period_max, w.bins_min, w.bins_max, 1,
+                             w.duty_cycle_max, w.width_trial_spacing)
+    pp = port.preprocess_plan(window_samples=w.window_samples(), min_points=w.min_points)
+    want = port.find_dm_peaks(rows, dms, w.tsamp, opts, pp, w.snr_threshold)
+    say(f"  {len(got)} GPU peaks, {len(want)} oracle peaks")
+    assert len(want) > 1000
+    assert_dm_peaks_equal(got, want)
+
+
+def test_lband_full_length_first_group_vs_oracle(gpu, port):
+    """1024 channels x 2^22 samples, 1024 DM trials: the complete peak list of
+    the first nominal group (DM trials 0..31) equals the oracle's."""
+    from gaffa_amd.pipeline import lband_workload
+
+    say("pipeline: lband at 2^22 samples, first nominal group vs oracle")
+    _full_length_group_vs_oracle(port, lband_workload(), segment=0, group=0)
+
+
+def test_large_full_length_last_group_vs_oracle(gpu, port):
+    """4096 channels x 2^22 samples with RFI, 4096 DM trials up to DM ~1000:
+    the complete peak list of the last nominal group (32 trials per group,
+    as configured) equals the oracle's."""
+    from gaffa_amd.pipeline import large_workload
+
+    w = large_workload()
+    say("pipeline: large at 2^22 samples, last nominal group vs oracle")
+    last = w.dm_segments[-1].ndm // w.ndm_per_nominal - 1
+    _full_length_group_vs_oracle(port, w, segment=len(w.dm_segments) - 1, group=last)
 
 
 def _check_series_peaks(got, want, s):
