@@ -7,6 +7,20 @@
   libgaffa_hip.so through an $ORIGIN rpath.
 
 Run ``python -m gaffa_amd.build`` (or ``__graft_entry__.build()``).
+
+``python gaffa_amd/build.py --sanitize`` rebuilds ``_core`` in place with
+AddressSanitizer and UndefinedBehaviorSanitizer on the host code (g++ only:
+the HIP library is unchanged, GPU code is never built with them), and
+``make -C oracle SANITIZE=1 -B oracle`` does the same for the CPU oracle.  Run
+the CPU tests with the runtimes preloaded::
+
+    LD_PRELOAD="$(g++ -print-file-name=libasan.so) $(g++ -print-file-name=libubsan.so)" \
+    ASAN_OPTIONS=detect_leaks=0 UBSAN_OPTIONS=halt_on_error=1:print_stacktrace=1 \
+    python -m pytest tests -m "not gpu"
+
+then ``python gaffa_amd/build.py`` and ``make -C oracle -B oracle`` again for
+the normal build (build.py run as a file: importing the package would load the
+sanitized module first).
 """
 
 from __future__ import annotations
@@ -76,7 +90,11 @@ def hip_library(force: bool = True) -> str:
     return lib
 
 
-def core_module(force: bool = True) -> str:
+SANITIZE_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer",
+                  "-fno-sanitize-recover=undefined"]
+
+
+def core_module(force: bool = True, sanitize: bool = False) -> str:
     import pybind11
 
     os.makedirs(BUILD, exist_ok=True)
@@ -92,24 +110,26 @@ def core_module(force: bool = True) -> str:
              "-ffp-contract=off", "-Wall", "-Wextra", "-Wno-unused-parameter",
              f"-I{INCLUDE}", f"-I{os.path.join(CSRC, 'host')}", f"-I{pybind11.get_include()}",
              f"-I{py_inc}"]
+    san = SANITIZE_FLAGS if sanitize else []
     objs, jobs = [], []
     for s in srcs:
-        o = os.path.join(BUILD, "host_" + os.path.basename(s).replace(".cpp", ".o"))
+        o = os.path.join(BUILD, ("san_" if sanitize else "host_") +
+                         os.path.basename(s).replace(".cpp", ".o"))
         objs.append(o)
-        jobs.append([CXX, *flags, "-c", s, "-o", o])
+        jobs.append([CXX, *flags, *san, "-c", s, "-o", o])
     with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
         list(ex.map(_run, jobs))
-    _run([CXX, "-shared", "-fopenmp", "-o", target, *objs, f"-L{PKG}", "-lgaffa_hip",
+    _run([CXX, "-shared", "-fopenmp", *san, "-o", target, *objs, f"-L{PKG}", "-lgaffa_hip",
           "-Wl,-rpath,$ORIGIN"])
     return target
 
 
-def build(force: bool = True) -> None:
-    hip_library(force)
+def build(force: bool = True, sanitize: bool = False) -> None:
+    hip_library(force and not sanitize)
     if os.path.isdir(os.path.join(CSRC, "host")):
-        core_module(force)
+        core_module(force or sanitize, sanitize)
 
 
 if __name__ == "__main__":
-    build(force="--incremental" not in sys.argv)
+    build(force="--incremental" not in sys.argv, sanitize="--sanitize" in sys.argv)
     print("built", os.path.join(PKG, "libgaffa_hip.so"))

@@ -988,8 +988,12 @@ PYBIND11_MODULE(_core, m) {
       py::arg("denominator_max") = 5, py::arg("frequency_tolerance_bins") = 1.5,
       py::arg("phase_distance_max") = 1.0, py::arg("dm_distance_max") = 3.0,
       py::arg("use_snr_consistency") = false, py::arg("snr_distance_max") = 3.0);
-  m.def("remove_harmonics", &remove_harmonics_cpu, py::arg("candidates"),
-        py::arg("max_candidates") = 0);
+  m.def(
+      "remove_harmonics",
+      [](const std::vector<HarmonicCandidate>& c, std::size_t max_candidates) {
+        return remove_harmonics_cpu(c, max_candidates);
+      },
+      py::arg("candidates"), py::arg("max_candidates") = 0);
 
   // batched FFA over series with their own plans
   py::class_<GpuFfaBatchedSearch>(m, "FfaBatchedSearch")

@@ -11,12 +11,14 @@
 //               own window in sorted order
 //   detrend   - FP64 linear interpolation of the low-resolution median back to
 //               full resolution (same operation order as the CPU), subtract,
-//               write float, per-block FP64 (sum, sum of squares) partials
-//   stats     - one workgroup per series: fixed-order (deterministic)
-//               reduction of the partials into mean and 1/stddev
+//               write float
+//   stats     - one lane per series: the FP64 sum and sum of squares in
+//               sample order, exactly the CPU loop's roundings, then mean and
+//               1/stddev (time_series.cpp:50-77)
 //   normalise - (x - mean) * (1/stddev) in FP64, rounded to float
-// Integer-valued inputs (dedispersed sums) make the scrunch sums exact; the
-// only difference from the CPU is the order of the FP64 statistics sums.
+// Integer-valued inputs (dedispersed sums) make the scrunch sums exact, and
+// every other FP operation runs in the CPU's order, so the output is the
+// CPU's bit for bit.
 #include "common.hpp"
 
 #include <cmath>
@@ -132,14 +134,12 @@ median_kernel(const T* __restrict__ src, uint64_t src_stride, uint64_t m,
   med[series * med_stride + static_cast<uint64_t>(k)] = value;
 }
 
-// x - interpolated median, written in place as float, plus FP64 partials.
+// x - interpolated median, written in place as float.
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 detrend_kernel(T* __restrict__ data, uint64_t stride, uint64_t n,
                int32_t detrend, uint64_t factor, const float* __restrict__ med,
-               uint64_t m, double2* __restrict__ partial, uint32_t nblocks,
-               int32_t* __restrict__ status) {
-  __shared__ double s1[kThreads], s2[kThreads];
+               uint64_t m, int32_t* __restrict__ status) {
   const uint32_t tid = threadIdx.x;
   const uint64_t series = blockIdx.y;
   T* x = data + series * stride;
@@ -147,7 +147,6 @@ detrend_kernel(T* __restrict__ data, uint64_t stride, uint64_t n,
   const float* md = med + series * (factor == 1 ? n : m);
   const double offset = 0.5 * static_cast<double>(factor - 1);
   const double fd = static_cast<double>(factor);
-  double a1 = 0.0, a2 = 0.0;
   bool bad = false;
   const uint64_t base = static_cast<uint64_t>(blockIdx.x) * kBlockSamples;
 #pragma unroll
@@ -180,53 +179,67 @@ detrend_kernel(T* __restrict__ data, uint64_t stride, uint64_t n,
       d = v - baseline;
     }
     out[i] = d;
-    const double dd = static_cast<double>(d);
-    a1 += dd;
-    a2 += dd * dd;
   }
   if (bad) atomicExch(&status[series], 1);
-  s1[tid] = a1;
-  s2[tid] = a2;
-  __syncthreads();
-  for (uint32_t h = kThreads / 2; h > 0; h >>= 1) {
-    if (tid < h) {
-      s1[tid] += s1[tid + h];
-      s2[tid] += s2[tid + h];
-    }
-    __syncthreads();
-  }
-  if (tid == 0) partial[series * nblocks + blockIdx.x] = make_double2(s1[0], s2[0]);
 }
 
-// One workgroup per series: fixed-order (deterministic) reduction of the
-// detrend partials into mean, 1/stddev and a status flag.
-__global__ void __launch_bounds__(kThreads)
-stats_kernel(const double2* __restrict__ partial, uint32_t nblocks, uint64_t n,
+// One lane per series: sum and sum of squares in FP64 in sample order, as
+// time_series_stats_cpu does (time_series.cpp:55-60), so every rounding is
+// the CPU's; then mean, 1/stddev and a status flag.  The series is read with
+// 16-byte loads, a batch of kStatsBatch loads in flight while the previous
+// batch is summed.
+constexpr int kStatsLanes = 64;
+constexpr int kStatsBatch = 8;
+
+struct SerialSums {
+  double sum = 0.0, squares = 0.0;
+  __device__ __forceinline__ void add(float v) {
+    const double x = static_cast<double>(v);
+    sum = sum + x;
+    squares = squares + x * x;  // separately rounded: the code is built without contraction
+  }
+  __device__ __forceinline__ void add4(float4 v) {
+    add(v.x);
+    add(v.y);
+    add(v.z);
+    add(v.w);
+  }
+};
+
+__global__ void __launch_bounds__(kStatsLanes)
+stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32_t nseries,
              int32_t reject_constant, double2* __restrict__ stats,
              int32_t* __restrict__ flags, int32_t* __restrict__ status) {
-  __shared__ double s1[kThreads], s2[kThreads];
-  const uint32_t tid = threadIdx.x;
-  const uint64_t series = blockIdx.x;
-  const double2* p = partial + series * nblocks;
-  double a1 = 0.0, a2 = 0.0;
-  for (uint32_t b = tid; b < nblocks; b += kThreads) {
-    a1 += p[b].x;
-    a2 += p[b].y;
-  }
-  s1[tid] = a1;
-  s2[tid] = a2;
-  __syncthreads();
-  for (uint32_t h = kThreads / 2; h > 0; h >>= 1) {
-    if (tid < h) {
-      s1[tid] += s1[tid + h];
-      s2[tid] += s2[tid + h];
+  const uint32_t series = blockIdx.x * kStatsLanes + threadIdx.x;
+  if (series >= nseries) return;
+  const float* x = data + static_cast<uint64_t>(series) * stride;
+  SerialSums acc;
+  uint64_t head = 0;
+  while (head < n && (reinterpret_cast<uintptr_t>(x + head) & 15u) != 0) acc.add(x[head++]);
+  const uint64_t nv = (n - head) / 4;
+  const float4* v = reinterpret_cast<const float4*>(x + head);
+  uint64_t k = 0;
+  if (nv >= kStatsBatch) {
+    float4 cur[kStatsBatch], nxt[kStatsBatch];
+#pragma unroll
+    for (int u = 0; u < kStatsBatch; ++u) cur[u] = v[u];
+    for (k = kStatsBatch; k + kStatsBatch <= nv; k += kStatsBatch) {
+#pragma unroll
+      for (int u = 0; u < kStatsBatch; ++u) nxt[u] = v[k + u];
+#pragma unroll
+      for (int u = 0; u < kStatsBatch; ++u) acc.add4(cur[u]);
+#pragma unroll
+      for (int u = 0; u < kStatsBatch; ++u) cur[u] = nxt[u];
     }
-    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kStatsBatch; ++u) acc.add4(cur[u]);
   }
-  if (tid != 0) return;
+  for (; k < nv; ++k) acc.add4(v[k]);
+  for (uint64_t i = head + 4 * nv; i < n; ++i) acc.add(x[i]);
+
   const double count = static_cast<double>(n);
-  const double mean = s1[0] / count;
-  double var = s2[0] / count - mean * mean;
+  const double mean = acc.sum / count;
+  double var = acc.squares / count - mean * mean;
   if (var < 0.0 && var > -1.0e-12) var = 0.0;
   int32_t flag = 0;
   if (var < 0.0 || !isfinite(var)) flag = 3;
@@ -281,7 +294,7 @@ void launch_median(const T* src, uint64_t src_stride, uint64_t m, uint32_t windo
 }
 
 struct Layout {
-  uint64_t m, low_floats, med_floats, nblocks, partial_bytes, partial_offset, stats_offset,
+  uint64_t m, low_floats, med_floats, nblocks, stats_offset,
       flags_offset, total;
 };
 
@@ -291,21 +304,18 @@ Layout layout(uint64_t nseries, uint64_t n, uint64_t factor) {
   l.low_floats = factor == 1 ? 0 : nseries * l.m;
   l.med_floats = nseries * l.m;
   l.nblocks = ceil_div(n, kBlockSamples);
-  l.partial_bytes = nseries * l.nblocks * sizeof(double2);
-  l.partial_offset = ((l.low_floats + l.med_floats) * 4 + 255) / 256 * 256;
-  l.stats_offset = l.partial_offset + (l.partial_bytes + 255) / 256 * 256;
+  l.stats_offset = ((l.low_floats + l.med_floats) * 4 + 255) / 256 * 256;
   l.flags_offset = l.stats_offset + (nseries * sizeof(double2) + 255) / 256 * 256;
   l.total = l.flags_offset + nseries * sizeof(int32_t);
   return l;
 }
 
 template <typename T>
-void run(const gaffa_hip_preprocess_args* a, hipStream_t st) {
+void run(const gaffa_hip_preprocess_args* a, void* workspace, hipStream_t st) {
   const Layout l = layout(a->nseries, a->nsamples, a->detrend ? a->scrunch_factor : 1);
-  float* low = static_cast<float*>(a->workspace);
+  float* low = static_cast<float*>(workspace);
   float* med = low + l.low_floats;
-  char* ws = static_cast<char*>(a->workspace);
-  double2* partial = reinterpret_cast<double2*>(ws + l.partial_offset);
+  char* ws = static_cast<char*>(workspace);
   double2* stats = reinterpret_cast<double2*>(ws + l.stats_offset);
   int32_t* flags = reinterpret_cast<int32_t*>(ws + l.flags_offset);
   T* data = static_cast<T*>(a->data);
@@ -333,18 +343,55 @@ void run(const gaffa_hip_preprocess_args* a, hipStream_t st) {
   }
   const dim3 g(static_cast<uint32_t>(l.nblocks), static_cast<uint32_t>(a->nseries));
   hipLaunchKernelGGL((detrend_kernel<T>), g, dim3(kThreads), 0, st, data, a->stride, a->nsamples,
-                     a->detrend, a->detrend ? f : 1, med, l.m, partial,
-                     static_cast<uint32_t>(l.nblocks), a->status);
+                     a->detrend, a->detrend ? f : 1, med, l.m, a->status);
   launch_check("detrend_kernel");
   if (a->normalise) {
-    hipLaunchKernelGGL(stats_kernel, dim3(static_cast<uint32_t>(a->nseries)), dim3(kThreads), 0,
-                       st, partial, static_cast<uint32_t>(l.nblocks), a->nsamples,
-                       a->reject_constant, stats, flags, a->status);
+    hipLaunchKernelGGL(stats_kernel,
+                       dim3(static_cast<uint32_t>(ceil_div(a->nseries, kStatsLanes))),
+                       dim3(kStatsLanes), 0, st, reinterpret_cast<const float*>(data), a->stride,
+                       a->nsamples, static_cast<uint32_t>(a->nseries), a->reject_constant, stats,
+                       flags, a->status);
     launch_check("stats_kernel");
     hipLaunchKernelGGL(normalise_kernel, g, dim3(kThreads), 0, st,
                        reinterpret_cast<float*>(data), a->stride, a->nsamples, stats, flags);
     launch_check("normalise_kernel");
   }
+}
+
+}  // namespace
+}  // namespace gaffa_hip
+
+namespace gaffa_hip {
+namespace {
+
+// Checks shared by both preprocess entry points; returns the scratch layout.
+Layout validate(const gaffa_hip_preprocess_args* a) {
+  check(a != nullptr, "preprocess arguments must not be null");
+  check(a->data != nullptr && a->status != nullptr, "preprocess pointers must not be null");
+  check(a->nseries > 0 && a->nsamples > 0, "preprocess batch must not be empty");
+  check(a->stride >= a->nsamples, "preprocess stride must be >= nsamples");
+  check(a->detrend || a->normalise || a->input_is_u32,
+        "preprocess plan must contain at least one step");
+  check(a->nseries < 65536, "preprocess batch has too many series for one launch");
+  const uint64_t f = a->detrend ? a->scrunch_factor : 1;
+  if (a->detrend) {
+    check(a->median_window > 0 && a->median_window % 2 == 1,
+          "running median detrend window must be odd and > 0");
+    check(f >= 1, "scrunch factor must be >= 1");
+    check(a->median_window + kThreads - 1 <= 4096,
+          "GPU running median supports windows up to 3841 points");
+    const uint64_t m = median_points(a->nsamples, f);
+    if (f == 1) {
+      check(a->median_window < a->nsamples,
+            "running median detrend window_samples must be < input size");
+    } else {
+      check(m > a->median_window,
+            "running median detrend low-resolution sample count must be > min_points");
+    }
+  }
+  const Layout l = layout(a->nseries, a->nsamples, f);
+  check_overflow(l.nblocks < 0x7FFFFFFFull, "preprocess series too long");
+  return l;
 }
 
 }  // namespace
@@ -363,37 +410,13 @@ uint64_t gaffa_hip_preprocess_workspace_bytes(uint64_t nseries, uint64_t nsample
 int gaffa_hip_detrend_normalise_batch(const gaffa_hip_preprocess_args* a,
                                       gaffa_hip_stream stream) {
   return guarded([&] {
-    check(a != nullptr, "preprocess arguments must not be null");
-    check(a->data != nullptr && a->status != nullptr, "preprocess pointers must not be null");
-    check(a->nseries > 0 && a->nsamples > 0, "preprocess batch must not be empty");
-    check(a->stride >= a->nsamples, "preprocess stride must be >= nsamples");
-    check(a->detrend || a->normalise || a->input_is_u32,
-          "preprocess plan must contain at least one step");
-    check(a->nseries < 65536, "preprocess batch has too many series for one launch");
-    const uint64_t f = a->detrend ? a->scrunch_factor : 1;
-    if (a->detrend) {
-      check(a->median_window > 0 && a->median_window % 2 == 1,
-            "running median detrend window must be odd and > 0");
-      check(f >= 1, "scrunch factor must be >= 1");
-      check(a->median_window + kThreads - 1 <= 4096,
-            "GPU running median supports windows up to 3841 points");
-      const uint64_t m = median_points(a->nsamples, f);
-      if (f == 1) {
-        check(a->median_window < a->nsamples,
-              "running median detrend window_samples must be < input size");
-      } else {
-        check(m > a->median_window,
-              "running median detrend low-resolution sample count must be > min_points");
-      }
-    }
-    const Layout l = layout(a->nseries, a->nsamples, f);
+    const Layout l = validate(a);
     check(a->workspace != nullptr && a->workspace_bytes >= l.total,
           "preprocess workspace is too small");
-    check_overflow(l.nblocks < 0x7FFFFFFFull, "preprocess series too long");
     if (a->input_is_u32) {
-      run<uint32_t>(a, as_stream(stream));
+      run<uint32_t>(a, a->workspace, as_stream(stream));
     } else {
-      run<float>(a, as_stream(stream));
+      run<float>(a, a->workspace, as_stream(stream));
     }
   });
 }

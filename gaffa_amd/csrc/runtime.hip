@@ -16,10 +16,12 @@ using namespace gaffa_hip;
 namespace gaffa_hip {
 // A kernel fill, for memory that a captured graph resets on every replay
 // (memset nodes are avoided in graphs).
+// Grid-stride, so one launch covers any count.
 __global__ void __launch_bounds__(256) fill_u32_kernel(uint32_t* dst, uint32_t value,
                                                        uint64_t count) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-  if (i < count) dst[i] = value;
+  const uint64_t step = static_cast<uint64_t>(gridDim.x) * 256;
+  for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x; i < count; i += step)
+    dst[i] = value;
 }
 }  // namespace gaffa_hip
 
@@ -197,8 +199,8 @@ int gaffa_hip_fill_u32(uint32_t* dst, uint32_t value, uint64_t count, gaffa_hip_
   return guarded([&] {
     check(dst != nullptr || count == 0, "fill destination must not be null");
     if (count == 0) return;
-    check_overflow(count < (1ull << 32), "fill too large");
-    hipLaunchKernelGGL(gaffa_hip::fill_u32_kernel, dim3(static_cast<uint32_t>(ceil_div(count, 256))),
+    const uint64_t blocks = std::min<uint64_t>(ceil_div(count, 256), 1ull << 20);
+    hipLaunchKernelGGL(gaffa_hip::fill_u32_kernel, dim3(static_cast<uint32_t>(blocks)),
                        dim3(256), 0, as_stream(stream), dst, value, count);
     launch_check("fill_u32_kernel");
   });
