@@ -453,15 +453,19 @@ def test_small_pipeline_end_to_end(gpu, port, ffa_series_tile):
     # the injected pulsar (P 0.714 s, DM 30) is the top candidate
     top = pipe.peaks[0]
     assert abs(top.peak.period - 0.714) < 0.005 and abs(top.dm - 30.0) <= 3.0
-    # the candidate stage of gaffa's DM-search benchmark on the same peaks
+    # the candidate stage of gaffa's DM-search benchmark on the same peaks,
+    # against gaffa's own candidate code (the reference CPU build)
     from gaffa_amd.pipeline import CANDIDATE_OPTIONS, HARMONIC_OPTIONS, search_candidates
+    from oracle.oracle import LIBS, Oracle
 
+    if not os.path.exists(LIBS["reference"]):
+        pytest.skip("reference CPU build (oracle/_ref) not present for the candidate stage")
+    ref = Oracle("reference")
     cands = search_candidates(pipe, w)
-    sel = port.select_candidates(want, pipe.out_nsamples * w.tsamp,
-                                 **CANDIDATE_OPTIONS)
+    sel = ref.select_candidates(want, pipe.out_nsamples * w.tsamp, **CANDIDATE_OPTIONS)
     f = w.frequencies
-    flagged, rel = port.flag_harmonics(sel, w.nsamples * w.tsamp, f.min(), f.max(),
-                                       **HARMONIC_OPTIONS)
+    flagged, rel = ref.flag_harmonics(sel, w.nsamples * w.tsamp, f.min(), f.max(),
+                                      **HARMONIC_OPTIONS)
     kept = flagged[rel["is_harmonic"] == 0]
     assert 0 < len(kept) < len(sel)
     assert [(c.dm_index, c.period, c.width, np.float32(c.snr), c.peak_count) for c in cands] == \
