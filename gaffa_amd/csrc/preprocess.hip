@@ -21,6 +21,8 @@
 // CPU's bit for bit.
 #include "common.hpp"
 
+#include <mutex>
+
 #include <cmath>
 #include <type_traits>
 
@@ -394,6 +396,35 @@ Layout validate(const gaffa_hip_preprocess_args* a) {
   return l;
 }
 
+// Scratch of the no-allocation entry point: the low-resolution series,
+// medians, statistics and flags of one call.  Every kernel
+// writes each element before reading it, so nothing needs clearing between
+// calls.  Each device holds its own copy for the life of the process; calls
+// that overlap on one device share it and race.
+constexpr uint64_t kNoAllocScratchBytes = GAFFA_HIP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` on the current device, looked up with the
+// device current on the first call there and kept per device (host code
+// never passes the symbol itself to a kernel).
+void* noalloc_scratch() {
+  constexpr int kMaxDevices = 64;
+  static std::mutex mu;
+  static void* ptr[kMaxDevices] = {};
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    throw status_error(GAFFA_HIP_RUNTIME, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (ptr[dev] == nullptr) {
+    void* p = nullptr;
+    hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+    if (p == nullptr) throw status_error(GAFFA_HIP_RUNTIME, "scratch symbol has no address");
+    ptr[dev] = p;
+  }
+  return ptr[dev];
+}
+
 }  // namespace
 }  // namespace gaffa_hip
 
@@ -420,5 +451,26 @@ int gaffa_hip_detrend_normalise_batch(const gaffa_hip_preprocess_args* a,
     }
   });
 }
+
+int gaffa_hip_detrend_normalise_batch_noalloc(const gaffa_hip_preprocess_args* a,
+                                              gaffa_hip_stream stream) {
+  return guarded([&] {
+    const Layout l = validate(a);
+    check(a->workspace == nullptr && a->workspace_bytes == 0,
+          "the no-allocation preprocess entry point uses its own scratch: workspace must be "
+          "null and workspace_bytes 0");
+    check_overflow(l.total <= kNoAllocScratchBytes,
+                   "preprocess batch needs more scratch than the no-allocation entry point "
+                   "holds (GAFFA_HIP_NOALLOC_SCRATCH_BYTES); split the batch");
+    void* ws = noalloc_scratch();
+    if (a->input_is_u32) {
+      run<uint32_t>(a, ws, as_stream(stream));
+    } else {
+      run<float>(a, ws, as_stream(stream));
+    }
+  });
+}
+
+uint64_t gaffa_hip_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes; }
 
 }  // extern "C"

@@ -285,6 +285,22 @@ uint64_t gaffa_hip_preprocess_workspace_bytes(uint64_t nseries,
 int gaffa_hip_detrend_normalise_batch(const gaffa_hip_preprocess_args* args,
                                       gaffa_hip_stream stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the same conversion, detrend and normalise as
+ * gaffa_hip_detrend_normalise_batch, with its scratch in a library-owned
+ * device array of GAFFA_HIP_NOALLOC_SCRATCH_BYTES (held on every device the
+ * library runs on until the process ends) instead of a workspace argument.
+ * `workspace` must be NULL and `workspace_bytes` 0; batches whose scratch
+ * (gaffa_hip_preprocess_workspace_bytes) exceeds the maximum are rejected
+ * with GAFFA_HIP_OVERFLOW.  The call allocates nothing.  All calls on one
+ * device share the array: calls that overlap on one device (other streams,
+ * other threads) race and must be serialised by the caller. */
+#define GAFFA_HIP_NOALLOC_SCRATCH_BYTES (64ull << 20)
+int gaffa_hip_detrend_normalise_batch_noalloc(const gaffa_hip_preprocess_args* args,
+                                              gaffa_hip_stream stream);
+uint64_t gaffa_hip_noalloc_scratch_bytes(void);
+
 /* ------------------------------------------------------------------ FFA ----
  * Device-side layout of one FFA execution plan (host-built, see
  * gaffa_amd/csrc/host/ffa_gpu.cpp).  A plan is shared by every series of a
