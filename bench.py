@@ -165,6 +165,9 @@ def cpu_baseline(workload, pipe, filterbank, threads: int) -> dict:
                            subband=workload.subband,
                            subband_channels=workload.subband_channels,
                            ndm_per_nominal=workload.ndm_per_nominal)
+    # the pipeline's series length is set by the largest delay over all its
+    # trials; a sample's sum does not depend on it
+    plane = np.ascontiguousarray(plane[:, :pipe.out_nsamples])
     t1 = time.perf_counter()
     opts = cpu.plan_options(workload.period_min, workload.period_max, workload.bins_min,
                             workload.bins_max, 1, workload.duty_cycle_max,

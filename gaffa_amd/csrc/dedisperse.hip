@@ -705,6 +705,95 @@ int gaffa_hip_dedisperse_check(gaffa_hip_stream stream) {
 
 }  // extern "C"
 
+// ------------------------------------------------- subband spectrum ----
+// Subband sums of the aligned dynamic spectrum at one DM, for folding:
+// out[k][t] = sum over the channels c of subband k of x[t + delay[c]][c],
+// read from the channel-major copy.  gaffa folds the aligned spectrum
+// (dedispersion.cpp:249-289) and averages each output channel's inputs in
+// FP64 (folding.cpp:354-360); the sum of uint8 samples is an exact integer
+// there, so folding these u16 sums divided by the channel count gives the
+// same values.  One wave per (512-sample tile, subband), 8 samples per lane,
+// with the stage-1 reads of subband dedispersion.
+namespace gaffa_hip {
+namespace {
+
+constexpr uint32_t kSsTile = 512;
+
+__global__ void __launch_bounds__(64)
+subband_spectrum_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride, uint32_t nsel,
+                        uint32_t sbc, const int32_t* __restrict__ delays,
+                        uint16_t* __restrict__ out, uint64_t out_stride, uint64_t out_nsamples) {
+  const uint32_t lane = threadIdx.x;
+  const uint32_t k = blockIdx.y;
+  const uint64_t t0 = static_cast<uint64_t>(blockIdx.x) * kSsTile;
+  const uint32_t cs = k * sbc;
+  const uint32_t ce = min(cs + sbc, nsel);
+  uint32_t a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+  uint32_t c0 = cs;
+  for (; c0 + kSbUnroll <= ce; c0 += kSbUnroll)
+    sb_channels<kSbUnroll>(cm + static_cast<uint64_t>(c0) * cm_stride, cm_stride, delays + c0,
+                           t0, lane, a0, a1, a2, a3);
+  for (; c0 < ce; ++c0)
+    sb_channels<1>(cm + static_cast<uint64_t>(c0) * cm_stride, cm_stride, delays + c0, t0, lane,
+                   a0, a1, a2, a3);
+  const uint64_t t = t0 + 8u * lane;
+  if (t >= out_nsamples) return;
+  uint16_t* o = out + static_cast<uint64_t>(k) * out_stride + t;
+  uint4 v;
+  v.x = __builtin_amdgcn_perm(a1, a0, 0x05040100u);  // s0 | s1 << 16
+  v.y = __builtin_amdgcn_perm(a1, a0, 0x07060302u);  // s2 | s3 << 16
+  v.z = __builtin_amdgcn_perm(a3, a2, 0x05040100u);
+  v.w = __builtin_amdgcn_perm(a3, a2, 0x07060302u);
+  if (t + 8 <= out_nsamples) {
+    *reinterpret_cast<uint4*>(o) = v;
+  } else {
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    for (uint32_t j = 0; t + j < out_nsamples; ++j)
+      o[j] = static_cast<uint16_t>(w[j >> 1] >> (16u * (j & 1u)));
+  }
+}
+
+}  // namespace
+}  // namespace gaffa_hip
+
+extern "C" int gaffa_hip_subband_spectrum_u8(const gaffa_hip_subband_spectrum_args* a,
+                                             gaffa_hip_stream stream) {
+  using namespace gaffa_hip;
+  return guarded([&] {
+    check(a != nullptr, "subband spectrum arguments must not be null");
+    check(a->channels != nullptr && a->delays != nullptr && a->out != nullptr,
+          "subband spectrum pointers must not be null");
+    check(a->nsel > 0 && a->nsel <= 0xFFFFFFFFull, "channel count out of range");
+    check(a->subband_channels > 0 && a->subband_channels <= 256,
+          "subband spectrum supports 1..256 channels per subband");
+    check(a->out_nsamples > 0, "subband spectrum output range is empty");
+    check(a->max_delay >= 0, "subband spectrum delays must be non-negative");
+    const uint64_t row_len = a->channel_length == 0 ? a->channel_stride : a->channel_length;
+    check(row_len <= a->channel_stride || a->nsel == 1, "channel_length exceeds the row pitch");
+    check(a->channel_stride % 4 == 0 && (reinterpret_cast<uintptr_t>(a->channels) & 3u) == 0,
+          "channel rows must be dword aligned");
+    const uint64_t nsub = ceil_div(a->nsel, a->subband_channels);
+    check(nsub <= 65535, "too many subbands for one launch");
+    check(a->out_stride >= a->out_nsamples && a->out_stride % 8 == 0 &&
+              (reinterpret_cast<uintptr_t>(a->out) & 15u) == 0,
+          "subband spectrum rows must be 16-byte aligned multiples of 8 samples");
+    // every read of the last tile stays inside the rows (the kernel trusts
+    // max_delay to bound the table)
+    check(ceil_div(a->out_nsamples, kSsTile) * kSsTile + static_cast<uint64_t>(a->max_delay) +
+                  kSbReadPad <=
+              row_len,
+          "channel rows are not padded far enough for the delays");
+    const uint64_t tiles = ceil_div(a->out_nsamples, kSsTile);
+    check_overflow(tiles < 0x7FFFFFFFull, "subband spectrum too long");
+    hipLaunchKernelGGL(subband_spectrum_kernel,
+                       dim3(static_cast<uint32_t>(tiles), static_cast<uint32_t>(nsub)), dim3(64),
+                       0, as_stream(stream), a->channels, a->channel_stride,
+                       static_cast<uint32_t>(a->nsel), static_cast<uint32_t>(a->subband_channels),
+                       a->delays, a->out, a->out_stride, a->out_nsamples);
+    launch_check("subband_spectrum_kernel");
+  });
+}
+
 // ------------------------------------------------------------ spectrum ----
 // Aligned dynamic spectrum (dedispersion.cpp:249-289): out[t][c] =
 // x[t + delay[c]][chan_begin + c] for t < out_nsamples, any element size.  One

@@ -19,11 +19,18 @@
 namespace gaffa_hip {
 namespace {
 
+// The value of output channel c0 / factor at one sample.  `divisor` != 1
+// marks an input whose elements are already sums of that many channels (exact
+// integers), which the CPU's serial FP64 sum would reproduce exactly.
 template <typename T>
-__device__ __forceinline__ double sample_value(const T* row, uint32_t c0, uint32_t factor) {
-  if (factor == 1) return static_cast<double>(row[c0]);
+__device__ __forceinline__ double sample_value(const T* row, uint64_t cstride, uint32_t c0,
+                                               uint32_t factor, double divisor) {
+  if (factor == 1) {
+    const double v = static_cast<double>(row[c0 * cstride]);
+    return divisor == 1.0 ? v : v / divisor;
+  }
   double v = 0.0;  // serial in channel order, then the mean (folding.cpp:354-360)
-  for (uint32_t o = 0; o < factor; ++o) v += static_cast<double>(row[c0 + o]);
+  for (uint32_t o = 0; o < factor; ++o) v += static_cast<double>(row[(c0 + o) * cstride]);
   return v / static_cast<double>(factor);
 }
 
@@ -58,6 +65,8 @@ fold_kernel(const T* __restrict__ data, gaffa_hip_fold_args a,
   int64_t m = (k_lo - static_cast<int64_t>(phase)) / static_cast<int64_t>(a.nbin);
   if (static_cast<int64_t>(phase) + m * static_cast<int64_t>(a.nbin) < k_lo) ++m;
   const T* base = data + job.input_offset;
+  const uint64_t cstride = a.channel_stride == 0 ? 1 : a.channel_stride;
+  const double divisor = a.divisor == 0.0 ? 1.0 : a.divisor;
 
   double sum = 0.0, expo = 0.0;
   for (int64_t k = static_cast<int64_t>(phase) + m * static_cast<int64_t>(a.nbin); k <= k_hi;
@@ -76,7 +85,8 @@ fold_kernel(const T* __restrict__ data, gaffa_hip_fold_args a,
       const double w = fmin(high, kd + 1.0) - fmax(low, kd);
       if (!(w > 0.0)) continue;
       const uint64_t t = static_cast<uint64_t>(i) - job.start_sample;
-      const double v = sample_value(base + t * a.row_stride, ch * factor, factor);
+      const double v = sample_value(base + t * a.row_stride, cstride, ch * factor, factor,
+                                    divisor);
       sum += v * w;
       expo += w;
     }
@@ -106,7 +116,13 @@ extern "C" int gaffa_hip_fold_batch(const gaffa_hip_fold_args* a, gaffa_hip_stre
     check(a->output_channels > 0 && a->output_channels <= a->nchans &&
               a->nchans % a->output_channels == 0,
           "fold input channel count must be divisible by output_channels");
-    check(a->row_stride >= a->nchans, "fold row stride must hold every channel");
+    const uint64_t cstride = a->channel_stride == 0 ? 1 : a->channel_stride;
+    check(cstride == 1 ? a->row_stride >= a->nchans
+                       : a->row_stride == 1 && cstride >= a->nsamples,
+          "fold strides must address every (sample, channel) once: sample-major rows of "
+          "nchans, or channel-major rows (row_stride 1) of at least nsamples");
+    check(a->divisor == 0.0 || (a->divisor >= 1.0 && std::isfinite(a->divisor)),
+          "fold divisor must be 0 (none) or >= 1");
     check(a->nsubint == ceil_div(a->nsamples, a->subint_samples),
           "fold nsubint must equal ceil(nsamples / subint_samples)");
     const uint64_t per_job = a->nsubint * a->output_channels * a->nbin;

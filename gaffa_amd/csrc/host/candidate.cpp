@@ -25,21 +25,47 @@ bool candidate_before(const Candidate& a, const Candidate& b) {
   return a.dm_index < b.dm_index;
 }
 
-// One cluster of peaks (indices into `peaks`) -> its candidate.
-Candidate make_candidate(std::span<const DmPeak> peaks, const std::uint32_t* begin,
-                         const std::uint32_t* end) {
-  const DmPeak* best = &peaks[*begin];
+// A peak's clustering key: frequency cluster order is set first, then DM
+// clusters need (width when widths stay apart, DM index, width, frequency)
+// order.  Peaks equal in all four keep no defined order, which changes
+// nothing: they split nowhere between them and the best of a cluster is
+// chosen by the full comparator.  The S/N rides along so the best peak of a
+// cluster needs the full records only for peaks at its highest S/N.
+struct DmKey {
+  std::size_t width_split, dm_index, width;
+  double frequency;
+  float snr;
+  std::uint32_t i;
+  bool operator<(const DmKey& o) const {
+    if (width_split != o.width_split) return width_split < o.width_split;
+    if (dm_index != o.dm_index) return dm_index < o.dm_index;
+    if (width != o.width) return width < o.width;
+    return frequency < o.frequency;
+  }
+};
+
+constexpr std::size_t kParallelClusterPeaks = std::size_t{1} << 18;
+
+// One DM cluster (keys [begin, end)) -> its candidate; every peak is finite.
+Candidate make_candidate(std::span<const DmPeak> peaks, const DmKey* begin, const DmKey* end) {
   Candidate c;
   c.dm_index_min = std::numeric_limits<std::size_t>::max();
   c.frequency_min = std::numeric_limits<double>::infinity();
   c.frequency_max = -std::numeric_limits<double>::infinity();
-  for (const std::uint32_t* it = begin; it != end; ++it) {
-    const DmPeak& p = peaks[*it];
-    if (peak_before(p, *best)) best = &p;  // strict: the first of equals stays
-    c.dm_index_min = std::min(c.dm_index_min, p.dm_index);
-    c.dm_index_max = std::max(c.dm_index_max, p.dm_index);
-    c.frequency_min = std::min(c.frequency_min, p.peak.frequency);
-    c.frequency_max = std::max(c.frequency_max, p.peak.frequency);
+  float top = -std::numeric_limits<float>::infinity();
+  for (const DmKey* k = begin; k != end; ++k) {
+    c.dm_index_min = std::min(c.dm_index_min, k->dm_index);
+    c.dm_index_max = std::max(c.dm_index_max, k->dm_index);
+    c.frequency_min = std::min(c.frequency_min, k->frequency);
+    c.frequency_max = std::max(c.frequency_max, k->frequency);
+    top = std::max(top, k->snr);
+  }
+  // is_better_dm_peak orders finite peaks by S/N first: the best has the top S/N
+  const DmPeak* best = nullptr;
+  for (const DmKey* k = begin; k != end; ++k) {
+    if (k->snr != top) continue;
+    const DmPeak& p = peaks[k->i];
+    if (best == nullptr || peak_before(p, *best)) best = &p;  // strict: the first of equals
   }
   c.dm = best->dm;
   c.dm_index = best->dm_index;
@@ -51,34 +77,6 @@ Candidate make_candidate(std::span<const DmPeak> peaks, const std::uint32_t* beg
   c.peak_count = static_cast<std::size_t>(end - begin);
   c.best_peak = *best;
   return c;
-}
-
-// The DM (and width) clusters of one frequency cluster.
-void dm_clusters(std::span<const DmPeak> peaks, std::vector<std::uint32_t>& idx,
-                 const CandidateSelectionOptions& o, std::vector<Candidate>& out) {
-  const auto key = [&](std::uint32_t i) {
-    const DmPeak& p = peaks[i];
-    // (width first only when widths stay apart), then DM index, width, frequency
-    return std::make_tuple(o.cluster_across_widths ? std::size_t{0} : p.peak.width, p.dm_index,
-                           p.peak.width, p.peak.frequency);
-  };
-  std::sort(idx.begin(), idx.end(), [&](std::uint32_t a, std::uint32_t b) { return key(a) < key(b); });
-  std::size_t start = 0;
-  for (std::size_t k = 1; k <= idx.size(); ++k) {
-    bool split = k == idx.size();
-    if (!split) {
-      const DmPeak& prev = peaks[idx[k - 1]];
-      const DmPeak& cur = peaks[idx[k]];
-      const bool same_width = o.cluster_across_widths || cur.peak.width == prev.peak.width;
-      const bool same_dm = cur.dm_index <= prev.dm_index ||
-                           cur.dm_index - prev.dm_index <= o.dm_cluster_radius;
-      split = !same_width || !same_dm;
-    }
-    if (split) {
-      out.push_back(make_candidate(peaks, idx.data() + start, idx.data() + k));
-      start = k;
-    }
-  }
 }
 
 struct Ratio {
@@ -169,36 +167,75 @@ std::vector<Candidate> select_candidates_cpu(std::span<const DmPeak> peaks,
             "Candidate selection frequency_cluster_radius must be finite and >= 0");
   if (peaks.empty()) return {};
   check_arg(peaks.size() < (std::size_t{1} << 32), "Candidate selection peak list too long");
-  for (const DmPeak& p : peaks) {
-    check_arg(std::isfinite(p.dm) && std::isfinite(p.peak.period) &&
-                  std::isfinite(p.peak.frequency) && std::isfinite(p.peak.duty_cycle) &&
-                  std::isfinite(p.peak.snr),
-              "Candidate selection peaks must contain finite scientific values");
-  }
+  const std::int64_t np = static_cast<std::int64_t>(peaks.size());
   // frequency order of (frequency, index) pairs, then the gaps that split
   struct F {
     double f;
     std::uint32_t i;
   };
   std::vector<F> by_f(peaks.size());
-  for (std::size_t i = 0; i < peaks.size(); ++i)
-    by_f[i] = F{peaks[i].peak.frequency, static_cast<std::uint32_t>(i)};
+  bool finite = true;
+#pragma omp parallel for reduction(&& : finite) schedule(static)
+  for (std::int64_t i = 0; i < np; ++i) {
+    const DmPeak& p = peaks[static_cast<std::size_t>(i)];
+    finite = finite && std::isfinite(p.dm) && std::isfinite(p.peak.period) &&
+             std::isfinite(p.peak.frequency) && std::isfinite(p.peak.duty_cycle) &&
+             std::isfinite(p.peak.snr);
+    by_f[static_cast<std::size_t>(i)] = F{p.peak.frequency, static_cast<std::uint32_t>(i)};
+  }
+  check_arg(finite, "Candidate selection peaks must contain finite scientific values");
   __gnu_parallel::sort(by_f.begin(), by_f.end(), [](const F& a, const F& b) { return a.f < b.f; });
   const double radius_hz = o.frequency_cluster_radius / observation_seconds;
   std::vector<std::size_t> starts{0};
   for (std::size_t k = 1; k < by_f.size(); ++k)
     if (by_f[k].f - by_f[k - 1].f > radius_hz) starts.push_back(k);
   starts.push_back(by_f.size());
-  const std::int64_t nclusters = static_cast<std::int64_t>(starts.size()) - 1;
-  std::vector<std::vector<Candidate>> parts(static_cast<std::size_t>(nclusters));
-#pragma omp parallel for schedule(dynamic, 16)
-  for (std::int64_t c = 0; c < nclusters; ++c) {
-    std::vector<std::uint32_t> idx;
-    for (std::size_t k = starts[c]; k < starts[c + 1]; ++k) idx.push_back(by_f[k].i);
-    dm_clusters(peaks, idx, o, parts[static_cast<std::size_t>(c)]);
+  // every peak's DM-cluster key, in frequency-cluster order
+  std::vector<DmKey> keys(peaks.size());
+#pragma omp parallel for schedule(static)
+  for (std::int64_t k = 0; k < np; ++k) {
+    const std::uint32_t i = by_f[static_cast<std::size_t>(k)].i;
+    const DmPeak& p = peaks[i];
+    keys[static_cast<std::size_t>(k)] =
+        DmKey{o.cluster_across_widths ? std::size_t{0} : p.peak.width, p.dm_index, p.peak.width,
+              p.peak.frequency, p.peak.snr, i};
   }
-  std::vector<Candidate> out;
-  for (auto& p : parts) out.insert(out.end(), p.begin(), p.end());
+  by_f = std::vector<F>();
+  const std::int64_t nclusters = static_cast<std::int64_t>(starts.size()) - 1;
+  // sort each frequency cluster: large ones one at a time with the parallel
+  // sort, the rest spread over the threads
+  for (std::int64_t c = 0; c < nclusters; ++c)
+    if (starts[c + 1] - starts[c] >= kParallelClusterPeaks)
+      __gnu_parallel::sort(keys.begin() + static_cast<std::ptrdiff_t>(starts[c]),
+                           keys.begin() + static_cast<std::ptrdiff_t>(starts[c + 1]));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (std::int64_t c = 0; c < nclusters; ++c)
+    if (starts[c + 1] - starts[c] < kParallelClusterPeaks)
+      std::sort(keys.begin() + static_cast<std::ptrdiff_t>(starts[c]),
+                keys.begin() + static_cast<std::ptrdiff_t>(starts[c + 1]));
+  // DM-cluster boundaries: a frequency-cluster start, or a width change (when
+  // widths stay apart) or a DM gap beyond the radius after the previous peak
+  std::vector<std::uint8_t> split(peaks.size(), 0);
+  for (std::int64_t c = 0; c < nclusters; ++c) split[starts[c]] = 1;
+#pragma omp parallel for schedule(static)
+  for (std::int64_t k = 1; k < np; ++k) {
+    const DmKey& prev = keys[static_cast<std::size_t>(k - 1)];
+    const DmKey& cur = keys[static_cast<std::size_t>(k)];
+    const bool same_width = o.cluster_across_widths || cur.width == prev.width;
+    const bool same_dm = cur.dm_index <= prev.dm_index ||
+                         cur.dm_index - prev.dm_index <= o.dm_cluster_radius;
+    if (!same_width || !same_dm) split[static_cast<std::size_t>(k)] = 1;
+  }
+  std::vector<std::size_t> seg;
+  for (std::size_t k = 0; k < split.size(); ++k)
+    if (split[k]) seg.push_back(k);
+  seg.push_back(split.size());
+  const std::int64_t nseg = static_cast<std::int64_t>(seg.size()) - 1;
+  std::vector<Candidate> out(static_cast<std::size_t>(nseg));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (std::int64_t j = 0; j < nseg; ++j)
+    out[static_cast<std::size_t>(j)] =
+        make_candidate(peaks, keys.data() + seg[j], keys.data() + seg[j + 1]);
   __gnu_parallel::sort(out.begin(), out.end(), candidate_before);
   if (o.max_candidates != 0 && out.size() > o.max_candidates) out.resize(o.max_candidates);
   return out;

@@ -65,6 +65,8 @@ std::size_t launch_fold(const GpuFoldInput& in, std::span<const FoldJob> jobs, d
   a.njobs = static_cast<std::uint32_t>(jobs.size());
   a.out = out;
   a.exposure = exposure;
+  a.channel_stride = in.channel_stride;
+  a.divisor = in.divisor;
   hip_ok(gaffa_hip_fold_batch(&a, stream), "fold_batch");
   // the job table must outlive the kernel
   hip_ok(gaffa_hip_stream_synchronize(stream), "hipStreamSynchronize");

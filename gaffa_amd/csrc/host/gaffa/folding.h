@@ -75,6 +75,8 @@ struct GpuFoldInput {
   std::size_t nchans = 1;
   std::size_t row_stride = 0;  // 0 = nchans
   int device_id = 0;
+  std::size_t channel_stride = 0;  // elements between channels, 0 = 1 (sample-major rows)
+  double divisor = 0.0;            // elements are sums of this many channels, 0 = 1
 };
 
 // One fold of a batch: input rows start at data + input_offset elements.

@@ -55,7 +55,7 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   ffa_plan_ = make_riptide_ffa_plan(n_out_, cfg_.tsamp, cfg_.ffa);
   if (!cfg_.preprocess.steps.empty()) {
     GpuPreprocessExecutionOptions pe;
-    pe.series_tile_size = std::min<std::size_t>(ndm_, 1024);
+    pe.series_tile_size = std::min<std::size_t>(ndm_, 65535);  // one launch: the statistics run a lane per series
     pe.max_nsamples = n_out_;
     pe.stream = stream_.get();
     prep_ = std::make_unique<GpuPreprocessProgram>(cfg_.preprocess,
@@ -435,10 +435,56 @@ double GpuSearchPipeline::fold_candidates_full(std::span<const Candidate> candid
             "fold output_channels must divide the channel count");
   GpuDeviceScope scope(cfg_.device_id);
   const gaffa_hip_stream st = stream_.get();
-  GpuDeviceBuffer<std::uint8_t> spectrum(cfg_.nsamples * cfg_.nchans);
   GpuDeviceBuffer<float> cube(nsubint * nout * nbin);
   if (cubes != nullptr) cubes->assign(candidates.size() * cube.size(), 0.0f);
   const double fmax = *std::max_element(cfg_.frequency_mhz.begin(), cfg_.frequency_mhz.end());
+  const std::size_t factor = cfg_.nchans / nout;
+  if (channels_.size() != 0 && factor <= 256) {
+    // subband workloads keep a channel-major copy: fold each candidate from
+    // its per-output-channel sums at the candidate's DM (the same cube)
+    const std::size_t sum_stride = (cfg_.nsamples + 7) / 8 * 8;
+    GpuDeviceBuffer<std::uint16_t> sums(nout * sum_stride);
+    GpuDeviceBuffer<std::int32_t> d_delays(cfg_.nchans);
+    hip_ok(gaffa_hip_event_record(events_[5], st), "hipEventRecord");
+    for (std::size_t i = 0; i < candidates.size(); ++i) {
+      const Candidate& c = candidates[i];
+      const MultiDmDedispersionPlan plan{c.dm, 1.0, 1, fmax, cfg_.tsamp, 0, cfg_.nchans};
+      const std::size_t n = multi_dm_output_nsamples(cfg_.nsamples, cfg_.frequency_mhz, plan);
+      check_arg(n > 0, "candidate DM leaves no aligned samples");
+      const std::vector<std::int32_t> delays = make_multi_dm_delay_table(cfg_.frequency_mhz, plan);
+      const std::int32_t dmax = *std::max_element(delays.begin(), delays.end());
+      check_arg(*std::min_element(delays.begin(), delays.end()) >= 0,
+                "dedispersion requires non-negative delays");
+      // the host table stays alive until the copy has run (synchronous copy)
+      hip_ok(gaffa_hip_memcpy(d_delays.data(), delays.data(), d_delays.bytes(), nullptr),
+             "hipMemcpy");
+      gaffa_hip_subband_spectrum_args a{};
+      a.channels = channels_.data();
+      a.channel_stride = channel_stride_;
+      a.nsel = cfg_.nchans;
+      a.subband_channels = factor;
+      a.delays = d_delays.data();
+      a.max_delay = dmax;
+      a.out = sums.data();
+      a.out_stride = sum_stride;
+      a.out_nsamples = n;
+      hip_ok(gaffa_hip_subband_spectrum_u8(&a, st), "subband_spectrum");
+      const FoldJob job{0, c.period, 0};
+      GpuFoldInput in{sums.data(), FoldDtype::U16, n, nout, 1, cfg_.device_id};
+      in.channel_stride = sum_stride;
+      in.divisor = static_cast<double>(factor);
+      fold_batch_gpu_device(in, std::span<const FoldJob>(&job, 1), cfg_.tsamp, nbin,
+                            (n + nsubint - 1) / nsubint, nout, cube.data(), st);
+      if (cubes != nullptr)
+        copy_to_host(cubes->data() + i * cube.size(), cube.data(), cube.bytes(), st);
+    }
+    hip_ok(gaffa_hip_event_record(events_[6], st), "hipEventRecord");
+    stream_.synchronize();
+    float ms = 0.0f;
+    hip_ok(gaffa_hip_event_elapsed_ms(events_[5], events_[6], &ms), "elapsed");
+    return ms;
+  }
+  GpuDeviceBuffer<std::uint8_t> spectrum(cfg_.nsamples * cfg_.nchans);
   GpuDedispersionOptions o;
   o.device_id = cfg_.device_id;
   o.stream = st;

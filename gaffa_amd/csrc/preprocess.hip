@@ -187,11 +187,22 @@ detrend_kernel(T* __restrict__ data, uint64_t stride, uint64_t n,
 
 // One lane per series: sum and sum of squares in FP64 in sample order, as
 // time_series_stats_cpu does (time_series.cpp:55-60), so every rounding is
-// the CPU's; then mean, 1/stddev and a status flag.  The series is read with
-// 16-byte loads, a batch of kStatsBatch loads in flight while the previous
-// batch is summed.
+// the CPU's; then mean, 1/stddev and a status flag.
+//
+// A one-wave workgroup owns 64 series.  The wave loads chunks of
+// 64 series x 128 samples with coalesced 16-byte loads (two 512-byte row
+// segments per instruction) into registers, stores them to LDS, and while
+// each lane sums its own row of the chunk in order, the next chunk's loads
+// are already in flight.  Rows must be 16-byte aligned (the pipeline's plane
+// is); other layouts take the per-lane path, which reads each row directly.
+// a native 4-float vector: HIP's float4 struct is copied through memory,
+// which would keep a register array of them in scratch
+typedef float vf4 __attribute__((ext_vector_type(4)));
 constexpr int kStatsLanes = 64;
-constexpr int kStatsBatch = 8;
+constexpr int kStatsChunk = 128;               // samples per series per chunk
+constexpr int kStatsPitch = kStatsChunk + 4;   // LDS floats per row (bank spread)
+constexpr int kStatsLoads = kStatsLanes * kStatsChunk / 4 / 64;  // float4 per lane
+constexpr uint32_t kStatsLdsBytes = 2u * kStatsLanes * kStatsPitch * 4u;
 
 struct SerialSums {
   double sum = 0.0, squares = 0.0;
@@ -208,37 +219,9 @@ struct SerialSums {
   }
 };
 
-__global__ void __launch_bounds__(kStatsLanes)
-stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32_t nseries,
-             int32_t reject_constant, double2* __restrict__ stats,
-             int32_t* __restrict__ flags, int32_t* __restrict__ status) {
-  const uint32_t series = blockIdx.x * kStatsLanes + threadIdx.x;
-  if (series >= nseries) return;
-  const float* x = data + static_cast<uint64_t>(series) * stride;
-  SerialSums acc;
-  uint64_t head = 0;
-  while (head < n && (reinterpret_cast<uintptr_t>(x + head) & 15u) != 0) acc.add(x[head++]);
-  const uint64_t nv = (n - head) / 4;
-  const float4* v = reinterpret_cast<const float4*>(x + head);
-  uint64_t k = 0;
-  if (nv >= kStatsBatch) {
-    float4 cur[kStatsBatch], nxt[kStatsBatch];
-#pragma unroll
-    for (int u = 0; u < kStatsBatch; ++u) cur[u] = v[u];
-    for (k = kStatsBatch; k + kStatsBatch <= nv; k += kStatsBatch) {
-#pragma unroll
-      for (int u = 0; u < kStatsBatch; ++u) nxt[u] = v[k + u];
-#pragma unroll
-      for (int u = 0; u < kStatsBatch; ++u) acc.add4(cur[u]);
-#pragma unroll
-      for (int u = 0; u < kStatsBatch; ++u) cur[u] = nxt[u];
-    }
-#pragma unroll
-    for (int u = 0; u < kStatsBatch; ++u) acc.add4(cur[u]);
-  }
-  for (; k < nv; ++k) acc.add4(v[k]);
-  for (uint64_t i = head + 4 * nv; i < n; ++i) acc.add(x[i]);
-
+__device__ __forceinline__ void finish_stats(const SerialSums& acc, uint64_t n, uint32_t series,
+                                             int32_t reject_constant, double2* stats,
+                                             int32_t* flags, int32_t* status) {
   const double count = static_cast<double>(n);
   const double mean = acc.sum / count;
   double var = acc.squares / count - mean * mean;
@@ -250,6 +233,91 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
   stats[series] = make_double2(mean, flag == 0 ? 1.0 / sd : 0.0);
   flags[series] = flag;
   if (flag == 2 || flag == 3) atomicCAS(&status[series], 0, flag);
+}
+
+// One or two waves per SIMD: the chunk's 32 float4 per lane then fit the
+// register budget (with four, the array would be placed in scratch memory).
+__global__ void __launch_bounds__(kStatsLanes) __attribute__((amdgpu_waves_per_eu(1, 2)))
+stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32_t nseries,
+             int32_t reject_constant, double2* __restrict__ stats,
+             int32_t* __restrict__ flags, int32_t* __restrict__ status) {
+  extern __shared__ __align__(16) float stage[];  // [2][kStatsLanes][kStatsPitch]
+  const uint32_t lane = threadIdx.x;
+  const uint32_t first = blockIdx.x * kStatsLanes;
+  const uint32_t rows = min(static_cast<uint32_t>(kStatsLanes), nseries - first);
+  const uint64_t nchunks = n / kStatsChunk;
+  const float* base = data + static_cast<uint64_t>(first) * stride;
+  // load k of a chunk: element e = k * 64 + lane is row e / 32, float4 e % 32
+  // (rows past the batch re-read the last row); every index is a constant
+  // after unrolling, so the chunk stays in registers
+  vf4 reg[kStatsLoads];
+  SerialSums acc;
+  if (nchunks > 0) {
+#pragma unroll
+    for (int k = 0; k < kStatsLoads; ++k) {
+      const uint32_t e = static_cast<uint32_t>(k) * 64u + lane;
+      const uint32_t r = min(e >> 5, rows - 1);
+      reg[k] = *reinterpret_cast<const vf4*>(base + static_cast<uint64_t>(r) * stride +
+                                                4u * (e & 31u));
+    }
+  }
+  for (uint64_t c = 0; c < nchunks; ++c) {
+    const uint32_t buf = static_cast<uint32_t>(c & 1u);
+#pragma unroll
+    for (int k = 0; k < kStatsLoads; ++k) {
+      const uint32_t e = static_cast<uint32_t>(k) * 64u + lane;
+      *reinterpret_cast<vf4*>(stage + (buf * kStatsLanes + (e >> 5)) * kStatsPitch +
+                                 4u * (e & 31u)) = reg[k];
+    }
+    __syncthreads();
+    if (c + 1 < nchunks) {  // in flight while this chunk is summed
+      const uint64_t col = (c + 1) * kStatsChunk;
+#pragma unroll
+      for (int k = 0; k < kStatsLoads; ++k) {
+        const uint32_t e = static_cast<uint32_t>(k) * 64u + lane;
+        const uint32_t r = min(e >> 5, rows - 1);
+        reg[k] = *reinterpret_cast<const vf4*>(base + static_cast<uint64_t>(r) * stride + col +
+                                                  4u * (e & 31u));
+      }
+    }
+    const vf4* row = reinterpret_cast<const vf4*>(stage + (buf * kStatsLanes + lane) *
+                                                  kStatsPitch);
+    // eight samples at a time: the conversions and squares first (independent
+    // of each other), then the two in-order add chains, which are all that
+    // remains on the critical path
+#pragma unroll 2
+    for (int j = 0; j < kStatsChunk / 4; j += 2) {
+      const vf4 v0 = row[j], v1 = row[j + 1];
+      const float f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+      double d[8], q[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) d[u] = static_cast<double>(f[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u) q[u] = d[u] * d[u];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        acc.sum = acc.sum + d[u];
+        acc.squares = acc.squares + q[u];
+      }
+    }
+  }
+  if (lane >= rows) return;
+  const float* x = base + static_cast<uint64_t>(lane) * stride;
+  for (uint64_t i = nchunks * kStatsChunk; i < n; ++i) acc.add(x[i]);  // < 128 samples
+  finish_stats(acc, n, first + lane, reject_constant, stats, flags, status);
+}
+
+// Rows that are not 16-byte aligned: each lane reads its row directly.
+__global__ void __launch_bounds__(kStatsLanes)
+stats_rowwise_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n,
+                     uint32_t nseries, int32_t reject_constant, double2* __restrict__ stats,
+                     int32_t* __restrict__ flags, int32_t* __restrict__ status) {
+  const uint32_t series = blockIdx.x * kStatsLanes + threadIdx.x;
+  if (series >= nseries) return;
+  const float* x = data + static_cast<uint64_t>(series) * stride;
+  SerialSums acc;
+  for (uint64_t i = 0; i < n; ++i) acc.add(x[i]);
+  finish_stats(acc, n, series, reject_constant, stats, flags, status);
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -348,12 +416,24 @@ void run(const gaffa_hip_preprocess_args* a, void* workspace, hipStream_t st) {
                      a->detrend, a->detrend ? f : 1, med, l.m, a->status);
   launch_check("detrend_kernel");
   if (a->normalise) {
-    hipLaunchKernelGGL(stats_kernel,
-                       dim3(static_cast<uint32_t>(ceil_div(a->nseries, kStatsLanes))),
-                       dim3(kStatsLanes), 0, st, reinterpret_cast<const float*>(data), a->stride,
-                       a->nsamples, static_cast<uint32_t>(a->nseries), a->reject_constant, stats,
-                       flags, a->status);
-    launch_check("stats_kernel");
+    const dim3 sg(static_cast<uint32_t>(ceil_div(a->nseries, kStatsLanes)));
+    const float* fdata = reinterpret_cast<const float*>(data);
+    if ((reinterpret_cast<uintptr_t>(fdata) & 15u) == 0 && a->stride % 4 == 0) {
+      // two chunk buffers exceed the default 64 KiB of dynamic LDS
+      hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(stats_kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(kStatsLdsBytes)),
+                "hipFuncSetAttribute");
+      hipLaunchKernelGGL(stats_kernel, sg, dim3(kStatsLanes), kStatsLdsBytes, st, fdata,
+                         a->stride, a->nsamples, static_cast<uint32_t>(a->nseries),
+                         a->reject_constant, stats, flags, a->status);
+      launch_check("stats_kernel");
+    } else {
+      hipLaunchKernelGGL(stats_rowwise_kernel, sg, dim3(kStatsLanes), 0, st, fdata, a->stride,
+                         a->nsamples, static_cast<uint32_t>(a->nseries), a->reject_constant,
+                         stats, flags, a->status);
+      launch_check("stats_rowwise_kernel");
+    }
     hipLaunchKernelGGL(normalise_kernel, g, dim3(kThreads), 0, st,
                        reinterpret_cast<float*>(data), a->stride, a->nsamples, stats, flags);
     launch_check("normalise_kernel");

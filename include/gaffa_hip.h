@@ -534,9 +534,38 @@ typedef struct gaffa_hip_fold_args {
   uint32_t njobs;
   float* out;
   double* exposure;
+  uint64_t channel_stride; /* elements between channels, 0 = 1 */
+  double divisor;          /* the input holds sums of this many channels, 0 = 1 */
 } gaffa_hip_fold_args;
 
 int gaffa_hip_fold_batch(const gaffa_hip_fold_args* args, gaffa_hip_stream stream);
+
+/* Subband sums of the aligned dynamic spectrum at one DM, the input of a
+ * full fold onto nsel / subband_channels output channels:
+ *   out[k * out_stride + t] = sum_{c in subband k} channels[c][t + delays[c]]
+ * for t < out_nsamples, from the channel-major copy
+ * (gaffa_hip_filterbank_transpose_u8).  Replaces dedisperse_spectrum_cuda
+ * (dedispersion_cuda.cu:295-318) plus the per-output-channel averaging of
+ * fold_spectrum_cpu (folding.cpp:354-360): folding these sums with
+ * divisor = subband_channels (gaffa_hip_fold_args) gives gaffa's cube bit for
+ * bit, since the uint8 channel sums are exact.  delays: device [nsel], each in
+ * [0, max_delay]; rows readable for ceil(out_nsamples / 512) * 512 +
+ * max_delay + 528 bytes; out 16-byte aligned, out_stride a multiple of 8. */
+typedef struct gaffa_hip_subband_spectrum_args {
+  const uint8_t* channels; /* [nsel][channel_stride] */
+  uint64_t channel_stride;
+  uint64_t channel_length; /* readable bytes per row, 0 = channel_stride */
+  uint64_t nsel;
+  uint64_t subband_channels;
+  const int32_t* delays;
+  int32_t max_delay;
+  uint16_t* out;           /* [nsel / subband_channels][out_stride] */
+  uint64_t out_stride;
+  uint64_t out_nsamples;
+} gaffa_hip_subband_spectrum_args;
+
+int gaffa_hip_subband_spectrum_u8(const gaffa_hip_subband_spectrum_args* args,
+                                  gaffa_hip_stream stream);
 
 /* --------------------------------------------------- synthetic inputs ----
  * Seeded, integer-exact synthetic filterbank (no gaffa counterpart): sample

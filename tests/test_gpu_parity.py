@@ -653,6 +653,32 @@ def test_fold_profile_and_candidates(gpu, ref):
     assert prof.max() - np.median(prof) > 5 * prof.std() / 3
 
 
+@pytest.mark.parametrize("name", ["lband", "small"])
+def test_full_fold_of_candidates_vs_reference(gpu, ref, name):
+    """gaffa's full fold of search candidates (aligned spectrum at the
+    candidate's DM, 32 subints x 64 output channels x 64 bins): subband
+    workloads fold from per-output-channel sums of the channel-major copy, the
+    brute-force one from the aligned spectrum; both give gaffa's pfold cube
+    (dedisperse_spectrum + fold_spectrum of the reference build) bit for bit."""
+    from gaffa_amd.pipeline import lband_workload, small_workload
+
+    say(f"fold: full folds of {name} candidates vs gaffa's pfold")
+    w = lband_workload(nsamples=1 << 18, ndm=256) if name == "lband" else \
+        small_workload(nsamples=1 << 16)
+    pipe = w.pipeline()
+    pipe.generate(w.synth_spec())
+    pipe.run()
+    cands = pipe.select_candidates(dm_cluster_radius=50)[:3]
+    assert len(cands) == 3
+    _, cubes = pipe.fold_candidates_full(cands, keep_cubes=True)
+    x = pipe.download_filterbank(w.nsamples, w.nchans)
+    for c, got in zip(cands, cubes):
+        spec = ref.dedisperse_spectrum(x, w.frequencies, c.dm, w.tsamp)
+        sub = -(-spec.shape[0] // 32)
+        want, _ = ref.fold_spectrum(spec, c.period, w.tsamp, 64, sub, output_channels=64)
+        np.testing.assert_array_equal(got, want)
+
+
 # ------------------------------------------------------------ spectrum ----
 @pytest.mark.parametrize("dtype", [np.uint8, np.uint16, np.float32])
 def test_spectrum_bit_exact(gpu, port, dtype):
