@@ -9,8 +9,8 @@ subband, 32 trials per nominal group) over 4096 DM trials 0-1000, in-place
 uint32->float conversion + 2 s running-median detrend + normalise, the FFA
 search (0.1-10 s, boxcar widths 1-32) of every DM trial with its dense
 periodogram, gaffa's candidate stage (clustering, harmonics removed), the
-learned scorer on every candidate and a full fold (gaffa's pfold of the
-filterbank at the candidate's DM) of those it keeps.  The plain pipeline is
+learned scorer on the candidates with S/N >= 8 and a full fold (gaffa's pfold
+of the filterbank at the candidate's DM) of those it keeps.  The plain pipeline is
 timed after it on the same data: the same search without the periodogram,
 the same candidate stage and a full fold of every candidate with S/N >= 8.
 Detections of both are the injected pulsars matched by a folded candidate
@@ -208,9 +208,10 @@ def pipeline_step(pipe, workload, net=None, threshold: float = 0.0,
     candidate stage (clustering, harmonics removed), then
 
     plain   (net None): full fold of every candidate with S/N >= snr_cut;
-    scored: the learned scorer on every candidate (features from the resident
-            plane and periodogram, network on the GPU), full fold of the
-            candidates it keeps.
+    scored: the learned scorer on the candidates with S/N >= snr_cut
+            (features from the resident plane and periodogram, network on the
+            GPU), full fold of those it keeps: the network prunes the plain
+            pipeline's fold list.
 
     Returns (times in ms, candidates, folded candidates)."""
     from gaffa_amd.pipeline import search_candidates
@@ -220,11 +221,12 @@ def pipeline_step(pipe, workload, net=None, threshold: float = 0.0,
     t1 = time.perf_counter()
     cands = search_candidates(pipe, workload)
     t2 = time.perf_counter()
+    strong = [c for c in cands if c.snr >= snr_cut]
     if net is None:
-        fold = [c for c in cands if c.snr >= snr_cut]
+        fold = strong
     else:
-        z = pipe.score_candidates(net, cands) if cands else []
-        fold = [c for c, v in zip(cands, z) if v >= threshold]
+        z = pipe.score_candidates(net, strong) if strong else []
+        fold = [c for c, v in zip(strong, z) if v >= threshold]
     t3 = time.perf_counter()
     fold_dev_ms = pipe.fold_candidates_full(fold)[0] if fold else 0.0
     t4 = time.perf_counter()
@@ -429,8 +431,8 @@ def main() -> int:
             "detections": len(plain_found),
         }
     scored = {
-        "pipeline": ("search (with periodogram) + gaffa's candidate stage + learned scorer + "
-                     "full fold of the candidates it keeps"),
+        "pipeline": ("search (with periodogram) + gaffa's candidate stage + learned scorer on "
+                     "the candidates with S/N >= 8 + full fold of those it keeps"),
         "threshold": threshold, "candidates": len(cands), "folded": len(kept),
         "detections": len(scored_found),
     }
