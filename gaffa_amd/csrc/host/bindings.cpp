@@ -920,6 +920,29 @@ PYBIND11_MODULE(_core, m) {
       .def_readonly("candidate", &HarmonicCandidate::candidate)
       .def_readonly("harmonic", &HarmonicCandidate::harmonic);
   m.def(
+      "_select_candidates_ranked_records",
+      [](const py::array& peaks, const py::array_t<std::uint32_t, py::array::c_style>& ranks,
+         double obs, double radius, std::size_t dm_radius, bool across,
+         std::size_t max_candidates) {
+        if (peaks.itemsize() != static_cast<py::ssize_t>(sizeof(DmPeak)))
+          throw std::invalid_argument("peak records must be gaffa DmPeak records");
+        const py::array c = py::array::ensure(peaks, py::array::c_style);
+        std::vector<DmPeak> v(static_cast<std::size_t>(c.size()));
+        if (!v.empty()) std::memcpy(v.data(), c.data(), v.size() * sizeof(DmPeak));
+        std::vector<std::uint32_t> r(ranks.data(), ranks.data() + ranks.size());
+        std::vector<Candidate> out;
+        {
+          py::gil_scoped_release release;
+          out = select_candidates_ranked_cpu(
+              v, r, obs, CandidateSelectionOptions{radius, dm_radius, across, max_candidates});
+        }
+        return py::bytes(reinterpret_cast<const char*>(out.data()),
+                         out.size() * sizeof(Candidate));
+      },
+      py::arg("peaks"), py::arg("period_rank"), py::arg("observation_seconds"),
+      py::arg("frequency_cluster_radius"), py::arg("dm_cluster_radius"),
+      py::arg("cluster_across_widths"), py::arg("max_candidates"));
+  m.def(
       "_select_candidates_records",
       [](const py::array& peaks, double obs, double radius, std::size_t dm_radius, bool across,
          std::size_t max_candidates) {
@@ -1306,10 +1329,8 @@ PYBIND11_MODULE(_core, m) {
           [](const GpuSearchPipeline& p, double radius, std::size_t dm_radius, bool across,
              std::size_t max_candidates) {
             py::gil_scoped_release release;
-            const double obs = static_cast<double>(p.out_nsamples()) * p.tsamp();
-            return select_candidates_cpu(
-                p.peaks(), obs, CandidateSelectionOptions{radius, dm_radius, across,
-                                                          max_candidates});
+            return p.select_candidates(
+                CandidateSelectionOptions{radius, dm_radius, across, max_candidates});
           },
           py::kw_only(), py::arg("frequency_cluster_radius") = 0.1,
           py::arg("dm_cluster_radius") = 1, py::arg("cluster_across_widths") = true,

@@ -3,8 +3,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <numeric>
 #include <optional>
+#include <omp.h>
 #include <parallel/algorithm>
 #include <stdexcept>
 
@@ -236,6 +238,228 @@ std::vector<Candidate> select_candidates_cpu(std::span<const DmPeak> peaks,
   for (std::int64_t j = 0; j < nseg; ++j)
     out[static_cast<std::size_t>(j)] =
         make_candidate(peaks, keys.data() + seg[j], keys.data() + seg[j + 1]);
+  __gnu_parallel::sort(out.begin(), out.end(), candidate_before);
+  if (o.max_candidates != 0 && out.size() > o.max_candidates) out.resize(o.max_candidates);
+  return out;
+}
+
+namespace {
+
+// A peak in the ranked selection: its index, its frequency position (period
+// ranks counted down, so ascending frequency), its (DM, width) bucket, its
+// frequency cluster and its S/N.
+struct RankedPeak {
+  std::uint32_t i, rdesc, dmw, cluster;
+  float snr;
+};
+
+// Stable parallel counting sort of `in` into `out` by bucket(e) < nb: each
+// thread counts a contiguous chunk, the offsets run bucket-major then
+// thread-major, and each thread scatters its chunk in order.
+template <class Bucket>
+void counting_sort(const std::vector<RankedPeak>& in, std::vector<RankedPeak>& out,
+                   std::size_t nb, Bucket bucket) {
+  const std::size_t n = in.size();
+  out.resize(n);
+  const int nt = static_cast<int>(
+      std::max<std::size_t>(1, std::min<std::size_t>(omp_get_max_threads(), n / 65536 + 1)));
+  std::vector<std::size_t> hist(static_cast<std::size_t>(nt) * nb, 0);
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+  for (int t = 0; t < nt; ++t) {
+    std::size_t* h = hist.data() + static_cast<std::size_t>(t) * nb;
+    for (std::size_t k = n * t / nt; k < n * (t + 1) / nt; ++k) ++h[bucket(in[k])];
+  }
+  // bucket totals and starts, then each thread's offset inside each bucket
+  std::vector<std::size_t> start(nb + 1, 0);
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (std::int64_t b = 0; b < static_cast<std::int64_t>(nb); ++b) {
+    std::size_t total = 0;
+    for (int t = 0; t < nt; ++t) total += hist[static_cast<std::size_t>(t) * nb + b];
+    start[b + 1] = total;
+  }
+  for (std::size_t b = 0; b < nb; ++b) start[b + 1] += start[b];
+#pragma omp parallel for num_threads(nt) schedule(static)
+  for (std::int64_t b = 0; b < static_cast<std::int64_t>(nb); ++b) {
+    std::size_t run = start[b];
+    for (int t = 0; t < nt; ++t) {
+      std::size_t& h = hist[static_cast<std::size_t>(t) * nb + b];
+      const std::size_t c = h;
+      h = run;
+      run += c;
+    }
+  }
+#pragma omp parallel for num_threads(nt) schedule(static, 1)
+  for (int t = 0; t < nt; ++t) {
+    std::size_t* h = hist.data() + static_cast<std::size_t>(t) * nb;
+    for (std::size_t k = n * t / nt; k < n * (t + 1) / nt; ++k) out[h[bucket(in[k])]++] = in[k];
+  }
+}
+
+}  // namespace
+
+std::vector<Candidate> select_candidates_ranked_cpu(std::span<const DmPeak> peaks,
+                                                    std::span<const std::uint32_t> period_rank,
+                                                    double observation_seconds,
+                                                    const CandidateSelectionOptions& o) {
+  check_arg(observation_seconds > 0.0 && std::isfinite(observation_seconds),
+            "Candidate selection observation_seconds must be finite and > 0");
+  check_arg(o.frequency_cluster_radius >= 0.0 && std::isfinite(o.frequency_cluster_radius),
+            "Candidate selection frequency_cluster_radius must be finite and >= 0");
+  if (peaks.empty()) return {};
+  check_arg(period_rank.size() == peaks.size(), "one period rank per peak is required");
+  check_arg(peaks.size() < (std::size_t{1} << 32), "Candidate selection peak list too long");
+  const std::int64_t np = static_cast<std::int64_t>(peaks.size());
+  bool finite = true;
+  std::uint32_t max_rank = 0;
+  std::size_t max_dm = 0;
+#pragma omp parallel for reduction(&& : finite) reduction(max : max_rank, max_dm) schedule(static)
+  for (std::int64_t i = 0; i < np; ++i) {
+    const DmPeak& p = peaks[static_cast<std::size_t>(i)];
+    finite = finite && std::isfinite(p.dm) && std::isfinite(p.peak.period) &&
+             std::isfinite(p.peak.frequency) && std::isfinite(p.peak.duty_cycle) &&
+             std::isfinite(p.peak.snr);
+    max_rank = std::max(max_rank, period_rank[static_cast<std::size_t>(i)]);
+    max_dm = std::max(max_dm, p.dm_index);
+  }
+  check_arg(finite, "Candidate selection peaks must contain finite scientific values");
+  // the distinct widths (a search has a handful), in ascending order
+  std::vector<std::size_t> widths;
+#pragma omp parallel
+  {
+    std::vector<std::size_t> local;
+#pragma omp for schedule(static) nowait
+    for (std::int64_t i = 0; i < np; ++i) {
+      const std::size_t w = peaks[static_cast<std::size_t>(i)].peak.width;
+      if (local.size() <= 64 && std::find(local.begin(), local.end(), w) == local.end())
+        local.push_back(w);
+    }
+#pragma omp critical(gaffa_ranked_widths)
+    for (std::size_t w : local)
+      if (std::find(widths.begin(), widths.end(), w) == widths.end()) widths.push_back(w);
+  }
+  const std::size_t nr = std::size_t{max_rank} + 1, ndm = max_dm + 1, nw = widths.size();
+  if (nw > 64 || nr > (std::size_t{1} << 26) || ndm * nw > (std::size_t{1} << 26))
+    return select_candidates_cpu(peaks, observation_seconds, o);
+  std::sort(widths.begin(), widths.end());
+  const auto widx = [&](std::size_t w) {
+    return static_cast<std::uint32_t>(std::lower_bound(widths.begin(), widths.end(), w) -
+                                      widths.begin());
+  };
+  // each element, and each rank's frequency: the first peak of a rank sets
+  // it (one compare-and-swap on its bits), every other peak of the rank must
+  // carry the same value
+  constexpr std::uint64_t kUnset = ~std::uint64_t{0};  // a NaN: never a finite frequency
+  std::vector<std::uint64_t> fbits(nr, kUnset);
+  std::vector<RankedPeak> a(peaks.size()), b;
+  bool consistent = true;
+#pragma omp parallel for reduction(&& : consistent) schedule(static)
+  for (std::int64_t i = 0; i < np; ++i) {
+    const DmPeak& p = peaks[static_cast<std::size_t>(i)];
+    const std::uint32_t rk = period_rank[static_cast<std::size_t>(i)];
+    std::uint64_t bits = 0;
+    std::memcpy(&bits, &p.peak.frequency, sizeof(bits));
+    std::uint64_t expected = kUnset;
+    if (!__atomic_compare_exchange_n(&fbits[rk], &expected, bits, false, __ATOMIC_RELAXED,
+                                     __ATOMIC_RELAXED))
+      consistent = consistent && expected == bits;
+    const std::uint32_t w = widx(p.peak.width);
+    const std::size_t dmw = o.cluster_across_widths ? p.dm_index * nw + w : w * ndm + p.dm_index;
+    a[static_cast<std::size_t>(i)] = RankedPeak{static_cast<std::uint32_t>(i),
+                                                static_cast<std::uint32_t>(nr - 1 - rk),
+                                                static_cast<std::uint32_t>(dmw), 0, p.peak.snr};
+  }
+  // the frequency clusters over the ranks present, ascending frequency; the
+  // ranks must order the frequencies (1 / period is monotone), else the
+  // general path
+  std::vector<std::uint32_t> cluster_of(nr, 0);
+  std::vector<double> freq_of(nr, 0.0);
+  const double radius_hz = o.frequency_cluster_radius / observation_seconds;
+  std::uint32_t nclusters = 0;
+  double prev = 0.0;
+  bool monotone = consistent;
+  for (std::size_t rd = 0; rd < nr && monotone; ++rd) {
+    const std::uint64_t bits = fbits[nr - 1 - rd];
+    if (bits == kUnset) continue;
+    double f = 0.0;
+    std::memcpy(&f, &bits, sizeof(f));
+    if (nclusters == 0) {
+      nclusters = 1;
+    } else {
+      monotone = f >= prev;
+      if (f - prev > radius_hz) ++nclusters;
+    }
+    cluster_of[rd] = nclusters - 1;
+    freq_of[rd] = f;
+    prev = f;
+  }
+  if (!monotone) return select_candidates_cpu(peaks, observation_seconds, o);
+  counting_sort(a, b, nr, [](const RankedPeak& e) { return e.rdesc; });
+#pragma omp parallel for schedule(static)
+  for (std::int64_t k = 0; k < np; ++k) b[static_cast<std::size_t>(k)].cluster =
+      cluster_of[b[static_cast<std::size_t>(k)].rdesc];
+  // (cluster, DM, width) order with frequency ascending inside: two more
+  // stable passes
+  counting_sort(b, a, ndm * nw, [](const RankedPeak& e) { return e.dmw; });
+  counting_sort(a, b, nclusters, [](const RankedPeak& e) { return e.cluster; });
+  // DM-cluster boundaries, as select_candidates_cpu draws them
+  const auto dm_of = [&](const RankedPeak& e) {
+    return o.cluster_across_widths ? e.dmw / nw : e.dmw % ndm;
+  };
+  const auto w_of = [&](const RankedPeak& e) {
+    return o.cluster_across_widths ? e.dmw % nw : e.dmw / ndm;
+  };
+  std::vector<std::uint8_t> split(peaks.size(), 0);
+  split[0] = 1;
+#pragma omp parallel for schedule(static)
+  for (std::int64_t k = 1; k < np; ++k) {
+    const RankedPeak& prev_e = b[static_cast<std::size_t>(k - 1)];
+    const RankedPeak& cur = b[static_cast<std::size_t>(k)];
+    const std::size_t dp = dm_of(prev_e), dc = dm_of(cur);
+    const bool same_width = o.cluster_across_widths || w_of(cur) == w_of(prev_e);
+    const bool same_dm = dc <= dp || dc - dp <= o.dm_cluster_radius;
+    if (cur.cluster != prev_e.cluster || !same_width || !same_dm)
+      split[static_cast<std::size_t>(k)] = 1;
+  }
+  std::vector<std::size_t> seg;
+  for (std::size_t k = 0; k < split.size(); ++k)
+    if (split[k]) seg.push_back(k);
+  seg.push_back(split.size());
+  const std::int64_t nseg = static_cast<std::int64_t>(seg.size()) - 1;
+  std::vector<Candidate> out(static_cast<std::size_t>(nseg));
+#pragma omp parallel for schedule(dynamic, 1)
+  for (std::int64_t j = 0; j < nseg; ++j) {
+    const RankedPeak* begin = b.data() + seg[j];
+    const RankedPeak* end = b.data() + seg[j + 1];
+    Candidate c;
+    c.dm_index_min = std::numeric_limits<std::size_t>::max();
+    c.frequency_min = std::numeric_limits<double>::infinity();
+    c.frequency_max = -std::numeric_limits<double>::infinity();
+    float top = -std::numeric_limits<float>::infinity();
+    for (const RankedPeak* e = begin; e != end; ++e) {
+      const std::size_t dm = dm_of(*e);
+      c.dm_index_min = std::min(c.dm_index_min, dm);
+      c.dm_index_max = std::max(c.dm_index_max, dm);
+      c.frequency_min = std::min(c.frequency_min, freq_of[e->rdesc]);
+      c.frequency_max = std::max(c.frequency_max, freq_of[e->rdesc]);
+      top = std::max(top, e->snr);
+    }
+    const DmPeak* best = nullptr;
+    for (const RankedPeak* e = begin; e != end; ++e) {
+      if (e->snr != top) continue;
+      const DmPeak& p = peaks[e->i];
+      if (best == nullptr || peak_before(p, *best)) best = &p;
+    }
+    c.dm = best->dm;
+    c.dm_index = best->dm_index;
+    c.period = best->peak.period;
+    c.frequency = best->peak.frequency;
+    c.width = best->peak.width;
+    c.duty_cycle = best->peak.duty_cycle;
+    c.snr = best->peak.snr;
+    c.peak_count = static_cast<std::size_t>(end - begin);
+    c.best_peak = *best;
+    out[static_cast<std::size_t>(j)] = c;
+  }
   __gnu_parallel::sort(out.begin(), out.end(), candidate_before);
   if (o.max_candidates != 0 && out.size() > o.max_candidates) out.resize(o.max_candidates);
   return out;

@@ -47,6 +47,17 @@ std::vector<Candidate> select_candidates_cpu(std::span<const DmPeak> peaks,
                                              double observation_seconds,
                                              const CandidateSelectionOptions& options = {});
 
+// The same candidates as select_candidates_cpu, for peaks whose trial
+// periods carry a dense rank (period_rank[i] < period_rank[j] exactly when
+// peaks[i].peak.period < peaks[j].peak.period; equal ranks, equal periods):
+// the frequency order is the rank order, so the sorts are stable counting
+// sorts instead of comparison sorts.  No gaffa counterpart: the search
+// pipeline's fast path (its ranks come from ffa_period_ranks).
+std::vector<Candidate> select_candidates_ranked_cpu(std::span<const DmPeak> peaks,
+                                                    std::span<const std::uint32_t> period_rank,
+                                                    double observation_seconds,
+                                                    const CandidateSelectionOptions& options = {});
+
 inline constexpr std::size_t kNoHarmonicParent = std::numeric_limits<std::size_t>::max();
 
 struct HarmonicContext {

@@ -139,6 +139,7 @@ class GpuPinnedBuffer {
   }
   ~GpuPinnedBuffer() { release(); }
   void* data() noexcept { return p_; }
+  const void* data() const noexcept { return p_; }
   std::size_t bytes() const noexcept { return bytes_; }
 
  private:

@@ -102,6 +102,10 @@ class GpuSearchPipeline {
   const GpuFfaProgram& ffa_program() const noexcept { return *ffa_; }
   GpuFfaProgram& ffa_program() noexcept { return *ffa_; }
   const std::vector<DmPeak>& peaks() const noexcept { return peaks_; }
+  // select_candidates_cpu of the last search's peaks (observation =
+  // out_nsamples * tsamp), through the ranked fast path: the peaks' trial
+  // periods carry the plan's dense period ranks.
+  std::vector<Candidate> select_candidates(const CandidateSelectionOptions& options) const;
   std::uint32_t* plane_u32() noexcept { return plane_.data(); }
   const std::uint8_t* filterbank() const noexcept { return filterbank_.data(); }
   gaffa_hip_stream stream() const noexcept { return stream_.get(); }

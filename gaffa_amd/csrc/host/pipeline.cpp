@@ -297,6 +297,34 @@ void GpuSearchPipeline::search() {
   }
 }
 
+std::vector<Candidate> GpuSearchPipeline::select_candidates(
+    const CandidateSelectionOptions& options) const {
+  const double obs = static_cast<double>(n_out_) * cfg_.tsamp;
+  const std::size_t n = peaks_.size();
+  const auto& tasks = ffa_plan_.tasks;
+  if (n == 0 || period_ranks_.empty() || expanded_.size() < n ||
+      compact_host_.bytes() < n * sizeof(gaffa_hip_ffa_peak))
+    return select_candidates_cpu(peaks_, obs, options);
+  // the peaks in arrival order (the same multiset as peaks_) and the dense
+  // rank of each one's trial period, from its task and row
+  std::vector<std::size_t> first_row(tasks.size() + 1, 0);
+  for (std::size_t k = 0; k < tasks.size(); ++k)
+    first_row[k + 1] = first_row[k] + tasks[k].rows_eval;
+  const auto* compact = static_cast<const gaffa_hip_ffa_peak*>(compact_host_.data());
+  std::vector<std::uint32_t> rank(n);
+  bool ok = period_ranks_.size() == first_row.back();
+#pragma omp parallel for reduction(&& : ok) schedule(static)
+  for (std::int64_t i = 0; i < static_cast<std::int64_t>(n); ++i) {
+    const gaffa_hip_ffa_peak& c = compact[i];
+    const bool in = c.task < tasks.size() && c.shift < tasks[c.task].rows_eval;
+    ok = ok && in;
+    rank[static_cast<std::size_t>(i)] = in ? period_ranks_[first_row[c.task] + c.shift] : 0u;
+  }
+  if (!ok) return select_candidates_cpu(peaks_, obs, options);
+  return select_candidates_ranked_cpu(std::span<const DmPeak>(expanded_.data(), n), rank, obs,
+                                      options);
+}
+
 StageTimes GpuSearchPipeline::run() {
   GpuDeviceScope scope(cfg_.device_id);
   gaffa_hip_stream st = stream_.get();

@@ -157,3 +157,28 @@ def test_harmonics_match_reference_build(core, ref):
         assert (r.frequency_error_bins, r.phase_distance, r.dm_distance, r.expected_snr,
                 r.snr_distance) == (w["frequency_error_bins"], w["phase_distance"],
                                     w["dm_distance"], w["expected_snr"], w["snr_distance"])
+
+
+@pytest.mark.parametrize("across,dm_radius,radius,grid", [(True, 50, 0.1, True), (False, 2, 0.5, True),
+                                                          (True, 1, 0.1, False), (True, 0, 0.0, True)])
+def test_ranked_selection_equals_general(core, across, dm_radius, radius, grid):
+    """The pipeline's ranked fast path (counting sorts over dense period ranks)
+    gives the same candidate records as the general selection, with many
+    peaks sharing a trial period (as a search's do), DM and width."""
+    rng = np.random.default_rng(31 + dm_radius)
+    peaks = random_peaks(rng, 30000)
+    if grid:  # frequencies on a trial grid: many peaks share each period
+        trials = np.sort(rng.uniform(0.1, 10.0, 3000))
+        f = trials[np.searchsorted(trials, peaks["peak"]["frequency"]).clip(0, 2999)]
+        pk = peaks["peak"]
+        pk["frequency"], pk["period"] = f, 1.0 / f
+        peaks["peak"] = pk
+    _, rank = np.unique(peaks["peak"]["period"], return_inverse=True)
+    rank = rank.astype(np.uint32)
+    want = select(core, peaks, obs=268.0, radius=radius, dm_radius=dm_radius, across=across)
+    raw = core._select_candidates_ranked_records(peaks, rank, 268.0, radius, dm_radius, across, 0)
+    got = np.frombuffer(raw, CANDIDATE_DTYPE)
+    assert len(got) == len(want) > 10
+    assert np.array_equal(got, want)
+    raw = core._select_candidates_ranked_records(peaks, rank, 268.0, radius, dm_radius, across, 7)
+    assert np.array_equal(np.frombuffer(raw, CANDIDATE_DTYPE), want[:7])
