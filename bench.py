@@ -271,7 +271,8 @@ def detections(pipe, workload, cands, snr_cut: float = 8.0) -> tuple:
 def bench_batched(args) -> int:
     """Batched FFA: float series of different lengths, each with its own
     octave plan, searched as tiles of one launch sequence each, replayed from
-    a HIP graph, with the tile size chosen by the autotuner; the per-series
+    a HIP graph, with the tile size and the rows per level and terminal work
+    item chosen by the autotuner; the per-series
     launch loop (one sequence per series) is timed beside it."""
     import numpy as np
 
@@ -283,7 +284,8 @@ def bench_batched(args) -> int:
     print(f"[bench] batched: {w.nseries} series, {int(w.lengths.sum())} samples, row "
           f"{b.row_length}, setup {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     best = b.autotune([8, 16, 32, 64])
-    print(f"[bench] autotuned series per tile: {best}", file=sys.stderr, flush=True)
+    tuning = b.tuning
+    print(f"[bench] autotuned: {json.dumps(tuning)}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.run(w.snr_threshold)
     total_ms, gpu_ms, npeaks, timing = 0.0, 0.0, 0, {}
@@ -310,7 +312,8 @@ def bench_batched(args) -> int:
         "config": {"workload": (f"batched FFA: {w.nseries} series, lengths 2^20-2^22 at "
                                 f"{w.tsamp} s, octave plans 0.1-10 s (bins 128-255), "
                                 "each series its own plan"),
-                   "series_per_tile": best, "tiles": timing.get("tiles"),
+                   "series_per_tile": best, "level_rows": tuning["level_rows"],
+                   "terminal_rows": tuning["terminal_rows"], "tiles": timing.get("tiles"),
                    "launches": timing.get("launches"), "peaks": npeaks},
         "roofline": {"bound": "hbm", "kernel": "batched FFA launch sequence (all tiles)",
                      "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",

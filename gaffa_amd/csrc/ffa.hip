@@ -38,11 +38,13 @@ namespace gaffa_hip {
 namespace {
 
 constexpr uint32_t kLevelThreads = 256;
-constexpr uint32_t kLevelRows = 16;  // rows per level work item (host side too)
+constexpr uint32_t kLevelRows = 16;     // default rows per level work item
+constexpr uint32_t kLevelRowsMax = 32;  // most rows a level item may hold
 constexpr uint32_t kSubtreeThreads = 1024;
 constexpr uint32_t kSubtreeMaxRows = 512;  // rows of a subtree node
 constexpr uint32_t kTerminalThreads = 256;
-constexpr uint32_t kTerminalRows = 32;  // rows per terminal work item (host side too)
+constexpr uint32_t kTerminalRows = 32;     // default rows per terminal work item
+constexpr uint32_t kTerminalRowsMax = 32;  // most rows a terminal item may hold
 constexpr uint32_t kPrepThreads = 256;
 constexpr uint32_t kPrepStage = 8192;  // floats of input staged per prepare block
 constexpr uint32_t kMaxLdsBytes = 160 * 1024;
@@ -575,11 +577,12 @@ struct Desc3 {
 
 __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
-  __shared__ Desc sdesc[kLevelRows];
-  __shared__ Desc2 sdesc2[kLevelRows];
-  __shared__ Desc3 sdesc3[kLevelRows];
+  __shared__ Desc sdesc[kLevelRowsMax];
+  __shared__ Desc2 sdesc2[kLevelRowsMax];
+  __shared__ Desc3 sdesc3[kLevelRowsMax];
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
+  if (item.c > kLevelRowsMax) return;  // the host builds items within the descriptor arrays
   const gaffa_hip_ffa_task& task = plan.tasks[item.task];
   const uint64_t sidx = blockIdx.y;
   const uint32_t b = task.bins;
@@ -599,7 +602,7 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
       sdesc[threadIdx.x] = merge_desc(find_node(0, task.rows, depth, r), r, b);
     }
     __syncthreads();
-    constexpr uint32_t kBatch = kLevelRows / kWaves;
+    constexpr uint32_t kBatch = kLevelRows / kWaves;  // rows per wave in flight
     merge_batch<kBatch>(
         wave, kWaves, item.c, b, lane,
         [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
@@ -755,7 +758,7 @@ template <bool kDetect>
 __global__ void __launch_bounds__(kTerminalThreads)
 ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   extern __shared__ float lds[];
-  __shared__ Desc2 sdesc[kTerminalRows];  // one-level merges use a, b, rh
+  __shared__ Desc2 sdesc[kTerminalRowsMax];  // one-level merges use a, b, rh
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.terminal_items[blockIdx.x];
   if (!kDetect && item.task != only_task) return;
@@ -773,6 +776,11 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
   uint32_t maxw = 0;
   for (uint32_t w = 0; w < task.nwidths; ++w) maxw = max(maxw, task.width[w]);
   const uint32_t sb = terminal_stride(b, maxw);
+  // the host sizes the LDS rows and the items together; an item beyond them
+  // is refused rather than written past the tile
+  if (nrows == 0 || nrows > kTerminalRowsMax ||
+      static_cast<uint64_t>(nrows) * sb * 4u > plan.terminal_lds_bytes)
+    return;
 
   // depth of the rows read: 1, or 2 when this kernel merges two levels
   const uint32_t tl = task.rows >= 2 && task.terminal_levels == 2 ? 2u : 1u;
@@ -867,12 +875,11 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
     }
     __syncthreads();
 
-    // One lane per (row, width): wave-uniform width, lanes = rows.
-    const uint32_t npairs = kTerminalRows * task.nwidths;
+    // One lane per (row, width) of this item: rows fastest.
+    const uint32_t npairs = nrows * task.nwidths;
     for (uint32_t e = threadIdx.x; e < npairs; e += kTerminalThreads) {
-      const uint32_t i = e % kTerminalRows;
-      const uint32_t widx = e / kTerminalRows;
-      if (i >= nrows) continue;
+      const uint32_t i = e % nrows;
+      const uint32_t widx = e / nrows;
       const uint32_t w = task.width[widx];
       const float* x = lds + static_cast<size_t>(i) * sb;
       float total;
@@ -1126,9 +1133,16 @@ uint32_t gaffa_hip_ffa_prepare_outputs_per_block(double factor) {
 
 uint32_t gaffa_hip_ffa_terminal_rows(void) { return kTerminalRows; }
 uint32_t gaffa_hip_ffa_level_rows(void) { return kLevelRows; }
+uint32_t gaffa_hip_ffa_terminal_rows_max(void) { return kTerminalRowsMax; }
+uint32_t gaffa_hip_ffa_level_rows_max(void) { return kLevelRowsMax; }
 
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {
   return kTerminalRows * terminal_stride(bins, max_width) * 4u;
+}
+
+uint32_t gaffa_hip_ffa_terminal_lds_bytes_rows(uint32_t bins, uint32_t max_width, uint32_t rows) {
+  if (rows == 0 || rows > kTerminalRowsMax) return 0xFFFFFFFFu;
+  return rows * terminal_stride(bins, max_width) * 4u;
 }
 
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins) {

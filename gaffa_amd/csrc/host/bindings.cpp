@@ -1088,10 +1088,35 @@ PYBIND11_MODULE(_core, m) {
           "Seeded N(0,1) series (+ Gaussian pulse trains where amplitude != 0) in the row")
       .def("set_series_per_tile", &GpuFfaBatchedSearch::set_series_per_tile)
       .def("autotune",
-           [](GpuFfaBatchedSearch& b, const std::vector<std::size_t>& candidates) {
+           [](GpuFfaBatchedSearch& b, const std::vector<std::size_t>& candidates,
+              const std::vector<std::uint32_t>& level_rows,
+              const std::vector<std::uint32_t>& terminal_rows) {
              py::gil_scoped_release release;
-             return b.autotune(candidates);
-           })
+             return b.autotune(candidates, level_rows, terminal_rows);
+           },
+           py::arg("candidates"), py::arg("level_rows") = std::vector<std::uint32_t>{8, 16, 32},
+           py::arg("terminal_rows") = std::vector<std::uint32_t>{16, 32},
+           "Tune series per tile, then rows per level item, then rows per terminal item; "
+           "cached per batch shape")
+      .def_property_readonly("tuning",
+                             [](const GpuFfaBatchedSearch& b) {
+                               const FfaBatchTuning& t = b.tuning();
+                               py::dict d;
+                               d["series_per_tile"] = t.series_per_tile;
+                               d["level_rows"] = t.item_shape.level_rows;
+                               d["terminal_rows"] = t.item_shape.terminal_rows;
+                               d["gpu_ms"] = t.gpu_ms;
+                               d["cached"] = t.cached;
+                               return d;
+                             })
+      .def(
+          "set_item_shape",
+          [](GpuFfaBatchedSearch& b, std::uint32_t level_rows, std::uint32_t terminal_rows) {
+            py::gil_scoped_release release;
+            b.set_item_shape(FfaItemShape{level_rows, terminal_rows});
+          },
+          py::arg("level_rows") = 0, py::arg("terminal_rows") = 0,
+          "Rows per level / terminal work item (0 = the kernels' defaults)")
       .def(
           "run",
           [](GpuFfaBatchedSearch& b, float snr_threshold, bool per_series) {

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <parallel/algorithm>
 #include <stdexcept>
 
@@ -84,6 +85,11 @@ void GpuFfaBatchedSearch::set_series_per_tile(std::size_t n) {
   build_tiles();
 }
 
+void GpuFfaBatchedSearch::set_item_shape(FfaItemShape shape) {
+  opt_.item_shape = shape;
+  build_tiles();
+}
+
 void GpuFfaBatchedSearch::build_tiles() {
   GpuDeviceScope scope(opt_.device_id);
   if (graph_ != nullptr) {
@@ -110,7 +116,7 @@ void GpuFfaBatchedSearch::build_tiles() {
     auto t = std::make_unique<Tile>();
     t->first = i0;
     t->count = i - i0;
-    t->plan = std::make_unique<GpuFfaExecutionPlan>(members);
+    t->plan = std::make_unique<GpuFfaExecutionPlan>(members, opt_.item_shape);
     const GpuFfaExecutionPlan& e = *t->plan;
     t->groups = gaffa::upload(e.groups().data(), e.groups().size());
     t->tasks = gaffa::upload(e.tasks().data(), e.tasks().size());
@@ -251,24 +257,96 @@ FfaBatchSearchResult GpuFfaBatchedSearch::run_per_series(float thr, FfaBatchTimi
   return r;
 }
 
-std::size_t GpuFfaBatchedSearch::autotune(const std::vector<std::size_t>& candidates) {
-  check_arg(!candidates.empty(), "autotune needs at least one candidate");
-  std::size_t best = 0;
-  float best_ms = 0.0F;
-  for (std::size_t c : candidates) {
-    if (c == 0) continue;
-    set_series_per_tile(c);
+namespace {
+
+// A batch's shape for the tuning cache: the members' lengths and plan sizes
+// (the tuned knobs change the time only, never the results, so two batches
+// that merely collide share a configuration and stay correct).
+std::uint64_t batch_shape_key(const std::vector<FfaSearchPlan>& plans) {
+  std::uint64_t h = 1469598103934665603ULL;
+  const auto mix = [&](std::uint64_t v) {
+    h ^= v + 0x9E3779B97F4A7C15ULL + (h << 6) + (h >> 2);
+  };
+  mix(plans.size());
+  for (const FfaSearchPlan& p : plans) {
+    mix(p.tasks.front().input_nsamples);
+    mix(p.tasks.size());
+    mix(p.tasks.front().bins);
+    mix(p.tasks.back().rows);
+  }
+  return h;
+}
+
+std::map<std::uint64_t, FfaBatchTuning>& tuning_cache() {
+  static std::map<std::uint64_t, FfaBatchTuning> cache;
+  return cache;
+}
+
+std::mutex& tuning_mutex() {
+  static std::mutex m;
+  return m;
+}
+
+}  // namespace
+
+std::size_t GpuFfaBatchedSearch::autotune(const std::vector<std::size_t>& tile_candidates,
+                                          const std::vector<std::uint32_t>& level_rows,
+                                          const std::vector<std::uint32_t>& terminal_rows) {
+  check_arg(!tile_candidates.empty(), "autotune needs at least one candidate");
+  const std::uint64_t key = batch_shape_key(plans_);
+  {
+    std::lock_guard<std::mutex> lock(tuning_mutex());
+    const auto it = tuning_cache().find(key);
+    if (it != tuning_cache().end()) {
+      tuning_ = it->second;
+      tuning_.cached = true;
+      opt_.item_shape = tuning_.item_shape;
+      set_series_per_tile(tuning_.series_per_tile);
+      return tuning_.series_per_tile;
+    }
+  }
+  const auto time_current = [&]() {
     FfaBatchTiming t;
     (void)run(1e30F, &t);  // no peaks: times the transform and detection alone
     (void)run(1e30F, &t);  // second run: graph captured, caches warm
-    if (best == 0 || t.gpu_ms < best_ms) {
-      best = c;
-      best_ms = t.gpu_ms;
+    return t.gpu_ms;
+  };
+  FfaBatchTuning best;
+  for (std::size_t c : tile_candidates) {
+    if (c == 0) continue;
+    set_series_per_tile(c);
+    const float ms = time_current();
+    if (best.series_per_tile == 0 || ms < best.gpu_ms) {
+      best.series_per_tile = c;
+      best.gpu_ms = ms;
     }
   }
-  check_arg(best != 0, "autotune needs a positive candidate");
-  set_series_per_tile(best);
-  return best;
+  check_arg(best.series_per_tile != 0, "autotune needs a positive candidate");
+  series_per_tile_ = best.series_per_tile;
+  best.item_shape = opt_.item_shape;
+  const auto tune_rows = [&](const std::vector<std::uint32_t>& values, bool level) {
+    for (std::uint32_t v : values) {
+      FfaItemShape s = best.item_shape;
+      (level ? s.level_rows : s.terminal_rows) = v;
+      const std::uint32_t cur = level ? best.item_shape.level_rows : best.item_shape.terminal_rows;
+      const std::uint32_t dflt = level ? gaffa_hip_ffa_level_rows() : gaffa_hip_ffa_terminal_rows();
+      if (v == 0 || v == (cur == 0 ? dflt : cur)) continue;  // the configuration already timed
+      set_item_shape(s);
+      const float ms = time_current();
+      if (ms < best.gpu_ms) {
+        best.item_shape = s;
+        best.gpu_ms = ms;
+      }
+    }
+  };
+  tune_rows(level_rows, true);
+  tune_rows(terminal_rows, false);
+  opt_.item_shape = best.item_shape;
+  set_series_per_tile(best.series_per_tile);
+  tuning_ = best;
+  std::lock_guard<std::mutex> lock(tuning_mutex());
+  tuning_cache()[key] = best;
+  return best.series_per_tile;
 }
 
 FfaBatchSearchResult GpuFfaBatchedSearch::collect(std::uint32_t count) const {

@@ -85,7 +85,24 @@ void collect_nodes(std::uint32_t start, std::uint32_t size, std::uint32_t depth,
 
 }  // namespace
 
-GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan) {
+namespace {
+
+std::uint32_t pick_rows(std::uint32_t want, std::uint32_t dflt, std::uint32_t max,
+                        const char* message) {
+  if (want == 0) return dflt;
+  check_arg(want <= max, message);
+  return want;
+}
+
+}  // namespace
+
+GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan, FfaItemShape shape)
+    : plan_(plan) {
+  level_rows_ = pick_rows(shape.level_rows, gaffa_hip_ffa_level_rows(),
+                          gaffa_hip_ffa_level_rows_max(), "FFA level rows per item too large");
+  terminal_rows_ = pick_rows(shape.terminal_rows, gaffa_hip_ffa_terminal_rows(),
+                             gaffa_hip_ffa_terminal_rows_max(),
+                             "FFA terminal rows per item too large");
   check_arg(!plan.tasks.empty(), "FFA executor plan must contain at least one task");
   input_nsamples_ = plan.tasks.front().input_nsamples;
   check_arg(input_nsamples_ > 0, "FFA executor time series must not be empty");
@@ -94,7 +111,13 @@ GpuFfaExecutionPlan::GpuFfaExecutionPlan(const FfaSearchPlan& plan) : plan_(plan
   finish();
 }
 
-GpuFfaExecutionPlan::GpuFfaExecutionPlan(std::span<const FfaBatchMember> members) {
+GpuFfaExecutionPlan::GpuFfaExecutionPlan(std::span<const FfaBatchMember> members,
+                                         FfaItemShape shape) {
+  level_rows_ = pick_rows(shape.level_rows, gaffa_hip_ffa_level_rows(),
+                          gaffa_hip_ffa_level_rows_max(), "FFA level rows per item too large");
+  terminal_rows_ = pick_rows(shape.terminal_rows, gaffa_hip_ffa_terminal_rows(),
+                             gaffa_hip_ffa_terminal_rows_max(),
+                             "FFA terminal rows per item too large");
   check_arg(!members.empty(), "FFA batch must contain at least one series");
   plan_.width_trials = members.front().plan->width_trials;
   for (const FfaBatchMember& m : members)
@@ -120,8 +143,8 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
   check_arg(plan.width_trials.size() <= GAFFA_HIP_FFA_MAX_WIDTHS,
             "GPU FFA supports at most 16 width trials");
   const std::size_t n = plan.tasks.front().input_nsamples;
-  const std::uint32_t level_rows = gaffa_hip_ffa_level_rows();
-  const std::uint32_t terminal_rows = gaffa_hip_ffa_terminal_rows();
+  const std::uint32_t level_rows = level_rows_;
+  const std::uint32_t terminal_rows = terminal_rows_;
   std::map<double, std::uint32_t> group_of;
   for (const FfaSearchTask& t : plan.tasks) {
     validate_task(t, n);
@@ -171,7 +194,8 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
     max_bins_ = std::max(max_bins_, d.bins);
     terminal_lds_bytes_ = std::max(
         terminal_lds_bytes_,
-        gaffa_hip_ffa_terminal_lds_bytes(d.bins, static_cast<std::uint32_t>(det.max_width)));
+        gaffa_hip_ffa_terminal_lds_bytes_rows(d.bins, static_cast<std::uint32_t>(det.max_width),
+                                              terminal_rows));
     // rows of a node: two ping-pong buffers in what the fixed LDS leaves
     check_arg(subtree_fixed_lds(lds_max_bins_) < kSubtreeLdsBytes,
               "GPU FFA bins too large for an LDS subtree");

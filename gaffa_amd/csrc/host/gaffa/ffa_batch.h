@@ -8,8 +8,9 @@
 // subtree, level and detect launch sequence covers the whole tile whatever
 // its members' lengths and row counts.  Tiles share one workspace and one
 // compact peak buffer; the launch sequence of all tiles can be captured once
-// in a HIP graph and replayed.  The tile size is chosen by timing candidate
-// sizes on the actual batch (autotune()).
+// in a HIP graph and replayed.  The tile size and the rows per level and
+// terminal work item are chosen by timing candidates on the actual batch
+// (autotune()); the choice is cached per batch shape for the process.
 #pragma once
 
 #include "gaffa/ffa_gpu.h"
@@ -28,6 +29,16 @@ struct GpuFfaBatchedOptions {
   std::size_t initial_peak_buffer_bytes = std::size_t{64} << 20;
   std::size_t max_peak_buffer_bytes = std::size_t{1} << 30;
   bool use_graph = true;
+  FfaItemShape item_shape{};  // rows per level / terminal item (0 = defaults)
+};
+
+// What autotune() chose, the device time of that configuration and whether
+// it came from the cache of an earlier batch of the same shape.
+struct FfaBatchTuning {
+  std::size_t series_per_tile = 0;
+  FfaItemShape item_shape{};
+  float gpu_ms = 0.0F;
+  bool cached = false;
 };
 
 struct FfaBatchTiming {
@@ -58,8 +69,19 @@ class GpuFfaBatchedSearch {
 
   // Regroup into tiles of at most n series (drops a captured graph).
   void set_series_per_tile(std::size_t n);
-  // Time one run per candidate tile size and keep the fastest; returns it.
-  std::size_t autotune(const std::vector<std::size_t>& candidates);
+  // Rows per level / terminal work item of the following runs (drops a
+  // captured graph).
+  void set_item_shape(FfaItemShape shape);
+  const FfaItemShape& item_shape() const noexcept { return opt_.item_shape; }
+  // Tune the tile size over `tile_candidates`, then the rows per level item
+  // over `level_rows`, then the rows per terminal item over `terminal_rows`
+  // (one knob at a time, each timed on the actual batch, fastest kept), and
+  // cache the result under the batch's shape: a later batch of the same
+  // shape takes it without timing.  Returns the tile size.
+  std::size_t autotune(const std::vector<std::size_t>& tile_candidates,
+                       const std::vector<std::uint32_t>& level_rows = {8, 16, 32},
+                       const std::vector<std::uint32_t>& terminal_rows = {16, 32});
+  const FfaBatchTuning& tuning() const noexcept { return tuning_; }
 
   // Search every series: peaks sorted by (series, is_better_ffa_peak).
   FfaBatchSearchResult run(float snr_threshold, FfaBatchTiming* timing = nullptr);
@@ -89,6 +111,7 @@ class GpuFfaBatchedSearch {
   GpuDeviceBuffer<std::uint32_t> count_;
   gaffa_hip_graph graph_ = nullptr;
   float graph_threshold_ = 0.0F;
+  FfaBatchTuning tuning_{};
   gaffa_hip_event ev_[2] = {nullptr, nullptr};
 };
 

@@ -57,14 +57,23 @@ struct FfaBatchMember {
   std::uint32_t series = 0;
 };
 
+// Rows per level and per terminal work item (0 = the kernels' defaults,
+// gaffa_hip_ffa_level_rows() / gaffa_hip_ffa_terminal_rows()); the batched
+// search's autotuner picks them per input shape.  Results do not depend on
+// them.
+struct FfaItemShape {
+  std::uint32_t level_rows = 0;
+  std::uint32_t terminal_rows = 0;
+};
+
 class GpuFfaExecutionPlan {
  public:
-  explicit GpuFfaExecutionPlan(const FfaSearchPlan& plan);
+  explicit GpuFfaExecutionPlan(const FfaSearchPlan& plan, FfaItemShape shape = {});
   // Several series with their own lengths and plans laid end to end in one
   // row: every group reads its member's offset, every task reports its
   // member's series index, and plan().tasks is the concatenation of the
   // members' tasks (indexed by the compact peaks' task field).
-  explicit GpuFfaExecutionPlan(std::span<const FfaBatchMember> members);
+  explicit GpuFfaExecutionPlan(std::span<const FfaBatchMember> members, FfaItemShape shape = {});
 
   const FfaSearchPlan& plan() const noexcept { return plan_; }
   std::size_t input_nsamples() const noexcept { return input_nsamples_; }
@@ -103,6 +112,7 @@ class GpuFfaExecutionPlan {
   std::size_t max_depth_ = 0;
   std::size_t max_prepared_ = 0;
   std::size_t lds_max_bins_ = 0;  // bins of the widest task of every plan appended
+  std::uint32_t level_rows_ = 0, terminal_rows_ = 0;  // rows per level / terminal item
   std::uint32_t subtree_lds_bytes_ = 0;
   std::uint32_t subtree_half_ = 0;
   std::uint32_t subtree_desc_ = 0;

@@ -426,6 +426,12 @@ uint32_t gaffa_hip_ffa_terminal_rows(void);
 uint32_t gaffa_hip_ffa_level_rows(void);
 uint32_t gaffa_hip_ffa_prepare_outputs_per_block(double factor);
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width);
+/* The same for terminal items of `rows` rows (1 .. gaffa_hip_ffa_terminal_rows_max();
+ * 0xFFFFFFFF otherwise), and the largest row counts level and terminal items
+ * may hold (the host picks rows per item within them; no gaffa counterpart). */
+uint32_t gaffa_hip_ffa_terminal_lds_bytes_rows(uint32_t bins, uint32_t max_width, uint32_t rows);
+uint32_t gaffa_hip_ffa_terminal_rows_max(void);
+uint32_t gaffa_hip_ffa_level_rows_max(void);
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins);
 /* Fills the merge table of a subtree node of `rows` rows (host memory; a
  * null `out` only counts) with the kernels' own row-selection arithmetic

@@ -576,17 +576,33 @@ def test_ffa_batched_heterogeneous_bit_exact(gpu, port):
 
 
 def test_ffa_batched_autotune_keeps_results(gpu):
+    """The autotuner's knobs (series per tile, rows per level item, rows per
+    terminal item) change the time only: every configuration, including odd
+    row counts that split tasks unevenly, returns the same peaks; the chosen
+    configuration is cached for a second batch of the same shape."""
     from gaffa_amd.pipeline import batched_workload
 
-    say("ffa batch: autotune over 1/2/4 series per tile")
+    say("ffa batch: autotune over tiles and rows per level / terminal item")
     w = batched_workload(5, seed=9, log2_min=15, log2_max=16, tsamp=256e-6, injected=1.0)
     b = w.search(series_per_tile=5, workspace_bytes=1 << 30)
-    before, _ = b.run(6.0)
+
+    def key(peaks):
+        return [(i, p.shift, p.phase, p.width, p.bins, p.snr) for i, p in peaks]
+
+    before = key(b.run(6.0)[0])
+    assert len(before) > 0
+    for level_rows, terminal_rows in ((8, 16), (32, 32), (5, 7), (1, 1), (32, 3)):
+        b.set_item_shape(level_rows=level_rows, terminal_rows=terminal_rows)
+        assert key(b.run(6.0)[0]) == before, (level_rows, terminal_rows)
+    b.set_item_shape()
     best = b.autotune([1, 2, 4])
-    assert best in (1, 2, 4) and b.series_per_tile == best
-    after, _ = b.run(6.0)
-    assert [(i, p.shift, p.phase, p.width, p.bins, p.snr) for i, p in before] == \
-        [(i, p.shift, p.phase, p.width, p.bins, p.snr) for i, p in after]
+    t = b.tuning
+    assert best in (1, 2, 4) and b.series_per_tile == best and not t["cached"]
+    assert key(b.run(6.0)[0]) == before
+    again = w.search(series_per_tile=5, workspace_bytes=1 << 30)
+    assert again.autotune([1, 2, 4]) == best and again.tuning["cached"]
+    assert again.tuning["level_rows"] == t["level_rows"]
+    assert key(again.run(6.0)[0]) == before
 
 
 # ------------------------------------------------------------ folding ----
