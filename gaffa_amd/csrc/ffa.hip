@@ -359,6 +359,12 @@ __host__ __device__ __forceinline__ uint32_t subtree_zero_floats(uint32_t max_bi
   return (need + 63u) & ~63u;
 }
 
+// Passes over a subtree node of `levels` tree levels: two levels each, the
+// bottom one single when `levels` is odd.
+__host__ __device__ __forceinline__ uint32_t subtree_tree_passes(uint32_t levels) {
+  return (levels + 1) / 2;
+}
+
 // Passes per wave per level of a node of `rows` rows: wave w merges rows w,
 // w + 16, ..., kSubtreeRowsPerPass of them per pass.
 __host__ __device__ __forceinline__ uint32_t subtree_passes(uint32_t rows) {
@@ -376,6 +382,10 @@ __device__ __forceinline__ gaffa_hip_ffa_merge load_merge(ConstMerge* p) {
   g.tail = p->tail;
   g.wrap = p->wrap;
   g.out = p->out;
+  g.head2 = p->head2;
+  g.wrap2 = p->wrap2;
+  g.tail2 = p->tail2;
+  g.wrap3 = p->wrap3;
   return g;
 }
 
@@ -449,20 +459,22 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
     if (levels == 0) {
       for (uint32_t e = tid; e < total; e += kSubtreeThreads) out[e] = cur[e];
     }
-    // One level: the wave's merges m[0 .. R * passes) of `src` (LDS) into
-    // `store`, which is written once per destination kind so every access
-    // keeps its address space (LDS rows are ds_* operations, the top level's
-    // stores global_*).  The table gives each wave its rows w, w + 16, ...,
-    // padded to whole passes with entries that are skipped.  A row is swept in
-    // blocks of 256 phases, NC <= 4 chunks of 64 lanes.  The roll splits it at
-    // w = bins - roll: phases p < w read tail[p + roll], phases p >= w read
-    // tail[p + roll - b].  Per chunk a lane picks one of two tail addresses
-    // (one compare, one select); every other address is one add per row plus
-    // the chunk's immediate offset.  The last chunk of a row loads up to 63
-    // floats past it (inside the next row or the LDS slack) and masks only
-    // its stores.  A copied row's tail is the zero rows.
+    // One pass of NL = 1 or 2 tree levels: the wave's merges m[0 .. R *
+    // passes) of `src` (LDS) into `store`, which is written once per
+    // destination kind so every access keeps its address space (LDS rows are
+    // ds_* operations, the top pass's stores global_*).  The table gives each
+    // wave its rows w, w + 16, ..., padded to whole passes with entries that
+    // are skipped.  A row is swept in blocks of 256 phases, NC <= 4 chunks of
+    // 64 lanes.  A rolled operand X (B, and C, D in a two-level pass) splits
+    // the row at its wrap point w = bins - roll: phases p < w read X[p +
+    // roll], phases p >= w read X[p + roll - b].  Per chunk a lane picks one
+    // of two addresses (one compare, one select); every other address is one
+    // add per row plus the chunk's immediate offset.  The last chunk of a row
+    // loads up to 63 floats past it (inside the next row or the LDS slack)
+    // and masks only its stores.  A copied operand reads the zero rows.
     const uint32_t passes = subtree_passes(nrows);
-    auto level = [&](ConstMerge* m, const float* src, auto&& store) {
+    auto level = [&](auto nl, ConstMerge* m, const float* src, auto&& store) {
+      constexpr uint32_t NL = decltype(nl)::value;
       if (wave >= nrows) return;
       constexpr uint32_t R = kSubtreeRowsPerPass;
       for (uint32_t p0 = 0; p0 < b; p0 += 256) {
@@ -472,30 +484,42 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
         auto pass = [&](auto ncc) {
           constexpr uint32_t NC = decltype(ncc)::value;
           using LdsFloat = __attribute__((address_space(3))) const float;
-          auto load_row = [&](const gaffa_hip_ffa_merge& g, float (&hv)[NC], float (&tv)[NC]) {
-            const uint32_t before = g.wrap - min(g.wrap, p0);  // phases of the block before w
-            const float* hp = src + g.head + p0 + lane;
-            const float* tbase = g.tail == GAFFA_HIP_FFA_COPY_ROW ? lds : src + g.tail;
-            LdsFloat* ta = (LdsFloat*)(tbase + p0 + lane);
-            // tb[64 j] lies inside the tail row for every lane that reads it
-            // (p >= w); tb itself may sit up to 192 floats before the row,
-            // inside the zero rows.  The empty asm keeps ta and tb two
-            // registers, so that each chunk's choice is one select rather
-            // than a rebuilt address.
+          // X[(p + roll) mod b] for the chunk's phases p0 + lane + 64 j.
+          // tb[64 j] lies inside X's row for every lane that reads it
+          // (p >= w); tb itself may sit up to 192 floats before the row,
+          // inside the zero rows.  The empty asm keeps ta and tb two
+          // registers, so that each chunk's choice is one select rather than
+          // a rebuilt address.
+          auto load_rolled = [&](uint32_t x, uint32_t wrap, float (&v)[NC]) {
+            const uint32_t before = wrap - min(wrap, p0);  // phases of the block before w
+            const float* base = x == GAFFA_HIP_FFA_COPY_ROW ? lds : src + x;
+            LdsFloat* ta = (LdsFloat*)(base + p0 + lane);
             LdsFloat* tb = ta - b;
             asm("" : "+v"(tb));
 #pragma unroll
             for (uint32_t j = 0; j < NC; ++j) {
               LdsFloat* tp = lane + 64u * j < before ? ta : tb;
-              hv[j] = hp[64u * j];
-              tv[j] = tp[64u * j];
+              v[j] = tp[64u * j];
             }
           };
-          auto store_row = [&](uint32_t orow, const float (&hv)[NC], const float (&tv)[NC]) {
+          auto load_row = [&](const gaffa_hip_ffa_merge& g, float (&v)[2 * NL][NC]) {
+            const float* hp = src + g.head + p0 + lane;
 #pragma unroll
-            for (uint32_t j = 0; j + 1 < NC; ++j)
-              store(orow + p0 + lane + 64u * j, hv[j] + tv[j]);
-            if (lane < live) store(orow + p0 + lane + 64u * (NC - 1), hv[NC - 1] + tv[NC - 1]);
+            for (uint32_t j = 0; j < NC; ++j) v[0][j] = hp[64u * j];
+            load_rolled(g.tail, g.wrap, v[1]);
+            if constexpr (NL == 2) {
+              load_rolled(g.head2, g.wrap2, v[2]);
+              load_rolled(g.tail2, g.wrap3, v[3]);
+            }
+          };
+          auto store_row = [&](uint32_t orow, const float (&v)[2 * NL][NC]) {
+            auto sum = [&](uint32_t j) {
+              if constexpr (NL == 2) return (v[0][j] + v[1][j]) + (v[2][j] + v[3][j]);
+              else return v[0][j] + v[1][j];
+            };
+#pragma unroll
+            for (uint32_t j = 0; j + 1 < NC; ++j) store(orow + p0 + lane + 64u * j, sum(j));
+            if (lane < live) store(orow + p0 + lane + 64u * (NC - 1), sum(NC - 1));
           };
           gaffa_hip_ffa_merge g[R];
 #pragma unroll
@@ -513,17 +537,17 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
             // (the last one or two entries of its list)
             static_assert(R == 2, "the padding cases below assume two rows per pass");
             if (cg[1].out != GAFFA_HIP_FFA_SKIP_ROW) {
-              float hv[R][NC], tv[R][NC];
+              float v[R][2 * NL][NC];
 #pragma unroll
-              for (uint32_t k = 0; k < R; ++k) load_row(cg[k], hv[k], tv[k]);
+              for (uint32_t k = 0; k < R; ++k) load_row(cg[k], v[k]);
               __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-              for (uint32_t k = 0; k < R; ++k) store_row(cg[k].out, hv[k], tv[k]);
+              for (uint32_t k = 0; k < R; ++k) store_row(cg[k].out, v[k]);
             } else if (cg[0].out != GAFFA_HIP_FFA_SKIP_ROW) {
-              float hv[NC], tv[NC];
-              load_row(cg[0], hv, tv);
+              float v[2 * NL][NC];
+              load_row(cg[0], v);
               __builtin_amdgcn_sched_barrier(0);
-              store_row(cg[0].out, hv, tv);
+              store_row(cg[0].out, v);
             }
           }
         };
@@ -533,18 +557,29 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
         else pass(std::integral_constant<uint32_t, 1>{});
       }
     };
-    // level d of the table: kWaves x (R * passes) merges, wave-major
+    // tree pass k of the table (k = 0 the node's own): kWaves x (R * passes)
+    // merges, wave-major; two levels per pass, the bottom pass single when
+    // the level count is odd
     ConstMerge* const table = merges + first_merge + wave * kSubtreeRowsPerPass * passes;
-    const uint32_t level_size = kWaves * kSubtreeRowsPerPass * passes;
-    for (int32_t d = static_cast<int32_t>(levels) - 1; d >= 1; --d) {
+    const uint32_t pass_size = kWaves * kSubtreeRowsPerPass * passes;
+    const uint32_t npasses = subtree_tree_passes(levels);
+    using One = std::integral_constant<uint32_t, 1>;
+    using Two = std::integral_constant<uint32_t, 2>;
+    for (int32_t k = static_cast<int32_t>(npasses) - 1; k >= 1; --k) {
       float* dst = nxt;
-      level(table + static_cast<uint32_t>(d) * level_size, cur,
-            [&](uint32_t i, float v) { dst[i] = v; });
+      auto to_lds = [&](uint32_t i, float v) { dst[i] = v; };
+      ConstMerge* const mk = table + static_cast<uint32_t>(k) * pass_size;
+      if (2u * static_cast<uint32_t>(k) + 1u < levels) level(Two{}, mk, cur, to_lds);
+      else level(One{}, mk, cur, to_lds);
       __syncthreads();
       nxt = cur;
       cur = dst;
     }
-    if (levels > 0) level(table, cur, [&](uint32_t i, float v) { out[i] = v; });
+    if (levels > 0) {
+      auto to_global = [&](uint32_t i, float v) { out[i] = v; };
+      if (levels >= 2) level(Two{}, table, cur, to_global);
+      else level(One{}, table, cur, to_global);
+    }
     __syncthreads();  // the LDS rows are rewritten by the next unit
   }
 }
@@ -1156,35 +1191,69 @@ uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins) {
 uint32_t gaffa_hip_ffa_subtree_merges(uint32_t rows, uint32_t bins, gaffa_hip_ffa_merge* out) {
   if (rows == 0 || rows > kSubtreeMaxRows || bins < 2 || bins > 65535) return 0;
   const uint32_t levels = tree_levels(rows);
+  const uint32_t npasses = subtree_tree_passes(levels);
   const uint32_t per_wave = kSubtreeRowsPerPass * subtree_passes(rows);
-  const uint32_t level_size = kSubtreeWaves * per_wave;
-  if (out == nullptr) return levels * level_size;
-  for (uint32_t d = 0; d < levels; ++d) {
+  const uint32_t pass_size = kSubtreeWaves * per_wave;
+  if (out == nullptr) return npasses * pass_size;
+  constexpr uint32_t kCopy = GAFFA_HIP_FFA_COPY_ROW;
+  // row r of the merge of the node containing r at `depth`, read from the
+  // rows one level below: head row, tail row and roll, or a copied leaf
+  auto one = [&](uint32_t depth, uint32_t r, uint32_t& head, uint32_t& tail, uint32_t& roll) {
+    const Node nd = find_node(0, rows, depth, r);
+    if (nd.size < 2) {
+      head = r;
+      tail = kCopy;
+      roll = 0;
+    } else {
+      const Desc ds = merge_desc(nd, r, bins);
+      head = ds.h;
+      tail = ds.t;
+      roll = ds.roll;
+    }
+  };
+  for (uint32_t k = 0; k < npasses; ++k) {
+    const uint32_t depth = 2 * k;                      // output depth of the pass
+    const bool two = depth + 1 < levels;               // else the bottom single level
     for (uint32_t w = 0; w < kSubtreeWaves; ++w) {
-      for (uint32_t k = 0; k < per_wave; ++k) {
-        // row w + 16 k, or padding past the wave's last row
-        const uint32_t r = w + kSubtreeWaves * k;
-        gaffa_hip_ffa_merge& m = out[d * level_size + w * per_wave + k];
-        if (r >= rows) {
-          m = gaffa_hip_ffa_merge{0, GAFFA_HIP_FFA_COPY_ROW, bins, GAFFA_HIP_FFA_SKIP_ROW};
+      for (uint32_t j = 0; j < per_wave; ++j) {
+        // row w + 16 j, or padding past the wave's last row
+        const uint32_t r = w + kSubtreeWaves * j;
+        gaffa_hip_ffa_merge& m = out[k * pass_size + w * per_wave + j];
+        m = gaffa_hip_ffa_merge{0, kCopy, bins, GAFFA_HIP_FFA_SKIP_ROW, kCopy, bins, kCopy, bins};
+        if (r >= rows) continue;
+        m.out = r * bins;
+        uint32_t h, t, roll;
+        one(depth, r, h, t, roll);
+        if (!two) {
+          m.head = h * bins;
+          if (t != kCopy) {
+            m.tail = t * bins + roll;
+            m.wrap = bins - roll;
+          }
           continue;
         }
-        const Node nd = find_node(0, rows, d, r);
-        m.out = r * bins;
-        if (nd.size < 2) {  // a leaf above depth d: copied down unchanged
-          m.head = r * bins;
-          m.tail = GAFFA_HIP_FFA_COPY_ROW;
-          m.wrap = bins;
-        } else {
-          const Desc ds = merge_desc(nd, r, bins);
-          m.head = ds.h * bins;
-          m.tail = ds.t * bins + ds.roll;
-          m.wrap = bins - ds.roll;
+        // head child row h: A + B; tail child row t (read at `roll`): C + D
+        uint32_t a, b, rb;
+        one(depth + 1, h, a, b, rb);
+        m.head = a * bins;
+        if (b != kCopy) {
+          m.tail = b * bins + rb;
+          m.wrap = bins - rb;
+        }
+        if (t == kCopy) continue;  // a leaf above depth: (A + -0) + (-0 + -0)
+        uint32_t c, d, rd;
+        one(depth + 1, t, c, d, rd);
+        m.head2 = c * bins + roll;
+        m.wrap2 = bins - roll;
+        if (d != kCopy) {
+          const uint32_t rcd = (roll + rd) % bins;
+          m.tail2 = d * bins + rcd;
+          m.wrap3 = bins - rcd;
         }
       }
     }
   }
-  return levels * level_size;
+  return npasses * pass_size;
 }
 
 }  // extern "C"

@@ -362,22 +362,34 @@ typedef struct gaffa_hip_ffa_peak {
   float snr;
 } gaffa_hip_ffa_peak;
 
-/* One row of one subtree level: the LDS merge out[i] = head[i] +
- * tail[(i + roll) mod bins] in float offsets from the level's row buffer
- * (16 bytes).  A subtree item's table holds, per level (level 0, the node's
- * own merge, first), 16 wave lists of 2 * ceil(ceil(rows / 16) / 2) merges:
- * wave w's rows w, w + 16, ..., then padding entries (out =
- * GAFFA_HIP_FFA_SKIP_ROW) up to the list length.  The item's `c` field is
- * the index of its first entry; items with the same rows and bins share a
- * table.  No gaffa counterpart: gaffa recurses per row (ffa.cpp:110-251). */
+/* One row of one subtree pass (32 bytes), in float offsets from the pass's
+ * source row buffer.  A pass runs one or two tree levels:
+ *   one level   out[i] = A[i] + B[(i + rb) mod bins]
+ *   two levels  out[i] = (A[i] + B[(i + rb) mod bins])
+ *                        + (C[(i + rc) mod bins] + D[(i + rd) mod bins]),
+ * the two one-level merges' FP32 sums in their order (the depth-(d + 1) row
+ * in between is never stored): A + B is the head child row and C + D the
+ * tail child row read at the output merge's roll rc.  A child that is a leaf
+ * (or a row that is a leaf above the pass) adds the -0.0 row
+ * (GAFFA_HIP_FFA_COPY_ROW), which leaves every value as it is.  A subtree
+ * item of L levels has ceil(L / 2) passes, the node's own (top) pass first;
+ * every pass has two levels except the bottom one when L is odd.  Each pass
+ * holds 16 wave lists of 2 * ceil(ceil(rows / 16) / 2) entries: wave w's
+ * rows w, w + 16, ..., then padding entries (out = GAFFA_HIP_FFA_SKIP_ROW) up
+ * to the list length.  The item's `c` field is the index of its first
+ * entry; items with the same rows and bins share a table.  No gaffa
+ * counterpart: gaffa recurses per row (ffa.cpp:110-251). */
 #define GAFFA_HIP_FFA_COPY_ROW 0xFFFFFFFFu
 #define GAFFA_HIP_FFA_SKIP_ROW 0xFFFFFFFFu
 typedef struct gaffa_hip_ffa_merge {
-  uint32_t head; /* head row * bins */
-  uint32_t tail; /* tail row * bins + roll, or GAFFA_HIP_FFA_COPY_ROW: the
-                    row is a leaf above this level and is copied */
-  uint32_t wrap; /* bins - roll: phases whose tail index does not wrap */
-  uint32_t out;  /* row * bins, or GAFFA_HIP_FFA_SKIP_ROW: a padding entry */
+  uint32_t head;  /* A: row * bins */
+  uint32_t tail;  /* B: row * bins + rb, or GAFFA_HIP_FFA_COPY_ROW */
+  uint32_t wrap;  /* bins - rb: phases whose B index does not wrap */
+  uint32_t out;   /* output row * bins, or GAFFA_HIP_FFA_SKIP_ROW: padding */
+  uint32_t head2; /* C: row * bins + rc (two-level passes; else unused) */
+  uint32_t wrap2; /* bins - rc */
+  uint32_t tail2; /* D: row * bins + rd, or GAFFA_HIP_FFA_COPY_ROW */
+  uint32_t wrap3; /* bins - rd */
 } gaffa_hip_ffa_merge;
 
 typedef struct gaffa_hip_ffa_plan_desc {

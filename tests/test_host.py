@@ -337,18 +337,20 @@ def test_large_workload_shape(core):
 
 
 class _Merge(ctypes.Structure):
-    _fields_ = [("head", ctypes.c_uint32), ("tail", ctypes.c_uint32),
-                ("wrap", ctypes.c_uint32), ("out", ctypes.c_uint32)]
+    _fields_ = [(name, ctypes.c_uint32) for name in
+                ("head", "tail", "wrap", "out", "head2", "wrap2", "tail2", "wrap3")]
 
 
-@pytest.mark.parametrize("rows,bins", [(2, 7), (3, 5), (7, 9), (16, 4), (37, 11), (84, 180),
-                                       (113, 181), (512, 2)])
+@pytest.mark.parametrize("rows,bins", [(2, 7), (3, 5), (5, 3), (7, 9), (16, 4), (37, 11),
+                                       (84, 180), (113, 181), (512, 2)])
 def test_subtree_merge_tables_replay_the_oracle_transform(port, rows, bins):
     """The host-built subtree merge tables (gaffa_hip_ffa_subtree_merges),
-    replayed level by level in float32 exactly as the subtree kernel applies
-    them (head + tail at the split-at-wrap index, -0.0 for a copied row,
-    padding entries skipped), give the oracle's FFA transform of the node bit
-    for bit, every row written exactly once per level."""
+    replayed pass by pass in float32 exactly as the subtree kernel applies
+    them (two levels per pass as (A + B) + (C + D), the bottom pass single
+    when the level count is odd, each rolled operand at its split-at-wrap
+    index, -0.0 for a copied operand, padding entries skipped), give the
+    oracle's FFA transform of the node bit for bit, every row written exactly
+    once per pass."""
     import gaffa_amd
 
     lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
@@ -356,33 +358,38 @@ def test_subtree_merge_tables_replay_the_oracle_transform(port, rows, bins):
     lib.gaffa_hip_ffa_subtree_merges.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p]
     n = lib.gaffa_hip_ffa_subtree_merges(rows, bins, None)
     levels = int(np.ceil(np.log2(rows))) if rows > 1 else 0
-    assert n % max(levels, 1) == 0 and (n == 0) == (levels == 0)
+    npasses = (levels + 1) // 2
+    assert n % max(npasses, 1) == 0 and (n == 0) == (levels == 0)
     table = (_Merge * max(n, 1))()
     assert lib.gaffa_hip_ffa_subtree_merges(rows, bins, ctypes.cast(table, ctypes.c_void_p)) == n
-    head = np.array([m.head for m in table[:n]], np.int64)
-    tail = np.array([m.tail for m in table[:n]], np.int64)
-    wrap = np.array([m.wrap for m in table[:n]], np.int64)
-    out = np.array([m.out for m in table[:n]], np.int64)
+    field = {name: np.array([getattr(m, name) for m in table[:n]], np.int64)
+             for name, _ in _Merge._fields_}
     rng = np.random.default_rng(rows * 1000 + bins)
     x = rng.normal(size=rows * bins).astype(np.float32)
     cur = x.copy()
-    per_level = n // levels if levels else 0
+    per_pass = n // npasses if npasses else 0
     p = np.arange(bins)
-    for d in range(levels - 1, -1, -1):
+    copy = 0xFFFFFFFF
+
+    def rolled(src, base, wrap):
+        if base == copy:
+            return np.full(bins, -0.0, np.float32)
+        return src[np.where(p < wrap, base + p, base + p - bins)]
+
+    for k in range(npasses - 1, -1, -1):
+        two = 2 * k + 1 < levels
         nxt = np.full(rows * bins, np.nan, np.float32)
-        sl = slice(d * per_level, (d + 1) * per_level)
-        for h, t, w, o in zip(head[sl], tail[sl], wrap[sl], out[sl]):
-            if o == 0xFFFFFFFF:  # padding of a wave list: skipped
+        sl = slice(k * per_pass, (k + 1) * per_pass)
+        for e in range(sl.start, sl.stop):
+            f = {name: int(v[e]) for name, v in field.items()}
+            if f["out"] == copy:  # padding of a wave list: skipped
                 continue
-            hv = cur[h + p]
-            if t == 0xFFFFFFFF:
-                tv = np.full(bins, -0.0, np.float32)
-            else:
-                tv = cur[np.where(p < w, t + p, t + p - bins)]
-            v = hv + tv
-            assert np.isnan(nxt[o])  # each row once per level
-            nxt[o + p] = v
-        assert not np.isnan(nxt).any()  # every row written at every level
+            v = cur[f["head"] + p] + rolled(cur, f["tail"], f["wrap"])
+            if two:
+                v = v + (rolled(cur, f["head2"], f["wrap2"]) + rolled(cur, f["tail2"], f["wrap3"]))
+            assert np.isnan(nxt[f["out"]])  # each row once per pass
+            nxt[f["out"] + p] = v
+        assert not np.isnan(nxt).any()  # every row written at every pass
         cur = nxt
     want = port.ffa_transform(x, bins).reshape(-1)
     assert cur.tobytes() == want.tobytes()
