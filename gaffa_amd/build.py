@@ -41,7 +41,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
 
 HIP_SOURCES = ["runtime.hip", "dedisperse.hip", "preprocess.hip", "ffa.hip", "synth.hip",
-               "fold.hip", "scorer.hip"]
+               "fold.hip", "scorer.hip", "candidate.hip"]
 HOST_SOURCES = ["host/dedispersion.cpp", "host/preprocessing.cpp", "host/ffa_plan.cpp",
                 "host/ffa_gpu.cpp", "host/hip_runtime.cpp", "host/synth.cpp", "host/pipeline.cpp",
                 "host/ffa_batch.cpp", "host/candidate.cpp", "host/folding.cpp",

@@ -1351,16 +1351,17 @@ PYBIND11_MODULE(_core, m) {
       })
       .def(
           "select_candidates",
-          [](const GpuSearchPipeline& p, double radius, std::size_t dm_radius, bool across,
-             std::size_t max_candidates) {
+          [](GpuSearchPipeline& p, double radius, std::size_t dm_radius, bool across,
+             std::size_t max_candidates, bool device) {
             py::gil_scoped_release release;
-            return p.select_candidates(
-                CandidateSelectionOptions{radius, dm_radius, across, max_candidates});
+            const CandidateSelectionOptions o{radius, dm_radius, across, max_candidates};
+            return device ? p.select_candidates(o) : p.select_candidates_host(o);
           },
           py::kw_only(), py::arg("frequency_cluster_radius") = 0.1,
           py::arg("dm_cluster_radius") = 1, py::arg("cluster_across_widths") = true,
-          py::arg("max_candidates") = 0,
-          "Candidates from the last search's peaks (observation = out_nsamples * tsamp)")
+          py::arg("max_candidates") = 0, py::arg("device") = true,
+          "Candidates from the last search's peaks (observation = out_nsamples * tsamp); "
+          "device=False runs the host selection")
       .def(
           "fold_candidates",
           [](GpuSearchPipeline& p, const std::vector<std::size_t>& dm_index,

@@ -19,14 +19,6 @@ void check_arg(bool ok, const char* message) {
 
 bool peak_before(const DmPeak& a, const DmPeak& b) { return is_better_dm_peak(a, b); }
 
-bool candidate_before(const Candidate& a, const Candidate& b) {
-  if (peak_before(a.best_peak, b.best_peak)) return true;
-  if (peak_before(b.best_peak, a.best_peak)) return false;
-  if (a.peak_count != b.peak_count) return a.peak_count > b.peak_count;
-  if (a.frequency != b.frequency) return a.frequency < b.frequency;
-  return a.dm_index < b.dm_index;
-}
-
 // A peak's clustering key: frequency cluster order is set first, then DM
 // clusters need (width when widths stay apart, DM index, width, frequency)
 // order.  Peaks equal in all four keep no defined order, which changes
@@ -159,6 +151,14 @@ std::optional<HarmonicRelation> relation(const Candidate& parent, const Candidat
 }
 
 }  // namespace
+
+bool candidate_before(const Candidate& a, const Candidate& b) {
+  if (peak_before(a.best_peak, b.best_peak)) return true;
+  if (peak_before(b.best_peak, a.best_peak)) return false;
+  if (a.peak_count != b.peak_count) return a.peak_count > b.peak_count;
+  if (a.frequency != b.frequency) return a.frequency < b.frequency;
+  return a.dm_index < b.dm_index;
+}
 
 std::vector<Candidate> select_candidates_cpu(std::span<const DmPeak> peaks,
                                              double observation_seconds,

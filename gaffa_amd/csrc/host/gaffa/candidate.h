@@ -39,6 +39,11 @@ struct CandidateSelectionOptions {
   std::size_t max_candidates = 0;  // 0 = all
 };
 
+// The order of the candidate list (is_better_candidate, candidate.cpp:48-62):
+// best peak first by is_better_dm_peak, then more peaks, lower frequency,
+// lower DM index.
+bool candidate_before(const Candidate& a, const Candidate& b);
+
 // candidate.cpp:175-212: single-linkage clusters along frequency (gaps larger
 // than the radius split), then along DM index inside each frequency cluster
 // (optionally per width); one candidate per cluster from its best peak,

@@ -158,6 +158,8 @@ class GpuFfaProgram {
   // ideally pinned) and waits; returns how many.
   std::size_t read_peaks_into(std::uint32_t count, gaffa_hip_ffa_peak* dst);
   std::size_t peak_capacity() const noexcept { return peaks_.size(); }
+  // The device compact peaks of the last search, in arrival order.
+  const gaffa_hip_ffa_peak* peak_data() const noexcept { return peaks_.data(); }
   // Enqueues on the program stream a copy of the running peak count into
   // `host_slot` (pinned host memory); valid once the stream reaches it.
   void enqueue_peak_count_copy(std::uint32_t* host_slot);

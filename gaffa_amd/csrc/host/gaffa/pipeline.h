@@ -103,9 +103,13 @@ class GpuSearchPipeline {
   GpuFfaProgram& ffa_program() noexcept { return *ffa_; }
   const std::vector<DmPeak>& peaks() const noexcept { return peaks_; }
   // select_candidates_cpu of the last search's peaks (observation =
-  // out_nsamples * tsamp), through the ranked fast path: the peaks' trial
-  // periods carry the plan's dense period ranks.
-  std::vector<Candidate> select_candidates(const CandidateSelectionOptions& options) const;
+  // out_nsamples * tsamp), equal candidate for candidate: on the device from
+  // the compact peaks where the sort key fits (frequency clusters numbered on
+  // the host from the ranks present), else through the host's ranked path;
+  // both carry the trial periods as the plan's dense period ranks.
+  std::vector<Candidate> select_candidates(const CandidateSelectionOptions& options);
+  // The host selection alone (the fallback of select_candidates).
+  std::vector<Candidate> select_candidates_host(const CandidateSelectionOptions& options) const;
   std::uint32_t* plane_u32() noexcept { return plane_.data(); }
   const std::uint8_t* filterbank() const noexcept { return filterbank_.data(); }
   gaffa_hip_stream stream() const noexcept { return stream_.get(); }
@@ -172,6 +176,18 @@ class GpuSearchPipeline {
   std::vector<DmPeak> expanded_;        // peaks in arrival order, expanded per tile
   std::vector<std::size_t> run_ends_;   // end of each tile's sorted keys
   std::vector<std::uint32_t> period_ranks_;  // ffa_period_ranks(ffa_plan_)
+  // device candidate selection: the rank table per evaluated (task, row),
+  // each task's first row, the frequency of each descending rank, and the
+  // buffers of the last selection (grown as needed)
+  std::uint32_t nranks_ = 0;
+  std::vector<double> freq_of_rdesc_;
+  GpuDeviceBuffer<std::uint32_t> cand_rank_table_;
+  GpuDeviceBuffer<std::uint64_t> cand_first_row_;
+  GpuDeviceBuffer<std::uint32_t> cand_peak_rank_, cand_cluster_of_, cand_counts_;
+  GpuDeviceBuffer<std::uint8_t> cand_present_, cand_workspace_;
+  GpuDeviceBuffer<gaffa_hip_candidate_segment> cand_segments_;
+  GpuDeviceBuffer<gaffa_hip_candidate_tie> cand_ties_;
+  std::vector<Candidate> select_candidates_gpu(const CandidateSelectionOptions& options);
   std::vector<double> trial_periods_;
   std::vector<std::uint32_t> trial_order_;
 };

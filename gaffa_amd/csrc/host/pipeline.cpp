@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <cmath>
+#include <limits>
 #include <parallel/algorithm>
 #include <stdexcept>
 
@@ -70,6 +71,23 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
   fe.initial_peak_buffer_bytes = std::max<std::size_t>(fe.initial_peak_buffer_bytes, 256u << 20);
   ffa_ = std::make_unique<GpuFfaProgram>(ffa_plan_, GpuFfaProgramOptions{cfg_.device_id}, fe);
   period_ranks_ = ffa_period_ranks(ffa_plan_);
+  if (!period_ranks_.empty()) {
+    // device candidate selection: the rank of each evaluated (task, row),
+    // each task's first row, and the frequency of each descending rank (the
+    // peaks' own 1 / period, make_ffa_peak)
+    const std::vector<double> periods = ffa_plan_trial_periods(ffa_plan_);
+    std::vector<std::uint64_t> first_row(ffa_plan_.tasks.size() + 1, 0);
+    for (std::size_t k = 0; k < ffa_plan_.tasks.size(); ++k)
+      first_row[k + 1] = first_row[k] + ffa_plan_.tasks[k].rows_eval;
+    std::uint32_t top = 0;
+    for (std::uint32_t r : period_ranks_) top = std::max(top, r);
+    nranks_ = top + 1;
+    freq_of_rdesc_.assign(nranks_, 0.0);
+    for (std::size_t i = 0; i < periods.size(); ++i)
+      freq_of_rdesc_[nranks_ - 1 - period_ranks_[i]] = 1.0 / periods[i];
+    cand_rank_table_ = gaffa::upload(period_ranks_.data(), period_ranks_.size());
+    cand_first_row_ = gaffa::upload(first_row.data(), first_row.size());
+  }
   events_.resize(7, nullptr);
   for (auto& e : events_) hip_ok(gaffa_hip_event_create(&e), "hipEventCreate");
 }
@@ -298,6 +316,154 @@ void GpuSearchPipeline::search() {
 }
 
 std::vector<Candidate> GpuSearchPipeline::select_candidates(
+    const CandidateSelectionOptions& options) {
+  std::vector<Candidate> out = select_candidates_gpu(options);
+  if (!out.empty()) return out;
+  return select_candidates_host(options);
+}
+
+namespace {
+
+std::uint32_t key_bits(std::uint64_t count) {  // bits to hold 0 .. count - 1
+  std::uint32_t b = 1;
+  while (b < 64 && (std::uint64_t{1} << b) < count) ++b;
+  return b;
+}
+
+}  // namespace
+
+// The device selection, or an empty list where it does not apply (no rank
+// table, peaks not on the device, or a sort key wider than 64 bits); every
+// peak of a search is finite except possibly its S/N, which the device checks.
+std::vector<Candidate> GpuSearchPipeline::select_candidates_gpu(
+    const CandidateSelectionOptions& o) {
+  const double obs = static_cast<double>(n_out_) * cfg_.tsamp;
+  check_arg(obs > 0.0 && std::isfinite(obs),
+            "Candidate selection observation_seconds must be finite and > 0");
+  check_arg(o.frequency_cluster_radius >= 0.0 && std::isfinite(o.frequency_cluster_radius),
+            "Candidate selection frequency_cluster_radius must be finite and >= 0");
+  const std::size_t n = peaks_.size();
+  const std::size_t nw = ffa_plan_.width_trials.size();
+  if (n == 0 || nranks_ == 0 || n >= (std::size_t{1} << 32) || expanded_.size() < n ||
+      ffa_->peak_capacity() < n || nw == 0 || nw >= (1u << 16))
+    return {};
+  for (double dm : dms_)
+    check_arg(std::isfinite(dm), "Candidate selection peaks must contain finite scientific values");
+  GpuDeviceScope scope(cfg_.device_id);
+  gaffa_hip_stream st = stream_.get();
+  if (cand_peak_rank_.size() < n) {
+    cand_peak_rank_ = GpuDeviceBuffer<std::uint32_t>(n + n / 4);
+    cand_segments_ = GpuDeviceBuffer<gaffa_hip_candidate_segment>(n + n / 4);
+    cand_ties_ = GpuDeviceBuffer<gaffa_hip_candidate_tie>(n + n / 4);
+    std::uint64_t bytes = 0;
+    hip_ok(gaffa_hip_candidate_workspace_bytes(n + n / 4, &bytes), "candidate workspace");
+    cand_workspace_ = GpuDeviceBuffer<std::uint8_t>(bytes);
+  }
+  if (cand_present_.size() < nranks_) {
+    cand_present_ = GpuDeviceBuffer<std::uint8_t>(nranks_);
+    cand_cluster_of_ = GpuDeviceBuffer<std::uint32_t>(nranks_);
+  }
+  if (cand_counts_.size() < 3) cand_counts_ = GpuDeviceBuffer<std::uint32_t>(3);
+  std::uint64_t ws_bytes = 0;
+  hip_ok(gaffa_hip_candidate_workspace_bytes(n, &ws_bytes), "candidate workspace");
+  if (cand_workspace_.size() < ws_bytes) cand_workspace_ = GpuDeviceBuffer<std::uint8_t>(ws_bytes);
+  gaffa_hip_candidate_args a{};
+  a.peaks = ffa_->peak_data();
+  a.npeaks = n;
+  a.period_rank = cand_rank_table_.data();
+  a.task_first_row = cand_first_row_.data();
+  a.ntasks = static_cast<std::uint32_t>(ffa_plan_.tasks.size());
+  a.nranks = nranks_;
+  a.ndm = static_cast<std::uint32_t>(ndm_);
+  a.nwidths = static_cast<std::uint32_t>(nw);
+  a.peak_rank = cand_peak_rank_.data();
+  a.present = cand_present_.data();
+  a.status = cand_counts_.data() + 2;
+  hip_ok(gaffa_hip_candidate_ranks(&a, st), "candidate ranks");
+  // frequency clusters over the ranks present, ascending frequency
+  std::vector<std::uint8_t> present(nranks_);
+  std::uint32_t status = 0;
+  copy_to_host(present.data(), cand_present_.data(), nranks_, st);
+  copy_to_host(&status, cand_counts_.data() + 2, sizeof(status), st);
+  check_arg((status & GAFFA_HIP_CANDIDATE_NONFINITE) == 0,
+            "Candidate selection peaks must contain finite scientific values");
+  if ((status & GAFFA_HIP_CANDIDATE_BAD_PEAK) != 0)
+    throw std::logic_error("candidate selection: a compact peak lies outside the plan");
+  const double radius_hz = o.frequency_cluster_radius / obs;
+  std::vector<std::uint32_t> cluster_of(nranks_, 0);
+  std::uint32_t nclusters = 0;
+  double prev = 0.0;
+  for (std::uint32_t rd = 0; rd < nranks_; ++rd) {
+    if (!present[nranks_ - 1 - rd]) continue;
+    const double f = freq_of_rdesc_[rd];
+    check_arg(std::isfinite(f), "Candidate selection peaks must contain finite scientific values");
+    if (nclusters == 0 || f - prev > radius_hz) ++nclusters;
+    cluster_of[rd] = nclusters - 1;
+    prev = f;
+  }
+  if (key_bits(nclusters) + key_bits(nranks_) +
+          key_bits(static_cast<std::uint64_t>(ndm_) * nw) > 64)
+    return {};
+  copy_to_device(cand_cluster_of_.data(), cluster_of.data(), nranks_ * sizeof(std::uint32_t), st);
+  a.cluster_of_rdesc = cand_cluster_of_.data();
+  a.nclusters = nclusters;
+  a.dm_cluster_radius = static_cast<std::uint32_t>(std::min<std::size_t>(o.dm_cluster_radius, ndm_));
+  a.cluster_across_widths = o.cluster_across_widths ? 1u : 0u;
+  a.workspace = cand_workspace_.data();
+  a.workspace_bytes = cand_workspace_.size();
+  a.segments = cand_segments_.data();
+  a.nsegments = cand_counts_.data();
+  a.ties = cand_ties_.data();
+  a.nties = cand_counts_.data() + 1;
+  hip_ok(gaffa_hip_candidate_clusters(&a, st), "candidate clusters");
+  std::uint32_t counts[2] = {0, 0};
+  copy_to_host(counts, cand_counts_.data(), sizeof(counts), st);
+  const std::size_t nseg = counts[0], nties = counts[1];
+  if (nseg == 0 || nseg > n || nties < nseg || nties > n)
+    throw std::logic_error("candidate selection: inconsistent cluster counts");
+  std::vector<gaffa_hip_candidate_segment> seg(nseg);
+  std::vector<gaffa_hip_candidate_tie> ties(nties);
+  copy_to_host(seg.data(), cand_segments_.data(), nseg * sizeof(seg[0]), st);
+  copy_to_host(ties.data(), cand_ties_.data(), nties * sizeof(ties[0]), st);
+  // each cluster's representative: the best of its peaks at the top S/N
+  std::vector<std::uint32_t> best(nseg, std::numeric_limits<std::uint32_t>::max());
+  for (const gaffa_hip_candidate_tie& t : ties) {
+    if (t.segment >= nseg || t.peak >= n)
+      throw std::logic_error("candidate selection: tie record out of range");
+    std::uint32_t& b = best[t.segment];
+    if (b == std::numeric_limits<std::uint32_t>::max() ||
+        is_better_dm_peak(expanded_[t.peak], expanded_[b]))
+      b = t.peak;
+  }
+  for (std::uint32_t b : best)
+    if (b == std::numeric_limits<std::uint32_t>::max())
+      throw std::logic_error("candidate selection: a cluster without its top peak");
+  std::vector<Candidate> out(nseg);
+#pragma omp parallel for schedule(static)
+  for (std::int64_t j = 0; j < static_cast<std::int64_t>(nseg); ++j) {
+    const gaffa_hip_candidate_segment& g = seg[static_cast<std::size_t>(j)];
+    const DmPeak& p = expanded_[best[static_cast<std::size_t>(j)]];
+    Candidate& c = out[static_cast<std::size_t>(j)];
+    c.dm = p.dm;
+    c.dm_index = p.dm_index;
+    c.period = p.peak.period;
+    c.frequency = p.peak.frequency;
+    c.width = p.peak.width;
+    c.duty_cycle = p.peak.duty_cycle;
+    c.snr = p.peak.snr;
+    c.peak_count = g.count;
+    c.dm_index_min = g.dm_min;
+    c.dm_index_max = g.dm_max;
+    c.frequency_min = freq_of_rdesc_[g.rdesc_min];
+    c.frequency_max = freq_of_rdesc_[g.rdesc_max];
+    c.best_peak = p;
+  }
+  __gnu_parallel::sort(out.begin(), out.end(), candidate_before);
+  if (o.max_candidates != 0 && out.size() > o.max_candidates) out.resize(o.max_candidates);
+  return out;
+}
+
+std::vector<Candidate> GpuSearchPipeline::select_candidates_host(
     const CandidateSelectionOptions& options) const {
   const double obs = static_cast<double>(n_out_) * cfg_.tsamp;
   const std::size_t n = peaks_.size();

@@ -511,6 +511,75 @@ int gaffa_hip_ffa_transform_rows(const gaffa_hip_ffa_search_args* args,
                                  uint32_t task, float* out,
                                  gaffa_hip_stream stream);
 
+/* ------------------------------------------------- candidate selection ----
+ * gaffa's candidate selection (select_candidates_cpu, candidate.cpp:175-212)
+ * over the compact peaks of a DM search, where `series` is the DM index:
+ * single-linkage clusters along frequency (a gap wider than the radius
+ * between consecutive present trial frequencies splits), then along DM index
+ * inside each frequency cluster (and width, unless clustered across widths).
+ * gaffa runs it on the host only; these entry points replace it for peaks
+ * already on the device.  Trial periods enter as dense ranks (equal periods
+ * share a rank, ascending period = ascending rank), and frequencies as
+ * descending ranks: rdesc = nranks - 1 - rank.
+ *
+ * 1. gaffa_hip_candidate_ranks: peak_rank[i] = period_rank[task_first_row[
+ *    task] + shift] of peak i, present[rank] = 1 for every rank a peak has
+ *    (present is cleared first), status = 0 or the GAFFA_HIP_CANDIDATE_*
+ *    bits of the peaks that fail (a task, row, DM or width out of range, or
+ *    a non-finite S/N).
+ * 2. The caller numbers the frequency clusters over the present ranks in
+ *    rdesc order: cluster_of_rdesc[rd] (the CPU's frequency gaps).
+ * 3. gaffa_hip_candidate_clusters: one record per DM cluster, in cluster
+ *    order (frequency cluster, then DM / width): its descending-rank and DM
+ *    index ranges, top S/N and peak count, plus a tie record (cluster, peak
+ *    index) for every peak whose S/N equals its cluster's top; the caller
+ *    picks each representative with gaffa's peak order.  segments and ties
+ *    hold npeaks entries; the workspace is
+ *    gaffa_hip_candidate_workspace_bytes(npeaks) bytes.  Steps 1 and 3 are
+ *    asynchronous on `stream`; step 2 needs `present` on the host. */
+#define GAFFA_HIP_CANDIDATE_BAD_PEAK 1u
+#define GAFFA_HIP_CANDIDATE_NONFINITE 2u
+
+typedef struct gaffa_hip_candidate_segment {
+  uint32_t rdesc_min, rdesc_max; /* frequency range as descending ranks */
+  uint32_t dm_min, dm_max;       /* DM index range */
+  float snr;                     /* top S/N */
+  uint32_t count;                /* peaks in the cluster */
+} gaffa_hip_candidate_segment;
+
+typedef struct gaffa_hip_candidate_tie {
+  uint32_t segment; /* DM cluster */
+  uint32_t peak;    /* index into the compact peaks */
+} gaffa_hip_candidate_tie;
+
+typedef struct gaffa_hip_candidate_args {
+  const gaffa_hip_ffa_peak* peaks; /* device [npeaks] */
+  uint64_t npeaks;
+  const uint32_t* period_rank;     /* device [task_first_row[ntasks]] */
+  const uint64_t* task_first_row;  /* device [ntasks + 1] */
+  uint32_t ntasks;
+  uint32_t nranks;
+  uint32_t ndm;
+  uint32_t nwidths;
+  uint32_t* peak_rank;             /* device [npeaks] (step 1 out, step 3 in) */
+  uint8_t* present;                /* device [nranks] */
+  uint32_t* status;                /* device [1] */
+  const uint32_t* cluster_of_rdesc; /* device [nranks] */
+  uint32_t nclusters;
+  uint32_t dm_cluster_radius;      /* DM index gap that still joins */
+  uint32_t cluster_across_widths;  /* 0: widths kept apart */
+  void* workspace;                 /* device */
+  uint64_t workspace_bytes;
+  gaffa_hip_candidate_segment* segments; /* device [npeaks] */
+  uint32_t* nsegments;             /* device [1] */
+  gaffa_hip_candidate_tie* ties;   /* device [npeaks] */
+  uint32_t* nties;                 /* device [1] */
+} gaffa_hip_candidate_args;
+
+int gaffa_hip_candidate_workspace_bytes(uint64_t npeaks, uint64_t* bytes);
+int gaffa_hip_candidate_ranks(const gaffa_hip_candidate_args* args, gaffa_hip_stream stream);
+int gaffa_hip_candidate_clusters(const gaffa_hip_candidate_args* args, gaffa_hip_stream stream);
+
 /* ------------------------------------------------------------- folding ----
  * Fractional-phase-overlap folds (folding.cpp:300-420): sample t of a job's
  * input (data + input_offset + t * row_stride, channels contiguous) covers

@@ -424,6 +424,57 @@ def assert_dm_peaks_equal(got, want):
     np.testing.assert_array_equal(got["peak"]["period"], want["peak"]["period"])
 
 
+CANDIDATE_FIELDS = ("dm", "dm_index", "period", "frequency", "width", "duty_cycle", "snr",
+                    "peak_count", "dm_index_min", "dm_index_max", "frequency_min",
+                    "frequency_max")
+PEAK_FIELDS = ("period", "frequency", "width", "duty_cycle", "width_index", "period_index",
+               "phase", "shift", "bins", "snr")
+
+
+def assert_candidates_equal(got, want):
+    """Candidate lists equal record by record, every field exact."""
+    assert len(got) == len(want), (len(got), len(want))
+    for k, (a, b) in enumerate(zip(got, want)):
+        for f in CANDIDATE_FIELDS:
+            assert getattr(a, f) == getattr(b, f), (k, f, getattr(a, f), getattr(b, f))
+        pa, pb = a.best_peak, b.best_peak
+        assert (pa.dm, pa.dm_index) == (pb.dm, pb.dm_index), k
+        for f in PEAK_FIELDS:
+            assert getattr(pa.peak, f) == getattr(pb.peak, f), (k, f)
+
+
+def test_candidate_selection_device_equals_host(gpu):
+    """gaffa's candidate selection on the device equals the host selection
+    (the ranked path, itself equal to select_candidates_cpu) candidate for
+    candidate, over frequency radii from 0 (every distinct trial frequency
+    its own cluster) up, DM radii from 0 up, widths together or apart, and a
+    max_candidates cut."""
+    from gaffa_amd.pipeline import small_workload
+
+    w = small_workload(nsamples=1 << 16)
+    pipe = w.pipeline()
+    pipe.generate(w.synth_spec())
+    pipe.run()
+    assert pipe.peak_count > 1000
+    for radius in (0.0, 0.1, 2.0):
+        for dm_radius in (0, 1, 50):
+            for across in (True, False):
+                for cut in (0, 7):
+                    opts = dict(frequency_cluster_radius=radius, dm_cluster_radius=dm_radius,
+                                cluster_across_widths=across, max_candidates=cut)
+                    dev = pipe.select_candidates(**opts)
+                    host = pipe.select_candidates(**opts, device=False)
+                    assert len(dev) > 0
+                    assert_candidates_equal(dev, host)
+    # and the host ranked path equals gaffa's general order-based selection
+    from gaffa_amd import _core as core
+
+    general = core.select_candidates(pipe.peaks, pipe.out_nsamples * w.tsamp,
+                                     frequency_cluster_radius=0.1, dm_cluster_radius=50)
+    assert_candidates_equal(pipe.select_candidates(frequency_cluster_radius=0.1,
+                                                   dm_cluster_radius=50), general)
+
+
 @pytest.mark.parametrize("ffa_series_tile", [0, 7])
 def test_small_pipeline_end_to_end(gpu, port, ffa_series_tile):
     """The 64-channel workload through the device-resident pipeline: the
@@ -483,6 +534,22 @@ def _full_length_group_vs_oracle(port, w, segment: int, group: int):
     pipe = w.pipeline()
     pipe.generate(w.synth_spec())
     pipe.run()
+    # the device candidate selection over every peak of the full search
+    # equals the host's, at gaffa's benchmark options and with widths apart
+    import time
+
+    for opts in (dict(frequency_cluster_radius=0.1, dm_cluster_radius=50,
+                      cluster_across_widths=True),
+                 dict(frequency_cluster_radius=0.3, dm_cluster_radius=2,
+                      cluster_across_widths=False)):
+        t0 = time.perf_counter()
+        dev = pipe.select_candidates(**opts)
+        t1 = time.perf_counter()
+        host = pipe.select_candidates(**opts, device=False)
+        t2 = time.perf_counter()
+        say(f"  candidates of {pipe.peak_count} peaks {opts}: {len(dev)} "
+            f"(device {1e3 * (t1 - t0):.0f} ms, host {1e3 * (t2 - t1):.0f} ms)")
+        assert_candidates_equal(dev, host)
     first = sum(s.ndm for s in w.dm_segments[:segment]) + group * w.ndm_per_nominal
     got = peak_records(pipe, first, first + w.ndm_per_nominal).copy()
     got["dm_index"] -= first
