@@ -185,23 +185,27 @@ detrend_kernel(T* __restrict__ data, uint64_t stride, uint64_t n,
   if (bad) atomicExch(&status[series], 1);
 }
 
-// One lane per series: sum and sum of squares in FP64 in sample order, as
+// Sum and sum of squares of each series in FP64 in sample order, as
 // time_series_stats_cpu does (time_series.cpp:55-60), so every rounding is
 // the CPU's; then mean, 1/stddev and a status flag.
 //
-// A one-wave workgroup owns 64 series.  The wave loads chunks of
-// 64 series x 128 samples with coalesced 16-byte loads (two 512-byte row
-// segments per instruction) into registers, stores them to LDS, and while
-// each lane sums its own row of the chunk in order, the next chunk's loads
-// are already in flight.  Rows must be 16-byte aligned (the pipeline's plane
-// is); other layouts take the per-lane path, which reads each row directly.
+// A two-wave workgroup owns 64 series, a lane of each wave per series: wave 0
+// runs the chain of sums, wave 1 the chain of squares (the two are
+// independent, so each wave has one serial chain instead of two and the
+// launch twice the waves).  The workgroup loads chunks of 64 series x 128
+// samples with coalesced 16-byte loads (two 512-byte row segments per
+// instruction) into registers, stores them to LDS, and while each lane adds
+// its own row of the chunk in order, the next chunk's loads are already in
+// flight.  Rows must be 16-byte aligned (the pipeline's plane is); other
+// layouts take the per-lane path, which reads each row directly.
 // a native 4-float vector: HIP's float4 struct is copied through memory,
 // which would keep a register array of them in scratch
 typedef float vf4 __attribute__((ext_vector_type(4)));
-constexpr int kStatsLanes = 64;
+constexpr int kStatsLanes = 64;                // series per workgroup
+constexpr int kStatsThreads = 2 * kStatsLanes; // one wave per chain
 constexpr int kStatsChunk = 128;               // samples per series per chunk
 constexpr int kStatsPitch = kStatsChunk + 4;   // LDS floats per row (bank spread)
-constexpr int kStatsLoads = kStatsLanes * kStatsChunk / 4 / 64;  // float4 per lane
+constexpr int kStatsLoads = kStatsLanes * kStatsChunk / 4 / kStatsThreads;  // float4 per thread
 constexpr uint32_t kStatsLdsBytes = 2u * kStatsLanes * kStatsPitch * 4u;
 
 struct SerialSums {
@@ -235,27 +239,28 @@ __device__ __forceinline__ void finish_stats(const SerialSums& acc, uint64_t n, 
   if (flag == 2 || flag == 3) atomicCAS(&status[series], 0, flag);
 }
 
-// One or two waves per SIMD: the chunk's 32 float4 per lane then fit the
-// register budget (with four, the array would be placed in scratch memory).
-__global__ void __launch_bounds__(kStatsLanes) __attribute__((amdgpu_waves_per_eu(1, 2)))
+__global__ void __launch_bounds__(kStatsThreads) __attribute__((amdgpu_waves_per_eu(1, 2)))
 stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32_t nseries,
              int32_t reject_constant, double2* __restrict__ stats,
              int32_t* __restrict__ flags, int32_t* __restrict__ status) {
   extern __shared__ __align__(16) float stage[];  // [2][kStatsLanes][kStatsPitch]
-  const uint32_t lane = threadIdx.x;
+  __shared__ double squares_of[kStatsLanes];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const bool squares = tid >= 64u;  // wave-uniform: wave 1 runs the squares
   const uint32_t first = blockIdx.x * kStatsLanes;
   const uint32_t rows = min(static_cast<uint32_t>(kStatsLanes), nseries - first);
   const uint64_t nchunks = n / kStatsChunk;
   const float* base = data + static_cast<uint64_t>(first) * stride;
-  // load k of a chunk: element e = k * 64 + lane is row e / 32, float4 e % 32
+  // load k of a chunk: element e = k * 128 + tid is row e / 32, float4 e % 32
   // (rows past the batch re-read the last row); every index is a constant
   // after unrolling, so the chunk stays in registers
   vf4 reg[kStatsLoads];
-  SerialSums acc;
+  double acc = 0.0;
   if (nchunks > 0) {
 #pragma unroll
     for (int k = 0; k < kStatsLoads; ++k) {
-      const uint32_t e = static_cast<uint32_t>(k) * 64u + lane;
+      const uint32_t e = static_cast<uint32_t>(k) * kStatsThreads + tid;
       const uint32_t r = min(e >> 5, rows - 1);
       reg[k] = *reinterpret_cast<const vf4*>(base + static_cast<uint64_t>(r) * stride +
                                                 4u * (e & 31u));
@@ -265,7 +270,7 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
     const uint32_t buf = static_cast<uint32_t>(c & 1u);
 #pragma unroll
     for (int k = 0; k < kStatsLoads; ++k) {
-      const uint32_t e = static_cast<uint32_t>(k) * 64u + lane;
+      const uint32_t e = static_cast<uint32_t>(k) * kStatsThreads + tid;
       *reinterpret_cast<vf4*>(stage + (buf * kStatsLanes + (e >> 5)) * kStatsPitch +
                                  4u * (e & 31u)) = reg[k];
     }
@@ -274,7 +279,7 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
       const uint64_t col = (c + 1) * kStatsChunk;
 #pragma unroll
       for (int k = 0; k < kStatsLoads; ++k) {
-        const uint32_t e = static_cast<uint32_t>(k) * 64u + lane;
+        const uint32_t e = static_cast<uint32_t>(k) * kStatsThreads + tid;
         const uint32_t r = min(e >> 5, rows - 1);
         reg[k] = *reinterpret_cast<const vf4*>(base + static_cast<uint64_t>(r) * stride + col +
                                                   4u * (e & 31u));
@@ -282,29 +287,53 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
     }
     const vf4* row = reinterpret_cast<const vf4*>(stage + (buf * kStatsLanes + lane) *
                                                   kStatsPitch);
-    // eight samples at a time: the conversions and squares first (independent
-    // of each other), then the two in-order add chains, which are all that
-    // remains on the critical path
+    // eight samples at a time: the conversions (and squares) first,
+    // independent of each other, then the in-order add chain, which is all
+    // that remains on the critical path
+    if (!squares) {
 #pragma unroll 2
-    for (int j = 0; j < kStatsChunk / 4; j += 2) {
-      const vf4 v0 = row[j], v1 = row[j + 1];
-      const float f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-      double d[8], q[8];
+      for (int j = 0; j < kStatsChunk / 4; j += 2) {
+        const vf4 v0 = row[j], v1 = row[j + 1];
+        const float f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        double d[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) d[u] = static_cast<double>(f[u]);
+        for (int u = 0; u < 8; ++u) d[u] = static_cast<double>(f[u]);
 #pragma unroll
-      for (int u = 0; u < 8; ++u) q[u] = d[u] * d[u];
+        for (int u = 0; u < 8; ++u) acc = acc + d[u];
+      }
+    } else {
+#pragma unroll 2
+      for (int j = 0; j < kStatsChunk / 4; j += 2) {
+        const vf4 v0 = row[j], v1 = row[j + 1];
+        const float f[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+        double q[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        acc.sum = acc.sum + d[u];
-        acc.squares = acc.squares + q[u];
+        for (int u = 0; u < 8; ++u) {
+          const double d = static_cast<double>(f[u]);
+          q[u] = d * d;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = acc + q[u];
       }
     }
+    // the other buffer is rewritten after the next barrier, once both waves
+    // are done with it
   }
-  if (lane >= rows) return;
-  const float* x = base + static_cast<uint64_t>(lane) * stride;
-  for (uint64_t i = nchunks * kStatsChunk; i < n; ++i) acc.add(x[i]);  // < 128 samples
-  finish_stats(acc, n, first + lane, reject_constant, stats, flags, status);
+  // the tail (< 128 samples) of each row, then the squares join the sums
+  if (lane < rows) {
+    const float* x = base + static_cast<uint64_t>(lane) * stride;
+    for (uint64_t i = nchunks * kStatsChunk; i < n; ++i) {
+      const double d = static_cast<double>(x[i]);
+      acc = squares ? acc + d * d : acc + d;
+    }
+    if (squares) squares_of[lane] = acc;
+  }
+  __syncthreads();
+  if (squares || lane >= rows) return;
+  SerialSums sums;
+  sums.sum = acc;
+  sums.squares = squares_of[lane];
+  finish_stats(sums, n, first + lane, reject_constant, stats, flags, status);
 }
 
 // Rows that are not 16-byte aligned: each lane reads its row directly.
@@ -424,7 +453,7 @@ void run(const gaffa_hip_preprocess_args* a, void* workspace, hipStream_t st) {
                                     hipFuncAttributeMaxDynamicSharedMemorySize,
                                     static_cast<int>(kStatsLdsBytes)),
                 "hipFuncSetAttribute");
-      hipLaunchKernelGGL(stats_kernel, sg, dim3(kStatsLanes), kStatsLdsBytes, st, fdata,
+      hipLaunchKernelGGL(stats_kernel, sg, dim3(kStatsThreads), kStatsLdsBytes, st, fdata,
                          a->stride, a->nsamples, static_cast<uint32_t>(a->nseries),
                          a->reject_constant, stats, flags, a->status);
       launch_check("stats_kernel");

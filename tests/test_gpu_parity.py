@@ -245,12 +245,13 @@ def test_preprocess_vs_golden(gpu, port, name):
 
 
 def test_preprocess_batch_many_series(gpu, port):
-    say("preprocess: 24 series x 2^16, window 3907/101")
+    say("preprocess: 100 series x 2^16, window 3907/101")
     rng = np.random.default_rng(2)
-    plane = rng.integers(30000, 31000, size=(24, 1 << 16)).astype(np.uint32)
+    # two statistics workgroups of 64 series, the second one partial
+    plane = rng.integers(30000, 31000, size=(100, 1 << 16)).astype(np.uint32)
     det = gpu._preprocess_batch_gpu(plane, window_samples=3907, min_points=101, normalise=False)
     full = gpu._preprocess_batch_gpu(plane, window_samples=3907, min_points=101)
-    for i in (0, 11, 23):
+    for i in (0, 11, 63, 64, 99):
         want = port.detrend(plane[i].astype(np.float32), 3907, 101)
         np.testing.assert_array_equal(det[i], want)
         np.testing.assert_array_equal(full[i], port.normalise(want))
