@@ -13,6 +13,8 @@
 // for bit (dedispersion.cpp:197-215, 291-329, 351-453).
 #include "common.hpp"
 
+#include <type_traits>
+
 namespace gaffa_hip {
 namespace {
 
@@ -409,48 +411,61 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
     }
     __syncthreads();
 
-    // Step 3: one wave pass per (DM, up to 512 outputs).
-    const uint32_t q = min(8u * lane, chunk - 8u);  // idle lanes of a short chunk re-read
-    for (uint32_t i = wave; i < nd; i += kSbWaves) {
-      const int32_t* arow = saddr + i * nsub4;
-      uint32_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-      for (uint32_t s0 = 0; s0 < nsub; s0 += flush_every) {
-        const uint32_t s1 = min(s0 + flush_every, nsub);
-        uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
+    // Step 3: one wave pass per (DM, chunk outputs).  Lane w sums the output
+    // pairs at 2w + k (chunk / 4), k < 4, so that for each k the lanes read
+    // consecutive LDS dwords (a pair and the next dword for the odd-residual
+    // shift: one ds_read2 per k, free of bank conflicts).  Lanes past
+    // chunk / 8 in a short chunk re-read the last pair and store nothing.
+    // The full chunk (512 samples) has its pair distance as a constant, so
+    // the four reads share one address with immediate offsets.
+    auto sum_dms = [&](auto slot_v) {
+      const uint32_t slot = slot_v;  // dwords between a lane's pairs: chunk / 8
+      const bool lane_live = lane < slot;
+      const uint32_t wdw = min(lane, slot - 1u);
+      for (uint32_t i = wave; i < nd; i += kSbWaves) {
+        const int32_t* arow = saddr + i * nsub4;
+        uint32_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (uint32_t s0 = 0; s0 < nsub; s0 += flush_every) {
+          const uint32_t s1 = min(s0 + flush_every, nsub);
+          uint32_t p0 = 0, p1 = 0, p2 = 0, p3 = 0;
 #pragma unroll 4
-        for (uint32_t s = s0; s < s1; ++s) {
-          const uint32_t u = static_cast<uint32_t>(arow[s]) + q;
-          const uint32_t* r32 = win32 + (u >> 1);
-          const uint32_t shb = (u & 1u) << 1;
-          const uint32_t d0 = r32[0], d1 = r32[1], d2 = r32[2], d3 = r32[3], d4 = r32[4];
-          p0 += __builtin_amdgcn_alignbyte(d1, d0, shb);
-          p1 += __builtin_amdgcn_alignbyte(d2, d1, shb);
-          p2 += __builtin_amdgcn_alignbyte(d3, d2, shb);
-          p3 += __builtin_amdgcn_alignbyte(d4, d3, shb);
+          for (uint32_t s = s0; s < s1; ++s) {
+            const uint32_t a = static_cast<uint32_t>(arow[s]);  // window u16 index
+            const uint32_t* r32 = win32 + (a >> 1) + wdw;
+            const uint32_t shb = (a & 1u) << 1;
+            p0 += __builtin_amdgcn_alignbyte(r32[1], r32[0], shb);
+            p1 += __builtin_amdgcn_alignbyte(r32[slot + 1u], r32[slot], shb);
+            p2 += __builtin_amdgcn_alignbyte(r32[2u * slot + 1u], r32[2u * slot], shb);
+            p3 += __builtin_amdgcn_alignbyte(r32[3u * slot + 1u], r32[3u * slot], shb);
+          }
+          tot[0] += p0 & 0xFFFFu;
+          tot[1] += p0 >> 16;
+          tot[2] += p1 & 0xFFFFu;
+          tot[3] += p1 >> 16;
+          tot[4] += p2 & 0xFFFFu;
+          tot[5] += p2 >> 16;
+          tot[6] += p3 & 0xFFFFu;
+          tot[7] += p3 >> 16;
         }
-        tot[0] += p0 & 0xFFFFu;
-        tot[1] += p0 >> 16;
-        tot[2] += p1 & 0xFFFFu;
-        tot[3] += p1 >> 16;
-        tot[4] += p2 & 0xFFFFu;
-        tot[5] += p2 >> 16;
-        tot[6] += p3 & 0xFFFFu;
-        tot[7] += p3 >> 16;
-      }
-      const uint64_t t = t_out + 8u * lane;
-      if (8u * lane < chunk && t < t_end) {
-        uint32_t* o = out + static_cast<uint64_t>(dm_first + i) * out_stride + t;
-        if (t + 8 <= t_end && (out_stride & 3u) == 0 &&
-            (reinterpret_cast<uintptr_t>(out) & 15u) == 0) {
-          reinterpret_cast<uint4*>(o)[0] = make_uint4(tot[0], tot[1], tot[2], tot[3]);
-          reinterpret_cast<uint4*>(o)[1] = make_uint4(tot[4], tot[5], tot[6], tot[7]);
-        } else {
+        if (lane_live) {
+          uint32_t* o = out + static_cast<uint64_t>(dm_first + i) * out_stride;
+          const bool pairs =
+              (out_stride & 1u) == 0 && (reinterpret_cast<uintptr_t>(out) & 7u) == 0;
 #pragma unroll
-          for (int jj = 0; jj < 8; ++jj)
-            if (t + jj < t_end) o[jj] = tot[jj];
+          for (uint32_t k = 0; k < 4; ++k) {
+            const uint64_t t = t_out + 2u * lane + 2u * slot * k;
+            if (t + 2 <= t_end && pairs) {
+              *reinterpret_cast<uint2*>(o + t) = make_uint2(tot[2 * k], tot[2 * k + 1]);
+            } else {
+              if (t < t_end) o[t] = tot[2 * k];
+              if (t + 1 < t_end) o[t + 1] = tot[2 * k + 1];
+            }
+          }
         }
       }
-    }
+    };
+    if (chunk == kSbPiece) sum_dms(std::integral_constant<uint32_t, kSbPiece / 8>{});
+    else sum_dms(chunk >> 3);
     __syncthreads();
   }
 }

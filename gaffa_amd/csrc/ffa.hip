@@ -521,21 +521,11 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
             for (uint32_t j = 0; j + 1 < NC; ++j) store(orow + p0 + lane + 64u * j, sum(j));
             if (lane < live) store(orow + p0 + lane + 64u * (NC - 1), sum(NC - 1));
           };
-          gaffa_hip_ffa_merge g[R];
-#pragma unroll
-          for (uint32_t k = 0; k < R; ++k) g[k] = load_merge(m + k);
-          for (uint32_t ps = 0; ps < passes; ++ps) {
-            gaffa_hip_ffa_merge cg[R];
-#pragma unroll
-            for (uint32_t k = 0; k < R; ++k) cg[k] = g[k];
-            if (ps + 1 < passes) {
-#pragma unroll
-              for (uint32_t k = 0; k < R; ++k) g[k] = load_merge(m + (ps + 1) * R + k);
-            }
-            // every load of the pass is issued before the first add waits;
-            // a wave with fewer rows than the longest list skips its padding
-            // (the last one or two entries of its list)
-            static_assert(R == 2, "the padding cases below assume two rows per pass");
+          // every load of the pass is issued before the first add waits; a
+          // wave with fewer rows than the longest list skips its padding (the
+          // last one or two entries of its list)
+          static_assert(R == 2, "the padding cases below assume two rows per pass");
+          auto run = [&](const gaffa_hip_ffa_merge (&cg)[R]) {
             if (cg[1].out != GAFFA_HIP_FFA_SKIP_ROW) {
               float v[R][2 * NL][NC];
 #pragma unroll
@@ -549,6 +539,26 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
               __builtin_amdgcn_sched_barrier(0);
               store_row(cg[0].out, v);
             }
+          };
+          // the merges of the next pass load while this one runs, into the
+          // other of two register sets (unrolled by two, so that no set is
+          // ever copied into the other)
+          gaffa_hip_ffa_merge ga[R], gb[R];
+#pragma unroll
+          for (uint32_t k = 0; k < R; ++k) ga[k] = load_merge(m + k);
+          for (uint32_t ps = 0; ps < passes; ps += 2) {
+            const bool second = ps + 1 < passes;
+            if (second) {
+#pragma unroll
+              for (uint32_t k = 0; k < R; ++k) gb[k] = load_merge(m + (ps + 1) * R + k);
+            }
+            run(ga);
+            if (!second) break;
+            if (ps + 2 < passes) {
+#pragma unroll
+              for (uint32_t k = 0; k < R; ++k) ga[k] = load_merge(m + (ps + 2) * R + k);
+            }
+            run(gb);
           }
         };
         if (nc == 4) pass(std::integral_constant<uint32_t, 4>{});

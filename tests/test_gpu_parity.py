@@ -137,9 +137,11 @@ def test_subband_large_slice_vs_oracle(gpu, port):
     fb = Filterbank.from_array(x, tsamp=w.tsamp, frequency_mhz=w.frequencies)
     # at DM ~990 a 32-trial nominal group spans ~1700 residual samples in the
     # lowest subband; 24 trials per group leave a short last group (64 = 24 +
-    # 24 + 16)
+    # 24 + 16); 48, 56 and 60 trials per group grow the residual windows until
+    # the kernel's time chunk drops from 512 to 256, 64 and 16 samples
     top = w.dm_segments[-1]
-    for seg, npn in ((w.dm_segments[0], 32), (top, 32), (top, 24)):
+    for seg, npn in ((w.dm_segments[0], 32), (top, 32), (top, 24), (top, 48), (top, 56),
+                     (top, 60)):
         say(f"  DM {seg.dm_low:.1f} step {seg.dm_step:.4f}, {npn} trials per nominal group")
         r = dedispersion.dedisperse_subband(fb, dm_low=seg.dm_low, dm_step=seg.dm_step, ndm=64,
                                             subband_channels=64, ndm_per_nominal=npn)
