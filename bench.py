@@ -230,10 +230,14 @@ def pipeline_step(pipe, workload, net=None, threshold: float = 0.0,
     t3 = time.perf_counter()
     fold_dev_ms = pipe.fold_candidates_full(fold)[0] if fold else 0.0
     t4 = time.perf_counter()
+    # the search's full peak list is ordered on a host thread meanwhile: the
+    # step ends once that is done too
+    pipe.wait_peaks()
+    t5 = time.perf_counter()
     times.update({"search_wall_ms": 1e3 * (t1 - t0), "candidates_ms": 1e3 * (t2 - t1),
                   "score_ms": 1e3 * (t3 - t2), "fold_wall_ms": 1e3 * (t4 - t3),
-                  "fold_device_ms": fold_dev_ms, "step_ms": 1e3 * (t4 - t0),
-                  "candidates": len(cands), "folded": len(fold)})
+                  "fold_device_ms": fold_dev_ms, "peak_order_wait_ms": 1e3 * (t5 - t4),
+                  "step_ms": 1e3 * (t5 - t0), "candidates": len(cands), "folded": len(fold)})
     return times, cands, fold
 
 

@@ -1531,8 +1531,15 @@ PYBIND11_MODULE(_core, m) {
       .def_property_readonly("dms", &GpuSearchPipeline::dms)
       .def_property_readonly("ffa_plan", &GpuSearchPipeline::ffa_plan)
       .def_property_readonly("peaks", &GpuSearchPipeline::peaks)
-      .def_property_readonly("peak_count",
-                             [](const GpuSearchPipeline& p) { return p.peaks().size(); })
+      .def_property_readonly("peak_count", &GpuSearchPipeline::peak_count)
+      .def(
+          "wait_peaks",
+          [](const GpuSearchPipeline& p) {
+            py::gil_scoped_release release;
+            p.wait_peaks();
+          },
+          "Wait for the last search's peaks to be ordered (the search ends by ordering "
+          "them on a host thread)")
       .def(
           "peak_records",
           [](const GpuSearchPipeline& p, std::size_t dm_begin, std::size_t dm_end) {

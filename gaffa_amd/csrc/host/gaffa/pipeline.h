@@ -15,8 +15,10 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <exception>
 #include <memory>
 #include <span>
+#include <thread>
 #include <vector>
 
 namespace gaffa {
@@ -101,7 +103,14 @@ class GpuSearchPipeline {
   const FfaSearchPlan& ffa_plan() const noexcept { return ffa_plan_; }
   const GpuFfaProgram& ffa_program() const noexcept { return *ffa_; }
   GpuFfaProgram& ffa_program() noexcept { return *ffa_; }
-  const std::vector<DmPeak>& peaks() const noexcept { return peaks_; }
+  // The last search's peaks in is_better_dm_peak order.  The search ends by
+  // merging its per-tile sorted parts on a host thread, beside whatever the
+  // caller enqueues next (candidate selection and folds read the arrival-order
+  // records instead); peaks() and the host candidate selection wait for it.
+  const std::vector<DmPeak>& peaks() const;
+  std::size_t peak_count() const noexcept { return npeaks_; }
+  // Waits for the ordering of the last search's peaks (rethrows its error).
+  void wait_peaks() const;
   // select_candidates_cpu of the last search's peaks (observation =
   // out_nsamples * tsamp), equal candidate for candidate: on the device from
   // the compact peaks where the sort key fits (frequency clusters numbered on
@@ -159,6 +168,9 @@ class GpuSearchPipeline {
   std::unique_ptr<GpuPreprocessProgram> prep_;
   std::unique_ptr<GpuFfaProgram> ffa_;
   std::vector<DmPeak> peaks_;
+  std::size_t npeaks_ = 0;
+  mutable std::thread merge_thread_;       // orders peaks_ after a search
+  mutable std::exception_ptr merge_error_;
   std::vector<gaffa_hip_event> events_;
   bool kernel_timing_ = false;
   FilterbankStreamStats stream_stats_{};

@@ -93,6 +93,7 @@ GpuSearchPipeline::GpuSearchPipeline(SearchPipelineConfig config) : cfg_(std::mo
 }
 
 GpuSearchPipeline::~GpuSearchPipeline() {
+  if (merge_thread_.joinable()) merge_thread_.join();
   for (auto e : events_)
     if (e != nullptr) (void)gaffa_hip_event_destroy(e);
   for (auto e : tile_done_)
@@ -233,7 +234,22 @@ void GpuSearchPipeline::preprocess() {
   }
 }
 
+void GpuSearchPipeline::wait_peaks() const {
+  if (merge_thread_.joinable()) merge_thread_.join();
+  if (merge_error_) {
+    std::exception_ptr e = merge_error_;
+    merge_error_ = nullptr;
+    std::rethrow_exception(e);
+  }
+}
+
+const std::vector<DmPeak>& GpuSearchPipeline::peaks() const {
+  wait_peaks();
+  return peaks_;
+}
+
 void GpuSearchPipeline::search() {
+  wait_peaks();  // the previous search's ordering still reads the buffers below
   GpuDeviceScope scope(cfg_.device_id);
   GpuFfaProgram& prog = *ffa_;
   const float* series = reinterpret_cast<const float*>(plane_.data());
@@ -309,8 +325,17 @@ void GpuSearchPipeline::search() {
     }
     peak_keys_.resize(done);
     if (run_ends_.empty()) run_ends_.push_back(0);
-    merge_sorted_dm_peaks(std::span<const DmPeak>(expanded_.data(), done), run_ends_, peak_keys_,
-                          merge_scratch_, peaks_);
+    // the ordering reads expanded_, peak_keys_ and run_ends_ and writes
+    // peaks_; every member function that touches them waits for it first
+    npeaks_ = done;
+    merge_thread_ = std::thread([this, done] {
+      try {
+        merge_sorted_dm_peaks(std::span<const DmPeak>(expanded_.data(), done), run_ends_,
+                              peak_keys_, merge_scratch_, peaks_);
+      } catch (...) {
+        merge_error_ = std::current_exception();
+      }
+    });
     break;
   }
 }
@@ -342,7 +367,7 @@ std::vector<Candidate> GpuSearchPipeline::select_candidates_gpu(
             "Candidate selection observation_seconds must be finite and > 0");
   check_arg(o.frequency_cluster_radius >= 0.0 && std::isfinite(o.frequency_cluster_radius),
             "Candidate selection frequency_cluster_radius must be finite and >= 0");
-  const std::size_t n = peaks_.size();
+  const std::size_t n = npeaks_;
   const std::size_t nw = ffa_plan_.width_trials.size();
   if (n == 0 || nranks_ == 0 || n >= (std::size_t{1} << 32) || expanded_.size() < n ||
       ffa_->peak_capacity() < n || nw == 0 || nw >= (1u << 16))
@@ -465,6 +490,7 @@ std::vector<Candidate> GpuSearchPipeline::select_candidates_gpu(
 
 std::vector<Candidate> GpuSearchPipeline::select_candidates_host(
     const CandidateSelectionOptions& options) const {
+  wait_peaks();
   const double obs = static_cast<double>(n_out_) * cfg_.tsamp;
   const std::size_t n = peaks_.size();
   const auto& tasks = ffa_plan_.tasks;
