@@ -57,6 +57,13 @@ __host__ __device__ __forceinline__ uint32_t pick_row(uint32_t child_rows, uint3
   return static_cast<uint32_t>(scale * static_cast<float>(s) + 0.5f);
 }
 
+// base[idx] with the byte offset computed in 32 bits (idx < 2^30: a task's
+// rows * bins, which the host checks), so the load is the uniform base in
+// scalar registers plus one 32-bit lane offset.
+__device__ __forceinline__ float ld32(const float* base, uint32_t idx) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + (idx << 2));
+}
+
 // Kernel-side view of gaffa_hip_ffa_search_args: the plan descriptor is
 // copied by value (it lives in host memory on the C-ABI side).
 struct KArgs {
@@ -146,15 +153,16 @@ __device__ __forceinline__ const float* task_input(const KArgs& a, const gaffa_h
 // skipped): out_i[p] = h_i[p] + t_i[(p + roll_i) mod b], lanes over p.  Every
 // load address is clamped in bounds and issued unconditionally, so all
 // 2 * RB * 4 loads of a batch are in flight together; only the stores are
-// predicated.  `row(i, h, t, roll)` supplies row i's operands and is only
-// called with i < count (skipped rows reuse row `first`'s operands).
+// predicated.  `row(i, h, t, roll)` supplies row i's operands as element
+// offsets from `base` (< 2^30) and is only called with i < count (skipped
+// rows reuse row `first`'s operands).
 template <uint32_t RB, typename RowFn, typename OutFn>
-__device__ __forceinline__ void merge_batch(uint32_t first, uint32_t step, uint32_t count,
-                                            uint32_t b, uint32_t lane, RowFn&& row,
-                                            OutFn&& out) {
+__device__ __forceinline__ void merge_batch(const float* base, uint32_t first, uint32_t step,
+                                            uint32_t count, uint32_t b, uint32_t lane,
+                                            RowFn&& row, OutFn&& out) {
   if (first >= count) return;  // wave-uniform: no row of this batch exists
-  const float* hp[RB];
-  const float* tp[RB];
+  uint32_t hp[RB];
+  uint32_t tp[RB];
   uint32_t roll[RB];
 #pragma unroll
   for (uint32_t k = 0; k < RB; ++k) {
@@ -170,7 +178,7 @@ __device__ __forceinline__ void merge_batch(uint32_t first, uint32_t step, uint3
         const uint32_t p = min(p0 + 64u * j, b - 1);
         uint32_t q = p + roll[k];
         q = min(q, q - b);  // (p + roll) mod b for p, roll < b
-        v[k][j] = hp[k][p] + tp[k][q];
+        v[k][j] = ld32(base, hp[k] + p) + ld32(base, tp[k] + q);
       }
     }
 #pragma unroll
@@ -649,11 +657,11 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
     __syncthreads();
     constexpr uint32_t kBatch = kLevelRows / kWaves;  // rows per wave in flight
     merge_batch<kBatch>(
-        wave, kWaves, item.c, b, lane,
-        [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
+        src, wave, kWaves, item.c, b, lane,
+        [&](uint32_t i, uint32_t& h, uint32_t& t, uint32_t& roll) {
           const Desc ds = sdesc[i];
-          h = src + static_cast<uint64_t>(ds.h) * b;
-          t = src + static_cast<uint64_t>(ds.t) * b;
+          h = ds.h * b;
+          t = ds.t * b;
           roll = ds.roll;
         },
         [&](uint32_t i, uint32_t p, float v) { dst[static_cast<uint64_t>(item.b + i) * b + p] = v; });
@@ -856,11 +864,11 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
     constexpr uint32_t kBatch = 4;
     for (uint32_t i0 = wave; i0 < nrows; i0 += kBatch * kWaves) {
       merge_batch<kBatch>(
-          i0, kWaves, nrows, b, lane,
-          [&](uint32_t i, const float*& h, const float*& t, uint32_t& roll) {
+          src, i0, kWaves, nrows, b, lane,
+          [&](uint32_t i, uint32_t& h, uint32_t& t, uint32_t& roll) {
             const Desc2 ds = sdesc[i];
-            h = src + static_cast<uint64_t>(ds.a) * b;
-            t = src + static_cast<uint64_t>(ds.b) * b;
+            h = ds.a * b;
+            t = ds.b * b;
             roll = ds.rh;
           },
           [&](uint32_t i, uint32_t p, float v) { lds[static_cast<size_t>(i) * sb + p] = v; });
@@ -878,10 +886,8 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
         float v[kBatch][4];
 #pragma unroll
         for (uint32_t k = 0; k < kBatch; ++k) {
-          const float* A = src + static_cast<uint64_t>(e[k].a) * b;
-          const float* B = src + static_cast<uint64_t>(e[k].b) * b;
-          const float* C = src + static_cast<uint64_t>(e[k].c) * b;
-          const float* D = src + static_cast<uint64_t>(e[k].d) * b;
+          // 32-bit element offsets from the uniform base (see ld32)
+          const uint32_t A = e[k].a * b, B = e[k].b * b, C = e[k].c * b, D = e[k].d * b;
 #pragma unroll
           for (uint32_t j = 0; j < 4; ++j) {
             const uint32_t p = min(p0 + 64u * j, b - 1);
@@ -891,7 +897,7 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
             q = min(q, q - b);
             uint32_t qd = q + e[k].rt;
             qd = min(qd, qd - b);
-            v[k][j] = (A[p] + B[pb]) + (C[q] + D[qd]);
+            v[k][j] = (ld32(src, A + p) + ld32(src, B + pb)) + (ld32(src, C + q) + ld32(src, D + qd));
           }
         }
 #pragma unroll

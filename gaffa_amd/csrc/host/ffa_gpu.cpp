@@ -69,6 +69,8 @@ void validate_task(const FfaSearchTask& t, std::size_t n) {
   }
   check_arg(t.bins < 65536, "GPU FFA supports bins < 65536");
   check_arg(t.rows < (1ULL << 31), "GPU FFA supports rows < 2^31");
+  // the kernels address a task's rows with 32-bit byte offsets
+  check_arg(t.rows * t.bins < (1ULL << 30), "GPU FFA supports rows * bins < 2^30 per task");
 }
 
 void collect_nodes(std::uint32_t start, std::uint32_t size, std::uint32_t depth,
