@@ -775,30 +775,59 @@ __device__ __host__ __forceinline__ uint32_t terminal_stride(uint32_t bins, uint
   return (bins + maxw + 8) | 1u;  // odd: lane-per-row reads hit distinct banks
 }
 
-// max over phases of P[p + w] - P[p] for one row x (circularly extended by
-// at least w + 7 entries) with P the serial FP32 prefix; `total` = P[bins].
+// max over phases p < b of P[p + w] - P[p] for one row x (circularly
+// extended by at least w + 7 entries) with P the serial FP32 prefix;
+// `total` = P[b].  The phases go in blocks of kScanBlock, and the next
+// block's LDS reads are issued before the current block's adds: two register
+// sets, the block loop unrolled by two, so that no set is copied into the
+// other.  Blocks of 4 keep both blocks' 2 x 4 ds_read2 within the LDS
+// counter's 15 outstanding operations, which blocks of 8 exceed.
+constexpr uint32_t kScanBlock = 4;
+
 __device__ __forceinline__ float boxcar_max(const float* x, uint32_t b, uint32_t w,
                                             float& total) {
   float A = 0.0f;  // P[p]
   float B = 0.0f;  // P[p + w]
   for (uint32_t k = 0; k < w; ++k) B += x[k];
   float best = -INFINITY;
-  uint32_t p = 0;
-  for (; p + 8 <= b; p += 8) {
-    float xa[8], xb[8];
+  struct Block {
+    float a[kScanBlock], b[kScanBlock];
+  };
+  auto load = [&](Block& v, uint32_t p) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
-      xa[u] = x[p + u];
-      xb[u] = x[p + w + u];
+    for (uint32_t u = 0; u < kScanBlock; ++u) {
+      v.a[u] = x[p + u];
+      v.b[u] = x[p + w + u];
     }
+  };
+  auto step = [&](const Block& v) {
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (uint32_t u = 0; u < kScanBlock; ++u) {
       best = fmaxf(best, B - A);
-      A += xa[u];
-      B += xb[u];
+      A += v.a[u];
+      B += v.b[u];
+    }
+  };
+  const uint32_t nblocks = b / kScanBlock;
+  if (nblocks > 0) {
+    Block X, Y;
+    load(X, 0);
+    uint32_t k = 0;
+    for (; k + 2 < nblocks; k += 2) {  // X holds block k
+      load(Y, kScanBlock * (k + 1));
+      step(X);
+      load(X, kScanBlock * (k + 2));
+      step(Y);
+    }
+    if (k + 1 < nblocks) {
+      load(Y, kScanBlock * (k + 1));
+      step(X);
+      step(Y);
+    } else {
+      step(X);
     }
   }
-  for (; p < b; ++p) {
+  for (uint32_t p = kScanBlock * nblocks; p < b; ++p) {
     best = fmaxf(best, B - A);
     A += x[p];
     B += x[p + w];
@@ -858,8 +887,18 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
     }
     __syncthreads();
   }
+  // A detection row x is stored with its circular extension x[q] = x[q mod
+  // b] for q in [b, sb): x[p + b] together with x[p], and, only where
+  // sb > 2b (widths near the bin count), q >= 2b in a pass of its own.
+  auto put = [&](uint32_t i, uint32_t p, float v) {
+    float* row = lds + static_cast<size_t>(i) * sb;
+    row[p] = v;
+    if constexpr (kDetect) {
+      if (p + b < sb) row[p + b] = v;
+    }
+  };
   if (task.rows == 1) {
-    for (uint32_t p = threadIdx.x; p < b; p += kTerminalThreads) lds[p] = src[p];
+    for (uint32_t p = threadIdx.x; p < b; p += kTerminalThreads) put(0, p, src[p]);
   } else if (tl == 1) {
     constexpr uint32_t kBatch = 4;
     for (uint32_t i0 = wave; i0 < nrows; i0 += kBatch * kWaves) {
@@ -871,7 +910,7 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
             t = ds.b * b;
             roll = ds.rh;
           },
-          [&](uint32_t i, uint32_t p, float v) { lds[static_cast<size_t>(i) * sb + p] = v; });
+          [&](uint32_t i, uint32_t p, float v) { put(i, p, v); });
     }
   } else {
     // two levels: out[p] = (A[p] + B[p+rh]) + (C[q] + D[q+rt]), q = p + ro
@@ -906,7 +945,7 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
 #pragma unroll
           for (uint32_t j = 0; j < 4; ++j) {
             const uint32_t p = p0 + 64u * j;
-            if (i < nrows && p < b) lds[static_cast<size_t>(i) * sb + p] = v[k][j];
+            if (i < nrows && p < b) put(i, p, v[k][j]);
           }
         }
       }
@@ -919,13 +958,13 @@ ffa_terminal_kernel(KArgs a, float* __restrict__ rows_out, uint32_t only_task) {
         rows_out[(sidx * task.rows_eval + s0 + i) * b + p] = lds[static_cast<size_t>(i) * sb + p];
     return;
   } else {
-    // circular extension [b, sb) = x[(q - b) mod b]
-    for (uint32_t i = wave; i < nrows; i += kWaves) {
-      float* row = lds + static_cast<size_t>(i) * sb;
-      for (uint32_t q = b + lane; q < sb; q += 64) row[q] = row[(q - b) % b];
+    if (sb > 2 * b) {  // uniform: the rest of the extension, from [0, b)
+      for (uint32_t i = wave; i < nrows; i += kWaves) {
+        float* row = lds + static_cast<size_t>(i) * sb;
+        for (uint32_t q = 2 * b + lane; q < sb; q += 64) row[q] = row[q % b];
+      }
+      __syncthreads();
     }
-    __syncthreads();
-
     // One lane per (row, width) of this item: rows fastest.
     const uint32_t npairs = nrows * task.nwidths;
     for (uint32_t e = threadIdx.x; e < npairs; e += kTerminalThreads) {
