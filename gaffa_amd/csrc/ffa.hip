@@ -31,6 +31,7 @@
 // stream whole rows (lanes over phase bins) reading only those.
 #include "common.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <type_traits>
 
@@ -604,170 +605,239 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
 
 // --------------------------------------------------------------- level ----
 
-// Rows [first, first + count) of the depth-d nodes of a task (item.a = d |
-// levels << 16).  One level: merged from the depth d + 1 rows.  Two or three
-// levels (Desc3 below):
-// merged straight from the depth d + 2 rows, each output phase p summed as
-// (A[p] + B[p + rh]) + (C[q] + D[q + rt]) with q = p + ro (indices mod b),
-// which is exactly the two one-level merges (the depth d + 1 values are
-// FP32 sums in between), without writing the depth d + 1 rows.  The work
-// region of each depth comes from the task's region bits.
+// One-level merge operands of a depth-d row for the two-level detection
+// merge: (A[p] + B[p + rh]) + (C[q] + D[q + rt]) with q = p + ro (indices
+// mod b), which is exactly the two one-level merges (the depth d + 1 values
+// are FP32 sums in between), without writing the depth d + 1 rows.
 struct Desc2 {
   uint32_t a, b, rh;  // head child row: its two inputs and roll
   uint32_t c, d, rt;  // tail child row: its two inputs and roll
   uint32_t ro;        // roll of the tail child row in the output merge
 };
 
-// Three levels: the eight depth-(d + 3) rows of an output row and the
-// combined roll of each (mod b, the first is 0):
-//   out[p] = ((r0[p] + r1[p+R1]) + (r2[p+R2] + r3[p+R3]))
-//          + ((r4[p+R4] + r5[p+R5]) + (r6[p+R6] + r7[p+R7])),
-// the FP32 sums of three one-level merges in their order.
-struct Desc3 {
+// A level pass of L = 1, 2 or 3 tree levels merges a depth-d row r straight
+// from the 2^L depth-(d + L) rows on its paths down (its leaves), each at the
+// combined roll R_j of its path (the sum of the rolls along it, mod b), in
+// the order of the L one-level merges:
+//   L = 1: v0 + v1
+//   L = 2: (v0 + v1) + (v2 + v3)
+//   L = 3: ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7))
+// with v_j = leaf_j[(p + R_j) mod b]; the depth d + 1 .. d + L - 1 values are
+// the FP32 sums in between and are never written.
+struct Leaves {
   uint32_t row[8];
-  uint32_t roll[7];
+  uint32_t roll[8];  // roll[0] = 0
 };
 
+__host__ __device__ __forceinline__ Leaves level_leaves(uint32_t task_rows, uint32_t depth,
+                                                        uint32_t levels, uint32_t r, uint32_t b) {
+  Leaves e{};
+  const Node nd = find_node(0, task_rows, depth, r);
+  const Desc top = merge_desc(nd, r, b);
+  if (levels == 1) {
+    e.row[0] = top.h;
+    e.row[1] = top.t;
+    e.roll[1] = top.roll;
+    return e;
+  }
+  const uint32_t hn = nd.size / 2;
+  const Node nh{nd.start, hn}, nt{nd.start + hn, nd.size - hn};
+  const Desc dh = merge_desc(nh, top.h, b);
+  const Desc dt = merge_desc(nt, top.t, b);
+  if (levels == 2) {
+    e.row[0] = dh.h;
+    e.row[1] = dh.t;
+    e.row[2] = dt.h;
+    e.row[3] = dt.t;
+    e.roll[1] = dh.roll;
+    e.roll[2] = top.roll;
+    e.roll[3] = (top.roll + dt.roll) % b;
+    return e;
+  }
+  const Desc da = merge_desc(Node{nh.start, nh.size / 2}, dh.h, b);
+  const Desc db = merge_desc(Node{nh.start + nh.size / 2, nh.size - nh.size / 2}, dh.t, b);
+  const Desc dc = merge_desc(Node{nt.start, nt.size / 2}, dt.h, b);
+  const Desc dd = merge_desc(Node{nt.start + nt.size / 2, nt.size - nt.size / 2}, dt.t, b);
+  e.row[0] = da.h;
+  e.row[1] = da.t;
+  e.row[2] = db.h;
+  e.row[3] = db.t;
+  e.row[4] = dc.h;
+  e.row[5] = dc.t;
+  e.row[6] = dd.h;
+  e.row[7] = dd.t;
+  e.roll[1] = da.roll;
+  e.roll[2] = dh.roll;
+  e.roll[3] = (dh.roll + db.roll) % b;
+  e.roll[4] = top.roll;
+  e.roll[5] = (top.roll + dc.roll) % b;
+  e.roll[6] = (top.roll + dt.roll) % b;
+  e.roll[7] = (top.roll + dt.roll + dd.roll) % b;
+  return e;
+}
+
+// Floats of slack after a level item's staged rows: the last chunk of a row
+// reads up to 255 floats past it (lanes beyond the row, whose stores are
+// masked).
+constexpr uint32_t kLevelGuard = 256;
+// Dynamic LDS a level launch may take: the LDS less the kernel's static
+// descriptors (kLevelRowsMax leaf sets and the runs).
+constexpr uint32_t kMaxLevelLdsBytes = kMaxLdsBytes - 4096;
+
+// Dynamic LDS an item of `count` rows from `first` needs to merge from LDS:
+// for each leaf j, the run of depth-(d + L) rows from the smallest to the
+// largest leaf_j over the item's rows (one short run per leaf when the item
+// lies in one node), all runs back to back, plus the guard.
+__host__ __device__ __forceinline__ uint64_t level_stage_bytes(uint32_t staged_rows, uint32_t b) {
+  return (static_cast<uint64_t>(staged_rows) * b + kLevelGuard) * 4u;
+}
+
+// Rows [item.b, item.b + item.c) of the depth-d nodes of a task (item.a = d
+// | L << 16).  The item's leaf runs are copied into LDS once
+// (global_load_lds: 64 consecutive floats per instruction, straight into
+// LDS), and each output phase then reads its 2^L operands there; a wave
+// merges one row at a time, four 64-phase chunks per block.  An item whose
+// runs exceed the launch's LDS (plan.level_lds_bytes) reads its operands
+// from global memory instead, with the same sums.  The work region of each
+// depth comes from the task's region bits.
 __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
-  __shared__ Desc sdesc[kLevelRowsMax];
-  __shared__ Desc2 sdesc2[kLevelRowsMax];
-  __shared__ Desc3 sdesc3[kLevelRowsMax];
+  extern __shared__ float stage[];
+  __shared__ Leaves sleaf[kLevelRowsMax];
+  __shared__ uint32_t srun[2][8];  // first row and row count of each leaf's run
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
-  if (item.c > kLevelRowsMax) return;  // the host builds items within the descriptor arrays
+  const uint32_t levels = item.a >> 16;
+  // the host builds items within the descriptor arrays and level counts
+  if (item.c == 0 || item.c > kLevelRowsMax || levels < 1 || levels > 3) return;
   const gaffa_hip_ffa_task& task = plan.tasks[item.task];
   const uint64_t sidx = blockIdx.y;
   const uint32_t b = task.bins;
   const uint32_t depth = item.a & 0xFFFFu;
-  const uint32_t levels = item.a >> 16;
-  const uint32_t lane = threadIdx.x & 63u;
-  const uint32_t wave = threadIdx.x >> 6;
+  const uint32_t nleaf = 1u << levels;
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63u;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr uint32_t kWaves = kLevelThreads / 64;
   const uint64_t region = a.nseries * plan.work_stride;
   const float* src = a.work + static_cast<uint64_t>((task.regions >> (depth + levels)) & 1u) * region +
                      sidx * plan.work_stride + task.work_offset;
   float* dst = a.work + static_cast<uint64_t>((task.regions >> depth) & 1u) * region +
                sidx * plan.work_stride + task.work_offset;
-  constexpr uint32_t kWaves = kLevelThreads / 64;
-  if (levels == 1) {
-    if (threadIdx.x < item.c) {
-      const uint32_t r = item.b + threadIdx.x;
-      sdesc[threadIdx.x] = merge_desc(find_node(0, task.rows, depth, r), r, b);
+  if (tid < item.c) sleaf[tid] = level_leaves(task.rows, depth, levels, item.b + tid, b);
+  __syncthreads();
+  if (tid < nleaf) {
+    uint32_t lo = sleaf[0].row[tid], hi = lo;
+    for (uint32_t i = 1; i < item.c; ++i) {
+      const uint32_t r = sleaf[i].row[tid];
+      lo = min(lo, r);
+      hi = max(hi, r);
     }
-    __syncthreads();
-    constexpr uint32_t kBatch = kLevelRows / kWaves;  // rows per wave in flight
-    merge_batch<kBatch>(
-        src, wave, kWaves, item.c, b, lane,
-        [&](uint32_t i, uint32_t& h, uint32_t& t, uint32_t& roll) {
-          const Desc ds = sdesc[i];
-          h = ds.h * b;
-          t = ds.t * b;
-          roll = ds.roll;
-        },
-        [&](uint32_t i, uint32_t p, float v) { dst[static_cast<uint64_t>(item.b + i) * b + p] = v; });
-    return;
-  }
-  if (levels == 3) {
-    // the host schedules three levels only where every node at depths d + 1
-    // and d + 2 has two rows or more
-    if (threadIdx.x < item.c) {
-      const uint32_t r = item.b + threadIdx.x;
-      const Node nd = find_node(0, task.rows, depth, r);
-      const Desc top = merge_desc(nd, r, b);
-      const Node nh{nd.start, nd.size / 2}, nt{nd.start + nd.size / 2, nd.size - nd.size / 2};
-      const Desc dh = merge_desc(nh, top.h, b);
-      const Desc dt = merge_desc(nt, top.t, b);
-      const Desc da = merge_desc(Node{nh.start, nh.size / 2}, dh.h, b);
-      const Desc db = merge_desc(Node{nh.start + nh.size / 2, nh.size - nh.size / 2}, dh.t, b);
-      const Desc dc = merge_desc(Node{nt.start, nt.size / 2}, dt.h, b);
-      const Desc dd = merge_desc(Node{nt.start + nt.size / 2, nt.size - nt.size / 2}, dt.t, b);
-      auto mod = [&](uint32_t x) { return x % b; };
-      Desc3 e;
-      e.row[0] = da.h; e.row[1] = da.t; e.row[2] = db.h; e.row[3] = db.t;
-      e.row[4] = dc.h; e.row[5] = dc.t; e.row[6] = dd.h; e.row[7] = dd.t;
-      e.roll[0] = da.roll;
-      e.roll[1] = dh.roll;
-      e.roll[2] = mod(dh.roll + db.roll);
-      e.roll[3] = top.roll;
-      e.roll[4] = mod(top.roll + dc.roll);
-      e.roll[5] = mod(top.roll + dt.roll);
-      e.roll[6] = mod(top.roll + dt.roll + dd.roll);
-      sdesc3[threadIdx.x] = e;
-    }
-    __syncthreads();
-    // one row per wave at a time, four phases per lane: 32 loads in flight
-    for (uint32_t i = wave; i < item.c; i += kWaves) {
-      const Desc3 e = sdesc3[i];
-      const float* r[8];
-#pragma unroll
-      for (uint32_t k = 0; k < 8; ++k) r[k] = src + static_cast<uint64_t>(e.row[k]) * b;
-      for (uint32_t p0 = lane; p0 < b; p0 += 256) {
-        float v[4];
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          const uint32_t p = min(p0 + 64u * j, b - 1);
-          uint32_t x[7];
-#pragma unroll
-          for (uint32_t k = 0; k < 7; ++k) {
-            x[k] = p + e.roll[k];
-            x[k] = min(x[k], x[k] - b);  // (p + roll) mod b for p, roll < b
-          }
-          v[j] = ((r[0][p] + r[1][x[0]]) + (r[2][x[1]] + r[3][x[2]])) +
-                 ((r[4][x[3]] + r[5][x[4]]) + (r[6][x[5]] + r[7][x[6]]));
-        }
-#pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          const uint32_t p = p0 + 64u * j;
-          if (p < b) dst[static_cast<uint64_t>(item.b + i) * b + p] = v[j];
-        }
-      }
-    }
-    return;
-  }
-  if (threadIdx.x < item.c) {
-    const uint32_t r = item.b + threadIdx.x;
-    const Node nd = find_node(0, task.rows, depth, r);
-    const Desc top = merge_desc(nd, r, b);
-    const uint32_t hn = nd.size / 2;
-    const Desc dh = merge_desc(Node{nd.start, hn}, top.h, b);
-    const Desc dt = merge_desc(Node{nd.start + hn, nd.size - hn}, top.t, b);
-    sdesc2[threadIdx.x] = Desc2{dh.h, dh.t, dh.roll, dt.h, dt.t, dt.roll, top.roll};
+    srun[0][tid] = lo;
+    srun[1][tid] = hi - lo + 1;
   }
   __syncthreads();
-  // two rows per wave at a time, four phases per lane: 32 loads in flight
-  for (uint32_t i0 = wave; i0 < item.c; i0 += 2 * kWaves) {
-    const uint32_t i1 = min(i0 + kWaves, item.c - 1);  // duplicate of a valid row
-    const Desc2 e[2] = {sdesc2[i0], sdesc2[i1]};
-    for (uint32_t p0 = lane; p0 < b; p0 += 256) {
-      float v[2][4];
+  uint32_t run_lo[8], run_off[8];  // a leaf's first row, and its run's first staged row
+  uint32_t staged = 0;
 #pragma unroll
-      for (uint32_t k = 0; k < 2; ++k) {
-        const float* A = src + static_cast<uint64_t>(e[k].a) * b;
-        const float* B = src + static_cast<uint64_t>(e[k].b) * b;
-        const float* C = src + static_cast<uint64_t>(e[k].c) * b;
-        const float* D = src + static_cast<uint64_t>(e[k].d) * b;
+  for (uint32_t j = 0; j < 8; ++j) {
+    run_lo[j] = 0;
+    run_off[j] = 0;
+    if (j < nleaf) {
+      run_lo[j] = __builtin_amdgcn_readfirstlane(srun[0][j]);
+      run_off[j] = staged;
+      staged += __builtin_amdgcn_readfirstlane(srun[1][j]);
+    }
+  }
+  const bool in_lds = level_stage_bytes(staged, b) <= plan.level_lds_bytes;
+  if (in_lds) {
 #pragma unroll
-        for (uint32_t j = 0; j < 4; ++j) {
-          const uint32_t p = min(p0 + 64u * j, b - 1);
-          uint32_t pb = p + e[k].rh;
-          pb = min(pb, pb - b);
-          uint32_t q = p + e[k].ro;
-          q = min(q, q - b);
-          uint32_t qd = q + e[k].rt;
-          qd = min(qd, qd - b);
-          v[k][j] = (A[p] + B[pb]) + (C[q] + D[qd]);
+    for (uint32_t j = 0; j < 8; ++j) {
+      if (j >= nleaf) break;
+      const float* g = src + static_cast<uint64_t>(run_lo[j]) * b;
+      float* l = stage + run_off[j] * b;
+      const uint32_t count = (j + 1 < nleaf ? run_off[j + 1] : staged) - run_off[j];
+      const uint32_t total = count * b;
+      for (uint32_t c0 = wave * 64u; c0 < total; c0 += kLevelThreads)
+        if (c0 + lane < total) __builtin_amdgcn_global_load_lds(g + c0 + lane, l + c0, 4, 0, 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the LDS writes of global_load_lds count in vmcnt
+    __syncthreads();
+  }
+
+  // out[p] for the row's phases, four 64-phase chunks per block.  From LDS,
+  // each operand picks one of two addresses per chunk (before or after its
+  // wrap point b - R_j) with one compare and one select, plus the chunk's
+  // immediate offset; lanes past the row read inside the next staged row or
+  // the guard.  From global memory, phases are clamped to the row and
+  // wrapped one by one (the work region has no guard).
+  auto merge_rows = [&](auto nlv, auto from_lds) {
+    constexpr uint32_t NL = 1u << decltype(nlv)::value;
+    constexpr bool kLds = decltype(from_lds)::value;
+    using LdsFloat = __attribute__((address_space(3))) const float;
+    for (uint32_t i = wave; i < item.c; i += kWaves) {
+      LdsFloat* pa[NL];
+      LdsFloat* pb[NL];
+      const float* rp[NL];
+      uint32_t roll[NL];
+#pragma unroll
+      for (uint32_t j = 0; j < NL; ++j) {
+        roll[j] = __builtin_amdgcn_readfirstlane(sleaf[i].roll[j]);
+        const uint32_t row = __builtin_amdgcn_readfirstlane(sleaf[i].row[j]);
+        if constexpr (kLds) {
+          pa[j] = (LdsFloat*)(stage + (run_off[j] + row - run_lo[j]) * b + roll[j] + lane);
+          pb[j] = pa[j] - b;
+        } else {
+          rp[j] = src + static_cast<uint64_t>(row) * b;
         }
       }
+      float* out = dst + static_cast<uint64_t>(item.b + i) * b;
+      for (uint32_t p0 = 0; p0 < b; p0 += 256) {
+        float v[NL][4];
 #pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) {
-        const uint32_t p = p0 + 64u * j;
-        if (p < b) {
-          dst[static_cast<uint64_t>(item.b + i0) * b + p] = v[0][j];
-          if (i0 + kWaves < item.c) dst[static_cast<uint64_t>(item.b + i1) * b + p] = v[1][j];
+        for (uint32_t j = 0; j < NL; ++j) {
+          if constexpr (kLds) {
+            const uint32_t wrap = b - roll[j];
+            const uint32_t before = wrap - min(wrap, p0);  // phases of the block before the wrap
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+              LdsFloat* ptr = lane + 64u * k < before ? pa[j] : pb[j];
+              v[j][k] = ptr[p0 + 64u * k];
+            }
+          } else {
+#pragma unroll
+            for (uint32_t k = 0; k < 4; ++k) {
+              const uint32_t p = min(p0 + 64u * k + lane, b - 1);
+              uint32_t x = p + roll[j];
+              x = min(x, x - b);  // (p + R_j) mod b for p, R_j < b
+              v[j][k] = rp[j][x];
+            }
+          }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          float sum;
+          if constexpr (NL == 2) {
+            sum = v[0][k] + v[1][k];
+          } else if constexpr (NL == 4) {
+            sum = (v[0][k] + v[1][k]) + (v[2][k] + v[3][k]);
+          } else {
+            sum = ((v[0][k] + v[1][k]) + (v[2][k] + v[3][k])) +
+                  ((v[4][k] + v[5][k]) + (v[6][k] + v[7][k]));
+          }
+          const uint32_t p = p0 + 64u * k + lane;
+          if (p < b) out[p] = sum;
         }
       }
     }
-  }
+  };
+  auto merge = [&](auto from_lds) {
+    if (levels == 1) merge_rows(std::integral_constant<uint32_t, 1>{}, from_lds);
+    else if (levels == 2) merge_rows(std::integral_constant<uint32_t, 2>{}, from_lds);
+    else merge_rows(std::integral_constant<uint32_t, 3>{}, from_lds);
+  };
+  if (in_lds) merge(std::true_type{});
+  else merge(std::false_type{});
 }
 
 // ------------------------------------------------------------ terminal ----
@@ -1024,6 +1094,7 @@ void validate(const gaffa_hip_ffa_search_args* a) {
         "FFA periodogram width count must be in 1..16");
   check(p.max_bins > 1 && p.max_bins < 65536, "FFA bins must be in (1, 65536)");
   check(p.terminal_lds_bytes <= kMaxLdsBytes, "FFA detection tile exceeds LDS");
+  check(p.level_lds_bytes <= kMaxLevelLdsBytes, "FFA level staging exceeds LDS");
   check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");
   check(p.nsubtree_items == 0 ||
             (p.subtree_half_floats > 0 &&
@@ -1043,9 +1114,10 @@ namespace {
 
 // Raises a kernel's dynamic-LDS limit to at least `bytes`, never lowering
 // it: launches captured into a graph replay under the largest limit any of
-// them needed.  Per device, for the two LDS-heavy kernels.
+// them needed.  Per device and kernel (slot): subtree, detection, transform
+// rows, level.
 void ensure_dynamic_lds(const void* kernel, int slot, uint32_t bytes) {
-  static uint32_t set[64][3] = {};
+  static uint32_t set[64][4] = {};
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   check(dev >= 0 && dev < 64, "device ordinal out of range");
@@ -1119,8 +1191,9 @@ void launch_levels(const gaffa_hip_ffa_search_args* a, hipStream_t st) {
     const uint32_t lo = p.level_item_offsets[d];
     const uint32_t hi = p.level_item_offsets[d + 1];
     if (hi <= lo) continue;
-    hipLaunchKernelGGL(ffa_level_kernel, dim3(hi - lo, ns), dim3(kLevelThreads), 0, st, kargs(a),
-                       lo);
+    ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_level_kernel), 3, p.level_lds_bytes);
+    hipLaunchKernelGGL(ffa_level_kernel, dim3(hi - lo, ns), dim3(kLevelThreads), p.level_lds_bytes,
+                       st, kargs(a), lo);
     launch_check("ffa_level_kernel");
   }
 }
@@ -1206,13 +1279,15 @@ int gaffa_hip_ffa_transform_rows(const gaffa_hip_ffa_search_args* a, uint32_t ta
 int gaffa_hip_ffa_reserve_launch(const gaffa_hip_ffa_plan_desc* p) {
   return guarded([&] {
     check(p != nullptr, "FFA plan must not be null");
-    check(p->subtree_lds_bytes <= kMaxLdsBytes && p->terminal_lds_bytes <= kMaxLdsBytes,
+    check(p->subtree_lds_bytes <= kMaxLdsBytes && p->terminal_lds_bytes <= kMaxLdsBytes &&
+              p->level_lds_bytes <= kMaxLevelLdsBytes,
           "FFA plan exceeds LDS");
     if (p->nsubtree_items > 0)
       ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_subtree_kernel), 0,
                          p->subtree_lds_bytes);
     ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_terminal_kernel<true>), 1,
                        p->terminal_lds_bytes);
+    ensure_dynamic_lds(reinterpret_cast<const void*>(ffa_level_kernel), 3, p->level_lds_bytes);
     (void)compute_units();
   });
 }
@@ -1225,6 +1300,25 @@ uint32_t gaffa_hip_ffa_terminal_rows(void) { return kTerminalRows; }
 uint32_t gaffa_hip_ffa_level_rows(void) { return kLevelRows; }
 uint32_t gaffa_hip_ffa_terminal_rows_max(void) { return kTerminalRowsMax; }
 uint32_t gaffa_hip_ffa_level_rows_max(void) { return kLevelRowsMax; }
+
+uint64_t gaffa_hip_ffa_level_lds_bytes(uint32_t task_rows, uint32_t bins, uint32_t depth,
+                                       uint32_t levels, uint32_t first, uint32_t count) {
+  if (levels < 1 || levels > 3 || bins < 2 || count == 0 || count > kLevelRowsMax ||
+      task_rows < 2 || first >= task_rows || count > task_rows - first)
+    return ~uint64_t{0};
+  const uint32_t nleaf = 1u << levels;
+  uint32_t lo[8], hi[8];
+  for (uint32_t i = 0; i < count; ++i) {
+    const Leaves e = level_leaves(task_rows, depth, levels, first + i, bins);
+    for (uint32_t j = 0; j < nleaf; ++j) {
+      lo[j] = i == 0 ? e.row[j] : std::min(lo[j], e.row[j]);
+      hi[j] = i == 0 ? e.row[j] : std::max(hi[j], e.row[j]);
+    }
+  }
+  uint32_t staged = 0;
+  for (uint32_t j = 0; j < nleaf; ++j) staged += hi[j] - lo[j] + 1;
+  return level_stage_bytes(staged, bins);
+}
 
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {
   return kTerminalRows * terminal_stride(bins, max_width) * 4u;

@@ -1585,6 +1585,7 @@ PYBIND11_MODULE(_core, m) {
             d["terminal_items"] = e.terminal_items().size();
             d["subtree_lds_bytes"] = e.subtree_lds_bytes();
             d["terminal_lds_bytes"] = e.terminal_lds_bytes();
+            d["level_lds_bytes"] = e.level_lds_bytes();
             return d;
           },
           "Device layout of the FFA plan: LDS depth per task, launches and items");

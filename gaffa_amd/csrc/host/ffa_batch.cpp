@@ -142,6 +142,7 @@ void GpuFfaBatchedSearch::build_tiles() {
     d.subtree_half_floats = e.subtree_half_floats();
     d.max_subtree_desc = e.max_subtree_desc();
     d.terminal_lds_bytes = e.terminal_lds_bytes();
+    d.level_lds_bytes = e.level_lds_bytes();
     d.prep_blocks = e.prep_blocks();
     d.max_prepared_nsamples = e.max_prepared();
     d.prepared_stride = e.prepared_stride();

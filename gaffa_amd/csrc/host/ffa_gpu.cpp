@@ -16,6 +16,8 @@ namespace gaffa {
 namespace {
 
 constexpr std::size_t kAlign = 64;               // floats
+// Most dynamic LDS a level launch is given: two workgroups per CU at least.
+constexpr std::uint64_t kLevelLdsCap = 64 * 1024;
 constexpr std::size_t kSubtreeMaxRows = 512;
 constexpr std::size_t kSubtreeLdsBytes = 160 * 1024;
 
@@ -261,9 +263,23 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
         const std::uint32_t levels = rem == 4 ? 2u : std::min<std::uint32_t>(rem, 3u);
         const std::uint32_t dout = din - levels;
         regions |= (1u - ((regions >> din) & 1u)) << dout;
-        for (std::uint32_t r = 0; r < d.rows; r += level_rows)
-          by_depth_[dout].push_back(gaffa_hip_ffa_item{task_index, dout | (levels << 16), r,
-                                                       std::min(level_rows, d.rows - r)});
+        // items of up to level_rows rows inside one depth-dout node, so that
+        // each leaf of an item's rows is one short run of rows; the level
+        // launch gets the LDS of the largest item within kLevelLdsCap, and
+        // items beyond it merge from global memory
+        std::vector<gaffa_hip_ffa_item> nodes;
+        collect_nodes(0, d.rows, 0, dout, task_index, nodes);
+        for (const gaffa_hip_ffa_item& nd : nodes) {
+          for (std::uint32_t r = nd.a; r < nd.a + nd.b; r += level_rows) {
+            const std::uint32_t count = std::min(level_rows, nd.a + nd.b - r);
+            by_depth_[dout].push_back(
+                gaffa_hip_ffa_item{task_index, dout | (levels << 16), r, count});
+            const std::uint64_t need =
+                gaffa_hip_ffa_level_lds_bytes(d.rows, d.bins, dout, levels, r, count);
+            if (need <= kLevelLdsCap)
+              level_lds_bytes_ = std::max(level_lds_bytes_, static_cast<std::uint32_t>(need));
+          }
+        }
         din = dout;
       }
       d.regions = regions;
@@ -352,6 +368,7 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.subtree_half_floats = exec_plan_.subtree_half_floats();
   desc_.max_subtree_desc = exec_plan_.max_subtree_desc();
   desc_.terminal_lds_bytes = exec_plan_.terminal_lds_bytes();
+  desc_.level_lds_bytes = exec_plan_.level_lds_bytes();
   desc_.prep_blocks = exec_plan_.prep_blocks();
   desc_.prep_fused_halo = exec_plan_.prep_fused_halo();
   desc_.prep_input_nsamples = exec_plan_.input_nsamples();

@@ -92,6 +92,7 @@ class GpuFfaExecutionPlan {
   std::uint32_t subtree_half_floats() const noexcept { return subtree_half_; }
   std::uint32_t max_subtree_desc() const noexcept { return subtree_desc_; }
   std::uint32_t terminal_lds_bytes() const noexcept { return terminal_lds_bytes_; }
+  std::uint32_t level_lds_bytes() const noexcept { return level_lds_bytes_; }
   std::uint32_t prep_blocks() const noexcept { return prep_blocks_; }
   // > 0 when one pass per series serves every downsample group (all read
   // the one series): samples a window reaches past its start.
@@ -117,6 +118,7 @@ class GpuFfaExecutionPlan {
   std::uint32_t subtree_half_ = 0;
   std::uint32_t subtree_desc_ = 0;
   std::uint32_t terminal_lds_bytes_ = 0;
+  std::uint32_t level_lds_bytes_ = 0;
   std::uint32_t prep_blocks_ = 0;
   std::uint32_t prep_fused_halo_ = 0;
   std::uint32_t max_bins_ = 0;

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define GAFFA_HIP_ABI_VERSION 2
+#define GAFFA_HIP_ABI_VERSION 3
 
 typedef enum gaffa_hip_status {
   GAFFA_HIP_OK = 0,
@@ -411,6 +411,11 @@ typedef struct gaffa_hip_ffa_plan_desc {
   uint32_t subtree_half_floats; /* max over subtree items of rows * bins */
   uint32_t max_subtree_desc;    /* max over subtree items of merges per item */
   uint32_t terminal_lds_bytes; /* max over tasks of ..._terminal_lds_bytes() */
+  uint32_t level_lds_bytes;    /* dynamic LDS of a level launch: items that need
+                                  at most this much (..._level_lds_bytes) merge
+                                  from LDS, the others from global memory (0:
+                                  every item from global memory); at most
+                                  155648 */
   uint32_t prep_blocks;        /* max over downsampled groups of
                                   ceil(prepared_nsamples / outputs per block) */
   uint32_t prep_fused_halo;    /* > 0: every downsampled group reads the one
@@ -444,6 +449,13 @@ uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width);
 uint32_t gaffa_hip_ffa_terminal_lds_bytes_rows(uint32_t bins, uint32_t max_width, uint32_t rows);
 uint32_t gaffa_hip_ffa_terminal_rows_max(void);
 uint32_t gaffa_hip_ffa_level_rows_max(void);
+/* LDS bytes a level work item of `count` rows from `first` (depth `depth` of
+ * a task of `task_rows` rows and `bins` bins, merged over `levels` = 1..3
+ * levels) needs to merge from LDS: its leaf rows' runs plus a guard
+ * (UINT64_MAX for invalid arguments).  No gaffa counterpart (gaffa merges
+ * every level from global memory, ffa_cuda_program.cuh:230-336). */
+uint64_t gaffa_hip_ffa_level_lds_bytes(uint32_t task_rows, uint32_t bins, uint32_t depth,
+                                       uint32_t levels, uint32_t first, uint32_t count);
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins);
 /* Fills the merge table of a subtree node of `rows` rows (host memory; a
  * null `out` only counts) with the kernels' own row-selection arithmetic

@@ -28,7 +28,7 @@ def test_c_abi_exports_every_declared_symbol():
     missing = [n for n in declared_functions() if not hasattr(lib, n)]
     assert missing == []
     lib.gaffa_hip_abi_version.restype = ctypes.c_int
-    assert lib.gaffa_hip_abi_version() == 2
+    assert lib.gaffa_hip_abi_version() == 3
 
 
 def test_c_abi_rejects_bad_arguments_without_a_device():
