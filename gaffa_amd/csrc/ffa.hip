@@ -38,8 +38,8 @@
 namespace gaffa_hip {
 namespace {
 
-constexpr uint32_t kLevelThreads = 256;
-constexpr uint32_t kLevelRows = 16;     // default rows per level work item
+constexpr uint32_t kLevelThreads = 512;
+constexpr uint32_t kLevelRows = 32;     // default rows per level work item
 constexpr uint32_t kLevelRowsMax = 32;  // most rows a level item may hold
 constexpr uint32_t kSubtreeThreads = 1024;
 constexpr uint32_t kSubtreeMaxRows = 512;  // rows of a subtree node
@@ -704,7 +704,7 @@ __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
   extern __shared__ float stage[];
   __shared__ Leaves sleaf[kLevelRowsMax];
-  __shared__ uint32_t srun[2][8];  // first row and row count of each leaf's run
+  __shared__ uint32_t srun[2][8];  // first and last row of each leaf's run
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
   const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
   const uint32_t levels = item.a >> 16;
@@ -725,16 +725,17 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
   float* dst = a.work + static_cast<uint64_t>((task.regions >> depth) & 1u) * region +
                sidx * plan.work_stride + task.work_offset;
   if (tid < item.c) sleaf[tid] = level_leaves(task.rows, depth, levels, item.b + tid, b);
+  if (tid < 8) {
+    srun[0][tid] = 0xFFFFFFFFu;
+    srun[1][tid] = 0;
+  }
   __syncthreads();
-  if (tid < nleaf) {
-    uint32_t lo = sleaf[0].row[tid], hi = lo;
-    for (uint32_t i = 1; i < item.c; ++i) {
-      const uint32_t r = sleaf[i].row[tid];
-      lo = min(lo, r);
-      hi = max(hi, r);
-    }
-    srun[0][tid] = lo;
-    srun[1][tid] = hi - lo + 1;
+  // each leaf's run: smallest and largest row over the item's rows, one
+  // (row, leaf) per thread
+  if (tid < item.c * nleaf) {
+    const uint32_t r = sleaf[tid >> levels].row[tid & (nleaf - 1)];
+    atomicMin(&srun[0][tid & (nleaf - 1)], r);
+    atomicMax(&srun[1][tid & (nleaf - 1)], r);
   }
   __syncthreads();
   uint32_t run_lo[8], run_off[8];  // a leaf's first row, and its run's first staged row
@@ -746,7 +747,7 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
     if (j < nleaf) {
       run_lo[j] = __builtin_amdgcn_readfirstlane(srun[0][j]);
       run_off[j] = staged;
-      staged += __builtin_amdgcn_readfirstlane(srun[1][j]);
+      staged += __builtin_amdgcn_readfirstlane(srun[1][j]) - run_lo[j] + 1;
     }
   }
   const bool in_lds = level_stage_bytes(staged, b) <= plan.level_lds_bytes;
