@@ -221,7 +221,7 @@ transpose_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples, uint64_t n
 // of one window need no barrier between them.  Workgroups are ordered span-
 // major and group-minor over the XCDs, so the groups running together on one
 // XCD read the same stretch of the channel rows.
-constexpr int kSbThreads = 512;
+constexpr int kSbThreads = 1024;    // 16 waves: a workgroup's LDS windows allow one or two per CU
 constexpr uint32_t kSbWaves = kSbThreads / 64;
 constexpr uint32_t kSbPiece = 512;   // intermediate samples per wave pass (64 lanes x 8)
 constexpr int kSbUnroll = 16;        // channels whose loads are in flight together

@@ -206,6 +206,7 @@ constexpr int kStatsThreads = 2 * kStatsLanes; // one wave per chain
 constexpr int kStatsChunk = 128;               // samples per series per chunk
 constexpr int kStatsPitch = kStatsChunk + 4;   // LDS floats per row (bank spread)
 constexpr int kStatsLoads = kStatsLanes * kStatsChunk / 4 / kStatsThreads;  // float4 per thread
+constexpr uint32_t kStatsVecs = kStatsChunk / 4;  // float4 per row of a chunk
 constexpr uint32_t kStatsLdsBytes = 2u * kStatsLanes * kStatsPitch * 4u;
 
 struct SerialSums {
@@ -252,7 +253,8 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
   const uint32_t rows = min(static_cast<uint32_t>(kStatsLanes), nseries - first);
   const uint64_t nchunks = n / kStatsChunk;
   const float* base = data + static_cast<uint64_t>(first) * stride;
-  // load k of a chunk: element e = k * 128 + tid is row e / 32, float4 e % 32
+  // load k of a chunk: element e = k * 128 + tid is row e / kStatsVecs,
+  // float4 e % kStatsVecs
   // (rows past the batch re-read the last row); every index is a constant
   // after unrolling, so the chunk stays in registers
   vf4 reg[kStatsLoads];
@@ -261,9 +263,9 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
 #pragma unroll
     for (int k = 0; k < kStatsLoads; ++k) {
       const uint32_t e = static_cast<uint32_t>(k) * kStatsThreads + tid;
-      const uint32_t r = min(e >> 5, rows - 1);
+      const uint32_t r = min(e / kStatsVecs, rows - 1);
       reg[k] = *reinterpret_cast<const vf4*>(base + static_cast<uint64_t>(r) * stride +
-                                                4u * (e & 31u));
+                                                4u * (e % kStatsVecs));
     }
   }
   for (uint64_t c = 0; c < nchunks; ++c) {
@@ -271,8 +273,8 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
 #pragma unroll
     for (int k = 0; k < kStatsLoads; ++k) {
       const uint32_t e = static_cast<uint32_t>(k) * kStatsThreads + tid;
-      *reinterpret_cast<vf4*>(stage + (buf * kStatsLanes + (e >> 5)) * kStatsPitch +
-                                 4u * (e & 31u)) = reg[k];
+      *reinterpret_cast<vf4*>(stage + (buf * kStatsLanes + e / kStatsVecs) * kStatsPitch +
+                                 4u * (e % kStatsVecs)) = reg[k];
     }
     __syncthreads();
     if (c + 1 < nchunks) {  // in flight while this chunk is summed
@@ -280,9 +282,9 @@ stats_kernel(const float* __restrict__ data, uint64_t stride, uint64_t n, uint32
 #pragma unroll
       for (int k = 0; k < kStatsLoads; ++k) {
         const uint32_t e = static_cast<uint32_t>(k) * kStatsThreads + tid;
-        const uint32_t r = min(e >> 5, rows - 1);
+        const uint32_t r = min(e / kStatsVecs, rows - 1);
         reg[k] = *reinterpret_cast<const vf4*>(base + static_cast<uint64_t>(r) * stride + col +
-                                                  4u * (e & 31u));
+                                                  4u * (e % kStatsVecs));
       }
     }
     const vf4* row = reinterpret_cast<const vf4*>(stage + (buf * kStatsLanes + lane) *
