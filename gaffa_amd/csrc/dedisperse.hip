@@ -221,8 +221,11 @@ transpose_u8_kernel(const uint8_t* __restrict__ x, uint64_t nsamples, uint64_t n
 // of one window need no barrier between them.  Workgroups are ordered span-
 // major and group-minor over the XCDs, so the groups running together on one
 // XCD read the same stretch of the channel rows.
-constexpr int kSbThreads = 1024;    // 16 waves: a workgroup's LDS windows allow one or two per CU
-constexpr uint32_t kSbWaves = kSbThreads / 64;
+// Threads per workgroup (a template argument, so loop strides stay
+// constants): 1024 (16 waves) where the LDS windows leave room for one
+// 512-thread workgroup per CU, else 512 (several per CU).
+constexpr int kSbThreads = 1024;
+constexpr int kSbThreadsSmall = 512;
 constexpr uint32_t kSbPiece = 512;   // intermediate samples per wave pass (64 lanes x 8)
 constexpr int kSbUnroll = 16;        // channels whose loads are in flight together
 constexpr uint32_t kXcds = 8;        // MI355X: 8 XCDs, one L2 each
@@ -267,7 +270,8 @@ __device__ inline void sb_channels(const uint8_t* rows0, uint64_t stride,
   }
 }
 
-__global__ void __launch_bounds__(kSbThreads)
+template <uint32_t kThreadsT>
+__global__ void __launch_bounds__(kThreadsT)
 dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_stride,
                                  uint64_t cm_length, uint32_t nsel, uint32_t sbc, uint32_t nsub,
                                  const SubbandGroupDev* __restrict__ groups, uint32_t ngroups,
@@ -291,6 +295,8 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63u;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  constexpr uint32_t threads = kThreadsT;
+  constexpr uint32_t nwaves = threads / 64u;
   const uint32_t nwork = ngroups * nspans;
   const uint32_t per_xcd = (nwork + kXcds - 1) / kXcds;
   const uint32_t w_id = (blockIdx.x % kXcds) * per_xcd + blockIdx.x / kXcds;
@@ -319,14 +325,14 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
     if (tid == 0) atomicExch(error_flag, 1);
     return;
   }
-  for (uint32_t e = tid; e < nd * nsub; e += kSbThreads) {
+  for (uint32_t e = tid; e < nd * nsub; e += threads) {
     const uint32_t i = e / nsub, s = e % nsub;
     const int32_t r = resid[static_cast<uint64_t>(dm_first + i) * nsub + s];
     if (r < 0 || r > (1 << 24)) sbad = 1;
     saddr[i * nsub4 + s] = r;
   }
   __syncthreads();
-  for (uint32_t s = tid; s < nsub; s += kSbThreads) {
+  for (uint32_t s = tid; s < nsub; s += threads) {
     int32_t h = 0;
     for (uint32_t i = 0; i < nd; ++i) h = max(h, saddr[i * nsub4 + s]);
     shalo[s] = (h + 7) & ~7;
@@ -345,7 +351,7 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
     if (tid == 0) atomicExch(error_flag, 1);
     return;
   }
-  for (uint32_t e = tid; e < nd * nsub; e += kSbThreads) {
+  for (uint32_t e = tid; e < nd * nsub; e += threads) {
     const uint32_t i = e / nsub, s = e % nsub;
     saddr[i * nsub4 + s] += soff[s];
   }
@@ -357,7 +363,7 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
                        static_cast<int64_t>(t_begin + static_cast<uint64_t>(nchunks) * chunk) -
                        hmax - static_cast<int64_t>(kSbReadPad);
   const int32_t* kdel = coarse + static_cast<uint64_t>(k) * nsel;
-  for (uint32_t c = tid; c < nsel; c += kSbThreads) {
+  for (uint32_t c = tid; c < nsel; c += threads) {
     const int32_t d = kdel[c];
     if (d < 0 || static_cast<int64_t>(d) > room) sbad = 1;
   }
@@ -370,7 +376,7 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
   for (uint32_t j = 0; j < nchunks; ++j) {
     const uint64_t t_out = t_begin + static_cast<uint64_t>(j) * chunk;
     // Steps 1 and 2, per owned subband window.
-    for (uint32_t s = wave; s < nsub; s += kSbWaves) {
+    for (uint32_t s = wave; s < nsub; s += nwaves) {
       const uint32_t h = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(shalo[s]));
       const uint32_t base = __builtin_amdgcn_readfirstlane(static_cast<uint32_t>(soff[s]));
       if (j > 0) {
@@ -422,7 +428,7 @@ dedisperse_subband_stream_kernel(const uint8_t* __restrict__ cm, uint64_t cm_str
       const uint32_t slot = slot_v;  // dwords between a lane's pairs: chunk / 8
       const bool lane_live = lane < slot;
       const uint32_t wdw = min(lane, slot - 1u);
-      for (uint32_t i = wave; i < nd; i += kSbWaves) {
+      for (uint32_t i = wave; i < nd; i += nwaves) {
         const int32_t* arow = saddr + i * nsub4;
         uint32_t tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
         for (uint32_t s0 = 0; s0 < nsub; s0 += flush_every) {
@@ -674,14 +680,18 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
           "gaffa_hip_subband_channel_stride)");
     const uint64_t nsub4 = (nsub + 3) & ~3ull;
     const uint64_t window_cap = (lds - 4 * (max_group * nsub4 + 2 * nsub4)) / 2;
-    hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(dedisperse_subband_stream_kernel),
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  static_cast<int>(lds)),
-              "hipFuncSetAttribute");
     const uint64_t nwork = ngroups * nspans;
     const uint64_t grid = ceil_div(nwork, kXcds) * kXcds;  // whole rounds over the XCDs
-    hipLaunchKernelGGL(dedisperse_subband_stream_kernel, dim3(static_cast<uint32_t>(grid)),
-                       dim3(kSbThreads), static_cast<uint32_t>(lds), as_stream(stream),
+    const bool wide = 2 * lds > kMaxLds;
+    const void* kernel = wide
+        ? reinterpret_cast<const void*>(dedisperse_subband_stream_kernel<kSbThreads>)
+        : reinterpret_cast<const void*>(dedisperse_subband_stream_kernel<kSbThreadsSmall>);
+    hip_check(hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  static_cast<int>(lds)),
+              "hipFuncSetAttribute");
+    auto launch = [&](auto kern, uint32_t threads) {
+      hipLaunchKernelGGL(kern, dim3(static_cast<uint32_t>(grid)),
+                       dim3(threads), static_cast<uint32_t>(lds), as_stream(stream),
                        a->channels, a->channel_stride, row_len, static_cast<uint32_t>(nsel),
                        static_cast<uint32_t>(a->subband_channels), static_cast<uint32_t>(nsub),
                        reinterpret_cast<const SubbandGroupDev*>(a->groups),
@@ -691,6 +701,9 @@ int gaffa_hip_dedisperse_subband_u8(const gaffa_hip_dedisperse_subband_args* a,
                        chunk, static_cast<uint32_t>(span), static_cast<uint32_t>(nspans),
                        static_cast<uint32_t>(max_group), static_cast<uint32_t>(window_cap),
                        flush, error_flag_for_device());
+    };
+    if (wide) launch(dedisperse_subband_stream_kernel<kSbThreads>, kSbThreads);
+    else launch(dedisperse_subband_stream_kernel<kSbThreadsSmall>, kSbThreadsSmall);
     launch_check("dedisperse_subband_stream_kernel");
   });
 }
