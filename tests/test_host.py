@@ -393,3 +393,84 @@ def test_subtree_merge_tables_replay_the_oracle_transform(port, rows, bins):
         cur = nxt
     want = port.ffa_transform(x, bins).reshape(-1)
     assert cur.tobytes() == want.tobytes()
+
+
+def _pick_row(child_rows, rows, s):
+    # the FFA's FP32 row selection (gaffa's ffa.cpp:133-143)
+    scale = np.float32(child_rows - 1) / np.float32(rows - 1)
+    return int(scale * np.float32(s) + np.float32(0.5))
+
+
+def _merge_desc(start, size, r, bins):
+    s = r - start
+    hn = size // 2
+    h = _pick_row(hn, size, s)
+    t = _pick_row(size - hn, size, s)
+    return start + h, start + hn + t, (s - t) % bins
+
+
+def _node(rows, depth, r):
+    start, size = 0, rows
+    for _ in range(depth):
+        if size <= 1:
+            break
+        h = size // 2
+        if r < start + h:
+            size = h
+        else:
+            start, size = start + h, size - h
+    return start, size
+
+
+def _leaves(rows, depth, levels, r, bins):
+    """Depth-(depth + levels) rows an output row r of a depth-`depth` node
+    sums, in the order of its one-level merges."""
+    out = [r]
+    nodes = [_node(rows, depth, r)]
+    for _ in range(levels):
+        nxt_rows, nxt_nodes = [], []
+        for row, (start, size) in zip(out, nodes):
+            h, t, _roll = _merge_desc(start, size, row, bins)
+            hn = size // 2
+            nxt_rows += [h, t]
+            nxt_nodes += [(start, hn), (start + hn, size - hn)]
+        out, nodes = nxt_rows, nxt_nodes
+    return out
+
+
+@pytest.mark.parametrize("rows,bins,depth,levels", [(5000, 200, 5, 3), (4100, 181, 3, 2),
+                                                    (777, 255, 2, 1), (96, 7, 1, 3)])
+def test_level_item_lds_need_matches_leaf_runs(rows, bins, depth, levels):
+    """gaffa_hip_ffa_level_lds_bytes gives a level item's leaf runs (from
+    the smallest to the largest row of each leaf over the item's rows, one
+    run per leaf) plus a 256-float guard, the LDS the level kernel stages.
+    Inside one node every leaf is monotone in the output row, so a run holds
+    only rows the item reads; an item across two nodes needs more."""
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    f = lib.gaffa_hip_ffa_level_lds_bytes
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint32] * 6
+    starts = sorted({_node(rows, depth, r)[0] for r in range(rows)})
+    bounds = list(zip(starts, starts[1:] + [rows]))
+    checked = 0
+    for a, b in bounds[:3] + bounds[-2:]:
+        for first in range(a, b, 32):
+            count = min(32, b - first)
+            leaves = np.array([_leaves(rows, depth, levels, r, bins)
+                               for r in range(first, first + count)])
+            assert np.all(np.diff(leaves, axis=0) >= 0)  # monotone inside the node
+            runs = int((leaves.max(axis=0) - leaves.min(axis=0) + 1).sum())
+            assert f(rows, bins, depth, levels, first, count) == (runs * bins + 256) * 4
+            checked += 1
+    assert checked > 0
+    if len(bounds) > 1:  # an item across the first node boundary
+        first = bounds[0][1] - 4
+        inside = f(rows, bins, depth, levels, first, 4)
+        across = f(rows, bins, depth, levels, first, 8)
+        assert across > inside
+    bad = (1 << 64) - 1
+    assert f(rows, bins, depth, 0, 0, 1) == bad and f(rows, bins, depth, 4, 0, 1) == bad
+    assert f(rows, bins, depth, levels, rows - 1, 2) == bad  # past the last row
+    assert f(rows, bins, depth, levels, 0, 33) == bad        # more rows than an item holds
