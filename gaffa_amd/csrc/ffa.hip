@@ -623,15 +623,13 @@ struct Desc2 {
 //   L = 2: (v0 + v1) + (v2 + v3)
 //   L = 3: ((v0 + v1) + (v2 + v3)) + ((v4 + v5) + (v6 + v7))
 // with v_j = leaf_j[(p + R_j) mod b]; the depth d + 1 .. d + L - 1 values are
-// the FP32 sums in between and are never written.
-struct Leaves {
-  uint32_t row[8];
-  uint32_t roll[8];  // roll[0] = 0
-};
-
-__host__ __device__ __forceinline__ Leaves level_leaves(uint32_t task_rows, uint32_t depth,
-                                                        uint32_t levels, uint32_t r, uint32_t b) {
-  Leaves e{};
+// the FP32 sums in between and are never written.  The host builds each
+// row's leaves (gaffa_hip_ffa_level_row) once per plan.
+__host__ __device__ __forceinline__ gaffa_hip_ffa_level_row level_leaves(uint32_t task_rows,
+                                                                         uint32_t depth,
+                                                                         uint32_t levels,
+                                                                         uint32_t r, uint32_t b) {
+  gaffa_hip_ffa_level_row e{};
   const Node nd = find_node(0, task_rows, depth, r);
   const Desc top = merge_desc(nd, r, b);
   if (levels == 1) {
@@ -681,7 +679,7 @@ __host__ __device__ __forceinline__ Leaves level_leaves(uint32_t task_rows, uint
 // masked).
 constexpr uint32_t kLevelGuard = 256;
 // Dynamic LDS a level launch may take: the LDS less the kernel's static
-// descriptors (kLevelRowsMax leaf sets and the runs).
+// leaf rows (kLevelRowsMax x 64 bytes).
 constexpr uint32_t kMaxLevelLdsBytes = kMaxLdsBytes - 4096;
 
 // Dynamic LDS an item of `count` rows from `first` needs to merge from LDS:
@@ -693,22 +691,25 @@ __host__ __device__ __forceinline__ uint64_t level_stage_bytes(uint32_t staged_r
 }
 
 // Rows [item.b, item.b + item.c) of the depth-d nodes of a task (item.a = d
-// | L << 16).  The item's leaf runs are copied into LDS once
-// (global_load_lds: 64 consecutive floats per instruction, straight into
-// LDS), and each output phase then reads its 2^L operands there; a wave
-// merges one row at a time, four 64-phase chunks per block.  An item whose
-// runs exceed the launch's LDS (plan.level_lds_bytes) reads its operands
-// from global memory instead, with the same sums.  The work region of each
-// depth comes from the task's region bits.
+// | L << 16).  The item's leaf runs (its span, host-built) are copied into
+// LDS once (global_load_lds: 64 consecutive floats per instruction, straight
+// into LDS) together with its rows' leaves (host-built), and each output
+// phase then reads its 2^L operands there; a wave merges one row at a time,
+// four 64-phase chunks per block.  An item whose runs exceed the launch's LDS
+// (plan.level_lds_bytes) reads its operands from global memory instead, with
+// the same sums.  The work region of each depth comes from the task's region
+// bits.
 __global__ void __launch_bounds__(kLevelThreads)
 ffa_level_kernel(KArgs a, uint32_t item_offset) {
   extern __shared__ float stage[];
-  __shared__ Leaves sleaf[kLevelRowsMax];
-  __shared__ uint32_t srun[2][8];  // first and last row of each leaf's run
+  using ConstSpan = __attribute__((address_space(4))) const gaffa_hip_ffa_level_span;
   const gaffa_hip_ffa_plan_desc& plan = a.plan;
-  const gaffa_hip_ffa_item item = plan.level_items[item_offset + blockIdx.x];
+  const uint32_t it = item_offset + blockIdx.x;
+  const gaffa_hip_ffa_item item = plan.level_items[it];
+  ConstSpan* const span = (ConstSpan*)plan.level_spans + it;
+  __shared__ gaffa_hip_ffa_level_row sleaf[kLevelRowsMax];
   const uint32_t levels = item.a >> 16;
-  // the host builds items within the descriptor arrays and level counts
+  // the host builds items within the level counts and the leaf tables
   if (item.c == 0 || item.c > kLevelRowsMax || levels < 1 || levels > 3) return;
   const gaffa_hip_ffa_task& task = plan.tasks[item.task];
   const uint64_t sidx = blockIdx.y;
@@ -724,20 +725,14 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
                      sidx * plan.work_stride + task.work_offset;
   float* dst = a.work + static_cast<uint64_t>((task.regions >> depth) & 1u) * region +
                sidx * plan.work_stride + task.work_offset;
-  if (tid < item.c) sleaf[tid] = level_leaves(task.rows, depth, levels, item.b + tid, b);
-  if (tid < 8) {
-    srun[0][tid] = 0xFFFFFFFFu;
-    srun[1][tid] = 0;
+  // the item's rows' leaves, to LDS beside the runs (read there per row)
+  {
+    const uint32_t* g = reinterpret_cast<const uint32_t*>(plan.level_leaf_rows + span->row_offset);
+    const uint32_t total = item.c * (sizeof(gaffa_hip_ffa_level_row) / 4);
+    uint32_t* l = reinterpret_cast<uint32_t*>(sleaf);
+    for (uint32_t c0 = wave * 64u; c0 < total; c0 += kLevelThreads)
+      if (c0 + lane < total) __builtin_amdgcn_global_load_lds(g + c0 + lane, l + c0, 4, 0, 0);
   }
-  __syncthreads();
-  // each leaf's run: smallest and largest row over the item's rows, one
-  // (row, leaf) per thread
-  if (tid < item.c * nleaf) {
-    const uint32_t r = sleaf[tid >> levels].row[tid & (nleaf - 1)];
-    atomicMin(&srun[0][tid & (nleaf - 1)], r);
-    atomicMax(&srun[1][tid & (nleaf - 1)], r);
-  }
-  __syncthreads();
   uint32_t run_lo[8], run_off[8];  // a leaf's first row, and its run's first staged row
   uint32_t staged = 0;
 #pragma unroll
@@ -745,26 +740,25 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
     run_lo[j] = 0;
     run_off[j] = 0;
     if (j < nleaf) {
-      run_lo[j] = __builtin_amdgcn_readfirstlane(srun[0][j]);
+      run_lo[j] = span->first[j];
       run_off[j] = staged;
-      staged += __builtin_amdgcn_readfirstlane(srun[1][j]) - run_lo[j] + 1;
+      staged += span->count[j];
     }
   }
-  const bool in_lds = level_stage_bytes(staged, b) <= plan.level_lds_bytes;
+  const bool in_lds = staged == span->staged && level_stage_bytes(staged, b) <= plan.level_lds_bytes;
   if (in_lds) {
 #pragma unroll
     for (uint32_t j = 0; j < 8; ++j) {
       if (j >= nleaf) break;
       const float* g = src + static_cast<uint64_t>(run_lo[j]) * b;
       float* l = stage + run_off[j] * b;
-      const uint32_t count = (j + 1 < nleaf ? run_off[j + 1] : staged) - run_off[j];
-      const uint32_t total = count * b;
+      const uint32_t total = span->count[j] * b;
       for (uint32_t c0 = wave * 64u; c0 < total; c0 += kLevelThreads)
         if (c0 + lane < total) __builtin_amdgcn_global_load_lds(g + c0 + lane, l + c0, 4, 0, 0);
     }
-    __builtin_amdgcn_s_waitcnt(0);  // the LDS writes of global_load_lds count in vmcnt
-    __syncthreads();
   }
+  __builtin_amdgcn_s_waitcnt(0);  // the LDS writes of global_load_lds count in vmcnt
+  __syncthreads();
 
   // out[p] for the row's phases, four 64-phase chunks per block.  From LDS,
   // each operand picks one of two addresses per chunk (before or after its
@@ -1096,6 +1090,9 @@ void validate(const gaffa_hip_ffa_search_args* a) {
   check(p.max_bins > 1 && p.max_bins < 65536, "FFA bins must be in (1, 65536)");
   check(p.terminal_lds_bytes <= kMaxLdsBytes, "FFA detection tile exceeds LDS");
   check(p.level_lds_bytes <= kMaxLevelLdsBytes, "FFA level staging exceeds LDS");
+  check(p.level_item_offsets == nullptr || p.level_item_offsets[p.max_depth] == 0 ||
+            (p.level_spans != nullptr && p.level_leaf_rows != nullptr),
+        "FFA level items need their spans and leaf rows");
   check(p.subtree_lds_bytes <= kMaxLdsBytes, "FFA subtree exceeds LDS");
   check(p.nsubtree_items == 0 ||
             (p.subtree_half_floats > 0 &&
@@ -1302,15 +1299,17 @@ uint32_t gaffa_hip_ffa_level_rows(void) { return kLevelRows; }
 uint32_t gaffa_hip_ffa_terminal_rows_max(void) { return kTerminalRowsMax; }
 uint32_t gaffa_hip_ffa_level_rows_max(void) { return kLevelRowsMax; }
 
-uint64_t gaffa_hip_ffa_level_lds_bytes(uint32_t task_rows, uint32_t bins, uint32_t depth,
-                                       uint32_t levels, uint32_t first, uint32_t count) {
+uint64_t gaffa_hip_ffa_level_item(uint32_t task_rows, uint32_t bins, uint32_t depth,
+                                  uint32_t levels, uint32_t first, uint32_t count,
+                                  gaffa_hip_ffa_level_span* span, gaffa_hip_ffa_level_row* rows) {
   if (levels < 1 || levels > 3 || bins < 2 || count == 0 || count > kLevelRowsMax ||
       task_rows < 2 || first >= task_rows || count > task_rows - first)
     return ~uint64_t{0};
   const uint32_t nleaf = 1u << levels;
-  uint32_t lo[8], hi[8];
+  uint32_t lo[8] = {}, hi[8] = {};
   for (uint32_t i = 0; i < count; ++i) {
-    const Leaves e = level_leaves(task_rows, depth, levels, first + i, bins);
+    const gaffa_hip_ffa_level_row e = level_leaves(task_rows, depth, levels, first + i, bins);
+    if (rows != nullptr) rows[i] = e;
     for (uint32_t j = 0; j < nleaf; ++j) {
       lo[j] = i == 0 ? e.row[j] : std::min(lo[j], e.row[j]);
       hi[j] = i == 0 ? e.row[j] : std::max(hi[j], e.row[j]);
@@ -1318,7 +1317,20 @@ uint64_t gaffa_hip_ffa_level_lds_bytes(uint32_t task_rows, uint32_t bins, uint32
   }
   uint32_t staged = 0;
   for (uint32_t j = 0; j < nleaf; ++j) staged += hi[j] - lo[j] + 1;
+  if (span != nullptr) {
+    span->staged = staged;
+    for (uint32_t j = 0; j < 8; ++j) {
+      span->first[j] = j < nleaf ? lo[j] : 0;
+      span->count[j] = j < nleaf ? hi[j] - lo[j] + 1 : 0;
+    }
+    span->reserved[0] = span->reserved[1] = 0;
+  }
   return level_stage_bytes(staged, bins);
+}
+
+uint64_t gaffa_hip_ffa_level_lds_bytes(uint32_t task_rows, uint32_t bins, uint32_t depth,
+                                       uint32_t levels, uint32_t first, uint32_t count) {
+  return gaffa_hip_ffa_level_item(task_rows, bins, depth, levels, first, count, nullptr, nullptr);
 }
 
 uint32_t gaffa_hip_ffa_terminal_lds_bytes(uint32_t bins, uint32_t max_width) {

@@ -40,6 +40,8 @@ struct GpuFfaBatchedSearch::Tile {
   GpuDeviceBuffer<gaffa_hip_ffa_group> groups;
   GpuDeviceBuffer<gaffa_hip_ffa_task> tasks;
   GpuDeviceBuffer<gaffa_hip_ffa_item> subtree, level, terminal;
+  GpuDeviceBuffer<gaffa_hip_ffa_level_span> level_spans;
+  GpuDeviceBuffer<gaffa_hip_ffa_level_row> level_rows;
   GpuDeviceBuffer<gaffa_hip_ffa_merge> merges;
   gaffa_hip_ffa_plan_desc desc{};
 };
@@ -123,6 +125,8 @@ void GpuFfaBatchedSearch::build_tiles() {
     t->subtree = gaffa::upload(e.subtree_items().data(), e.subtree_items().size());
     t->merges = gaffa::upload(e.subtree_merges().data(), e.subtree_merges().size());
     t->level = gaffa::upload(e.level_items().data(), e.level_items().size());
+    t->level_spans = gaffa::upload(e.level_spans().data(), e.level_spans().size());
+    t->level_rows = gaffa::upload(e.level_leaf_rows().data(), e.level_leaf_rows().size());
     t->terminal = gaffa::upload(e.terminal_items().data(), e.terminal_items().size());
     gaffa_hip_ffa_plan_desc& d = t->desc;
     d.groups = t->groups.data();
@@ -133,6 +137,8 @@ void GpuFfaBatchedSearch::build_tiles() {
     d.nsubtree_items = static_cast<std::uint32_t>(e.subtree_items().size());
     d.subtree_merges = t->merges.data();
     d.level_items = t->level.data();
+    d.level_spans = t->level_spans.data();
+    d.level_leaf_rows = t->level_rows.data();
     d.level_item_offsets = e.level_offsets().data();
     d.max_depth = static_cast<std::uint32_t>(e.max_depth());
     d.terminal_items = t->terminal.data();

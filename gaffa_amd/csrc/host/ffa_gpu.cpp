@@ -238,7 +238,10 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
       }
       subtree_lds_bytes_ =
           static_cast<std::uint32_t>(8u * subtree_half_ + subtree_fixed_lds(lds_max_bins_));
-      if (by_depth_.size() < depth) by_depth_.resize(depth);
+      if (by_depth_.size() < depth) {
+        by_depth_.resize(depth);
+        spans_by_depth_.resize(depth);
+      }
       // global passes from depth `depth` up to depth `tl`: three tree levels
       // per pass, two where two or four remain (2 + 2 reads less than
       // 3 + 1); the region bits make every pass read one work region and
@@ -274,8 +277,16 @@ void GpuFfaExecutionPlan::append(const FfaSearchPlan& plan, std::uint64_t input_
             const std::uint32_t count = std::min(level_rows, nd.a + nd.b - r);
             by_depth_[dout].push_back(
                 gaffa_hip_ffa_item{task_index, dout | (levels << 16), r, count});
-            const std::uint64_t need =
-                gaffa_hip_ffa_level_lds_bytes(d.rows, d.bins, dout, levels, r, count);
+            gaffa_hip_ffa_level_span span{};
+            check_arg(level_leaf_rows_.size() + count < (std::size_t{1} << 32),
+                      "FFA plan has too many level rows");
+            span.row_offset = static_cast<std::uint32_t>(level_leaf_rows_.size());
+            level_leaf_rows_.resize(level_leaf_rows_.size() + count);
+            const std::uint64_t need = gaffa_hip_ffa_level_item(
+                d.rows, d.bins, dout, levels, r, count, &span,
+                level_leaf_rows_.data() + span.row_offset);
+            check_arg(need != ~std::uint64_t{0}, "FFA level item outside its task");
+            spans_by_depth_[dout].push_back(span);
             if (need <= kLevelLdsCap)
               level_lds_bytes_ = std::max(level_lds_bytes_, static_cast<std::uint32_t>(need));
           }
@@ -308,11 +319,15 @@ void GpuFfaExecutionPlan::finish() {
   level_offsets_.assign(max_depth_ + 1, 0);
   for (std::size_t l = 0; l <= max_depth_; ++l) {
     level_offsets_[l] = static_cast<std::uint32_t>(level_.size());
-    if (l < by_depth_.size())
+    if (l < by_depth_.size()) {
       level_.insert(level_.end(), by_depth_[l].begin(), by_depth_[l].end());
+      level_spans_.insert(level_spans_.end(), spans_by_depth_[l].begin(),
+                          spans_by_depth_[l].end());
+    }
   }
   level_offsets_.push_back(static_cast<std::uint32_t>(level_.size()));
   by_depth_.clear();
+  spans_by_depth_.clear();
   check_arg(terminal_.size() < (1ULL << 31) && tasks_.size() < (1ULL << 31),
             "FFA plan has too many work items");
 }
@@ -344,6 +359,8 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   d_subtree_ = upload(exec_plan_.subtree_items().data(), exec_plan_.subtree_items().size());
   d_merges_ = upload(exec_plan_.subtree_merges().data(), exec_plan_.subtree_merges().size());
   d_level_ = upload(exec_plan_.level_items().data(), exec_plan_.level_items().size());
+  d_level_spans_ = upload(exec_plan_.level_spans().data(), exec_plan_.level_spans().size());
+  d_level_rows_ = upload(exec_plan_.level_leaf_rows().data(), exec_plan_.level_leaf_rows().size());
   d_terminal_ = upload(exec_plan_.terminal_items().data(), exec_plan_.terminal_items().size());
   prepared_ = GpuDeviceBuffer<float>(std::max<std::size_t>(1, tile_ * exec_plan_.prepared_stride()));
   work_ = GpuDeviceBuffer<float>(std::max<std::size_t>(1, 2 * tile_ * exec_plan_.work_stride()));
@@ -359,6 +376,8 @@ GpuFfaProgram::GpuFfaProgram(const FfaSearchPlan& plan, const GpuFfaProgramOptio
   desc_.nsubtree_items = static_cast<std::uint32_t>(exec_plan_.subtree_items().size());
   desc_.subtree_merges = d_merges_.data();
   desc_.level_items = d_level_.data();
+  desc_.level_spans = d_level_spans_.data();
+  desc_.level_leaf_rows = d_level_rows_.data();
   desc_.level_item_offsets = exec_plan_.level_offsets().data();
   desc_.max_depth = static_cast<std::uint32_t>(exec_plan_.max_depth());
   desc_.terminal_items = d_terminal_.data();

@@ -86,6 +86,10 @@ class GpuFfaExecutionPlan {
   // the subtree items' merge tables (item.c = first entry)
   const std::vector<gaffa_hip_ffa_merge>& subtree_merges() const noexcept { return merges_; }
   const std::vector<gaffa_hip_ffa_item>& level_items() const noexcept { return level_; }
+  const std::vector<gaffa_hip_ffa_level_span>& level_spans() const noexcept { return level_spans_; }
+  const std::vector<gaffa_hip_ffa_level_row>& level_leaf_rows() const noexcept {
+    return level_leaf_rows_;
+  }
   const std::vector<std::uint32_t>& level_offsets() const noexcept { return level_offsets_; }
   const std::vector<gaffa_hip_ffa_item>& terminal_items() const noexcept { return terminal_; }
   std::uint32_t subtree_lds_bytes() const noexcept { return subtree_lds_bytes_; }
@@ -106,6 +110,9 @@ class GpuFfaExecutionPlan {
   void append(const FfaSearchPlan& plan, std::uint64_t input_offset, std::uint32_t series);
   void finish();
   std::vector<std::vector<gaffa_hip_ffa_item>> by_depth_;
+  std::vector<std::vector<gaffa_hip_ffa_level_span>> spans_by_depth_;
+  std::vector<gaffa_hip_ffa_level_span> level_spans_;   // parallel to level_
+  std::vector<gaffa_hip_ffa_level_row> level_leaf_rows_;
   FfaSearchPlan plan_;
   std::size_t input_nsamples_ = 0;
   std::size_t prepared_stride_ = 0;
@@ -191,6 +198,8 @@ class GpuFfaProgram {
   GpuDeviceBuffer<gaffa_hip_ffa_group> d_groups_;
   GpuDeviceBuffer<gaffa_hip_ffa_task> d_tasks_;
   GpuDeviceBuffer<gaffa_hip_ffa_item> d_subtree_, d_level_, d_terminal_;
+  GpuDeviceBuffer<gaffa_hip_ffa_level_span> d_level_spans_;
+  GpuDeviceBuffer<gaffa_hip_ffa_level_row> d_level_rows_;
   GpuDeviceBuffer<gaffa_hip_ffa_merge> d_merges_;
   GpuDeviceBuffer<float> prepared_, work_;
   GpuDeviceBuffer<gaffa_hip_ffa_peak> peaks_;

@@ -392,6 +392,27 @@ typedef struct gaffa_hip_ffa_merge {
   uint32_t wrap3; /* bins - rd */
 } gaffa_hip_ffa_merge;
 
+/* Leaves of one output row of a level pass of L = 1..3 tree levels: the
+ * 2^L depth-(d + L) rows it sums and the combined roll of each (mod bins,
+ * roll[0] = 0), in the order of its one-level merges; entries past 2^L are
+ * 0.  No gaffa counterpart (gaffa merges one level per pass). */
+typedef struct gaffa_hip_ffa_level_row {
+  uint32_t row[8];
+  uint32_t roll[8];
+} gaffa_hip_ffa_level_row;
+
+/* A level item's leaf runs: leaf j of its rows spans rows first[j] ..
+ * first[j] + count[j] - 1 (smallest to largest over the item's rows); the
+ * runs are staged back to back, `staged` rows in all; the item's rows'
+ * leaves are level_leaf_rows[row_offset .. row_offset + item rows). */
+typedef struct gaffa_hip_ffa_level_span {
+  uint32_t row_offset;
+  uint32_t staged;
+  uint32_t first[8];
+  uint32_t count[8];
+  uint32_t reserved[2];
+} gaffa_hip_ffa_level_span;
+
 typedef struct gaffa_hip_ffa_plan_desc {
   const gaffa_hip_ffa_group* groups; /* device */
   uint32_t ngroups;
@@ -401,6 +422,8 @@ typedef struct gaffa_hip_ffa_plan_desc {
   uint32_t nsubtree_items;
   const gaffa_hip_ffa_merge* subtree_merges; /* device */
   const gaffa_hip_ffa_item* level_items;   /* device, sorted by depth desc */
+  const gaffa_hip_ffa_level_span* level_spans;    /* device, one per level item */
+  const gaffa_hip_ffa_level_row* level_leaf_rows; /* device, spans' row tables */
   const uint32_t* level_item_offsets;      /* host, [max_depth + 1] */
   uint32_t max_depth;
   const gaffa_hip_ffa_item* terminal_items; /* device */
@@ -456,6 +479,13 @@ uint32_t gaffa_hip_ffa_level_rows_max(void);
  * every level from global memory, ffa_cuda_program.cuh:230-336). */
 uint64_t gaffa_hip_ffa_level_lds_bytes(uint32_t task_rows, uint32_t bins, uint32_t depth,
                                        uint32_t levels, uint32_t first, uint32_t count);
+/* The same item's span (all but row_offset, which the caller sets) and its
+ * rows' leaves (`rows`: count entries), with the kernels' own row-selection
+ * arithmetic (gaffa's ffa.cpp:133-143 in FP32); returns the LDS need as
+ * gaffa_hip_ffa_level_lds_bytes.  Null outputs are not written. */
+uint64_t gaffa_hip_ffa_level_item(uint32_t task_rows, uint32_t bins, uint32_t depth,
+                                  uint32_t levels, uint32_t first, uint32_t count,
+                                  gaffa_hip_ffa_level_span* span, gaffa_hip_ffa_level_row* rows);
 uint32_t gaffa_hip_ffa_subtree_lds_bytes(uint32_t rows, uint32_t bins);
 /* Fills the merge table of a subtree node of `rows` rows (host memory; a
  * null `out` only counts) with the kernels' own row-selection arithmetic
