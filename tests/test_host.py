@@ -422,20 +422,64 @@ def _node(rows, depth, r):
     return start, size
 
 
-def _leaves(rows, depth, levels, r, bins):
+def _leaves(rows, depth, levels, r, bins, with_rolls=False):
     """Depth-(depth + levels) rows an output row r of a depth-`depth` node
-    sums, in the order of its one-level merges."""
-    out = [r]
+    sums, in the order of its one-level merges (and each one's roll: the
+    tail rolls along its path, mod bins)."""
+    out, rolls = [r], [0]
     nodes = [_node(rows, depth, r)]
     for _ in range(levels):
-        nxt_rows, nxt_nodes = [], []
-        for row, (start, size) in zip(out, nodes):
-            h, t, _roll = _merge_desc(start, size, row, bins)
+        nxt_rows, nxt_rolls, nxt_nodes = [], [], []
+        for row, acc, (start, size) in zip(out, rolls, nodes):
+            h, t, roll = _merge_desc(start, size, row, bins)
             hn = size // 2
             nxt_rows += [h, t]
+            nxt_rolls += [acc, (acc + roll) % bins]
             nxt_nodes += [(start, hn), (start + hn, size - hn)]
-        out, nodes = nxt_rows, nxt_nodes
-    return out
+        out, rolls, nodes = nxt_rows, nxt_rolls, nxt_nodes
+    return (out, rolls) if with_rolls else out
+
+
+class _LevelRow(ctypes.Structure):
+    _fields_ = [("row", ctypes.c_uint32 * 8), ("roll", ctypes.c_uint32 * 8)]
+
+
+class _LevelSpan(ctypes.Structure):
+    _fields_ = [("row_offset", ctypes.c_uint32), ("staged", ctypes.c_uint32),
+                ("first", ctypes.c_uint32 * 8), ("count", ctypes.c_uint32 * 8),
+                ("reserved", ctypes.c_uint32 * 2)]
+
+
+@pytest.mark.parametrize("rows,bins,depth,levels,first,count",
+                         [(5000, 200, 5, 3, 160, 32), (4100, 181, 3, 2, 530, 17),
+                          (777, 255, 2, 1, 0, 32), (96, 7, 1, 3, 48, 20)])
+def test_level_item_tables_match_the_tree(rows, bins, depth, levels, first, count):
+    """gaffa_hip_ffa_level_item writes each row's leaves (rows and combined
+    rolls, zero past 2^L) and the item's runs as the tree restated above
+    gives them."""
+    import gaffa_amd
+
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    f = lib.gaffa_hip_ffa_level_item
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint32] * 6 + [ctypes.c_void_p, ctypes.c_void_p]
+    span = _LevelSpan()
+    out = (_LevelRow * count)()
+    need = f(rows, bins, depth, levels, first, count, ctypes.addressof(span), ctypes.addressof(out))
+    nleaf = 1 << levels
+    got_rows = np.array([list(r.row) for r in out])
+    got_rolls = np.array([list(r.roll) for r in out])
+    want = [_leaves(rows, depth, levels, first + i, bins, True) for i in range(count)]
+    want_rows = np.array([w[0] for w in want])
+    want_rolls = np.array([w[1] for w in want])
+    assert np.array_equal(got_rows[:, :nleaf], want_rows)
+    assert np.array_equal(got_rolls[:, :nleaf], want_rolls)
+    assert not got_rows[:, nleaf:].any() and not got_rolls[:, nleaf:].any()
+    lo, hi = want_rows.min(axis=0), want_rows.max(axis=0)
+    assert list(span.first)[:nleaf] == lo.tolist()
+    assert list(span.count)[:nleaf] == (hi - lo + 1).tolist()
+    assert span.staged == int((hi - lo + 1).sum())
+    assert need == (span.staged * bins + 256) * 4
 
 
 @pytest.mark.parametrize("rows,bins,depth,levels", [(5000, 200, 5, 3), (4100, 181, 3, 2),
