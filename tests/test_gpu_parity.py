@@ -645,6 +645,39 @@ def test_ffa_batched_heterogeneous_bit_exact(gpu, port):
     say(f"ffa batch: batched {timing['gpu_ms']:.2f} ms, loop {t_loop['gpu_ms']:.2f} ms")
 
 
+def test_ffa_batched_full_length_slice_bit_exact(gpu, port):
+    """Sixteen series of the batched workload's own shape: 2^20 to 2^22
+    samples at 64 us, each with its own octave plan (840 tasks), half with
+    injected pulse trains, in uneven tiles of 6; the batched graphs' and the
+    per-series loop's peak lists equal the oracle's search of the same bytes,
+    peak for peak."""
+    from gaffa_amd.pipeline import batched_workload
+
+    n = 16
+    say(f"ffa batch: {n} series of 2^20..2^22 at 64 us, tiles of 6, vs oracle")
+    w = batched_workload(n, seed=23, log2_min=20, log2_max=22, tsamp=64e-6, injected=0.5)
+    plans = w.plans()
+    b = gpu.FfaBatchedSearch(plans, series_per_tile=6, workspace_bytes=8 << 30)
+    b.generate(w.seed, w.tsamp, list(w.period), list(w.phase0), list(w.sigma),
+               list(w.amplitude))
+    row = b.download_row()
+    offs, lens = list(b.offsets), list(b.lengths)
+    assert min(lens) >= 1 << 20 and max(lens) <= 1 << 22
+    first, timing = b.run(6.0)
+    assert timing["tiles"] == 3
+    loop, _ = b.run(6.0, per_series=True)
+    total = 0
+    for s in range(n):
+        x = row[offs[s]: offs[s] + lens[s]]
+        want = port.search_ffa(x, Plan(plans[s].tasks, np.array(plans[s].width_trials,
+                                                              np.uint64)), 6.0)
+        total += len(want)
+        for got in (first, loop):
+            _check_series_peaks(got, want, s)
+    assert total > 0
+    say(f"ffa batch: {total} peaks equal, batched {timing['gpu_ms']:.1f} ms")
+
+
 def test_ffa_batched_autotune_keeps_results(gpu):
     """The autotuner's knobs (series per tile, rows per level item, rows per
     terminal item) change the time only: every configuration, including odd
