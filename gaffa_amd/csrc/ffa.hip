@@ -595,7 +595,9 @@ ffa_subtree_kernel(KArgs a, uint32_t nunits) {
       cur = dst;
     }
     if (levels > 0) {
-      auto to_global = [&](uint32_t i, float v) { out[i] = v; };
+      // streaming stores: the node's rows are read back by the next kernel,
+      // long after L2 would have had to hold them
+      auto to_global = [&](uint32_t i, float v) { __builtin_nontemporal_store(v, &out[i]); };
       if (levels >= 2) level(Two{}, table, cur, to_global);
       else level(One{}, table, cur, to_global);
     }
@@ -821,7 +823,7 @@ ffa_level_kernel(KArgs a, uint32_t item_offset) {
                   ((v[4][k] + v[5][k]) + (v[6][k] + v[7][k]));
           }
           const uint32_t p = p0 + 64u * k + lane;
-          if (p < b) out[p] = sum;
+          if (p < b) __builtin_nontemporal_store(sum, &out[p]);  // streaming, as the subtree's
         }
       }
     }
