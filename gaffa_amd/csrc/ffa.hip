@@ -16,19 +16,21 @@
 //             LDS depth D, series) units: the node's rows live in LDS and
 //             every level below it runs there, ping-ponging between two LDS
 //             buffers, with merges from host-built tables
-//   level     one launch per output depth: one to three levels per pass
-//             through L2 / HBM
+//   level     one launch per output depth: one to three levels per pass,
+//             each item (32 rows of one node) copying its leaf rows' runs
+//             from HBM into LDS and merging there
 //   terminal  one workgroup per (32 final rows, series): the root merge, or
 //             the top two levels where that saves a global pass, into LDS
-//             (rows padded with a circular extension), then one lane per
+//             (rows stored with a circular extension), then one lane per
 //             (row, width) runs the serial FP32 prefix twice, P[phi] and
 //             P[phi + w], so every boxcar sum equals the CPU's circular-prefix
 //             difference bit for bit (ffa_detection.cpp:126-239); significant
 //             peaks are appended with one atomic each.
-// In the level and terminal kernels the (head row, tail row, roll)
-// descriptor of each output row is computed once, by one lane, into LDS; the
-// subtree kernel reads its merges from host-built tables.  The waves then
-// stream whole rows (lanes over phase bins) reading only those.
+// The subtree and level kernels read their merges and leaves from host-built
+// tables; in the terminal kernel the (head row, tail row, roll) descriptor of
+// each output row is computed once, by one lane, into LDS.  The waves then
+// stream whole rows (lanes over phase bins) reading only those.  The subtree
+// and level passes write their rows with streaming stores.
 #include "common.hpp"
 
 #include <algorithm>
