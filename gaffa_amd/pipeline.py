@@ -1,6 +1,7 @@
 """Search workloads: synthetic filterbank specs, DM plans and the GPU pipeline.
 
-The workloads are the survey shapes of the search hot path:
+The workloads are the observation shapes the search hot path is tested and
+measured on:
 
 * ``small``   - 64 channels, 1500 MHz / -4 MHz, 256 us, 2^17 samples, one
   pulsar (P 0.714 s, DM 30, 3 % duty, 0.5 sigma per channel), brute-force
