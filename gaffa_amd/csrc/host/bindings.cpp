@@ -713,6 +713,29 @@ PYBIND11_MODULE(_core, m) {
   m.def("_ffa_transform_rows_gpu", &py_ffa_transform_rows, py::arg("batch"), py::arg("plan"),
         py::arg("task"), py::kw_only(), py::arg("device_id") = 0);
   m.def("ffa_task_stdnoise", &ffa_task_stdnoise);
+  m.def(
+      "_ffa_level_layout",
+      [](const FfaSearchPlan& plan, std::uint32_t level_rows) {
+        // the host-built level tables of a plan (no device needed): items
+        // [n, 4], spans [n, 20] and leaf rows [m, 16] as uint32, in launch order
+        const GpuFfaExecutionPlan e(plan, FfaItemShape{level_rows, 0});
+        auto words = [](const auto& v, std::size_t per) {
+          std::vector<std::uint32_t> w(v.size() * per);
+          if (!v.empty()) std::memcpy(w.data(), v.data(), w.size() * sizeof(std::uint32_t));
+          return owning_array(std::move(w), {static_cast<py::ssize_t>(v.size()),
+                                             static_cast<py::ssize_t>(per)});
+        };
+        static_assert(sizeof(gaffa_hip_ffa_item) == 16 && sizeof(gaffa_hip_ffa_level_span) == 80 &&
+                      sizeof(gaffa_hip_ffa_level_row) == 64);
+        py::dict d;
+        d["items"] = words(e.level_items(), 4);
+        d["spans"] = words(e.level_spans(), 20);
+        d["leaf_rows"] = words(e.level_leaf_rows(), 16);
+        d["level_lds_bytes"] = e.level_lds_bytes();
+        return d;
+      },
+      py::arg("plan"), py::arg("level_rows") = 0,
+      "Host-built level work items, their leaf-run spans and leaf rows (tests)");
 
   // SIGPROC files (native reader / writer)
   m.def(

@@ -518,3 +518,55 @@ def test_level_item_lds_need_matches_leaf_runs(rows, bins, depth, levels):
     assert f(rows, bins, depth, 0, 0, 1) == bad and f(rows, bins, depth, 4, 0, 1) == bad
     assert f(rows, bins, depth, levels, rows - 1, 2) == bad  # past the last row
     assert f(rows, bins, depth, levels, 0, 33) == bad        # more rows than an item holds
+
+
+@pytest.mark.parametrize("nsamples,level_rows", [(1 << 20, 0), (1 << 21, 7)])
+def test_level_layout_items_lie_in_nodes_and_carry_their_tables(nsamples, level_rows):
+    """The plan builder's level work items each lie inside one node of their
+    output depth and cover every row of every pass exactly once; each item's
+    span and leaf rows (kept parallel to the items after they are ordered by
+    depth) are what gaffa_hip_ffa_level_item gives for it; and the launch LDS
+    is the largest item need within 64 KB."""
+    import collections
+
+    import gaffa_amd
+    from gaffa_amd import _core as core
+    from gaffa_amd.ffa import make_riptide_plan
+
+    plan = make_riptide_plan(nsamples=nsamples, tsamp=6.4e-5, period_min=0.1, period_max=10.0,
+                             bins_min=180, bins_max=256, duty_cycle_max=0.18,
+                             width_trial_spacing=2.0)
+    d = core._ffa_level_layout(plan, level_rows)
+    items, spans, leaf = d["items"], d["spans"], d["leaf_rows"]
+    assert len(items) == len(spans) > 0
+    lib = ctypes.CDLL(gaffa_amd.HIP_LIBRARY)
+    f = lib.gaffa_hip_ffa_level_item
+    f.restype = ctypes.c_uint64
+    f.argtypes = [ctypes.c_uint32] * 6 + [ctypes.c_void_p, ctypes.c_void_p]
+    tasks = plan.tasks
+    cover = collections.defaultdict(list)
+    max_need = 0
+    for k, (task, a, first, count) in enumerate(items.tolist()):
+        depth, levels = a & 0xFFFF, a >> 16
+        rows, bins = int(tasks["rows"][task]), int(tasks["bins"][task])
+        assert 1 <= levels <= 3 and 1 <= count <= (level_rows or 32)
+        assert _node(rows, depth, first) == _node(rows, depth, first + count - 1)
+        cover[(task, depth)].append((first, count))
+        span = _LevelSpan()
+        out = (_LevelRow * count)()
+        need = f(rows, bins, depth, levels, first, count, ctypes.addressof(span),
+                 ctypes.addressof(out))
+        sp = spans[k]
+        assert sp[1] == span.staged
+        assert sp[2:10].tolist() == list(span.first) and sp[10:18].tolist() == list(span.count)
+        want = np.array([list(r.row) + list(r.roll) for r in out], np.uint32)
+        assert np.array_equal(leaf[sp[0]: sp[0] + count], want)
+        if need <= 64 * 1024:
+            max_need = max(max_need, need)
+    assert d["level_lds_bytes"] == max_need
+    for (task, depth), runs in cover.items():
+        pos = 0
+        for first, count in sorted(runs):
+            assert first == pos
+            pos += count
+        assert pos == int(tasks["rows"][task])
