@@ -570,3 +570,16 @@ def test_level_layout_items_lie_in_nodes_and_carry_their_tables(nsamples, level_
             assert first == pos
             pos += count
         assert pos == int(tasks["rows"][task])
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and says
+    so before it touches a device."""
+    import subprocess
+    import sys as _sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([_sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 2
+    assert "measures one GPU" in r.stderr and r.stdout == ""
